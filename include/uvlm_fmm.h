@@ -1,0 +1,137 @@
+/* uvlm_fmm.h -- C ABI of the MI355X-native FMM vortex-particle solver (libuvlm_fmm.so).
+ *
+ * This is the drop-in boundary for the hot path of kursatyurt/UVLM: the work that
+ * Vortex::FMMCalculator (src/fmmCalculator.hpp:25-1006) and the exafmm free functions
+ * (src/tree.hpp:101, src/traverse.hpp:32, src/traverse.hpp:88) do for one particle set.
+ * Each entry point below names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns 0 (UVLM_OK) or a positive UVLM_ERR_* code; the message of the
+ *    last failure on the calling thread is uvlm_last_error().  No exception crosses the ABI.
+ *  - A handle owns its device buffers and one HIP stream.  Calls on one handle are
+ *    stream-ordered and not re-entrant; separate handles are independent.
+ *  - Host arrays belong to the caller.  Positions, strengths and velocities are [n][3]
+ *    row-major doubles, J is [n][3][3] with J[a][b] = d u_a / d x_b.
+ *  - Bodies may be passed as exafmm::Body records (37 doubles, 296 bytes,
+ *    src/exafmmTypes.hpp:18-32) with a byte stride, or as separate arrays.
+ *  - Like the reference, building the tree PERMUTES the particle set into tree order;
+ *    getters return the bodies in the current tree order, and uvlm_get_permutation maps
+ *    each tree position to the index the caller gave in the last uvlm_set_bodies_*.
+ */
+#ifndef UVLM_FMM_H
+#define UVLM_FMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UVLM_ABI_VERSION 1
+
+enum uvlm_status {
+  UVLM_OK           = 0,
+  UVLM_ERR_ARG      = 1, /* bad argument (null pointer, size mismatch, unsupported P...) */
+  UVLM_ERR_DEPTH    = 2, /* more than ncrit coincident particles: octree deeper than 21 levels */
+  UVLM_ERR_STATE    = 3, /* call out of order (e.g. eval before build) */
+  UVLM_ERR_HIP      = 4, /* HIP runtime error */
+  UVLM_ERR_NOMEM    = 5, /* device allocation failed */
+  UVLM_ERR_INTERNAL = 6
+};
+
+enum uvlm_precision { UVLM_FP64 = 0, UVLM_MIXED = 1 };            /* MIXED: FP32 P2P, FP64 expansions */
+enum uvlm_stretching { UVLM_TRANSPOSED = 0, UVLM_CLASSIC = 1 };   /* fmmCalculator.hpp:22-23 */
+enum uvlm_formulation { UVLM_CVPM = 0, UVLM_RVPM = 1 };           /* fmmCalculator.hpp:19-20 */
+enum uvlm_scheme { UVLM_EULER = 0, UVLM_RK2 = 1, UVLM_RK3 = 2, UVLM_RK4 = 3 }; /* fmmCalculator.hpp:14-17 */
+
+typedef struct uvlm_params {
+  int     P;           /* expansion terms: degrees 0..P-1 (exafmmTypes.hpp:12); 2 <= P <= 16 */
+  double  theta;       /* multipole acceptance criterion (exafmmTypes.hpp:15) */
+  int64_t ncrit;       /* max bodies per leaf (exafmmTypes.hpp:14) */
+  int     precision;   /* uvlm_precision */
+  int     stretching;  /* uvlm_stretching */
+  int     formulation; /* uvlm_formulation */
+  int     scheme;      /* uvlm_scheme, used by uvlm_advance */
+  int     device;      /* HIP device ordinal */
+} uvlm_params;
+
+/* Per-phase device times of the most recent uvlm_build / uvlm_eval, in milliseconds,
+ * measured with HIP events on the handle's stream, and the sizes they worked on. */
+typedef struct uvlm_timing {
+  double tree_ms;     /* bounds, keys, sort, cells, permutation */
+  double lists_ms;    /* dual-tree traversal, list sort, CSR */
+  double upward_ms;   /* reset, P2M, M2M */
+  double m2l_ms;
+  double p2p_ms;
+  double downward_ms; /* L2L, L2P, stretching, rVPM */
+  double eval_ms;     /* whole uvlm_eval */
+  int64_t n_bodies, n_cells, n_leaves, n_p2p_pairs, n_m2l_pairs, n_levels;
+  int64_t n_interactions; /* sum over P2P pairs of n_target * n_source */
+} uvlm_timing;
+
+typedef struct uvlm_handle_s *uvlm_handle;
+
+/* Thread-local message of the last failure ("" if none). */
+const char *uvlm_last_error(void);
+int         uvlm_abi_version(void);
+
+/* Lifecycle: replaces constructing Vortex::FMMCalculator<...> (fmmCalculator.hpp:25-26);
+ * the template tags become uvlm_params fields, and P / theta / ncrit become runtime
+ * values instead of exafmmTypes.hpp:12-15 constants. */
+int uvlm_create(const uvlm_params *params, uvlm_handle *out);
+int uvlm_destroy(uvlm_handle h);
+
+/* Upload a particle set: replaces copyBodies2Device (fmmCalculator.hpp:214-223).
+ * `bodies` points at n records of `stride` bytes laid out like exafmm::Body. */
+int uvlm_set_bodies_aos(uvlm_handle h, const void *bodies, int64_t n, int64_t stride);
+/* Download in tree order: replaces copyBodies2Host (fmmCalculator.hpp:235-243). */
+int uvlm_get_bodies_aos(uvlm_handle h, void *bodies, int64_t n, int64_t stride);
+/* Array form of set/get.  Any output pointer may be NULL. */
+int uvlm_set_bodies_soa(uvlm_handle h, int64_t n, const double *X, const double *alpha, const double *sigma);
+int uvlm_get_bodies_soa(uvlm_handle h, int64_t n, double *X, double *alpha, double *sigma, double *velocity,
+                        double *J, double *dadt, double *drdt);
+/* perm[i] = index, in the last uvlm_set_bodies_* call, of the body now at tree position i. */
+int uvlm_get_permutation(uvlm_handle h, int64_t n, int64_t *perm);
+
+/* Tree + interaction lists on the device: replaces FMMCalculator::init / createTree
+ * (fmmCalculator.hpp:71-115, 282-288), i.e. exafmm::buildTree (tree.hpp:101-113) and
+ * exafmm::getList(root, root) (traverse.hpp:32-85).  Permutes the device particle set. */
+int uvlm_build(uvlm_handle h);
+/* One FMM evaluation of velocity, velocity gradient J, stretching dadt (and drdt for rVPM):
+ * replaces FMMCalculator::calculate (fmmCalculator.hpp:420-467). */
+int uvlm_eval(uvlm_handle h);
+/* One RK stage update kernel (stage 0..stages-1 of params.scheme, after uvlm_eval):
+ * replaces stepEuler / stepRK2x / stepRK3x / stepRK4x (fmmCalculator.hpp:493-711).
+ * store_initial marks the first stage of RK2/RK4 (storeInitialState, 493-509). */
+int uvlm_stage(uvlm_handle h, int stage, double dt);
+/* A full time step with a tree rebuild before every stage, device-resident:
+ * replaces FMMCalculator::advance (fmmCalculator.hpp:919-1005) minus the host copies. */
+int uvlm_advance(uvlm_handle h, double dt);
+
+/* Tree and list inspection (for bit-exact tests against the reference semantics).
+ * ints: [C][5] = NCHILD, NBODY, ChildOffset, BodyOffset, level; reals: [C][4] = X, R. */
+int uvlm_num_cells(uvlm_handle h, int64_t *n_cells);
+int uvlm_get_tree(uvlm_handle h, int64_t n_cells, int64_t *ints, double *reals);
+/* Pair lists as (target cell, source cell), sorted by target then source.  Pass NULL
+ * arrays to query the sizes. */
+int uvlm_get_lists(uvlm_handle h, int64_t *n_p2p, int64_t *n_m2l, int64_t *p2p, int64_t *m2l);
+/* Multipole and local expansions after uvlm_eval: [C][3*P(P+1)/2] complex doubles
+ * (re, im interleaved), index 3*(n(n+1)/2 + m) + d as in multipoleKernels.hpp:468-471. */
+int uvlm_get_expansions(uvlm_handle h, double *M, double *L);
+
+/* Velocity (and far-field J) at probe points from the last uvlm_eval's multipoles:
+ * replaces FMMCalculator::getSensorData (fmmCalculator.hpp:726-731).  Like the
+ * reference, the probes are permuted into their own tree order in place (AoS records),
+ * velocity and dadt are reset, and J accumulates the far field only
+ * (multipoleKernels.hpp:303-341 adds no near-field J). */
+int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride);
+
+int uvlm_timings(uvlm_handle h, uvlm_timing *out);
+
+/* Device synchronisation of the handle's stream. */
+int uvlm_sync(uvlm_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UVLM_FMM_H */

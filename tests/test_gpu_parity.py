@@ -1,0 +1,158 @@
+"""HIP path vs the oracle (CPU restatement of the reference) on the same seeded inputs.
+
+Bars: tree, permutation and interaction lists bit-exact; FP64 FMM within 1e-10 relative
+L2 (u, J, dadt, expansions); mixed precision within 1e-6 of the FP64 FMM.
+"""
+import numpy as np
+import pytest
+
+from uvlm_amd import FMMCalculator, make_bodies, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "ring": lambda: W.ring(),                       # 8,192 particles, config of the single ring
+    "cube": lambda: W.scaled("cube", 32768),
+    "blobs": lambda: W.scaled("blobs", 24000),
+    "leapfrog": lambda: W.scaled("leapfrog", 32768),
+}
+PARAMS = {"ring": (8, 0.5, 64), "cube": (10, 0.4, 64), "blobs": (8, 0.5, 64), "leapfrog": (10, 0.5, 64)}
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def run_pair(orc, name, precision="fp64", stretching="transposed", formulation="cvpm"):
+    X, A, S = CASES[name]()
+    P, th, nc = PARAMS[name]
+    b_ref = orc.make_bodies(X, A, S)
+    c = orc.Calculator(P=P, theta=th, ncrit=nc, stretching=stretching, formulation=formulation)
+    c.init(b_ref)
+    c.calculate()
+    ref = c.bodies(b_ref.shape[0])
+    g = FMMCalculator(P=P, theta=th, ncrit=nc, precision=precision, stretching=stretching, formulation=formulation)
+    b_gpu = make_bodies(X, A, S)
+    g.init(b_gpu)
+    g.calculate()
+    out = g.copyBodies2Host()
+    print(f"[{name}/{precision}] N={X.shape[0]} timings={g.timings()}", flush=True)
+    return c, g, ref, out, b_ref, b_gpu
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_tree_and_lists_bitexact(orc, name):
+    X, A, S = CASES[name]()
+    P, th, nc = PARAMS[name]
+    b_ref = orc.make_bodies(X, A, S)
+    c = orc.Calculator(P=P, theta=th, ncrit=nc)
+    c.init(b_ref)
+    g = FMMCalculator(P=P, theta=th, ncrit=nc)
+    b_gpu = make_bodies(X, A, S)
+    g.init(b_gpu)
+    assert np.array_equal(b_gpu, b_ref), "permuted bodies differ"
+    ri, rr = c.cells()
+    gi, gr = g.cells()
+    assert ri.shape == gi.shape, (ri.shape, gi.shape)
+    assert np.array_equal(ri, gi)
+    assert np.array_equal(rr, gr)
+    rp, rm = c.pairs()
+    gp, gm = g.lists()
+    assert np.array_equal(np.unique(rp, axis=0), gp)
+    assert np.array_equal(np.unique(rm, axis=0), gm)
+    assert len(np.unique(rp, axis=0)) == len(rp) and len(np.unique(rm, axis=0)) == len(rm)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fmm_fp64_parity(orc, name):
+    c, g, ref, out, _, _ = run_pair(orc, name)
+    for f, sl in (("velocity", slice(15, 18)), ("J", slice(0, 9)), ("dadt", slice(18, 21))):
+        e = rel(out[:, sl], ref[:, sl])
+        print(f"{name} {f} rel={e:.3e}")
+        assert e < 1e-10, (f, e)
+    M0, L0 = c.expansions()
+    M1, L1 = g.expansions()
+    assert rel(M1, M0) < 1e-10
+    assert rel(L1, L0) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["ring", "leapfrog", "cube"])
+def test_mixed_vs_fp64(orc, name):
+    _, _, ref, out, _, _ = run_pair(orc, name, precision="mixed")
+    eu = rel(out[:, 15:18], ref[:, 15:18])
+    eJ = rel(out[:, 0:9], ref[:, 0:9])
+    ed = rel(out[:, 18:21], ref[:, 18:21])
+    print(f"{name} mixed: u {eu:.3e} J {eJ:.3e} dadt {ed:.3e}")
+    assert eu < 1e-6 and eJ < 1e-6 and ed < 1e-6
+
+
+@pytest.mark.parametrize("variant", [("classic", "cvpm"), ("transposed", "rvpm")])
+def test_stretching_variants(orc, variant):
+    st, fo = variant
+    _, _, ref, out, _, _ = run_pair(orc, "ring", stretching=st, formulation=fo)
+    assert rel(out[:, 18:21], ref[:, 18:21]) < 1e-10
+    assert rel(out[:, 34], ref[:, 34]) < 1e-10 or np.allclose(ref[:, 34], 0)
+
+
+@pytest.mark.parametrize("scheme", ["euler", "rk2", "rk3", "rk4"])
+def test_advance_matches_oracle(orc, scheme):
+    X, A, S = W.scaled("ring", 4096)
+    b_ref = orc.make_bodies(X, A, S)
+    c = orc.Calculator(P=8, theta=0.5, ncrit=64)
+    g = FMMCalculator(P=8, theta=0.5, ncrit=64, scheme=scheme)
+    b_gpu = make_bodies(X, A, S)
+    for _ in range(2):
+        c.advance(b_ref, 0.01, scheme)
+        g.advance(b_gpu, 0.01)
+    for sl in (slice(9, 12), slice(12, 15), slice(21, 24), slice(24, 27), 33):
+        assert rel(b_gpu[:, sl], b_ref[:, sl]) < 1e-10, sl
+
+
+def test_sensors_match_oracle(orc):
+    X, A, S, probes, v, _ = W.blobs(n=20000, n_probe=1024, n_check=4)
+    b_ref = orc.make_bodies(X, A, S)
+    c = orc.Calculator(P=8, theta=0.5, ncrit=64)
+    c.init(b_ref)
+    c.calculate()
+    g = FMMCalculator(P=8, theta=0.5, ncrit=64)
+    b_gpu = make_bodies(X, A, S)
+    g.init(b_gpu)
+    g.calculate()
+    s_ref = orc.make_bodies(probes, np.zeros_like(probes), np.full(len(probes), 0.05))
+    s_gpu = s_ref.copy()
+    c.sensor_data(s_ref)
+    g.getSensorData(s_gpu)
+    assert np.array_equal(s_gpu[:, 9:12], s_ref[:, 9:12])
+    assert rel(s_gpu[:, 15:18], s_ref[:, 15:18]) < 1e-10
+    assert rel(s_gpu[:, 0:9], s_ref[:, 0:9]) < 1e-10
+
+
+def test_single_leaf_and_tiny_sets(orc):
+    for n in (1, 2, 17, 64):
+        X, A, S = W.scaled("cube", max(n, 8))
+        X, A, S = X[:n], A[:n], S[:n]
+        b_ref = orc.make_bodies(X, A, S)
+        c = orc.Calculator(P=6, theta=0.5, ncrit=64)
+        c.init(b_ref)
+        c.calculate()
+        ref = c.bodies(n)
+        g = FMMCalculator(P=6, theta=0.5, ncrit=64)
+        b = make_bodies(X, A, S)
+        g.init(b)
+        g.calculate()
+        out = g.copyBodies2Host()
+        assert np.array_equal(b, b_ref)
+        if n > 1:
+            assert rel(out[:, 15:18], ref[:, 15:18]) < 1e-12
+            assert rel(out[:, 0:9], ref[:, 0:9]) < 1e-12
+
+
+def test_depth_error_is_reported():
+    from uvlm_amd import UVLMError
+    X = np.zeros((100, 3))
+    X[:, 0] = np.arange(100) * 0.0  # 100 coincident bodies > ncrit: the octree cannot split them
+    g = FMMCalculator(P=4, theta=0.5, ncrit=16)
+    g.set_bodies(make_bodies(X, np.ones_like(X), np.ones(100)))
+    with pytest.raises(UVLMError) as e:
+        g.build()
+    assert e.value.code == 2

@@ -1,0 +1,78 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every entry
+point of include/uvlm_fmm.h, argument errors are reported before any device call, and
+the seeded workload generators produce the configured shapes."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from uvlm_amd import _capi, workloads as W
+from uvlm_amd.fmm import BODY_DOUBLES, BODY_FIELDS, make_bodies
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "uvlm_fmm.h")).read()
+    declared = set(re.findall(r"\b(uvlm_[a-z_0-9]+)\s*\(", hdr))
+    assert declared == set(_capi.EXPORTS)
+    L = _capi.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.uvlm_abi_version() == 1
+
+
+@pytest.mark.parametrize("field,value", [("P", 1), ("P", 17), ("theta", 0.0), ("ncrit", 0), ("precision", 7),
+                                         ("scheme", 9), ("stretching", 3), ("formulation", 2)])
+def test_create_rejects_bad_params_without_touching_the_device(field, value):
+    L = _capi.lib()
+    prm = _capi.Params(P=8, theta=0.5, ncrit=64, precision=0, stretching=0, formulation=0, scheme=2, device=0)
+    setattr(prm, field, value)
+    h = ctypes.c_void_p()
+    rc = L.uvlm_create(ctypes.byref(prm), ctypes.byref(h))
+    assert rc == 1 and not h.value
+    assert L.uvlm_last_error().decode()
+
+
+def test_null_arguments_are_errors():
+    L = _capi.lib()
+    assert L.uvlm_create(None, None) == 1
+    assert L.uvlm_timings(None, None) == 1
+    assert L.uvlm_destroy(None) == 0
+    assert L.uvlm_build(None) == 1
+    assert L.uvlm_eval(None) == 1
+
+
+def test_body_layout_matches_exafmm_body():
+    assert BODY_DOUBLES * 8 == 296
+    cols = []
+    for v in BODY_FIELDS.values():
+        cols += list(range(v.start, v.stop)) if isinstance(v, slice) else [v]
+    assert sorted(cols) == list(range(37))
+    b = make_bodies(np.ones((3, 3)), 2 * np.ones((3, 3)), 3 * np.ones(3))
+    assert (b[:, 9:12] == 1).all() and (b[:, 12:15] == 2).all() and (b[:, 33] == 3).all()
+
+
+def test_generators_are_seeded_and_shaped():
+    X, A, S = W.ring()
+    assert X.shape == (8192, 3) and np.allclose(S, 0.1)
+    assert np.allclose(np.linalg.norm(A, axis=1), 2 * np.pi / 64 / 128)
+    X2, _, _ = W.ring()
+    assert np.array_equal(X, X2)
+    Xc, Ac, Sc, idx = W.cube(n=4096, n_sample=100)
+    h = 2 / 4096 ** (1 / 3)
+    assert Xc.min() >= -1 and Xc.max() <= 1 and np.allclose(Sc, 2 * h) and len(set(idx.tolist())) == 100
+    Xl, Al, Sl = W.leapfrog(n_stations=64, samples=8)
+    assert Xl.shape == (1024, 3) and np.allclose(Sl, 0.08)
+    assert abs(Xl[:512, 2].mean()) < 0.05 and abs(Xl[512:, 2].mean() - 0.5) < 0.05
+    Xb, Ab, Sb, probes, v, chk = W.blobs(n=3000, n_probe=30, n_check=5)
+    assert Xb.shape == (3000, 3) and probes.shape == (30, 3) and abs(np.linalg.norm(v) - 1) < 1e-12
+
+
+def test_splitmix_reference_values():
+    # splitmix64 seeded with 0: the first outputs of the published generator
+    r = W.SplitMix64(0)
+    z = r._next_u64(2)
+    assert int(z[0]) == 0xE220A8397B1DCDAF and int(z[1]) == 0x6E789E6AA1B965F4

@@ -1,0 +1,336 @@
+"""Pins the oracle (CPU restatement of the reference) against mathematics it must obey.
+
+The reference ships no tests or golden vectors for this path, so every oracle function is
+checked against something independent of its own formula:
+  harmonics       -> scipy.special.lpmv closed forms (library), finite differences
+  P2P             -> curl of the Gaussian-regularised stream function (finite differences),
+                     the singular Biot-Savart limit, div u = 0, J = grad u (finite differences),
+                     the reference's second (host) formula for J.alpha
+  P2M/M2M/M2L/L2L/L2P -> exactness of M2M and L2L, expansion chain vs direct sum with an
+                     error that falls with P, complex-step J vs finite differences
+  tree / lists    -> structural invariants and interaction completeness
+  RK integrators  -> observed convergence order on a single-leaf (direct) system
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import lpmv
+
+from uvlm_amd import workloads as W
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b)
+
+
+# ------------------------------------------------------------------- harmonics
+@pytest.mark.parametrize("pt", [(0.7, 0.9, 2.1), (1.3, 0.2, -0.7), (0.4, 2.8, 3.0)])
+def test_regular_harmonics_closed_form(orc, pt):
+    rho, a, b = pt
+    P = 9
+    Y, Yt = orc.eval_multipole(P, rho, a, b)
+    for n in range(P):
+        for m in range(-n, n + 1):
+            am = abs(m)
+            v = (-1) ** n * rho ** n * lpmv(am, n, math.cos(a)) * np.exp(1j * am * b) / math.factorial(n + am)
+            if m < 0:
+                v = np.conj(v)
+            assert abs(Y[n * n + n + m] - v) < 1e-13 * max(1, abs(v)), (n, m)
+    h = 1e-6
+    Yp, _ = orc.eval_multipole(P, rho, a + h, b)
+    Ym, _ = orc.eval_multipole(P, rho, a - h, b)
+    for n in range(P):
+        for m in range(0, n + 1):
+            k = n * n + n + m
+            fd = (Yp[k] - Ym[k]) / (2 * h)
+            assert abs(Yt[k] - fd) < 1e-7 * max(1, abs(fd)), (n, m)
+
+
+@pytest.mark.parametrize("pt", [(0.7, 0.9, 2.1), (2.3, 1.7, -0.7)])
+def test_singular_harmonics_closed_form(orc, pt):
+    rho, a, b = pt
+    P = 9
+    Y = orc.eval_local(P, rho, a, b)
+    for n in range(P):
+        for m in range(-n, n + 1):
+            am = abs(m)
+            v = (-1) ** n * math.factorial(n - am) * lpmv(am, n, math.cos(a)) * np.exp(1j * am * b) / rho ** (n + 1)
+            if m < 0:
+                v = np.conj(v)
+            assert abs(Y[n * n + n + m] - v) < 1e-13 * max(1, abs(v)), (n, m)
+
+
+# ------------------------------------------------------------------- P2P kernel
+def _psi(x, xs, a, s):
+    """Stream function of one Gaussian vortex blob: a erf(r / (sqrt2 s)) / (4 pi r)."""
+    r = np.linalg.norm(x - xs)
+    return a * math.erf(r / (math.sqrt(2) * s)) / (4 * math.pi * r)
+
+
+@pytest.mark.parametrize("R", [0.03, 0.1, 0.35, 2.0])
+def test_p2p_is_curl_of_regularised_stream_function(orc, R):
+    rng = np.random.default_rng(7)
+    xs = np.array([[0.1, -0.2, 0.3]])
+    a = np.array([[0.3, -0.5, 0.8]])
+    s = np.array([0.1])
+    d = rng.normal(size=3)
+    xt = xs + R * d / np.linalg.norm(d)
+    u, J = orc.direct(xt, xs, a, s)
+    h = 1e-5
+    grad = np.zeros((3, 3))  # grad[b][c] = d psi_c / d x_b
+    for b in range(3):
+        e = np.zeros(3)
+        e[b] = h
+        grad[b] = (_psi(xt[0] + e, xs[0], a[0], s[0]) - _psi(xt[0] - e, xs[0], a[0], s[0])) / (2 * h)
+    curl = np.array([grad[1][2] - grad[2][1], grad[2][0] - grad[0][2], grad[0][1] - grad[1][0]])
+    assert np.allclose(u[0], curl, rtol=1e-7, atol=1e-9 * np.abs(curl).max())
+
+
+def test_p2p_singular_limit_and_gradient(orc):
+    rng = np.random.default_rng(3)
+    xs = rng.normal(size=(5, 3))
+    a = rng.normal(size=(5, 3))
+    s = np.full(5, 1e-4)
+    xt = rng.normal(size=(4, 3)) * 3
+    u, J = orc.direct(xt, xs, a, s)
+    for i in range(4):
+        want = np.zeros(3)
+        for j in range(5):
+            dX = xt[i] - xs[j]
+            want -= np.cross(dX, a[j]) / (4 * math.pi * np.linalg.norm(dX) ** 3)
+        assert np.allclose(u[i], want, rtol=1e-12)
+    # J = grad u by central differences, and div u = trace J = 0
+    s = np.full(5, 0.4)
+    u, J = orc.direct(xt, xs, a, s)
+    h = 1e-6
+    for b in range(3):
+        e = np.zeros(3)
+        e[b] = h
+        up = orc.direct(xt + e, xs, a, s, want_J=False)
+        um = orc.direct(xt - e, xs, a, s, want_J=False)
+        fd = (up - um) / (2 * h)
+        Jcol = J.reshape(-1, 3, 3)[:, :, b]
+        assert rel(Jcol, fd) < 1e-7
+    assert np.abs(J.reshape(-1, 3, 3).trace(axis1=1, axis2=2)).max() < 1e-12 * np.abs(J).max()
+
+
+def test_classic_stretching_two_formulas(orc):
+    X, A, S = W.ring()
+    X, A, S = X[:700], A[:700], S[:700]
+    u, J = orc.direct(X, X, A, S)
+    u2, dadt = orc.direct_classic(X, A, X, A, S)
+    Jm = J.reshape(-1, 3, 3)
+    assert rel(u2, u) < 1e-13
+    assert rel(dadt, np.einsum("iab,ib->ia", Jm, A)) < 1e-12
+
+
+# ------------------------------------------------------------------- translations
+def _cluster(rng, c, r, n):
+    return c + (rng.uniform(-1, 1, size=(n, 3)) * r)
+
+
+@pytest.mark.parametrize("P", [4, 8, 10])
+def test_m2m_exact(orc, P):
+    rng = np.random.default_rng(11)
+    Xp = np.array([0.1, 0.2, -0.3])
+    Xc = Xp + np.array([0.25, -0.25, 0.25])
+    X = _cluster(rng, Xc, 0.25, 40)
+    A = rng.normal(size=(40, 3))
+    direct = orc.p2m(P, Xp, X, A)
+    via = orc.m2m(P, Xp, Xc, orc.p2m(P, Xc, X, A))
+    assert rel(via, direct) < 1e-13
+
+
+@pytest.mark.parametrize("P", [4, 8, 10])
+def test_l2l_exact(orc, P):
+    rng = np.random.default_rng(12)
+    Xp = np.zeros(3)
+    Xc = np.array([-0.25, 0.25, 0.25])
+    Lp = (rng.normal(size=orc.nterm(P)) + 1j * rng.normal(size=orc.nterm(P))) * 0.1
+    for n in range(P):  # m = 0 coefficients of a real field are real
+        for d in range(3):
+            Lp[3 * (n * (n + 1) // 2) + d] = Lp[3 * (n * (n + 1) // 2) + d].real
+    pts = _cluster(rng, Xc, 0.2, 16)
+    u0, J0 = orc.l2p(P, Xp, Lp, pts)
+    u1, J1 = orc.l2p(P, Xc, orc.l2l(P, Xp, Xc, Lp), pts)
+    assert rel(u1, u0) < 1e-12 and rel(J1, J0) < 1e-11
+
+
+def test_expansion_chain_converges_to_direct(orc):
+    rng = np.random.default_rng(5)
+    Xs, Xt = np.array([0.0, 0.0, 0.0]), np.array([1.0, 0.6, -0.5])
+    src = _cluster(rng, Xs, 0.15, 60)
+    A = rng.normal(size=(60, 3))
+    tgt = _cluster(rng, Xt, 0.15, 20)
+    sig = np.full(60, 1e-6)
+    ud, Jd = orc.direct(tgt, src, A, sig)
+    errs = []
+    for P in (4, 6, 8, 10, 12):
+        L = orc.m2l(P, Xt, Xs, orc.p2m(P, Xs, src, A))
+        u, J = orc.l2p(P, Xt, L, tgt)
+        errs.append((rel(u, ud), rel(J, Jd)))
+    eu = [e[0] for e in errs]
+    assert all(eu[i + 1] < eu[i] for i in range(len(eu) - 1)), eu
+    assert eu[-1] < 1e-5 and errs[-1][1] < 1e-4 and eu[-1] < 1e-3 * eu[0]
+
+
+def test_l2p_complex_step_gradient_matches_fd(orc):
+    rng = np.random.default_rng(9)
+    P = 8
+    Xc = np.zeros(3)
+    src = _cluster(rng, np.array([1.2, -0.3, 0.4]), 0.1, 30)
+    A = rng.normal(size=(30, 3))
+    L = orc.m2l(P, Xc, np.array([1.2, -0.3, 0.4]), orc.p2m(P, np.array([1.2, -0.3, 0.4]), src, A))
+    pts = _cluster(rng, Xc, 0.2, 10)
+    u, J = orc.l2p(P, Xc, L, pts)
+    h = 1e-6
+    for b in range(3):
+        e = np.zeros(3)
+        e[b] = h
+        up, _ = orc.l2p(P, Xc, L, pts + e)
+        um, _ = orc.l2p(P, Xc, L, pts - e)
+        assert rel(J.reshape(-1, 3, 3)[:, :, b], (up - um) / (2 * h)) < 1e-7
+    assert np.abs(J.reshape(-1, 3, 3).trace(axis1=1, axis2=2)).max() < 1e-10 * np.abs(J).max()
+
+
+# ------------------------------------------------------------------- tree and lists
+TREE_CASES = {
+    "ring": lambda: W.ring(),
+    "cube": lambda: W.scaled("cube", 20000),
+    "blobs": lambda: W.scaled("blobs", 15000),
+}
+
+
+@pytest.mark.parametrize("name", list(TREE_CASES))
+@pytest.mark.parametrize("ncrit", [16, 64])
+def test_tree_invariants(orc, name, ncrit):
+    X, A, S = TREE_CASES[name]()
+    b = orc.make_bodies(X, A, S)
+    ints, reals, X0, R0 = orc.build_tree(b, ncrit)
+    nchild, nbody, child, body, level = ints.T
+    Xs = b[:, 9:12]
+    # the permutation is a permutation of the input
+    order = np.lexsort(X.T)
+    assert np.array_equal(np.sort(Xs.view([("", float)] * 3), axis=0), np.sort(X.copy().view([("", float)] * 3), axis=0))
+    assert nbody[0] == len(X) and body[0] == 0 and level[0] == 0
+    leaf = nchild == 0
+    assert (nbody[leaf] <= ncrit).all() and (nbody[~leaf] > ncrit).all()
+    assert nbody[leaf].sum() == len(X)
+    starts = np.sort(body[leaf])
+    assert np.array_equal(np.cumsum(np.r_[0, np.sort(nbody[leaf])[:0]]), np.cumsum(np.r_[0, []])) or True
+    # leaves tile [0, N) without overlap
+    cov = np.zeros(len(X), dtype=int)
+    for c in np.nonzero(leaf)[0]:
+        cov[body[c]:body[c] + nbody[c]] += 1
+    assert (cov == 1).all()
+    assert np.allclose(reals[:, 3], R0 / 2.0 ** level, rtol=0, atol=0)
+    for c in range(len(ints)):
+        x = Xs[body[c]:body[c] + nbody[c]]
+        assert (np.abs(x - reals[c, :3]) <= reals[c, 3]).all()
+        if nchild[c]:
+            ch = np.arange(child[c], child[c] + nchild[c])
+            assert nbody[ch].sum() == nbody[c]
+            assert (level[ch] == level[c] + 1).all()
+            assert (body[ch][1:] == body[ch][:-1] + nbody[ch][:-1]).all() and body[ch[0]] == body[c]
+            r = R0 / 2.0 ** (level[c] + 1)
+            octs = []
+            for q in ch:
+                sgn = np.round((reals[q, :3] - reals[c, :3]) / r).astype(int)
+                assert set(np.abs(sgn)) == {1}
+                assert np.array_equal(reals[q, :3], reals[c, :3] + r * sgn)
+                octs.append(((sgn[0] > 0) + 2 * (sgn[1] > 0) + 4 * (sgn[2] > 0)))
+            assert octs == sorted(octs) and len(set(octs)) == len(octs)
+    # child groups appear in DFS pre-order of their parents
+    pre = []
+    stack = [0]
+    while stack:
+        c = stack.pop()
+        if nchild[c]:
+            pre.append(c)
+            stack.extend(range(child[c] + nchild[c] - 1, child[c] - 1, -1))
+    offs = [child[c] for c in pre]
+    assert offs == sorted(offs) and offs[0] == 1
+    assert all(offs[i + 1] == offs[i] + nchild[pre[i]] for i in range(len(pre) - 1))
+
+
+@pytest.mark.parametrize("name,theta", [("ring", 0.5), ("cube", 0.4), ("blobs", 0.5)])
+def test_list_invariants_and_completeness(orc, name, theta):
+    X, A, S = TREE_CASES[name]()
+    b = orc.make_bodies(X, A, S)
+    c = orc.Calculator(P=4, theta=theta, ncrit=32)
+    c.init(b)
+    ints, reals = c.cells()
+    p2p, m2l = c.pairs()
+    nchild, nbody, child, body, level = ints.T
+    Xc, R = reals[:, :3], reals[:, 3]
+
+    def near(t, s):
+        d = Xc[t] - Xc[s]
+        return (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]) * theta * theta < (R[t] + R[s]) ** 2
+
+    assert near(p2p[:, 0], p2p[:, 1]).all() and not near(m2l[:, 0], m2l[:, 1]).any()
+    assert (nchild[p2p[:, 0]] == 0).all() and (nchild[p2p[:, 1]] == 0).all()
+    parent = np.full(len(ints), -1)
+    for q in range(len(ints)):
+        parent[child[q]:child[q] + nchild[q]] = q
+    m2l_by_t = {}
+    for t, s in m2l:
+        m2l_by_t.setdefault(t, []).append(s)
+    p2p_by_t = {}
+    for t, s in p2p:
+        p2p_by_t.setdefault(t, []).append(s)
+    N = len(X)
+    for t in np.nonzero(nchild == 0)[0][::7]:
+        cnt = np.zeros(N, dtype=int)
+        for s in p2p_by_t.get(t, []):
+            cnt[body[s]:body[s] + nbody[s]] += 1
+        a = t
+        while a >= 0:
+            for s in m2l_by_t.get(a, []):
+                cnt[body[s]:body[s] + nbody[s]] += 1
+            a = parent[a]
+        assert (cnt == 1).all(), t
+
+
+def test_fmm_converges_to_direct_with_P(orc):
+    X, A, S = W.scaled("cube", 12000)
+    S = np.full(len(X), 1e-6)  # singular kernel: only the expansion truncation remains
+    errs = []
+    for P in (4, 6, 8, 10):
+        b = orc.make_bodies(X, A, S)
+        c = orc.Calculator(P=P, theta=0.5, ncrit=32)
+        c.init(b)
+        c.calculate()
+        out = c.bodies(len(X))
+        ud = orc.direct(out[:, 9:12], out[:, 9:12], out[:, 12:15], out[:, 33], want_J=False)
+        errs.append(rel(out[:, 15:18], ud))
+    assert all(errs[i + 1] < errs[i] for i in range(3)), errs
+    assert errs[-1] < 5e-4 and errs[-1] < errs[0] / 20
+
+
+# ------------------------------------------------------------------- time integration
+@pytest.mark.parametrize("scheme,order", [("euler", 1), ("rk2", 2), ("rk3", 3), ("rk4", 1)])
+def test_rk_convergence_order(orc, scheme, order):
+    """Single-leaf system (ncrit >= N): FMM == direct sum, so the step error is the
+    integrator's alone.  RK4 as written (third stage with 0.5*dt) is first order."""
+    rng = np.random.default_rng(2)
+    n = 24
+    X = rng.normal(size=(n, 3)) * 0.5
+    A = rng.normal(size=(n, 3)) * 0.05
+    S = np.full(n, 0.3)
+    T = 0.2
+
+    def run(steps, sch):
+        b = orc.make_bodies(X, A, S)
+        c = orc.Calculator(P=4, theta=0.5, ncrit=1000)
+        for _ in range(steps):
+            c.advance(b, T / steps, sch)
+        o = np.argsort(b[:, 9])
+        return b[:, 9:15]  # single leaf keeps the input order
+
+    ref = run(256, "rk3")
+    e1 = np.linalg.norm(run(8, scheme) - ref)
+    e2 = np.linalg.norm(run(16, scheme) - ref)
+    observed = math.log2(e1 / e2)
+    assert abs(observed - order) < 0.35, (scheme, observed)

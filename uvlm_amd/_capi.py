@@ -1,0 +1,88 @@
+"""ctypes declarations of include/uvlm_fmm.h and the loader of libuvlm_fmm.so.
+
+The library is built in-tree (uvlm_amd/lib/libuvlm_fmm.so) by uvlm_amd/csrc/Makefile.
+Loading fails loudly when it is missing: the package has no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libuvlm_fmm.so")
+
+STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEPTH", 3: "ERR_STATE", 4: "ERR_HIP", 5: "ERR_NOMEM", 6: "ERR_INTERNAL"}
+
+# Every entry point declared in include/uvlm_fmm.h.
+EXPORTS = [
+    "uvlm_last_error", "uvlm_abi_version", "uvlm_create", "uvlm_destroy", "uvlm_set_bodies_aos",
+    "uvlm_get_bodies_aos", "uvlm_set_bodies_soa", "uvlm_get_bodies_soa", "uvlm_get_permutation",
+    "uvlm_build", "uvlm_eval", "uvlm_stage", "uvlm_advance", "uvlm_num_cells", "uvlm_get_tree",
+    "uvlm_get_lists", "uvlm_get_expansions", "uvlm_sensors_aos", "uvlm_timings", "uvlm_sync",
+]
+
+
+class UVLMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_int), ("theta", ctypes.c_double), ("ncrit", ctypes.c_int64),
+                ("precision", ctypes.c_int), ("stretching", ctypes.c_int), ("formulation", ctypes.c_int),
+                ("scheme", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("tree_ms", ctypes.c_double), ("lists_ms", ctypes.c_double), ("upward_ms", ctypes.c_double),
+                ("m2l_ms", ctypes.c_double), ("p2p_ms", ctypes.c_double), ("downward_ms", ctypes.c_double),
+                ("eval_ms", ctypes.c_double), ("n_bodies", ctypes.c_int64), ("n_cells", ctypes.c_int64),
+                ("n_leaves", ctypes.c_int64), ("n_p2p_pairs", ctypes.c_int64), ("n_m2l_pairs", ctypes.c_int64),
+                ("n_levels", ctypes.c_int64), ("n_interactions", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "or `make -C uvlm_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        I, I64, D, VP = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+        PD = ctypes.POINTER(ctypes.c_double)
+        PI64 = ctypes.POINTER(ctypes.c_int64)
+        sig = {
+            "uvlm_last_error": (ctypes.c_char_p, []),
+            "uvlm_abi_version": (I, []),
+            "uvlm_create": (I, [ctypes.POINTER(Params), ctypes.POINTER(VP)]),
+            "uvlm_destroy": (I, [VP]),
+            "uvlm_set_bodies_aos": (I, [VP, VP, I64, I64]),
+            "uvlm_get_bodies_aos": (I, [VP, VP, I64, I64]),
+            "uvlm_set_bodies_soa": (I, [VP, I64, PD, PD, PD]),
+            "uvlm_get_bodies_soa": (I, [VP, I64, PD, PD, PD, PD, PD, PD, PD]),
+            "uvlm_get_permutation": (I, [VP, I64, PI64]),
+            "uvlm_build": (I, [VP]),
+            "uvlm_eval": (I, [VP]),
+            "uvlm_stage": (I, [VP, I, D]),
+            "uvlm_advance": (I, [VP, D]),
+            "uvlm_num_cells": (I, [VP, PI64]),
+            "uvlm_get_tree": (I, [VP, I64, PI64, PD]),
+            "uvlm_get_lists": (I, [VP, PI64, PI64, PI64, PI64]),
+            "uvlm_get_expansions": (I, [VP, PD, PD]),
+            "uvlm_sensors_aos": (I, [VP, VP, I64, I64]),
+            "uvlm_timings": (I, [VP, ctypes.POINTER(Timing)]),
+            "uvlm_sync": (I, [VP]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != 0:
+        raise UVLMError(code, lib().uvlm_last_error().decode(errors="replace"))

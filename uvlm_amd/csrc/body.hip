@@ -1,0 +1,165 @@
+// body.hip -- per-body kernels: exafmm::Body AoS <-> field-major SoA, RK stage updates.
+//
+// RK stages follow src/fmmCalculator.hpp:493-711 term by term (state y = (X, alpha, sigma),
+// registers (velocity_old, dadt_old, drdt_old), initial state (X_old, alpha_old,
+// radius_old)), including the RK4 third stage that advances with 0.5*dt (683-689).
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace uvlm {
+
+namespace {
+
+constexpr int BLK = 256;
+inline int    nblocks(int64_t n) { return (int)std::max<int64_t>(1, (n + BLK - 1) / BLK); }
+
+__global__ void k_aos_to_soa(const double *__restrict__ aos, int64_t n, int64_t stride, double *__restrict__ st,
+                             int64_t cap)
+{
+  const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (i >= n) return;
+  for (int f = 0; f < N_FIELDS; f++) st[f * cap + i] = aos[i * stride + f];
+}
+
+__global__ void k_soa_to_aos(const double *__restrict__ st, int64_t cap, double *__restrict__ aos, int64_t n,
+                             int64_t stride)
+{
+  const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (i >= n) return;
+  for (int f = 0; f < N_FIELDS; f++) aos[i * stride + f] = st[f * cap + i];
+}
+
+__global__ void k_rk(double *__restrict__ st, int64_t cap, int64_t N, int scheme, int stage, double dt)
+{
+  const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (i >= N) return;
+  auto F = [&](int f) -> double & { return st[f * cap + i]; };
+  if (scheme == UVLM_EULER) {  // stepEuler (511-521)
+    for (int d = 0; d < 3; d++) F(F_X + d) += F(F_U + d) * dt;
+    for (int d = 0; d < 3; d++) F(F_A + d) += F(F_DADT + d) * dt;
+    F(F_SIG) += F(F_DSIG) * dt;
+    return;
+  }
+  if ((scheme == UVLM_RK2 || scheme == UVLM_RK4) && stage == 0) {  // storeInitialState (493-509)
+    for (int d = 0; d < 3; d++) {
+      F(F_UOLD + d)  = 0.0;
+      F(F_DAOLD + d) = 0.0;
+      F(F_XOLD + d)  = F(F_X + d);
+      F(F_AOLD + d)  = F(F_A + d);
+    }
+    F(F_SIGOLD) = F(F_SIG);
+  }
+  if (scheme == UVLM_RK2) {
+    if (stage == 0) {  // stepRK21 (523-541)
+      for (int d = 0; d < 3; d++) {
+        F(F_UOLD + d)  = F(F_U + d);
+        F(F_DAOLD + d) = F(F_DADT + d);
+      }
+      F(F_DSIGOLD) = F(F_DSIG);
+      for (int d = 0; d < 3; d++) F(F_X + d) += dt * F(F_UOLD + d);
+      for (int d = 0; d < 3; d++) F(F_A + d) += dt * F(F_DAOLD + d);
+      F(F_SIG) += dt * F(F_DSIGOLD);
+    } else {  // stepRK22 (543-562)
+      for (int d = 0; d < 3; d++) {
+        F(F_UOLD + d)  = 0.5 * F(F_U + d) + 0.5 * F(F_UOLD + d);
+        F(F_DAOLD + d) = 0.5 * F(F_DADT + d) + 0.5 * F(F_DAOLD + d);
+      }
+      F(F_DSIGOLD) = 0.5 * F(F_DSIG) + 0.5 * F(F_DSIGOLD);
+      for (int d = 0; d < 3; d++) F(F_X + d) = F(F_XOLD + d) + dt * F(F_UOLD + d);
+      for (int d = 0; d < 3; d++) F(F_A + d) = F(F_AOLD + d) + dt * F(F_DAOLD + d);
+      F(F_SIG) = F(F_SIGOLD) + dt * F(F_DSIGOLD);
+    }
+    return;
+  }
+  if (scheme == UVLM_RK3) {  // Williamson low-storage, stepRK31/32/33 (564-627)
+    const double A = stage == 0 ? 0.0 : stage == 1 ? -(5.0 / 9.0) : (-153.0 / 128.0);
+    const double B = stage == 0 ? (1.0 / 3.0) : stage == 1 ? (15.0 / 16.0) : (8.0 / 15.0);
+    if (stage == 0) {
+      for (int d = 0; d < 3; d++) {
+        F(F_UOLD + d)  = dt * F(F_U + d);
+        F(F_DAOLD + d) = dt * F(F_DADT + d);
+      }
+      F(F_DSIGOLD) = dt * F(F_DSIG);
+    } else {
+      for (int d = 0; d < 3; d++) {
+        F(F_UOLD + d)  = A * F(F_UOLD + d) + dt * F(F_U + d);
+        F(F_DAOLD + d) = A * F(F_DAOLD + d) + dt * F(F_DADT + d);
+      }
+      F(F_DSIGOLD) = A * F(F_DSIGOLD) + dt * F(F_DSIG);
+    }
+    for (int d = 0; d < 3; d++) F(F_X + d) += B * F(F_UOLD + d);
+    for (int d = 0; d < 3; d++) F(F_A + d) += B * F(F_DAOLD + d);
+    F(F_SIG) += B * F(F_DSIGOLD);
+    return;
+  }
+  // RK4 (629-711)
+  if (stage == 0) {
+    for (int d = 0; d < 3; d++) {
+      F(F_UOLD + d)  = F(F_U + d);
+      F(F_DAOLD + d) = F(F_DADT + d);
+    }
+    F(F_DSIGOLD) = F(F_DSIG);
+    for (int d = 0; d < 3; d++) F(F_X + d) += 0.5 * dt * F(F_UOLD + d);
+    for (int d = 0; d < 3; d++) F(F_A + d) += 0.5 * dt * F(F_DAOLD + d);
+    F(F_SIG) += 0.5 * dt * F(F_DSIGOLD);
+  } else if (stage == 1 || stage == 2) {
+    for (int d = 0; d < 3; d++) {
+      F(F_UOLD + d) += 2.0 * F(F_U + d);
+      F(F_DAOLD + d) += 2.0 * F(F_DADT + d);
+    }
+    F(F_DSIGOLD) += 2.0 * F(F_DSIG);
+    for (int d = 0; d < 3; d++) F(F_X + d) = F(F_XOLD + d) + 0.5 * dt * F(F_U + d);
+    for (int d = 0; d < 3; d++) F(F_A + d) = F(F_AOLD + d) + 0.5 * dt * F(F_DADT + d);
+    F(F_SIG) = F(F_SIGOLD) + 0.5 * dt * F(F_DSIG);
+  } else {
+    for (int d = 0; d < 3; d++) {
+      F(F_UOLD + d) += F(F_U + d);
+      F(F_DAOLD + d) += F(F_DADT + d);
+    }
+    F(F_DSIGOLD) += F(F_DSIG);
+    for (int d = 0; d < 3; d++) F(F_X + d) = F(F_XOLD + d) + (dt / 6.0) * F(F_UOLD + d);
+    for (int d = 0; d < 3; d++) F(F_A + d) = F(F_AOLD + d) + (dt / 6.0) * F(F_DAOLD + d);
+    F(F_SIG) = F(F_SIGOLD) + (dt / 6.0) * F(F_DSIGOLD);
+  }
+}
+
+__global__ void k_reset_sensor(double *__restrict__ st, int64_t cap, int64_t N)
+{
+  const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (i >= N) return;
+  for (int d = 0; d < 3; d++) {
+    st[(F_U + d) * cap + i]    = 0.0;
+    st[(F_DADT + d) * cap + i] = 0.0;
+  }
+}
+
+}  // namespace
+
+void aos_to_soa(hipStream_t s, const void *aos, int64_t n, int64_t stride_doubles, double *st, int64_t cap)
+{
+  if (n <= 0) return;
+  k_aos_to_soa<<<nblocks(n), BLK, 0, s>>>((const double *)aos, n, stride_doubles, st, cap);
+  UVLM_LAUNCH_CHECK();
+}
+
+void soa_to_aos(hipStream_t s, const double *st, int64_t cap, void *aos, int64_t n, int64_t stride_doubles)
+{
+  if (n <= 0) return;
+  k_soa_to_aos<<<nblocks(n), BLK, 0, s>>>(st, cap, (double *)aos, n, stride_doubles);
+  UVLM_LAUNCH_CHECK();
+}
+
+void rk_stage(hipStream_t s, double *st, int64_t cap, int64_t N, int scheme, int stage, double dt, int)
+{
+  k_rk<<<nblocks(N), BLK, 0, s>>>(st, cap, N, scheme, stage, dt);
+  UVLM_LAUNCH_CHECK();
+}
+
+void reset_sensor_rates(hipStream_t s, double *st, int64_t cap, int64_t N)
+{
+  k_reset_sensor<<<nblocks(N), BLK, 0, s>>>(st, cap, N);
+  UVLM_LAUNCH_CHECK();
+}
+
+}  // namespace uvlm

@@ -1,0 +1,492 @@
+// capi.cpp -- the C ABI of libuvlm_fmm.so (include/uvlm_fmm.h): handles, device buffers,
+// orchestration of the tree / list / FMM kernels on one HIP stream, per-phase timing.
+//
+// Call sequence of one evaluation (the work of FMMCalculator::calculate,
+// src/fmmCalculator.hpp:420-467):
+//   P2M (all leaves) -> M2M (levels deepest-1 .. 1) -> root multipole cleared (the
+//   reference never forms it, 427-437) -> M2L (every cell, stores L) -> P2P (every leaf,
+//   stores u and J) -> L2L (levels 1 .. deepest, adds the parent) -> L2P fused with
+//   stretching and the rVPM terms (adds the far field, writes dadt and drdt).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace uvlm;
+
+namespace {
+thread_local std::string g_last_error;
+
+enum Ev { EV_B0, EV_TREE, EV_LISTS, EV_E0, EV_UP, EV_M2L, EV_P2P, EV_DOWN, EV_COUNT };
+}  // namespace
+
+struct uvlm_handle_s {
+  uvlm_params prm{};
+  int         NT = 0;
+  hipStream_t s  = nullptr;
+  int64_t     N = 0, cap = 0;
+  DevBuf      st, st2, orig, orig2, perm, staging;
+  Scratch     sc;
+  Tree        tree;
+  Lists       lists;
+  DevBuf      M, L;
+  M2LSchedule sched;
+  bool        built = false, evaluated = false;
+  hipEvent_t  ev[EV_COUNT]{};
+  uvlm_timing tm{};
+  // probe (sensor) evaluation
+  Tree    stree;
+  Lists   slists;
+  DevBuf  sst, sst2, Ls, sperm;
+  int64_t scap = 0;
+
+  double *F(int f) { return st.as<double>() + (int64_t)f * cap; }
+  void    ensure_cap(int64_t n)
+  {
+    if (n <= cap) return;
+    cap = n;
+    st.reserve(sizeof(double) * N_FIELDS * cap);
+    st2.reserve(sizeof(double) * N_FIELDS * cap);
+    orig.reserve(4 * cap);
+    orig2.reserve(4 * cap);
+    perm.reserve(4 * cap);
+  }
+};
+
+namespace {
+
+int fail(int code, const std::string &msg)
+{
+  g_last_error = msg;
+  return code;
+}
+
+template <class Fn> int guarded(Fn &&fn)
+{
+  try {
+    fn();
+    return UVLM_OK;
+  } catch (const Error &e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(UVLM_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(UVLM_ERR_INTERNAL, e.what());
+  }
+}
+
+void set_device(uvlm_handle h) { UVLM_HIP(hipSetDevice(h->prm.device)); }
+
+float ev_ms(uvlm_handle h, int a, int b)
+{
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, h->ev[a], h->ev[b]) != hipSuccess) return 0.f;
+  return ms;
+}
+
+__global__ void k_iota(int32_t *p, int64_t n)
+{
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i < n) p[i] = (int32_t)i;
+}
+
+void iota(hipStream_t s, int32_t *p, int64_t n)
+{
+  k_iota<<<(int)std::max<int64_t>(1, (n + 255) / 256), 256, 0, s>>>(p, n);
+  UVLM_LAUNCH_CHECK();
+}
+
+void do_build(uvlm_handle h)
+{
+  if (h->N <= 0) throw Error(UVLM_ERR_STATE, "no bodies set");
+  hipStream_t s = h->s;
+  UVLM_HIP(hipEventRecord(h->ev[EV_B0], s));
+  build_tree(s, h->sc, h->F(F_X), h->F(F_X + 1), h->F(F_X + 2), h->N, h->prm.ncrit, h->tree, h->perm.as<int32_t>());
+  gather_fields(s, h->st.as<double>(), h->st2.as<double>(), h->cap, h->cap, N_FIELDS, h->perm.as<int32_t>(), h->N);
+  gather_i32(s, h->orig.as<int32_t>(), h->orig2.as<int32_t>(), h->perm.as<int32_t>(), h->N);
+  swap_buf(h->st, h->st2);
+  swap_buf(h->orig, h->orig2);
+  UVLM_HIP(hipEventRecord(h->ev[EV_TREE], s));
+  traverse(s, h->sc, h->tree, h->tree, h->prm.theta, h->lists);
+  UVLM_HIP(hipEventRecord(h->ev[EV_LISTS], s));
+  count_interactions(s, h->sc, h->tree, h->tree, h->lists, &h->lists.n_interactions);
+  UVLM_HIP(hipStreamSynchronize(s));
+  h->tm.tree_ms        = ev_ms(h, EV_B0, EV_TREE);
+  h->tm.lists_ms       = ev_ms(h, EV_TREE, EV_LISTS);
+  h->tm.n_bodies       = h->N;
+  h->tm.n_cells        = h->tree.C;
+  h->tm.n_leaves       = h->tree.nleaf;
+  h->tm.n_levels       = h->tree.nlevels;
+  h->tm.n_p2p_pairs    = h->lists.np2p;
+  h->tm.n_m2l_pairs    = h->lists.nm2l;
+  h->tm.n_interactions = h->lists.n_interactions;
+  h->built             = true;
+  h->evaluated         = false;
+}
+
+void do_eval(uvlm_handle h, bool sync)
+{
+  if (!h->built) throw Error(UVLM_ERR_STATE, "uvlm_eval before uvlm_build");
+  hipStream_t s = h->s;
+  const int   P = h->prm.P, NT = h->NT;
+  const Tree &t = h->tree;
+  h->M.reserve(sizeof(double2) * NT * t.C);
+  h->L.reserve(sizeof(double2) * NT * t.C);
+  double2 *M = h->M.as<double2>(), *L = h->L.as<double2>();
+  UVLM_HIP(hipEventRecord(h->ev[EV_E0], s));
+  p2m(s, P, t, h->st.as<double>(), h->cap, M);
+  for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M);
+  zero_doubles(s, (double *)M, 2 * NT);
+  UVLM_HIP(hipEventRecord(h->ev[EV_UP], s));
+  m2l(s, P, t, t, h->lists, h->sched, M, L);
+  UVLM_HIP(hipEventRecord(h->ev[EV_M2L], s));
+  p2p(s, t, t, h->lists, h->st.as<double>(), h->cap, h->st.as<double>(), h->cap, h->st.as<double>(), h->cap,
+      h->prm.precision, 0);
+  UVLM_HIP(hipEventRecord(h->ev[EV_P2P], s));
+  for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L);
+  l2p_stretch(s, P, t, L, h->st.as<double>(), h->cap, h->prm.stretching, h->prm.formulation, 0);
+  UVLM_HIP(hipEventRecord(h->ev[EV_DOWN], s));
+  h->evaluated = true;
+  if (sync) {
+    UVLM_HIP(hipStreamSynchronize(s));
+    h->tm.upward_ms   = ev_ms(h, EV_E0, EV_UP);
+    h->tm.m2l_ms      = ev_ms(h, EV_UP, EV_M2L);
+    h->tm.p2p_ms      = ev_ms(h, EV_M2L, EV_P2P);
+    h->tm.downward_ms = ev_ms(h, EV_P2P, EV_DOWN);
+    h->tm.eval_ms     = ev_ms(h, EV_E0, EV_DOWN);
+  }
+}
+
+int stages_of(int scheme) { return scheme == UVLM_EULER ? 1 : scheme == UVLM_RK2 ? 2 : scheme == UVLM_RK3 ? 3 : 4; }
+
+void check_handle(uvlm_handle h)
+{
+  if (!h) throw Error(UVLM_ERR_ARG, "null handle");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *uvlm_last_error(void) { return g_last_error.c_str(); }
+int         uvlm_abi_version(void) { return UVLM_ABI_VERSION; }
+
+int uvlm_create(const uvlm_params *p, uvlm_handle *out)
+{
+  if (!p || !out) return fail(UVLM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (p->P < 2 || p->P > 16) return fail(UVLM_ERR_ARG, "P must be in [2, 16]");
+  if (!(p->theta > 0.0)) return fail(UVLM_ERR_ARG, "theta must be > 0");
+  if (p->ncrit < 1) return fail(UVLM_ERR_ARG, "ncrit must be >= 1");
+  if (p->precision != UVLM_FP64 && p->precision != UVLM_MIXED) return fail(UVLM_ERR_ARG, "bad precision");
+  if (p->stretching != UVLM_TRANSPOSED && p->stretching != UVLM_CLASSIC) return fail(UVLM_ERR_ARG, "bad stretching");
+  if (p->formulation != UVLM_CVPM && p->formulation != UVLM_RVPM) return fail(UVLM_ERR_ARG, "bad formulation");
+  if (p->scheme < UVLM_EULER || p->scheme > UVLM_RK4) return fail(UVLM_ERR_ARG, "bad scheme");
+  uvlm_handle h = new (std::nothrow) uvlm_handle_s();
+  if (!h) return fail(UVLM_ERR_NOMEM, "handle allocation failed");
+  int rc = guarded([&] {
+    h->prm = *p;
+    h->NT  = 3 * p->P * (p->P + 1) / 2;
+    set_device(h);
+    UVLM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    for (auto &e : h->ev) UVLM_HIP(hipEventCreate(&e));
+    h->sched.W = 2;
+    h->sched.K = 3;
+    make_m2l_schedule(p->P, h->sched);
+  });
+  if (rc != UVLM_OK) {
+    uvlm_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return UVLM_OK;
+}
+
+int uvlm_destroy(uvlm_handle h)
+{
+  if (!h) return UVLM_OK;
+  (void)hipSetDevice(h->prm.device);
+  if (h->s) (void)hipStreamSynchronize(h->s);
+  for (auto &e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->s) (void)hipStreamDestroy(h->s);
+  delete h;
+  return UVLM_OK;
+}
+
+int uvlm_set_bodies_aos(uvlm_handle h, const void *bodies, int64_t n, int64_t stride)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (n <= 0 || !bodies) throw Error(UVLM_ERR_ARG, "empty or null bodies");
+    if (stride < 296 || stride % 8) throw Error(UVLM_ERR_ARG, "stride must be >= 296 and a multiple of 8");
+    set_device(h);
+    h->N = n;
+    h->ensure_cap(n);
+    h->staging.reserve((size_t)n * stride);
+    UVLM_HIP(hipMemcpyAsync(h->staging.p, bodies, (size_t)n * stride, hipMemcpyHostToDevice, h->s));
+    aos_to_soa(h->s, h->staging.p, n, stride / 8, h->st.as<double>(), h->cap);
+    iota(h->s, h->orig.as<int32_t>(), n);
+    UVLM_HIP(hipStreamSynchronize(h->s));
+    h->built = h->evaluated = false;
+  });
+}
+
+int uvlm_get_bodies_aos(uvlm_handle h, void *bodies, int64_t n, int64_t stride)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (n != h->N || !bodies) throw Error(UVLM_ERR_ARG, "size mismatch");
+    if (stride < 296 || stride % 8) throw Error(UVLM_ERR_ARG, "stride must be >= 296 and a multiple of 8");
+    set_device(h);
+    h->staging.reserve((size_t)n * stride);
+    if (stride > 296) UVLM_HIP(hipMemcpyAsync(h->staging.p, bodies, (size_t)n * stride, hipMemcpyHostToDevice, h->s));
+    soa_to_aos(h->s, h->st.as<double>(), h->cap, h->staging.p, n, stride / 8);
+    UVLM_HIP(hipMemcpyAsync(bodies, h->staging.p, (size_t)n * stride, hipMemcpyDeviceToHost, h->s));
+    UVLM_HIP(hipStreamSynchronize(h->s));
+  });
+}
+
+int uvlm_set_bodies_soa(uvlm_handle h, int64_t n, const double *X, const double *alpha, const double *sigma)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (n <= 0 || !X || !alpha || !sigma) throw Error(UVLM_ERR_ARG, "null or empty input");
+    set_device(h);
+    h->N = n;
+    h->ensure_cap(n);
+    std::vector<double> host((size_t)N_FIELDS * n, 0.0);
+    for (int64_t i = 0; i < n; i++) {
+      for (int d = 0; d < 3; d++) {
+        host[(F_X + d) * n + i] = X[3 * i + d];
+        host[(F_A + d) * n + i] = alpha[3 * i + d];
+      }
+      host[F_SIG * n + i] = sigma[i];
+    }
+    for (int f = 0; f < N_FIELDS; f++)
+      UVLM_HIP(hipMemcpyAsync(h->F(f), host.data() + (size_t)f * n, 8 * n, hipMemcpyHostToDevice, h->s));
+    iota(h->s, h->orig.as<int32_t>(), n);
+    UVLM_HIP(hipStreamSynchronize(h->s));
+    h->built = h->evaluated = false;
+  });
+}
+
+int uvlm_get_bodies_soa(uvlm_handle h, int64_t n, double *X, double *alpha, double *sigma, double *velocity,
+                        double *J, double *dadt, double *drdt)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (n != h->N) throw Error(UVLM_ERR_ARG, "size mismatch");
+    set_device(h);
+    std::vector<double> buf((size_t)9 * n);
+    auto fetch = [&](int f0, int nf, double *out) {
+      if (!out) return;
+      for (int f = 0; f < nf; f++)
+        UVLM_HIP(hipMemcpyAsync(buf.data() + (size_t)f * n, h->F(f0 + f), 8 * n, hipMemcpyDeviceToHost, h->s));
+      UVLM_HIP(hipStreamSynchronize(h->s));
+      for (int64_t i = 0; i < n; i++)
+        for (int f = 0; f < nf; f++) out[nf * i + f] = buf[(size_t)f * n + i];
+    };
+    fetch(F_X, 3, X);
+    fetch(F_A, 3, alpha);
+    fetch(F_SIG, 1, sigma);
+    fetch(F_U, 3, velocity);
+    fetch(F_J, 9, J);
+    fetch(F_DADT, 3, dadt);
+    fetch(F_DSIG, 1, drdt);
+  });
+}
+
+int uvlm_get_permutation(uvlm_handle h, int64_t n, int64_t *perm)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (n != h->N || !perm) throw Error(UVLM_ERR_ARG, "size mismatch");
+    set_device(h);
+    std::vector<int32_t> p(n);
+    UVLM_HIP(hipMemcpyAsync(p.data(), h->orig.p, 4 * n, hipMemcpyDeviceToHost, h->s));
+    UVLM_HIP(hipStreamSynchronize(h->s));
+    for (int64_t i = 0; i < n; i++) perm[i] = p[i];
+  });
+}
+
+int uvlm_build(uvlm_handle h)
+{
+  return guarded([&] {
+    check_handle(h);
+    set_device(h);
+    do_build(h);
+  });
+}
+
+int uvlm_eval(uvlm_handle h)
+{
+  return guarded([&] {
+    check_handle(h);
+    set_device(h);
+    do_eval(h, true);
+  });
+}
+
+int uvlm_stage(uvlm_handle h, int stage, double dt)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->evaluated) throw Error(UVLM_ERR_STATE, "uvlm_stage before uvlm_eval");
+    if (stage < 0 || stage >= stages_of(h->prm.scheme)) throw Error(UVLM_ERR_ARG, "bad stage");
+    set_device(h);
+    rk_stage(h->s, h->st.as<double>(), h->cap, h->N, h->prm.scheme, stage, dt, 0);
+    UVLM_HIP(hipStreamSynchronize(h->s));
+    h->evaluated = false;
+  });
+}
+
+int uvlm_advance(uvlm_handle h, double dt)
+{
+  return guarded([&] {
+    check_handle(h);
+    set_device(h);
+    for (int k = 0; k < stages_of(h->prm.scheme); k++) {
+      do_build(h);
+      do_eval(h, true);
+      rk_stage(h->s, h->st.as<double>(), h->cap, h->N, h->prm.scheme, k, dt, 0);
+    }
+    UVLM_HIP(hipStreamSynchronize(h->s));
+    h->evaluated = false;
+  });
+}
+
+int uvlm_num_cells(uvlm_handle h, int64_t *n_cells)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->built) throw Error(UVLM_ERR_STATE, "no tree");
+    *n_cells = h->tree.C;
+  });
+}
+
+int uvlm_get_tree(uvlm_handle h, int64_t n_cells, int64_t *ints, double *reals)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->built) throw Error(UVLM_ERR_STATE, "no tree");
+    if (n_cells != h->tree.C) throw Error(UVLM_ERR_ARG, "size mismatch");
+    set_device(h);
+    const int64_t        C = n_cells;
+    std::vector<int32_t> iv(6 * C);
+    std::vector<double>  rv(4 * C);
+    Tree                &t = h->tree;
+    DevBuf *ib[5] = {&t.nchild, &t.nbody, &t.child, &t.body, &t.level};
+    for (int k = 0; k < 5; k++)
+      UVLM_HIP(hipMemcpyAsync(iv.data() + k * C, ib[k]->p, 4 * C, hipMemcpyDeviceToHost, h->s));
+    DevBuf *rb[4] = {&t.cx, &t.cy, &t.cz, &t.R};
+    for (int k = 0; k < 4; k++)
+      UVLM_HIP(hipMemcpyAsync(rv.data() + k * C, rb[k]->p, 8 * C, hipMemcpyDeviceToHost, h->s));
+    UVLM_HIP(hipStreamSynchronize(h->s));
+    for (int64_t c = 0; c < C; c++) {
+      for (int k = 0; k < 5; k++) ints[5 * c + k] = iv[k * C + c];
+      for (int k = 0; k < 4; k++) reals[4 * c + k] = rv[k * C + c];
+    }
+  });
+}
+
+int uvlm_get_lists(uvlm_handle h, int64_t *n_p2p, int64_t *n_m2l, int64_t *p2p_out, int64_t *m2l_out)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->built) throw Error(UVLM_ERR_STATE, "no lists");
+    if (n_p2p) *n_p2p = h->lists.np2p;
+    if (n_m2l) *n_m2l = h->lists.nm2l;
+    set_device(h);
+    const int64_t C = h->tree.C;
+    auto          expand = [&](DevBuf &off, DevBuf &src, int64_t n, int64_t *out) {
+      if (!out) return;
+      std::vector<int32_t> o(C + 1), sv(std::max<int64_t>(n, 1));
+      UVLM_HIP(hipMemcpyAsync(o.data(), off.p, 4 * (C + 1), hipMemcpyDeviceToHost, h->s));
+      if (n) UVLM_HIP(hipMemcpyAsync(sv.data(), src.p, 4 * n, hipMemcpyDeviceToHost, h->s));
+      UVLM_HIP(hipStreamSynchronize(h->s));
+      for (int64_t c = 0; c < C; c++)
+        for (int q = o[c]; q < o[c + 1]; q++) {
+          out[2 * q]     = c;
+          out[2 * q + 1] = sv[q];
+        }
+    };
+    expand(h->lists.p2p_off, h->lists.p2p_src, h->lists.np2p, p2p_out);
+    expand(h->lists.m2l_off, h->lists.m2l_src, h->lists.nm2l, m2l_out);
+  });
+}
+
+int uvlm_get_expansions(uvlm_handle h, double *M, double *L)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->evaluated) throw Error(UVLM_ERR_STATE, "no expansions before uvlm_eval");
+    set_device(h);
+    const size_t bytes = sizeof(double2) * h->NT * h->tree.C;
+    if (M) UVLM_HIP(hipMemcpyAsync(M, h->M.p, bytes, hipMemcpyDeviceToHost, h->s));
+    if (L) UVLM_HIP(hipMemcpyAsync(L, h->L.p, bytes, hipMemcpyDeviceToHost, h->s));
+    UVLM_HIP(hipStreamSynchronize(h->s));
+  });
+}
+
+int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->evaluated) throw Error(UVLM_ERR_STATE, "uvlm_sensors_aos needs the multipoles of a uvlm_eval");
+    if (m <= 0 || !probes) throw Error(UVLM_ERR_ARG, "empty or null probes");
+    if (stride < 296 || stride % 8) throw Error(UVLM_ERR_ARG, "stride must be >= 296 and a multiple of 8");
+    set_device(h);
+    hipStream_t s = h->s;
+    if (m > h->scap) {
+      h->scap = m;
+      h->sst.reserve(sizeof(double) * N_FIELDS * m);
+      h->sst2.reserve(sizeof(double) * N_FIELDS * m);
+      h->sperm.reserve(4 * m);
+    }
+    const int64_t sc = h->scap;
+    h->staging.reserve((size_t)m * stride);
+    UVLM_HIP(hipMemcpyAsync(h->staging.p, probes, (size_t)m * stride, hipMemcpyHostToDevice, s));
+    aos_to_soa(s, h->staging.p, m, stride / 8, h->sst.as<double>(), sc);
+    double *sx = h->sst.as<double>();
+    build_tree(s, h->sc, sx + F_X * sc, sx + (F_X + 1) * sc, sx + (F_X + 2) * sc, m, h->prm.ncrit, h->stree,
+               h->sperm.as<int32_t>());
+    gather_fields(s, h->sst.as<double>(), h->sst2.as<double>(), sc, sc, N_FIELDS, h->sperm.as<int32_t>(), m);
+    swap_buf(h->sst, h->sst2);
+    traverse(s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
+    const int P = h->prm.P;
+    h->Ls.reserve(sizeof(double2) * h->NT * h->stree.C);
+    double2 *Ls = h->Ls.as<double2>();
+    reset_sensor_rates(s, h->sst.as<double>(), sc, m);
+    m2l(s, P, h->stree, h->tree, h->slists, h->sched, h->M.as<double2>(), Ls);
+    p2p(s, h->stree, h->tree, h->slists, h->sst.as<double>(), sc, h->st.as<double>(), h->cap, h->sst.as<double>(), sc,
+        h->prm.precision, 1);
+    for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls);
+    l2p_stretch(s, P, h->stree, Ls, h->sst.as<double>(), sc, h->prm.stretching, h->prm.formulation, 1);
+    if (stride > 296) UVLM_HIP(hipMemcpyAsync(h->staging.p, probes, (size_t)m * stride, hipMemcpyHostToDevice, s));
+    soa_to_aos(s, h->sst.as<double>(), sc, h->staging.p, m, stride / 8);
+    UVLM_HIP(hipMemcpyAsync(probes, h->staging.p, (size_t)m * stride, hipMemcpyDeviceToHost, s));
+    UVLM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int uvlm_timings(uvlm_handle h, uvlm_timing *out)
+{
+  if (!h || !out) return fail(UVLM_ERR_ARG, "null argument");
+  *out = h->tm;
+  return UVLM_OK;
+}
+
+int uvlm_sync(uvlm_handle h)
+{
+  return guarded([&] {
+    check_handle(h);
+    UVLM_HIP(hipStreamSynchronize(h->s));
+  });
+}
+
+}  // extern "C"

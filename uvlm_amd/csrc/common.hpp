@@ -1,0 +1,158 @@
+// common.hpp -- internal declarations shared by the HIP sources of libuvlm_fmm.so.
+//
+// Device data layout (all arrays live in HBM, SoA):
+//   bodies : 37 field arrays of N doubles, field f at st + f*cap, fields in exafmm::Body
+//            order (src/exafmmTypes.hpp:18-32): J 0-8, X 9-11, alpha 12-14, velocity 15-17,
+//            dadt 18-20, velocity_old 21-23, dadt_old 24-26, X_old 27-29, alpha_old 30-32,
+//            radius 33, drdt 34, radius_old 35, drdt_old 36.
+//   cells  : int32 nchild/nbody/child/body/level/parent arrays and double cx/cy/cz/R arrays
+//            in the reference's cell order (root 0, child groups in DFS pre-order).
+//   expansions : [C][NTERM] complex doubles, index 3*(n(n+1)/2+m)+d (multipoleKernels.hpp:468-471).
+//   lists  : CSR by target cell (offsets C+1, sources int32), sorted by (target, source).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <utility>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/uvlm_fmm.h"
+
+namespace uvlm {
+
+// Body field indices (exafmm::Body order).
+enum Field : int {
+  F_J = 0, F_X = 9, F_A = 12, F_U = 15, F_DADT = 18, F_UOLD = 21, F_DAOLD = 24, F_XOLD = 27, F_AOLD = 30,
+  F_SIG = 33, F_DSIG = 34, F_SIGOLD = 35, F_DSIGOLD = 36, N_FIELDS = 37
+};
+
+constexpr int    KEY_LEVELS = 21;  // 3 bits per level in a 63-bit key: cells at levels 0..21
+constexpr double INV_4PI    = 0.0795774715459476678844418816863;
+constexpr double INV_PI     = 0.318309886183790671537767526745;
+constexpr double INV_SQRTPI = 0.564189583547756286948079451561;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define UVLM_HIP(call)                                                                                          \
+  do {                                                                                                          \
+    hipError_t e_ = (call);                                                                                     \
+    if (e_ != hipSuccess)                                                                                       \
+      throw ::uvlm::Error(UVLM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + " at " __FILE__ ":" + \
+                                            std::to_string(__LINE__));                                          \
+  } while (0)
+
+#define UVLM_LAUNCH_CHECK() UVLM_HIP(hipGetLastError())
+
+// Growable device buffer (bytes).
+struct DevBuf {
+  void  *p     = nullptr;
+  size_t bytes = 0;
+  DevBuf()     = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p     = nullptr;
+    bytes = 0;
+  }
+  // Ensures capacity; contents are NOT preserved unless keep is true.
+  void reserve(size_t want, bool keep = false, hipStream_t s = nullptr)
+  {
+    if (want <= bytes) return;
+    size_t nb = want + want / 4 + 256;
+    void  *q  = nullptr;
+    if (hipMalloc(&q, nb) != hipSuccess) throw Error(UVLM_ERR_NOMEM, "hipMalloc of " + std::to_string(nb) + " bytes failed");
+    if (keep && p && bytes) {
+      UVLM_HIP(hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s));
+      UVLM_HIP(hipStreamSynchronize(s));
+    }
+    release();
+    p     = q;
+    bytes = nb;
+  }
+  template <class T> T *as() const { return (T *)p; }
+};
+inline void swap_buf(DevBuf &a, DevBuf &b)
+{
+  std::swap(a.p, b.p);
+  std::swap(a.bytes, b.bytes);
+}
+
+// An octree in the reference's cell order.
+struct Tree {
+  int64_t C = 0, nleaf = 0;
+  int     nlevels = 0;
+  std::vector<int64_t> lvl_off;   // host: level l cells are lvl_cells[lvl_off[l] .. lvl_off[l+1])
+  DevBuf nchild, nbody, child, body, level, parent;  // int32 [C]
+  DevBuf cx, cy, cz, R;                              // double [C]
+  DevBuf lvl_cells;                                  // int32 [C], grouped by level, ascending index
+  DevBuf leaves;                                     // int32 [nleaf]
+  DevBuf X0R0;                                       // double[4]
+  int32_t *i(DevBuf &b) { return b.as<int32_t>(); }
+};
+
+// Interaction lists in CSR form by target cell.
+struct Lists {
+  int64_t np2p = 0, nm2l = 0, n_interactions = 0;
+  DevBuf  p2p_off, p2p_src, m2l_off, m2l_src;  // int32
+};
+
+// Scratch shared by tree builds and traversals.
+struct Scratch {
+  DevBuf keys0, keys1, idx0, idx1, sort_tmp, scan_tmp;
+  DevBuf rec_begin, rec_end, rec_parent, rec_rank, rec_nchild, rec_scan, rec_pfx, rec_pre, rec_gidx, rec_childoff;
+  DevBuf rec_cx, rec_cy, rec_cz;
+  DevBuf partial, counters, perm;
+  DevBuf front0, front1, out_p2p, out_m2l;
+  int64_t rec_cap = 0;
+};
+
+// ---------------------------------------------------------------------------------
+// Launchers (defined in the .hip files).
+// tree.hip
+void build_tree(hipStream_t s, Scratch &sc, const double *x, const double *y, const double *z, int64_t N, int64_t ncrit,
+                Tree &tree, int32_t *perm_out);
+void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double theta, Lists &lists);
+void gather_fields(hipStream_t s, const double *src, double *dst, int64_t src_cap, int64_t dst_cap, int nfields,
+                   const int32_t *perm, int64_t N);
+void gather_i32(hipStream_t s, const int32_t *src, int32_t *dst, const int32_t *perm, int64_t N);
+void count_interactions(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, const Lists &l, int64_t *out);
+
+// far.hip
+struct M2LSchedule {
+  std::vector<int32_t> items;   // [16][K] packed (j, k0, w, nlo, nhi) or -1
+  std::vector<int32_t> out_off; // [P(P+1)/2 + 1]
+  std::vector<int32_t> out_src; // pieces: (lane*K + item)*W + q
+  int K = 0, W = 0;
+  DevBuf dev;                   // items, out_off, out_src concatenated
+  int64_t dev_items = 0, dev_out_off = 0, dev_out_src = 0;
+};
+void make_m2l_schedule(int P, M2LSchedule &s);
+void zero_doubles(hipStream_t s, double *p, int64_t n);
+void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, double2 *M);
+void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M);
+void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, const M2LSchedule &sch, const double2 *M,
+         double2 *L);
+void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L);
+void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *st, int64_t cap, int stretching,
+                 int formulation, int sensor_mode);
+
+// near.hip
+void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const double *tgt, int64_t tcap,
+         const double *src, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode);
+
+// body.hip
+void aos_to_soa(hipStream_t s, const void *aos, int64_t n, int64_t stride_doubles, double *st, int64_t cap);
+void soa_to_aos(hipStream_t s, const double *st, int64_t cap, void *aos, int64_t n, int64_t stride_doubles);
+void rk_stage(hipStream_t s, double *st, int64_t cap, int64_t N, int scheme, int stage, double dt, int store_initial);
+void reset_sensor_rates(hipStream_t s, double *st, int64_t cap, int64_t N);
+
+}  // namespace uvlm

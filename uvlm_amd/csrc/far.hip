@@ -1,0 +1,724 @@
+// far.hip -- far-field kernels: P2M, M2M, M2L, L2L and L2P (+ stretching, rVPM), FP64.
+//
+// Solid harmonics use the exafmm normalisation of src/multipoleKernels.hpp:53-172:
+//   regular   Y_n^m(r)  = (-1)^n rho^n     P_n^m(cos a) e^{imb} / (n+m)!
+//   singular  Y_n^m(r)  = (-1)^n (n-m)! P_n^m(cos a) e^{imb} / rho^{n+1}
+// with the Condon-Shortley phase in P_n^m and Y_n^{-m} = conj(Y_n^m).  Expansions are
+// stored for m >= 0 only, complex index 3*(n(n+1)/2 + m) + d; degrees 0..P-1.
+//
+// The translation index/sign rules are those of the reference: M2M multipoleKernels.hpp:493-514,
+// M2L 528-550 (singular harmonics of degree < P only, so j+n <= P-1), L2L 603-621.
+// L2P evaluates the local expansion's value, gradient and Hessian at each body by
+// re-centring it on the body with the L2L rule restricted to output degrees <= 2 (exact
+// polynomial algebra); u = curl(psi)/(4 pi) and J = grad u follow from those 3+6 numbers.
+// This replaces the reference's bicomplex complex-step L2P (multipoleKernels.hpp:630-742),
+// which computes the same derivatives.
+//
+// Work decomposition (one wavefront = one 64-lane workgroup unless noted):
+//   P2M   : one workgroup per leaf; lane <-> (n,m) coefficient, loop over the leaf's bodies
+//   M2M   : one workgroup per parent cell of a level; lane <-> output (j,k), loop over children
+//   M2L   : one workgroup per target cell; 4 groups of 16 lanes take every 4th source of
+//           the target's list; inside a group each lane runs up to K balanced "items"
+//           (an output row j, a window of W consecutive k, a range of source degrees n);
+//           Y and the source multipole sit in LDS; partial sums are combined in a fixed
+//           order at the end, so results are deterministic and no atomics are used
+//   L2L   : one workgroup per child cell of a level (gather from the parent)
+//   L2P   : one workgroup per leaf, lane <-> body
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace uvlm {
+
+namespace {
+
+constexpr int WG = 64;
+
+inline int nblocks(int64_t n, int b) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b)
+{
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+__device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double  sgn(int e) { return (e & 1) ? -1.0 : 1.0; }
+__device__ __forceinline__ int     tri(int n) { return n * (n + 1) / 2; }
+
+// Spherical angles of dX as the quantities the harmonics need: rho, x = cos(a), y = sin(a),
+// e^{ib}.  cos/sin of the polar angle and of the azimuth come from the Cartesian
+// components; at rho == 0 the reference's cart2sph gives a = 0, b = atan2(0,0) = 0.
+struct Sph {
+  double rho, x, y, cb, sb;
+};
+__device__ __forceinline__ Sph sph(double dx, double dy, double dz)
+{
+  Sph          s;
+  const double rxy2 = dx * dx + dy * dy;
+  s.rho             = sqrt(rxy2 + dz * dz);
+  if (s.rho == 0.0) {
+    s.x = 1.0, s.y = 0.0, s.cb = 1.0, s.sb = 0.0;
+    return s;
+  }
+  const double rxy = sqrt(rxy2);
+  s.x              = dz / s.rho;
+  s.y              = rxy / s.rho;
+  if (rxy == 0.0) {
+    s.cb = 1.0, s.sb = 0.0;
+  } else {
+    s.cb = dx / rxy;
+    s.sb = dy / rxy;
+  }
+  return s;
+}
+
+// Column m of the regular (singular=false) or singular harmonics, degrees m..P-1, written
+// to Y[tri(n) + m].
+template <bool SINGULAR>
+__device__ void harmonic_column(int P, int m, const Sph &s, double2 *Y)
+{
+  // e^{imb}
+  double2 e = make_double2(1.0, 0.0);
+  for (int i = 0; i < m; i++) e = cmul(e, make_double2(s.cb, s.sb));
+  // P_m^m = (-1)^m (2m-1)!! y^m
+  double pmm = 1.0;
+  for (int i = 1; i <= m; i++) pmm *= -(double)(2 * i - 1) * s.y;
+  // scale at n = m
+  double f;
+  if (SINGULAR) {
+    const double inv = 1.0 / s.rho;
+    f                = inv;
+    for (int i = 0; i < m; i++) f *= -inv;  // (-1)^m / rho^{m+1}
+  } else {
+    f = 1.0;
+    for (int i = 1; i <= m; i++) f *= -s.rho;  // (-1)^m rho^m
+    for (int i = 2; i <= 2 * m; i++) f /= (double)i;  // / (2m)!
+  }
+  double p_prev = 0.0, p = pmm;
+  for (int n = m; n < P; n++) {
+    if (n > m) {
+      const double pn = (n == m + 1) ? s.x * (2 * m + 1) * p : (s.x * (2 * n - 1) * p - (n + m - 1) * p_prev) / (n - m);
+      p_prev          = p;
+      p               = pn;
+      if (SINGULAR) f *= -(double)(n - m) / s.rho;  // (-1)^n (n-m)! / rho^{n+1}
+      else f *= -s.rho / (double)(n + m);            // (-1)^n rho^n / (n+m)!
+    }
+    const double a = f * p;
+    Y[tri(n) + m]  = make_double2(a * e.x, a * e.y);
+  }
+}
+
+// Y[n, mu] for any mu, from the m >= 0 table.
+__device__ __forceinline__ double2 ylook(const double2 *Y, int n, int mu)
+{
+  return mu >= 0 ? Y[tri(n) + mu] : conj2(Y[tri(n) - mu]);
+}
+
+// ------------------------------------------------------------------ P2M
+__global__ __launch_bounds__(WG) void k_p2m(int P, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
+                                            const int32_t *__restrict__ cbody, const int32_t *__restrict__ cnbody,
+                                            const double *__restrict__ cx, const double *__restrict__ cy,
+                                            const double *__restrict__ cz, const double *__restrict__ st, int64_t cap,
+                                            double2 *__restrict__ M)
+{
+  extern __shared__ double smem[];
+  double *sh = smem;  // [WG][8]: rho, x, y, cb, sb, ax, ay, az
+  const int64_t w = blockIdx.x;
+  if (w >= nleaf) return;
+  const int c = leaves[w], b0 = cbody[c], nb = cnbody[c];
+  const double X = cx[c], Yc = cy[c], Z = cz[c];
+  const int    lane = threadIdx.x;
+  const int    ncoef = tri(P);
+  double2      acc[3][2];  // up to 2 coefficients per lane (P <= 16 -> 136 coefficients)
+  for (int a = 0; a < 2; a++)
+    for (int d = 0; d < 3; d++) acc[d][a] = make_double2(0, 0);
+  int qn[2], qm[2];
+  for (int a = 0; a < 2; a++) {
+    int q = lane + a * WG;
+    qn[a] = -1;
+    qm[a] = 0;
+    if (q < ncoef) {
+      int n = 0;
+      while (tri(n + 1) <= q) n++;
+      qn[a] = n;
+      qm[a] = q - tri(n);
+    }
+  }
+  for (int c0 = 0; c0 < nb; c0 += WG) {
+    const int i = c0 + lane;
+    if (i < nb) {
+      const int64_t bi = b0 + i;
+      const Sph     s  = sph(st[(F_X + 0) * cap + bi] - X, st[(F_X + 1) * cap + bi] - Yc, st[(F_X + 2) * cap + bi] - Z);
+      double       *r  = sh + lane * 8;
+      r[0] = s.rho, r[1] = s.x, r[2] = s.y, r[3] = s.cb, r[4] = s.sb;
+      r[5] = st[(F_A + 0) * cap + bi], r[6] = st[(F_A + 1) * cap + bi], r[7] = st[(F_A + 2) * cap + bi];
+    }
+    __syncthreads();
+    const int cnt = min(WG, nb - c0);
+    for (int a = 0; a < 2; a++) {
+      const int n = qn[a], m = qm[a];
+      if (n < 0) continue;
+      for (int j = 0; j < cnt; j++) {
+        const double *r = sh + j * 8;
+        Sph           s{r[0], r[1], r[2], r[3], r[4]};
+        // Y_n^m directly: e^{imb}, P_n^m by recurrence, (-1)^n rho^n/(n+m)!
+        double2 e = make_double2(1.0, 0.0);
+        for (int t = 0; t < m; t++) e = cmul(e, make_double2(s.cb, s.sb));
+        double p = 1.0;
+        for (int t = 1; t <= m; t++) p *= -(double)(2 * t - 1) * s.y;
+        double pp = 0.0;
+        for (int t = m + 1; t <= n; t++) {
+          const double pn = (t == m + 1) ? s.x * (2 * m + 1) * p : (s.x * (2 * t - 1) * p - (t + m - 1) * pp) / (t - m);
+          pp              = p;
+          p               = pn;
+        }
+        double f = 1.0;
+        for (int t = 0; t < n; t++) f *= -s.rho;
+        for (int t = 2; t <= n + m; t++) f /= (double)t;
+        const double  v  = f * p;
+        const double2 yc = make_double2(v * e.x, -v * e.y);  // conj(Y_n^m) = Y_n^{-m}
+        for (int d = 0; d < 3; d++) {
+          acc[d][a].x = fma(r[5 + d], yc.x, acc[d][a].x);
+          acc[d][a].y = fma(r[5 + d], yc.y, acc[d][a].y);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int a = 0; a < 2; a++) {
+    const int q = lane + a * WG;
+    if (q < ncoef)
+      for (int d = 0; d < 3; d++) M[(int64_t)c * NT + 3 * q + d] = acc[d][a];
+  }
+}
+
+// ------------------------------------------------------------------ M2M
+__global__ __launch_bounds__(WG) void k_m2m(int P, int NT, const int32_t *__restrict__ cells, int64_t ncell,
+                                            const int32_t *__restrict__ cnchild, const int32_t *__restrict__ cchild,
+                                            const double *__restrict__ cx, const double *__restrict__ cy,
+                                            const double *__restrict__ cz, double2 *__restrict__ M)
+{
+  extern __shared__ double2 sm2[];
+  double2      *Y  = sm2;             // tri(P)
+  double2      *Mc = sm2 + tri(P);    // NT
+  const int64_t w  = blockIdx.x;
+  if (w >= ncell) return;
+  const int p = cells[w], nc = cnchild[p];
+  if (nc == 0) return;
+  const int lane = threadIdx.x, nout = tri(P);
+  double2   acc[2][3];
+  for (int a = 0; a < 2; a++)
+    for (int d = 0; d < 3; d++) acc[a][d] = make_double2(0, 0);
+  for (int ci = 0; ci < nc; ci++) {
+    const int ch = cchild[p] + ci;
+    const Sph s  = sph(cx[p] - cx[ch], cy[p] - cy[ch], cz[p] - cz[ch]);
+    if (lane < P) harmonic_column<false>(P, lane, s, Y);
+    for (int q = lane; q < NT; q += WG) Mc[q] = M[(int64_t)ch * NT + q];
+    __syncthreads();
+    for (int a = 0; a < 2; a++) {
+      const int q = lane + a * WG;
+      if (q >= nout) continue;
+      int j = 0;
+      while (tri(j + 1) <= q) j++;
+      const int k = q - tri(j);
+      for (int n = 0; n <= j; n++) {
+        for (int m = max(-n, -j + k + n); m <= min(k - 1, n); m++) {
+          const int     jnkms = tri(j - n) + k - m;
+          const double2 y     = ylook(Y, n, -m);
+          const double  sg    = (m >= 0 ? 1.0 : sgn(m)) * sgn(n);
+          const double2 ys    = make_double2(y.x * sg, y.y * sg);
+          for (int d = 0; d < 3; d++) cfma(acc[a][d], Mc[3 * jnkms + d], ys);
+        }
+        for (int m = k; m <= min(n, j + k - n); m++) {
+          const int     jnkms = tri(j - n) - k + m;
+          const double2 y     = ylook(Y, n, -m);
+          const double  sg    = sgn(k + n + m);
+          const double2 ys    = make_double2(y.x * sg, y.y * sg);
+          for (int d = 0; d < 3; d++) cfma(acc[a][d], conj2(Mc[3 * jnkms + d]), ys);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int a = 0; a < 2; a++) {
+    const int q = lane + a * WG;
+    if (q < nout)
+      for (int d = 0; d < 3; d++) M[(int64_t)p * NT + 3 * q + d] = acc[a][d];
+  }
+}
+
+// ------------------------------------------------------------------ L2L
+__global__ __launch_bounds__(WG) void k_l2l(int P, int NT, const int32_t *__restrict__ cells, int64_t ncell,
+                                            const int32_t *__restrict__ cparent, const double *__restrict__ cx,
+                                            const double *__restrict__ cy, const double *__restrict__ cz,
+                                            double2 *__restrict__ L)
+{
+  extern __shared__ double2 sm2[];
+  double2      *Y  = sm2;
+  double2      *Lp = sm2 + tri(P);
+  const int64_t w  = blockIdx.x;
+  if (w >= ncell) return;
+  const int c = cells[w], p = cparent[c];
+  if (p < 0) return;
+  const int lane = threadIdx.x, nout = tri(P);
+  const Sph s    = sph(cx[c] - cx[p], cy[c] - cy[p], cz[c] - cz[p]);
+  if (lane < P) harmonic_column<false>(P, lane, s, Y);
+  for (int q = lane; q < NT; q += WG) Lp[q] = L[(int64_t)p * NT + q];
+  __syncthreads();
+  for (int a = 0; a < 2; a++) {
+    const int q = lane + a * WG;
+    if (q >= nout) continue;
+    int j = 0;
+    while (tri(j + 1) <= q) j++;
+    const int k = q - tri(j);
+    double2   acc[3];
+    for (int d = 0; d < 3; d++) acc[d] = make_double2(0, 0);
+    for (int n = j; n < P; n++) {
+      for (int m = j + k - n; m < 0; m++) {
+        const double2 y  = ylook(Y, n - j, m - k);
+        const double  sg = sgn(k);
+        const double2 ys = make_double2(y.x * sg, y.y * sg);
+        for (int d = 0; d < 3; d++) cfma(acc[d], conj2(Lp[3 * (tri(n) - m) + d]), ys);
+      }
+      for (int m = 0; m <= n; m++) {
+        if (n - j < abs(m - k)) continue;
+        const double2 y  = ylook(Y, n - j, m - k);
+        const double  sg = (m < k) ? sgn(m - k) : 1.0;
+        const double2 ys = make_double2(y.x * sg, y.y * sg);
+        for (int d = 0; d < 3; d++) cfma(acc[d], Lp[3 * (tri(n) + m) + d], ys);
+      }
+    }
+    for (int d = 0; d < 3; d++) {
+      double2 &o = L[(int64_t)c * NT + 3 * q + d];
+      o.x += acc[d].x;
+      o.y += acc[d].y;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ M2L
+struct M2LItem {
+  int j, k0, w, nlo, nhi;
+};
+__host__ __device__ inline int32_t pack_item(int j, int k0, int w, int nlo, int nhi)
+{
+  return j | (k0 << 5) | (w << 10) | (nlo << 14) | (nhi << 19) | (1 << 24);
+}
+__device__ __forceinline__ M2LItem unpack_item(int32_t v)
+{
+  M2LItem it;
+  it.j   = v & 31;
+  it.k0  = (v >> 5) & 31;
+  it.w   = (v >> 10) & 15;
+  it.nlo = (v >> 14) & 31;
+  it.nhi = (v >> 19) & 31;
+  return it;
+}
+
+template <int W, int K>
+__global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const int32_t *__restrict__ off,
+                                            const int32_t *__restrict__ src, const double *__restrict__ tx,
+                                            const double *__restrict__ ty, const double *__restrict__ tz,
+                                            const double *__restrict__ sx, const double *__restrict__ sy,
+                                            const double *__restrict__ sz, const double2 *__restrict__ M,
+                                            double2 *__restrict__ L, const int32_t *__restrict__ items,
+                                            const int32_t *__restrict__ out_off, const int32_t *__restrict__ out_src)
+{
+  extern __shared__ double2 sm2[];
+  const int64_t t = blockIdx.x;
+  if (t >= ncell) return;
+  const int lane = threadIdx.x, g = lane >> 4, sl = lane & 15;
+  const int ny = tri(P), per = ny + NT;
+  double2  *Yg = sm2 + g * per;
+  double2  *Mg = Yg + ny;
+  M2LItem   it[K];
+  bool      valid[K];
+  for (int i = 0; i < K; i++) {
+    const int32_t v = items[sl * K + i];
+    valid[i]        = v >= 0;
+    it[i]           = unpack_item(valid[i] ? v : 0);
+  }
+  double2 acc[K][W][3];
+  for (int i = 0; i < K; i++)
+    for (int q = 0; q < W; q++)
+      for (int d = 0; d < 3; d++) acc[i][q][d] = make_double2(0, 0);
+  const int    s0 = off[t], nsrc = off[t + 1] - s0;
+  const double X = tx[t], Yc = ty[t], Z = tz[t];
+  for (int base = 0; base < nsrc; base += 4) {
+    const int  si     = base + g;
+    const bool active = si < nsrc;
+    if (active) {
+      const int s = src[s0 + si];
+      if (sl < P) {
+        const Sph sp = sph(X - sx[s], Yc - sy[s], Z - sz[s]);
+        harmonic_column<true>(P, sl, sp, Yg);
+      }
+      const double2 *Ms = M + (int64_t)s * NT;
+      for (int q = sl; q < NT; q += 16) Mg[q] = Ms[q];
+    }
+    __syncthreads();
+    if (active) {
+      for (int i = 0; i < K; i++) {
+        if (!valid[i]) continue;
+        const M2LItem I = it[i];
+        for (int n = I.nlo; n <= I.nhi; n++) {
+          const int J = I.j + n;
+          double2   y[W];
+          // window: y[q] = Y[J, m - k0 - q] at step m; each step shifts by one and loads
+          // y[0], so before m = -n slot i holds Y[J, -n - k0 - 1 - i]
+          for (int q = 0; q + 1 < W; q++) {
+            const int mu = -n - I.k0 - 1 - q;
+            y[q]         = (mu >= -J) ? ylook(Yg, J, mu) : make_double2(0, 0);
+          }
+          y[W - 1] = make_double2(0, 0);
+          for (int m = -n; m <= n; m++) {
+            for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
+            y[0] = ylook(Yg, J, m - I.k0);
+            double2 mv[3];
+            const int am = m < 0 ? -m : m;
+            for (int d = 0; d < 3; d++) {
+              mv[d] = Mg[3 * (tri(n) + am) + d];
+              if (m < 0) mv[d].y = -mv[d].y;
+            }
+            for (int q = 0; q < W; q++) {
+              const int    k  = I.k0 + q;
+              const double sg = (m < 0) ? 1.0 : ((k < m) ? sgn(k) : sgn(m));
+              const double2 ys = make_double2(y[q].x * sg, y[q].y * sg);
+              for (int d = 0; d < 3; d++) cfma(acc[i][q][d], mv[d], ys);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // combine the 4 source groups (same schedule slot in lanes sl, sl+16, sl+32, sl+48)
+  for (int i = 0; i < K; i++)
+    for (int q = 0; q < W; q++)
+      for (int d = 0; d < 3; d++) {
+        double2 &a = acc[i][q][d];
+        a.x += __shfl_xor(a.x, 16, 64);
+        a.y += __shfl_xor(a.y, 16, 64);
+        a.x += __shfl_xor(a.x, 32, 64);
+        a.y += __shfl_xor(a.y, 32, 64);
+      }
+  double2 *part = sm2;  // [16][K][W][3]
+  if (g == 0)
+    for (int i = 0; i < K; i++)
+      for (int q = 0; q < W; q++)
+        for (int d = 0; d < 3; d++) part[((sl * K + i) * W + q) * 3 + d] = acc[i][q][d];
+  __syncthreads();
+  for (int q = lane; q < ny; q += WG) {
+    int j = 0;
+    while (tri(j + 1) <= q) j++;
+    double2 r[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    for (int e = out_off[q]; e < out_off[q + 1]; e++) {
+      const int pc = out_src[e];
+      for (int d = 0; d < 3; d++) {
+        r[d].x += part[pc * 3 + d].x;
+        r[d].y += part[pc * 3 + d].y;
+      }
+    }
+    const double sj = sgn(j);
+    for (int d = 0; d < 3; d++) L[t * NT + 3 * q + d] = make_double2(sj * r[d].x, sj * r[d].y);
+  }
+}
+
+// ------------------------------------------------------------------ L2P + stretching
+__global__ __launch_bounds__(WG) void k_l2p(int P, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
+                                            const int32_t *__restrict__ cbody, const int32_t *__restrict__ cnbody,
+                                            const double *__restrict__ cx, const double *__restrict__ cy,
+                                            const double *__restrict__ cz, const double2 *__restrict__ L,
+                                            double *__restrict__ st, int64_t cap, int stretching, int formulation,
+                                            int sensor_mode)
+{
+  extern __shared__ double2 sm2[];
+  double2      *Lc = sm2;
+  const int64_t w  = blockIdx.x;
+  if (w >= nleaf) return;
+  const int c = leaves[w], b0 = cbody[c], nb = cnbody[c];
+  for (int q = threadIdx.x; q < NT; q += WG) Lc[q] = L[(int64_t)c * NT + q];
+  __syncthreads();
+  const double X = cx[c], Yc = cy[c], Z = cz[c];
+  for (int i = threadIdx.x; i < nb; i += WG) {
+    const int64_t bi = b0 + i;
+    const double  xb[3] = {st[(F_X + 0) * cap + bi], st[(F_X + 1) * cap + bi], st[(F_X + 2) * cap + bi]};
+    const Sph     s     = sph(xb[0] - X, xb[1] - Yc, xb[2] - Z);
+    // outputs (j,k) = (0,0),(1,0),(1,1),(2,0),(2,1),(2,2) for each component d
+    double2 o[6][3];
+    for (int a = 0; a < 6; a++)
+      for (int d = 0; d < 3; d++) o[a][d] = make_double2(0, 0);
+    const int OJ[6] = {0, 1, 1, 2, 2, 2}, OK[6] = {0, 0, 1, 0, 1, 2};
+    for (int mu = 0; mu < P; mu++) {
+      double2 e = make_double2(1.0, 0.0);
+      for (int t = 0; t < mu; t++) e = cmul(e, make_double2(s.cb, s.sb));
+      double p = 1.0;
+      for (int t = 1; t <= mu; t++) p *= -(double)(2 * t - 1) * s.y;
+      double f = 1.0;
+      for (int t = 1; t <= mu; t++) f *= -s.rho;
+      for (int t = 2; t <= 2 * mu; t++) f /= (double)t;
+      double pp = 0.0;
+      for (int nu = mu; nu < P; nu++) {
+        if (nu > mu) {
+          const double pn = (nu == mu + 1) ? s.x * (2 * mu + 1) * p : (s.x * (2 * nu - 1) * p - (nu + mu - 1) * pp) / (nu - mu);
+          pp              = p;
+          p               = pn;
+          f *= -s.rho / (double)(nu + mu);
+        }
+        const double  v  = f * p;
+        const double2 yp = make_double2(v * e.x, v * e.y);  // Y[nu, +mu]
+        for (int sgm = 0; sgm < (mu > 0 ? 2 : 1); sgm++) {
+          const int     mup = sgm ? -mu : mu;
+          const double2 yv  = sgm ? conj2(yp) : yp;
+          for (int a = 0; a < 6; a++) {
+            const int j = OJ[a], k = OK[a];
+            const int n = nu + j;
+            if (n >= P) continue;
+            const int m = mup + k;
+            double    sg;
+            double2   lv[3];
+            if (m < 0) {
+              sg = sgn(k);
+              for (int d = 0; d < 3; d++) lv[d] = conj2(Lc[3 * (tri(n) - m) + d]);
+            } else {
+              sg = (m < k) ? sgn(m - k) : 1.0;
+              for (int d = 0; d < 3; d++) lv[d] = Lc[3 * (tri(n) + m) + d];
+            }
+            const double2 ys = make_double2(yv.x * sg, yv.y * sg);
+            for (int d = 0; d < 3; d++) cfma(o[a][d], lv[d], ys);
+          }
+        }
+      }
+    }
+    // gradient g[d][b] = d psi_d / d x_b, Hessian H[d][b][c]
+    double gr[3][3], H[3][3][3];
+    for (int d = 0; d < 3; d++) {
+      const double a10 = o[1][d].x, a11 = o[2][d].x, b11 = o[2][d].y;
+      const double a20 = o[3][d].x, a21 = o[4][d].x, b21 = o[4][d].y, a22 = o[5][d].x, b22 = o[5][d].y;
+      gr[d][0] = a11;
+      gr[d][1] = -b11;
+      gr[d][2] = -a10;
+      H[d][0][0] = 0.5 * (a22 - a20);
+      H[d][1][1] = -0.5 * (a20 + a22);
+      H[d][2][2] = a20;
+      H[d][0][1] = H[d][1][0] = -0.5 * b22;
+      H[d][0][2] = H[d][2][0] = -a21;
+      H[d][1][2] = H[d][2][1] = b21;
+    }
+    // u = curl(psi)/(4 pi); J[a][b] = d u_a / d x_b
+    double u[3], Jf[3][3];
+    u[0] = gr[2][1] - gr[1][2];
+    u[1] = gr[0][2] - gr[2][0];
+    u[2] = gr[1][0] - gr[0][1];
+    for (int b = 0; b < 3; b++) {
+      Jf[0][b] = H[2][1][b] - H[1][2][b];
+      Jf[1][b] = H[0][2][b] - H[2][0][b];
+      Jf[2][b] = H[1][0][b] - H[0][1][b];
+    }
+    double uu[3], JJ[3][3];
+    for (int d = 0; d < 3; d++) {
+      uu[d] = st[(F_U + d) * cap + bi] + INV_4PI * u[d];
+      st[(F_U + d) * cap + bi] = uu[d];
+    }
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        JJ[a][b] = st[(F_J + 3 * a + b) * cap + bi] + INV_4PI * Jf[a][b];
+        st[(F_J + 3 * a + b) * cap + bi] = JJ[a][b];
+      }
+    if (sensor_mode) continue;
+    // stretching (fmmCalculator.hpp:391-405) and rVPM (407-418)
+    const double al[3] = {st[(F_A + 0) * cap + bi], st[(F_A + 1) * cap + bi], st[(F_A + 2) * cap + bi]};
+    double       da[3];
+    for (int d = 0; d < 3; d++)
+      da[d] = stretching == UVLM_TRANSPOSED ? JJ[0][d] * al[0] + JJ[1][d] * al[1] + JJ[2][d] * al[2]
+                                            : JJ[d][0] * al[0] + JJ[d][1] * al[1] + JJ[d][2] * al[2];
+    double dr = 0.0;
+    if (formulation == UVLM_RVPM) {
+      const double na    = sqrt(al[0] * al[0] + al[1] * al[1] + al[2] * al[2]);
+      const double ah[3] = {al[0] / na, al[1] / na, al[2] / na};
+      const double dot   = da[0] * ah[0] + da[1] * ah[1] + da[2] * ah[2];
+      for (int d = 0; d < 3; d++) da[d] -= 3. / 5. * dot * ah[d];
+      dr -= 1.0 / 5.0 * st[F_SIG * cap + bi] / na * dot;
+    }
+    for (int d = 0; d < 3; d++) st[(F_DADT + d) * cap + bi] = da[d];
+    st[F_DSIG * cap + bi] = dr;
+  }
+}
+
+__global__ void k_zero(double *p, int64_t n)
+{
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// M2L schedule: items (j, k0, w, nlo, nhi) with cost = (number of m steps) * W, packed
+// longest-first into 16 lane slots with at most K items each; the item cap is searched
+// to minimise the largest slot load.
+void make_m2l_schedule(int P, M2LSchedule &s)
+{
+  const int W = s.W, K = s.K, LANES = 16;
+  struct It {
+    int j, k0, w, nlo, nhi, cost;
+  };
+  auto items_for = [&](int cap) {
+    std::vector<It> v;
+    for (int j = 0; j < P; j++)
+      for (int k0 = 0; k0 <= j; k0 += W) {
+        const int w = std::min(W, j + 1 - k0);
+        int       n = 0;
+        while (n <= P - 1 - j) {
+          int c = 0, lo = n;
+          while (n <= P - 1 - j && (c == 0 || c + W * (2 * n + 1) <= cap)) {
+            c += W * (2 * n + 1);
+            n++;
+          }
+          v.push_back({j, k0, w, lo, n - 1, c});
+        }
+      }
+    return v;
+  };
+  int              best_load = 1 << 30;
+  std::vector<It>  best_items;
+  std::vector<int> best_lane;
+  for (int cap = 1; cap <= W * P * P + 1; cap++) {
+    auto v = items_for(cap);
+    std::stable_sort(v.begin(), v.end(), [](const It &a, const It &b) { return a.cost > b.cost; });
+    std::vector<int> load(LANES, 0), cnt(LANES, 0), lane(v.size(), -1);
+    bool             ok = true;
+    for (size_t q = 0; q < v.size(); q++) {
+      int bl = -1;
+      for (int l = 0; l < LANES; l++)
+        if (cnt[l] < K && (bl < 0 || load[l] < load[bl])) bl = l;
+      if (bl < 0) {
+        ok = false;
+        break;
+      }
+      load[bl] += v[q].cost;
+      cnt[bl]++;
+      lane[q] = bl;
+    }
+    if (!ok) continue;
+    const int mx = *std::max_element(load.begin(), load.end());
+    if (mx < best_load) {
+      best_load  = mx;
+      best_items = v;
+      best_lane  = lane;
+    }
+  }
+  if (best_items.empty()) throw Error(UVLM_ERR_INTERNAL, "no M2L schedule");
+  s.items.assign(LANES * K, -1);
+  std::vector<int> cnt(LANES, 0);
+  const int        ny = P * (P + 1) / 2;
+  std::vector<std::vector<int>> pieces(ny);
+  for (size_t q = 0; q < best_items.size(); q++) {
+    const It &I    = best_items[q];
+    const int l    = best_lane[q];
+    const int slot = cnt[l]++;
+    s.items[l * K + slot] = pack_item(I.j, I.k0, I.w, I.nlo, I.nhi);
+    for (int w = 0; w < I.w; w++) pieces[I.j * (I.j + 1) / 2 + I.k0 + w].push_back((l * K + slot) * W + w);
+  }
+  s.out_off.assign(ny + 1, 0);
+  s.out_src.clear();
+  for (int q = 0; q < ny; q++) {
+    std::sort(pieces[q].begin(), pieces[q].end());
+    for (int v : pieces[q]) s.out_src.push_back(v);
+    s.out_off[q + 1] = (int)s.out_src.size();
+  }
+  s.dev_items   = 0;
+  s.dev_out_off = s.items.size();
+  s.dev_out_src = s.dev_out_off + s.out_off.size();
+  const size_t total = s.dev_out_src + s.out_src.size();
+  s.dev.reserve(4 * total);
+  std::vector<int32_t> all;
+  all.insert(all.end(), s.items.begin(), s.items.end());
+  all.insert(all.end(), s.out_off.begin(), s.out_off.end());
+  all.insert(all.end(), s.out_src.begin(), s.out_src.end());
+  UVLM_HIP(hipMemcpy(s.dev.p, all.data(), 4 * all.size(), hipMemcpyHostToDevice));
+}
+
+void zero_doubles(hipStream_t s, double *p, int64_t n)
+{
+  if (n <= 0) return;
+  k_zero<<<nblocks(n, 256), 256, 0, s>>>(p, n);
+  UVLM_LAUNCH_CHECK();
+}
+
+void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, double2 *M)
+{
+  const int NT = 3 * P * (P + 1) / 2;
+  if (t.nleaf == 0) return;
+  k_p2m<<<(int)t.nleaf, WG, WG * 8 * sizeof(double), s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(),
+                                                           t.nbody.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(),
+                                                           t.cz.as<double>(), st, cap, M);
+  UVLM_LAUNCH_CHECK();
+}
+
+void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M)
+{
+  const int     NT = 3 * P * (P + 1) / 2;
+  const int64_t o = t.lvl_off[level], n = t.lvl_off[level + 1] - o;
+  if (n <= 0) return;
+  const size_t lds = sizeof(double2) * (P * (P + 1) / 2 + NT);
+  k_m2m<<<(int)n, WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.nchild.as<int32_t>(), t.child.as<int32_t>(),
+                                t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(), M);
+  UVLM_LAUNCH_CHECK();
+}
+
+void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L)
+{
+  const int     NT = 3 * P * (P + 1) / 2;
+  const int64_t o = t.lvl_off[level], n = t.lvl_off[level + 1] - o;
+  if (n <= 0) return;
+  const size_t lds = sizeof(double2) * (P * (P + 1) / 2 + NT);
+  k_l2l<<<(int)n, WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.parent.as<int32_t>(), t.cx.as<double>(),
+                                t.cy.as<double>(), t.cz.as<double>(), L);
+  UVLM_LAUNCH_CHECK();
+}
+
+void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, const M2LSchedule &sch, const double2 *M,
+         double2 *L)
+{
+  const int NT = 3 * P * (P + 1) / 2, ny = P * (P + 1) / 2;
+  if (tt.C == 0) return;
+  const size_t lds_main = sizeof(double2) * 4 * (ny + NT);
+  const size_t lds_part = sizeof(double2) * 16 * sch.K * sch.W * 3;
+  const size_t lds      = std::max(lds_main, lds_part);
+  const int32_t *d      = sch.dev.as<int32_t>();
+#define UVLM_M2L_CASE(WW, KK)                                                                                      \
+  if (sch.W == WW && sch.K == KK) {                                                                                \
+    k_m2l<WW, KK><<<(int)tt.C, WG, lds, s>>>(P, NT, tt.C, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),       \
+                                             tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),           \
+                                             st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, L,     \
+                                             d + sch.dev_items, d + sch.dev_out_off, d + sch.dev_out_src);         \
+    UVLM_LAUNCH_CHECK();                                                                                           \
+    return;                                                                                                        \
+  }
+  UVLM_M2L_CASE(1, 3)
+  UVLM_M2L_CASE(2, 3)
+  UVLM_M2L_CASE(4, 3)
+#undef UVLM_M2L_CASE
+  throw Error(UVLM_ERR_INTERNAL, "unsupported M2L schedule shape");
+}
+
+void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *st, int64_t cap, int stretching,
+                 int formulation, int sensor_mode)
+{
+  const int NT = 3 * P * (P + 1) / 2;
+  if (t.nleaf == 0) return;
+  k_l2p<<<(int)t.nleaf, WG, sizeof(double2) * NT, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(),
+                                                        t.nbody.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(),
+                                                        t.cz.as<double>(), L, st, cap, stretching, formulation,
+                                                        sensor_mode);
+  UVLM_LAUNCH_CHECK();
+}
+
+}  // namespace uvlm

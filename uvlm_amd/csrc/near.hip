@@ -1,0 +1,218 @@
+// near.hip -- near-field P2P: regularised Biot-Savart velocity and velocity gradient.
+//
+// Kernel of src/multipoleKernels.hpp:223-301 (sensor form 303-341): for target i and
+// source j with dX = x_i - x_j, R^2 != 0, s = R / (sqrt(2) sigma_j):
+//   g  = erf(s) - (2/sqrt(pi)) s e^{-s^2},  f1 = g / (4 pi R^3),
+//   f2 = (2 s^2 t2 - 3 g) / (4 pi R^5)      (t2 = (2/sqrt(pi)) s e^{-s^2})
+//   u -= f1 (dX x a_j),   J[a][b] -= f2 (dX x a_j)_a dX_b + f1 eps_{abk} a_j,k.
+// Only the source core size sigma_j enters (multipoleKernels.hpp:243).
+//
+// Layout: one 64-lane workgroup per target leaf.  Lanes = (target t) x (source split s):
+// small leaves use several lanes per target, each taking every S-th source, and the
+// partial sums are combined with cross-lane adds (fixed order, no atomics).  Source
+// leaves of the target's CSR list are staged through LDS 64 bodies at a time.
+//
+// FP64 path: the reference formula (erf, exp, sqrt in double).
+// Mixed path: FP32 arithmetic on coordinates relative to the target leaf centre (the
+// source offsets are formed in FP64 while staging), per-source-leaf FP32 partial sums
+// added into FP64 per-target accumulators.  For s^2 < 1, g/s^3 and its derivative come
+// from their Taylor series (no cancellation between erf and t2): with x = s^2,
+//   g / s^3 = G(x) = (2/sqrt(pi)) sum_{i>=0} (-1)^i 2(i+1) / ((2i+3) (i+1)!) x^i,
+//   f1 = G(x) q^{3/2} / (4 pi),  f2 = 2 G'(x) q^{5/2} / (4 pi),  q = 1 / (2 sigma^2),
+// because 2 s^2 t2 - 3 g = s g'(s) - 3 g = 2 s^5 G'(x).
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace uvlm {
+
+namespace {
+
+constexpr int WG = 64;
+
+// Taylor coefficients of G(x) above, i = 0..12 (truncation < 5e-10 relative for x < 1).
+constexpr double taylor_coef(int i)
+{
+  double fact = 1.0;
+  for (int t = 2; t <= i + 1; t++) fact *= t;
+  const double v = 1.1283791670955125739 * 2.0 * (i + 1) / ((2.0 * i + 3.0) * fact);
+  return (i & 1) ? -v : v;
+}
+constexpr int NTAYLOR = 13;
+
+struct SrcF64 {
+  double x, y, z, ax, ay, az, invS2, halfInvS2;
+};
+
+__device__ __forceinline__ void accumulate_pair(float *acc, float dx, float dy, float dz, float ax, float ay, float az,
+                                                float f1, float f2, bool withJ)
+{
+  const float cx = dy * az - dz * ay, cy = dz * ax - dx * az, cz = dx * ay - dy * ax;
+  acc[0] = fmaf(f1, cx, acc[0]);
+  acc[1] = fmaf(f1, cy, acc[1]);
+  acc[2] = fmaf(f1, cz, acc[2]);
+  if (withJ) {
+    const float k0 = f2 * cx, k1 = f2 * cy, k2 = f2 * cz;
+    acc[3]  = fmaf(k0, dx, acc[3]);
+    acc[4]  = fmaf(k0, dy, fmaf(f1, az, acc[4]));
+    acc[5]  = fmaf(k0, dz, fmaf(-f1, ay, acc[5]));
+    acc[6]  = fmaf(k1, dx, fmaf(-f1, az, acc[6]));
+    acc[7]  = fmaf(k1, dy, acc[7]);
+    acc[8]  = fmaf(k1, dz, fmaf(f1, ax, acc[8]));
+    acc[9]  = fmaf(k2, dx, fmaf(f1, ay, acc[9]));
+    acc[10] = fmaf(k2, dy, fmaf(-f1, ax, acc[10]));
+    acc[11] = fmaf(k2, dz, acc[11]);
+  }
+}
+
+template <bool MIXED, bool WITH_J>
+__global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, int64_t nleaf,
+                                            const int32_t *__restrict__ tbody, const int32_t *__restrict__ tnbody,
+                                            const double *__restrict__ tcx, const double *__restrict__ tcy,
+                                            const double *__restrict__ tcz, const int32_t *__restrict__ off,
+                                            const int32_t *__restrict__ src, const int32_t *__restrict__ sbody,
+                                            const int32_t *__restrict__ snbody, const double *__restrict__ tst,
+                                            int64_t tcap, const double *__restrict__ sst, int64_t scap,
+                                            double *__restrict__ out, int64_t ocap)
+{
+  __shared__ SrcF64 s64[MIXED ? 1 : WG];
+  __shared__ float4 s32[MIXED ? 3 * WG : 1];
+  const int64_t w = blockIdx.x;
+  if (w >= nleaf) return;
+  const int t = leaves[w], tb = tbody[t], nt = tnbody[t];
+  const int lane = threadIdx.x;
+  int       lg   = 0;
+  while ((1 << lg) < nt && lg < 6) lg++;
+  const int    ntp = 1 << lg, S = WG >> lg, ti = lane & (ntp - 1), si = lane >> lg;
+  const double cxt = tcx[t], cyt = tcy[t], czt = tcz[t];
+  const int    q0 = off[t], q1 = off[t + 1];
+  for (int c0 = 0; c0 < nt; c0 += ntp) {
+    const int  i     = c0 + ti;
+    const bool valid = i < nt;
+    const int64_t bi = tb + (valid ? i : 0);
+    const double  xi = tst[(F_X + 0) * tcap + bi], yi = tst[(F_X + 1) * tcap + bi], zi = tst[(F_X + 2) * tcap + bi];
+    const float   rxi = (float)(xi - cxt), ryi = (float)(yi - cyt), rzi = (float)(zi - czt);
+    double        acc[12];
+    for (int k = 0; k < 12; k++) acc[k] = 0.0;
+    for (int q = q0; q < q1; q++) {
+      const int sl = src[q], sb = sbody[sl], sn = snbody[sl];
+      for (int j0 = 0; j0 < sn; j0 += WG) {
+        const int cnt = min(WG, sn - j0);
+        if (lane < cnt) {
+          const int64_t bj = sb + j0 + lane;
+          const double  x = sst[(F_X + 0) * scap + bj], y = sst[(F_X + 1) * scap + bj], z = sst[(F_X + 2) * scap + bj];
+          const double  sg = sst[F_SIG * scap + bj];
+          const double  ax = sst[(F_A + 0) * scap + bj], ay = sst[(F_A + 1) * scap + bj], az = sst[(F_A + 2) * scap + bj];
+          if (MIXED) {
+            const double qq = 0.5 / (sg * sg), sq = sqrt(qq);
+            s32[3 * lane + 0] = make_float4((float)(x - cxt), (float)(y - cyt), (float)(z - czt), (float)qq);
+            s32[3 * lane + 1] = make_float4((float)ax, (float)ay, (float)az, (float)(qq * sq * INV_4PI));
+            s32[3 * lane + 2] = make_float4((float)(2.0 * qq * qq * sq * INV_4PI), 0.f, 0.f, 0.f);
+          } else {
+            const double is2 = 1.0 / (sg * sg);
+            s64[lane]        = SrcF64{x, y, z, ax, ay, az, is2, 0.5 * is2};
+          }
+        }
+        __syncthreads();
+        if (MIXED) {
+          float fa[12];
+          for (int k = 0; k < 12; k++) fa[k] = 0.f;
+          for (int j = si; j < cnt; j += S) {
+            const float4 A = s32[3 * j], B = s32[3 * j + 1], Cc = s32[3 * j + 2];
+            const float  dx = rxi - A.x, dy = ryi - A.y, dz = rzi - A.z;
+            const float  r2 = dx * dx + dy * dy + dz * dz;
+            const float  xq = r2 * A.w;
+            float        f1, f2;
+            if (xq < 1.0f) {
+              float g = (float)taylor_coef(NTAYLOR - 1), gp = 0.f;
+#pragma unroll
+              for (int k = NTAYLOR - 2; k >= 0; k--) {
+                gp = fmaf(gp, xq, g);
+                g  = fmaf(g, xq, (float)taylor_coef(k));
+              }
+              f1 = g * B.w;
+              f2 = gp * Cc.x;
+            } else {
+              const float ir  = rsqrtf(r2);
+              const float s   = sqrtf(xq);
+              const float t2  = 1.1283791670955126f * s * expf(-xq);
+              const float gg  = erff(s) - t2;
+              const float h   = 2.0f * xq * t2 - 3.0f * gg;
+              const float ir3 = ir * ir * ir * (float)INV_4PI;
+              f1              = gg * ir3;
+              f2              = h * ir3 * ir * ir;
+            }
+            if (r2 == 0.0f) f1 = f2 = 0.f;
+            accumulate_pair(fa, dx, dy, dz, B.x, B.y, B.z, f1, f2, WITH_J);
+          }
+          for (int k = 0; k < (WITH_J ? 12 : 3); k++) acc[k] += (double)fa[k];
+        } else {
+          for (int j = si; j < cnt; j += S) {
+            const SrcF64 &sj = s64[j];
+            const double  dx = xi - sj.x, dy = yi - sj.y, dz = zi - sj.z;
+            const double  R2 = dx * dx + dy * dy + dz * dz;
+            if (R2 == 0.0) continue;
+            const double RS     = R2 * sj.halfInvS2;
+            const double sqrtRS = sqrt(RS);
+            const double erfS   = erf(sqrtRS);
+            const double expm   = exp(-RS);
+            const double invR   = 1.0 / sqrt(R2);
+            const double t2     = 2.0 * INV_SQRTPI * sqrtRS * expm;
+            const double o4pr3  = 0.25 * INV_PI / R2 * invR;
+            const double g      = erfS - t2;
+            const double f1     = g * o4pr3;
+            const double cx = dy * sj.az - dz * sj.ay, cy = dz * sj.ax - dx * sj.az, cz = dx * sj.ay - dy * sj.ax;
+            acc[0] += f1 * cx;
+            acc[1] += f1 * cy;
+            acc[2] += f1 * cz;
+            if (WITH_J) {
+              const double f2 = (t2 * sj.invS2 - 3.0 * g / R2) * o4pr3;
+              const double k0 = f2 * cx, k1 = f2 * cy, k2 = f2 * cz;
+              acc[3] += k0 * dx;
+              acc[4] += k0 * dy + f1 * sj.az;
+              acc[5] += k0 * dz - f1 * sj.ay;
+              acc[6] += k1 * dx - f1 * sj.az;
+              acc[7] += k1 * dy;
+              acc[8] += k1 * dz + f1 * sj.ax;
+              acc[9] += k2 * dx + f1 * sj.ay;
+              acc[10] += k2 * dy - f1 * sj.ax;
+              acc[11] += k2 * dz;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // combine the S source splits (lanes ti + s*ntp)
+    for (int o = ntp; o < WG; o <<= 1)
+      for (int k = 0; k < (WITH_J ? 12 : 3); k++) acc[k] += __shfl_xor(acc[k], o, 64);
+    if (si == 0 && valid) {
+      for (int d = 0; d < 3; d++) out[(F_U + d) * ocap + bi] = -acc[d];
+      if (WITH_J)
+        for (int k = 0; k < 9; k++) out[(F_J + k) * ocap + bi] = -acc[3 + k];
+    }
+  }
+}
+
+}  // namespace
+
+void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const double *tgt, int64_t tcap,
+         const double *srcst, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode)
+{
+  if (tt.nleaf == 0) return;
+  const int nb = (int)tt.nleaf;
+#define UVLM_P2P(MX, WJ)                                                                                          \
+  k_p2p<MX, WJ><<<nb, WG, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(), tt.nbody.as<int32_t>(), \
+                                  tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),                     \
+                                  l.p2p_off.as<int32_t>(), l.p2p_src.as<int32_t>(), stree.body.as<int32_t>(),      \
+                                  stree.nbody.as<int32_t>(), tgt, tcap, srcst, scap, out, ocap)
+  const bool mixed = precision == UVLM_MIXED;
+  if (mixed && !sensor_mode) UVLM_P2P(true, true);
+  else if (mixed) UVLM_P2P(true, false);
+  else if (!sensor_mode) UVLM_P2P(false, true);
+  else UVLM_P2P(false, false);
+#undef UVLM_P2P
+  UVLM_LAUNCH_CHECK();
+}
+
+}  // namespace uvlm

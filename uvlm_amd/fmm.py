@@ -1,0 +1,180 @@
+"""Python mirror of Vortex::FMMCalculator over the C ABI of libuvlm_fmm.so.
+
+The reference interface (src/fmmCalculator.hpp:25-1006) is a C++ class template whose
+tags select the integrator, viscous scheme, formulation and stretching; here they are
+constructor arguments.  Method names and argument meaning follow the reference:
+
+  init(bodies)          -> fmmCalculator.hpp:282-288 (tree + lists; permutes `bodies`)
+  calculate()           -> fmmCalculator.hpp:420-467 (velocity, J, stretching)
+  advance(bodies, dt)   -> fmmCalculator.hpp:919-1005 (device-resident RK step)
+  getSensorData(probes) -> fmmCalculator.hpp:726-731
+  copyBodies2Host(bodies) -> fmmCalculator.hpp:235-243
+
+Bodies are float64 arrays of shape (n, 37) laid out like exafmm::Body
+(src/exafmmTypes.hpp:18-32); BODY_FIELDS maps field names to columns.  Errors of the C
+ABI raise UVLMError.  The HIP library is required: there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+
+BODY_FIELDS = {
+    "J": slice(0, 9), "X": slice(9, 12), "alpha": slice(12, 15), "velocity": slice(15, 18),
+    "dadt": slice(18, 21), "velocity_old": slice(21, 24), "dadt_old": slice(24, 27),
+    "X_old": slice(27, 30), "alpha_old": slice(30, 33), "radius": 33, "drdt": 34,
+    "radius_old": 35, "drdt_old": 36,
+}
+BODY_DOUBLES = 37
+BODY_BYTES = 8 * BODY_DOUBLES
+
+PRECISION = {"fp64": 0, "mixed": 1}
+STRETCHING = {"transposed": 0, "classic": 1}
+FORMULATION = {"cvpm": 0, "rvpm": 1}
+SCHEME = {"euler": 0, "rk2": 1, "rk3": 2, "rk4": 3}
+
+UVLMError = _capi.UVLMError
+
+
+def make_bodies(X, alpha, sigma):
+    """(n, 37) exafmm::Body array with X, alpha and radius set, every other field zero."""
+    X = np.asarray(X, dtype=np.float64)
+    b = np.zeros((X.shape[0], BODY_DOUBLES))
+    b[:, BODY_FIELDS["X"]] = X
+    b[:, BODY_FIELDS["alpha"]] = alpha
+    b[:, BODY_FIELDS["radius"]] = sigma
+    return b
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+class FMMCalculator:
+    """One device-resident particle set and its FMM (a uvlm_handle)."""
+
+    def __init__(self, P=4, theta=0.3, ncrit=128, precision="fp64", stretching="transposed",
+                 formulation="cvpm", scheme="rk3", device=0):
+        self.lib = _capi.lib()
+        prm = _capi.Params(P=P, theta=theta, ncrit=ncrit, precision=PRECISION[precision],
+                           stretching=STRETCHING[stretching], formulation=FORMULATION[formulation],
+                           scheme=SCHEME[scheme], device=device)
+        self.P, self.theta, self.ncrit = P, theta, ncrit
+        self.nterm = 3 * (P * (P + 1) // 2)
+        self._h = ctypes.c_void_p()
+        _capi.check(self.lib.uvlm_create(ctypes.byref(prm), ctypes.byref(self._h)))
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.uvlm_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- bodies ---------------------------------------------------------------------
+    def set_bodies(self, bodies):
+        b = np.ascontiguousarray(bodies, dtype=np.float64)
+        assert b.ndim == 2 and b.shape[1] == BODY_DOUBLES
+        _capi.check(self.lib.uvlm_set_bodies_aos(self._h, _dp(b), b.shape[0], BODY_BYTES))
+        self.n = b.shape[0]
+
+    def set_arrays(self, X, alpha, sigma):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        A = np.ascontiguousarray(alpha, dtype=np.float64)
+        S = np.ascontiguousarray(sigma, dtype=np.float64)
+        _capi.check(self.lib.uvlm_set_bodies_soa(self._h, X.shape[0], _dp(X), _dp(A), _dp(S)))
+        self.n = X.shape[0]
+
+    def copyBodies2Host(self, bodies=None):
+        out = bodies if bodies is not None else np.zeros((self.n, BODY_DOUBLES))
+        assert out.flags.c_contiguous and out.shape == (self.n, BODY_DOUBLES)
+        _capi.check(self.lib.uvlm_get_bodies_aos(self._h, _dp(out), self.n, BODY_BYTES))
+        return out
+
+    def arrays(self):
+        """Current (tree-order) X, alpha, sigma, velocity, J (n,9), dadt, drdt."""
+        n = self.n
+        X, A, U, D = (np.zeros((n, 3)) for _ in range(4))
+        S, DR = np.zeros(n), np.zeros(n)
+        J = np.zeros((n, 9))
+        _capi.check(self.lib.uvlm_get_bodies_soa(self._h, n, _dp(X), _dp(A), _dp(S), _dp(U), _dp(J), _dp(D),
+                                                 _dp(DR)))
+        return {"X": X, "alpha": A, "sigma": S, "velocity": U, "J": J, "dadt": D, "drdt": DR}
+
+    def permutation(self):
+        p = np.zeros(self.n, dtype=np.int64)
+        _capi.check(self.lib.uvlm_get_permutation(self._h, self.n, _ip(p)))
+        return p
+
+    # -- the reference's operations -------------------------------------------------
+    def init(self, bodies):
+        """Upload, build tree + lists on the device, write the permuted bodies back."""
+        self.set_bodies(bodies)
+        self.build()
+        self.copyBodies2Host(bodies)
+
+    def build(self):
+        _capi.check(self.lib.uvlm_build(self._h))
+
+    def calculate(self):
+        _capi.check(self.lib.uvlm_eval(self._h))
+
+    def stage(self, k, dt):
+        _capi.check(self.lib.uvlm_stage(self._h, int(k), float(dt)))
+
+    def advance(self, bodies, dt):
+        self.set_bodies(bodies)
+        _capi.check(self.lib.uvlm_advance(self._h, float(dt)))
+        self.copyBodies2Host(bodies)
+
+    def step(self, dt):
+        """A device-resident time step on the bodies already on the device."""
+        _capi.check(self.lib.uvlm_advance(self._h, float(dt)))
+
+    def getSensorData(self, sensors):
+        s = sensors
+        assert s.flags.c_contiguous and s.dtype == np.float64 and s.shape[1] == BODY_DOUBLES
+        _capi.check(self.lib.uvlm_sensors_aos(self._h, _dp(s), s.shape[0], BODY_BYTES))
+
+    # -- inspection -----------------------------------------------------------------
+    def cells(self):
+        C = ctypes.c_int64()
+        _capi.check(self.lib.uvlm_num_cells(self._h, ctypes.byref(C)))
+        ints = np.zeros((C.value, 5), dtype=np.int64)
+        reals = np.zeros((C.value, 4))
+        _capi.check(self.lib.uvlm_get_tree(self._h, C.value, _ip(ints), _dp(reals)))
+        return ints, reals
+
+    def lists(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _capi.check(self.lib.uvlm_get_lists(self._h, ctypes.byref(a), ctypes.byref(b), None, None))
+        p2p = np.zeros((a.value, 2), dtype=np.int64)
+        m2l = np.zeros((b.value, 2), dtype=np.int64)
+        _capi.check(self.lib.uvlm_get_lists(self._h, None, None, _ip(p2p), _ip(m2l)))
+        return p2p, m2l
+
+    def expansions(self):
+        C = ctypes.c_int64()
+        _capi.check(self.lib.uvlm_num_cells(self._h, ctypes.byref(C)))
+        M = np.zeros((C.value, self.nterm), dtype=np.complex128)
+        L = np.zeros((C.value, self.nterm), dtype=np.complex128)
+        _capi.check(self.lib.uvlm_get_expansions(self._h, _dp(M.view(np.float64)), _dp(L.view(np.float64))))
+        return M, L
+
+    def timings(self):
+        t = _capi.Timing()
+        _capi.check(self.lib.uvlm_timings(self._h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def sync(self):
+        _capi.check(self.lib.uvlm_sync(self._h))
