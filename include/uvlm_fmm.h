@@ -126,6 +126,12 @@ int uvlm_get_expansions(uvlm_handle h, double *M, double *L);
  * (multipoleKernels.hpp:303-341 adds no near-field J). */
 int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride);
 
+/* FP64 O(m*N) direct sum from the handle's current bodies (sources) to m target points
+ * xt [m][3]: u [m][3] and, if non-NULL, J [m][9]; with target strengths at [m][3], dadt
+ * [m][3] = J.at (classic stretching).  Replaces exafmm::direct (traverse.hpp:88-98) with
+ * the device P2P formula (multipoleKernels.hpp:223-301); coincident points are skipped. */
+int uvlm_direct(uvlm_handle h, int64_t m, const double *xt, const double *at, double *u, double *J, double *dadt);
+
 int uvlm_timings(uvlm_handle h, uvlm_timing *out);
 
 /* Device synchronisation of the handle's stream. */

@@ -16,7 +16,7 @@ EXPORTS = [
     "uvlm_last_error", "uvlm_abi_version", "uvlm_create", "uvlm_destroy", "uvlm_set_bodies_aos",
     "uvlm_get_bodies_aos", "uvlm_set_bodies_soa", "uvlm_get_bodies_soa", "uvlm_get_permutation",
     "uvlm_build", "uvlm_eval", "uvlm_stage", "uvlm_advance", "uvlm_num_cells", "uvlm_get_tree",
-    "uvlm_get_lists", "uvlm_get_expansions", "uvlm_sensors_aos", "uvlm_timings", "uvlm_sync",
+    "uvlm_get_lists", "uvlm_get_expansions", "uvlm_sensors_aos", "uvlm_direct", "uvlm_timings", "uvlm_sync",
 ]
 
 
@@ -72,6 +72,7 @@ def lib():
             "uvlm_get_lists": (I, [VP, PI64, PI64, PI64, PI64]),
             "uvlm_get_expansions": (I, [VP, PD, PD]),
             "uvlm_sensors_aos": (I, [VP, VP, I64, I64]),
+            "uvlm_direct": (I, [VP, I64, PD, PD, PD, PD, PD]),
             "uvlm_timings": (I, [VP, ctypes.POINTER(Timing)]),
             "uvlm_sync": (I, [VP]),
         }

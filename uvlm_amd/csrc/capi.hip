@@ -474,6 +474,30 @@ int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride)
   });
 }
 
+int uvlm_direct(uvlm_handle h, int64_t m, const double *xt, const double *at, double *u, double *J, double *dadt)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (h->N <= 0) throw Error(UVLM_ERR_STATE, "no bodies set");
+    if (m <= 0 || !xt || !u) throw Error(UVLM_ERR_ARG, "null or empty targets");
+    if (dadt && !at) throw Error(UVLM_ERR_ARG, "dadt needs target strengths");
+    set_device(h);
+    DevBuf buf;
+    const int64_t nin = 3 * m + (at ? 3 * m : 0), nout = 3 * m + (J ? 9 * m : 0) + (dadt ? 3 * m : 0);
+    buf.reserve(sizeof(double) * (nin + nout));
+    double *d   = buf.as<double>();
+    double *dxt = d, *dat = at ? d + 3 * m : nullptr;
+    double *du = d + nin, *dJ = J ? du + 3 * m : nullptr, *dd = dadt ? du + 3 * m + (J ? 9 * m : 0) : nullptr;
+    UVLM_HIP(hipMemcpyAsync(dxt, xt, 24 * m, hipMemcpyHostToDevice, h->s));
+    if (at) UVLM_HIP(hipMemcpyAsync(dat, at, 24 * m, hipMemcpyHostToDevice, h->s));
+    direct(h->s, m, dxt, dat, h->st.as<double>(), h->cap, h->N, du, dJ, dd);
+    UVLM_HIP(hipMemcpyAsync(u, du, 24 * m, hipMemcpyDeviceToHost, h->s));
+    if (J) UVLM_HIP(hipMemcpyAsync(J, dJ, 72 * m, hipMemcpyDeviceToHost, h->s));
+    if (dadt) UVLM_HIP(hipMemcpyAsync(dadt, dd, 24 * m, hipMemcpyDeviceToHost, h->s));
+    UVLM_HIP(hipStreamSynchronize(h->s));
+  });
+}
+
 int uvlm_timings(uvlm_handle h, uvlm_timing *out)
 {
   if (!h || !out) return fail(UVLM_ERR_ARG, "null argument");

@@ -149,6 +149,9 @@ void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *
 void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const double *tgt, int64_t tcap,
          const double *src, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode);
 
+void direct(hipStream_t s, int64_t m, const double *xt, const double *at, const double *sst, int64_t scap, int64_t n,
+            double *u, double *J, double *dadt);
+
 // body.hip
 void aos_to_soa(hipStream_t s, const void *aos, int64_t n, int64_t stride_doubles, double *st, int64_t cap);
 void soa_to_aos(hipStream_t s, const double *st, int64_t cap, void *aos, int64_t n, int64_t stride_doubles);

@@ -194,7 +194,79 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
   }
 }
 
+// O(M*N) direct sum, FP64, reference formula; one thread per target, sources tiled
+// through LDS.  Output: u (m,3), J (m,9) and, when target strengths are given, the
+// classic stretching J.alpha_t (the quantity exafmm::direct produces, traverse.hpp:88-98).
+__global__ __launch_bounds__(256) void k_direct(int64_t m, const double *__restrict__ xt,
+                                                const double *__restrict__ at, const double *__restrict__ sst,
+                                                int64_t scap, int64_t n, double *__restrict__ u,
+                                                double *__restrict__ J, double *__restrict__ dadt)
+{
+  __shared__ SrcF64 sh[256];
+  const int64_t i  = blockIdx.x * (int64_t)256 + threadIdx.x;
+  const bool    ok = i < m;
+  const double  xi = ok ? xt[3 * i] : 0.0, yi = ok ? xt[3 * i + 1] : 0.0, zi = ok ? xt[3 * i + 2] : 0.0;
+  double        acc[12];
+  for (int k = 0; k < 12; k++) acc[k] = 0.0;
+  for (int64_t j0 = 0; j0 < n; j0 += 256) {
+    const int64_t j = j0 + threadIdx.x;
+    if (j < n) {
+      const double sg  = sst[F_SIG * scap + j];
+      const double is2 = 1.0 / (sg * sg);
+      sh[threadIdx.x]  = SrcF64{sst[(F_X + 0) * scap + j], sst[(F_X + 1) * scap + j], sst[(F_X + 2) * scap + j],
+                                sst[(F_A + 0) * scap + j], sst[(F_A + 1) * scap + j], sst[(F_A + 2) * scap + j], is2,
+                                0.5 * is2};
+    }
+    __syncthreads();
+    const int cnt = (int)min<int64_t>(256, n - j0);
+    for (int q = 0; q < cnt; q++) {
+      const SrcF64 &sj = sh[q];
+      const double  dx = xi - sj.x, dy = yi - sj.y, dz = zi - sj.z;
+      const double  R2 = dx * dx + dy * dy + dz * dz;
+      if (R2 == 0.0) continue;
+      const double RS     = R2 * sj.halfInvS2;
+      const double sqrtRS = sqrt(RS);
+      const double t2     = 2.0 * INV_SQRTPI * sqrtRS * exp(-RS);
+      const double invR   = 1.0 / sqrt(R2);
+      const double o4pr3  = 0.25 * INV_PI / R2 * invR;
+      const double g      = erf(sqrtRS) - t2;
+      const double f1     = g * o4pr3;
+      const double f2     = (t2 * sj.invS2 - 3.0 * g / R2) * o4pr3;
+      const double cx = dy * sj.az - dz * sj.ay, cy = dz * sj.ax - dx * sj.az, cz = dx * sj.ay - dy * sj.ax;
+      acc[0] += f1 * cx;
+      acc[1] += f1 * cy;
+      acc[2] += f1 * cz;
+      const double k0 = f2 * cx, k1 = f2 * cy, k2 = f2 * cz;
+      acc[3] += k0 * dx;
+      acc[4] += k0 * dy + f1 * sj.az;
+      acc[5] += k0 * dz - f1 * sj.ay;
+      acc[6] += k1 * dx - f1 * sj.az;
+      acc[7] += k1 * dy;
+      acc[8] += k1 * dz + f1 * sj.ax;
+      acc[9] += k2 * dx + f1 * sj.ay;
+      acc[10] += k2 * dy - f1 * sj.ax;
+      acc[11] += k2 * dz;
+    }
+    __syncthreads();
+  }
+  if (!ok) return;
+  for (int d = 0; d < 3; d++) u[3 * i + d] = -acc[d];
+  if (J)
+    for (int k = 0; k < 9; k++) J[9 * i + k] = -acc[3 + k];
+  if (dadt && at)
+    for (int a = 0; a < 3; a++)
+      dadt[3 * i + a] = -(acc[3 + 3 * a] * at[3 * i] + acc[4 + 3 * a] * at[3 * i + 1] + acc[5 + 3 * a] * at[3 * i + 2]);
+}
+
 }  // namespace
+
+void direct(hipStream_t s, int64_t m, const double *xt, const double *at, const double *sst, int64_t scap, int64_t n,
+            double *u, double *J, double *dadt)
+{
+  if (m <= 0) return;
+  k_direct<<<(int)((m + 255) / 256), 256, 0, s>>>(m, xt, at, sst, scap, n, u, J, dadt);
+  UVLM_LAUNCH_CHECK();
+}
 
 void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const double *tgt, int64_t tcap,
          const double *srcst, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode)
