@@ -1,0 +1,253 @@
+// uvlm/fmm_calculator.hpp -- C++ drop-in for the FMM path of kursatyurt/UVLM.
+//
+// Same spellings as the reference so that a caller switches by changing includes:
+//   exafmm::Body / Bodies / Cell / Cells      (src/exafmmTypes.hpp:18-47, same Body layout)
+//   exafmm::buildTree(Bodies&)                (src/tree.hpp:101-113)
+//   exafmm::direct(Bodies&, Bodies&)          (src/traverse.hpp:88-98)
+//   Vortex tags Inviscid/PSE/CSM, Euler/RK2/RK3/RK4, cVPM/rVPM, Transposed/Classic
+//                                             (src/fmmCalculator.hpp:9-23)
+//   Vortex::FMMCalculator<Host, Device, Integration, Viscous, Formulation, Stretching>
+//     init / calculate / advance / getSensorData / calculateViscosity /
+//     copyBodies2Host / copyBodies2Device     (src/fmmCalculator.hpp:25-1006)
+// Differences: P, theta and ncrit are constructor arguments (the reference has them as
+// constexpr in exafmmTypes.hpp:12-15); Host/Device tags are accepted and ignored; the
+// viscous schemes PSE/CSM are rejected at compile time; failures throw std::runtime_error
+// carrying uvlm_last_error().  All compute runs on the GPU through include/uvlm_fmm.h.
+#ifndef UVLM_FMM_CALCULATOR_HPP
+#define UVLM_FMM_CALCULATOR_HPP
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../uvlm_fmm.h"
+
+namespace exafmm {
+typedef double real_t;
+
+struct Body {
+  real_t J[3][3]{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  real_t X[3]{0.0, 0.0, 0.0};
+  real_t alpha[3]{0.0, 0.0, 0.0};
+  real_t velocity[3]{0.0, 0.0, 0.0};
+  real_t dadt[3]{0.0, 0.0, 0.0};
+  real_t velocity_old[3]{0.0, 0.0, 0.0};
+  real_t dadt_old[3]{0.0, 0.0, 0.0};
+  real_t X_old[3]{0.0, 0.0, 0.0};
+  real_t alpha_old[3]{0.0, 0.0, 0.0};
+  real_t radius{0.0};
+  real_t drdt{0.0};
+  real_t radius_old{0.0};
+  real_t drdt_old{0.0};
+};
+static_assert(sizeof(Body) == 296, "exafmm::Body must keep the reference layout (37 doubles)");
+static_assert(offsetof(Body, X) == 72 && offsetof(Body, alpha) == 96 && offsetof(Body, radius) == 264,
+              "exafmm::Body field offsets");
+typedef std::vector<Body> Bodies;
+
+struct Cell {
+  size_t NCHILD;
+  size_t NBODY;
+  size_t ChildOffset;
+  size_t BodyOffset;
+  real_t X[3];
+  Cell  *CHILD;
+  Body  *BODY;
+  real_t R;
+  size_t level;
+};
+typedef std::vector<Cell> Cells;
+
+namespace detail {
+inline void check(int rc)
+{
+  if (rc != UVLM_OK) throw std::runtime_error(std::string("uvlm: ") + uvlm_last_error());
+}
+
+struct Handle {
+  uvlm_handle h = nullptr;
+  explicit Handle(const uvlm_params &p) { check(uvlm_create(&p, &h)); }
+  ~Handle() { uvlm_destroy(h); }
+  Handle(const Handle &)            = delete;
+  Handle &operator=(const Handle &) = delete;
+};
+
+inline uvlm_params params(int P, double theta, size_t ncrit, int precision, int stretching, int formulation,
+                          int scheme, int device)
+{
+  uvlm_params p{};
+  p.P           = P;
+  p.theta       = theta;
+  p.ncrit       = (int64_t)ncrit;
+  p.precision   = precision;
+  p.stretching  = stretching;
+  p.formulation = formulation;
+  p.scheme      = scheme;
+  p.device      = device;
+  return p;
+}
+
+// Fills a Cells vector (with CHILD/BODY pointers into `cells` / `bodies`) from the device tree.
+inline Cells fetch_cells(uvlm_handle h, Bodies &bodies)
+{
+  int64_t C = 0;
+  check(uvlm_num_cells(h, &C));
+  std::vector<int64_t> ints(5 * C);
+  std::vector<double>  reals(4 * C);
+  check(uvlm_get_tree(h, C, ints.data(), reals.data()));
+  Cells cells(C);
+  for (int64_t c = 0; c < C; c++) {
+    Cell &q       = cells[c];
+    q.NCHILD      = (size_t)ints[5 * c + 0];
+    q.NBODY       = (size_t)ints[5 * c + 1];
+    q.ChildOffset = (size_t)ints[5 * c + 2];
+    q.BodyOffset  = (size_t)ints[5 * c + 3];
+    q.level       = (size_t)ints[5 * c + 4];
+    for (int d = 0; d < 3; d++) q.X[d] = reals[4 * c + d];
+    q.R     = reals[4 * c + 3];
+    q.CHILD = q.NCHILD ? &cells[0] + q.ChildOffset : nullptr;
+    q.BODY  = bodies.data() + q.BodyOffset;
+  }
+  return cells;
+}
+}  // namespace detail
+
+// Octree of `bodies` built on the GPU; permutes `bodies` into tree order like tree.hpp:101.
+inline Cells buildTree(Bodies &bodies, size_t ncrit = 128, int device = 0)
+{
+  detail::Handle h(detail::params(4, 0.3, ncrit, UVLM_FP64, UVLM_TRANSPOSED, UVLM_CVPM, UVLM_RK3, device));
+  detail::check(uvlm_set_bodies_aos(h.h, bodies.data(), (int64_t)bodies.size(), sizeof(Body)));
+  detail::check(uvlm_build(h.h));
+  detail::check(uvlm_get_bodies_aos(h.h, bodies.data(), (int64_t)bodies.size(), sizeof(Body)));
+  return detail::fetch_cells(h.h, bodies);
+}
+
+// FP64 direct sum of jbodies onto bodies: velocity += u, dadt += J.alpha (traverse.hpp:88-98).
+inline void direct(Bodies &bodies, Bodies &jbodies, int device = 0)
+{
+  detail::Handle h(detail::params(4, 0.3, 128, UVLM_FP64, UVLM_CLASSIC, UVLM_CVPM, UVLM_RK3, device));
+  detail::check(uvlm_set_bodies_aos(h.h, jbodies.data(), (int64_t)jbodies.size(), sizeof(Body)));
+  const size_t        m = bodies.size();
+  std::vector<double> xt(3 * m), at(3 * m), u(3 * m), da(3 * m);
+  for (size_t i = 0; i < m; i++)
+    for (int d = 0; d < 3; d++) {
+      xt[3 * i + d] = bodies[i].X[d];
+      at[3 * i + d] = bodies[i].alpha[d];
+    }
+  detail::check(uvlm_direct(h.h, (int64_t)m, xt.data(), at.data(), u.data(), nullptr, da.data()));
+  for (size_t i = 0; i < m; i++)
+    for (int d = 0; d < 3; d++) {
+      bodies[i].velocity[d] += u[3 * i + d];
+      bodies[i].dadt[d] += da[3 * i + d];
+    }
+}
+}  // namespace exafmm
+
+namespace Vortex {
+struct Inviscid {};
+struct PSE {};
+struct CSM {};
+struct Euler {};
+struct RK2 {};
+struct RK3 {};
+struct RK4 {};
+struct cVPM {};
+struct rVPM {};
+struct Transposed {};
+struct Classic {};
+
+enum class Precision { FP64 = UVLM_FP64, Mixed = UVLM_MIXED };
+
+template <typename Host, typename Device, typename IntegrationScheme, typename ViscousScheme, typename Formulation,
+          typename Stretching>
+struct FMMCalculator {
+  static_assert(std::is_same<ViscousScheme, Inviscid>::value,
+                "only the inviscid scheme is on the GPU path (PSE/CSM are outside it)");
+  static_assert(std::is_same<Stretching, Transposed>::value || std::is_same<Stretching, Classic>::value,
+                "Stretching should be either Transposed or Classic");
+
+  static constexpr int scheme()
+  {
+    return std::is_same<IntegrationScheme, Euler>::value ? UVLM_EULER
+           : std::is_same<IntegrationScheme, RK2>::value ? UVLM_RK2
+           : std::is_same<IntegrationScheme, RK3>::value ? UVLM_RK3
+                                                          : UVLM_RK4;
+  }
+
+  explicit FMMCalculator(int P = 4, double theta = 0.3, size_t ncrit = 128, Precision precision = Precision::FP64,
+                         int device = 0)
+      : h_(detail_params(P, theta, ncrit, precision, device))
+  {
+  }
+
+  // fmmCalculator.hpp:282-288: tree + lists; permutes `bodies` into tree order.
+  void init(exafmm::Bodies &bodies)
+  {
+    copyBodies2Device(bodies);
+    exafmm::detail::check(uvlm_build(h_.h));
+    copyBodies2Host(bodies);
+  }
+  // fmmCalculator.hpp:420-467
+  void calculate() { exafmm::detail::check(uvlm_eval(h_.h)); }
+  // fmmCalculator.hpp:713-724 (inviscid: nothing to do)
+  void calculateViscosity() {}
+  // fmmCalculator.hpp:919-1005: every stage rebuilds the tree; the state stays on the GPU.
+  void advance(exafmm::Bodies &bodies, exafmm::real_t dt)
+  {
+    copyBodies2Device(bodies);
+    exafmm::detail::check(uvlm_advance(h_.h, dt));
+    copyBodies2Host(bodies);
+  }
+  // fmmCalculator.hpp:726-731: uses the multipoles of the last calculate().
+  void getSensorData(exafmm::Bodies &sensors)
+  {
+    exafmm::detail::check(uvlm_sensors_aos(h_.h, sensors.data(), (int64_t)sensors.size(), sizeof(exafmm::Body)));
+  }
+  void copyBodies2Device(exafmm::Bodies &bodies)
+  {
+    exafmm::detail::check(
+        uvlm_set_bodies_aos(h_.h, bodies.data(), (int64_t)bodies.size(), sizeof(exafmm::Body)));
+    n_ = bodies.size();
+  }
+  void copyBodies2Host(exafmm::Bodies &bodies)
+  {
+    if (bodies.size() != n_) bodies.resize(n_);
+    exafmm::detail::check(uvlm_get_bodies_aos(h_.h, bodies.data(), (int64_t)n_, sizeof(exafmm::Body)));
+  }
+  // Interaction lists as (target cell, source cell) pairs (the reference keeps them in
+  // _P2PPairsView / _M2LPairsView, fmmCalculator.hpp:50-51).
+  std::pair<std::vector<std::pair<size_t, size_t>>, std::vector<std::pair<size_t, size_t>>> lists() const
+  {
+    int64_t np = 0, nm = 0;
+    exafmm::detail::check(uvlm_get_lists(h_.h, &np, &nm, nullptr, nullptr));
+    std::vector<int64_t> p(2 * np), m(2 * nm);
+    exafmm::detail::check(uvlm_get_lists(h_.h, nullptr, nullptr, p.data(), m.data()));
+    std::vector<std::pair<size_t, size_t>> P2P(np), M2L(nm);
+    for (int64_t i = 0; i < np; i++) P2P[i] = {(size_t)p[2 * i], (size_t)p[2 * i + 1]};
+    for (int64_t i = 0; i < nm; i++) M2L[i] = {(size_t)m[2 * i], (size_t)m[2 * i + 1]};
+    return {std::move(P2P), std::move(M2L)};
+  }
+  uvlm_timing timings() const
+  {
+    uvlm_timing t{};
+    exafmm::detail::check(uvlm_timings(h_.h, &t));
+    return t;
+  }
+  uvlm_handle handle() const { return h_.h; }
+
+private:
+  static uvlm_params detail_params(int P, double theta, size_t ncrit, Precision precision, int device)
+  {
+    return exafmm::detail::params(P, theta, ncrit, (int)precision,
+                                  std::is_same<Stretching, Transposed>::value ? UVLM_TRANSPOSED : UVLM_CLASSIC,
+                                  std::is_same<Formulation, rVPM>::value ? UVLM_RVPM : UVLM_CVPM, scheme(), device);
+  }
+  exafmm::detail::Handle h_;
+  size_t                 n_ = 0;
+};
+}  // namespace Vortex
+
+#endif  // UVLM_FMM_CALCULATOR_HPP

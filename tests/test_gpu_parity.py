@@ -156,3 +156,34 @@ def test_depth_error_is_reported():
     with pytest.raises(UVLMError) as e:
         g.build()
     assert e.value.code == 2
+
+
+def test_cpp_facade_drop_in(orc, tmp_path):
+    """A C++ caller written against the reference's Vortex::FMMCalculator spelling
+    (tests/cpp/facade_demo.cpp) gives the oracle's results."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "uvlm_amd", "lib",
+                       "uvlm_facade_demo")
+    X, A, S = W.scaled("ring", 4096)
+    b = orc.make_bodies(X, A, S)
+    fin, fev, fad = tmp_path / "in.bin", tmp_path / "eval.bin", tmp_path / "adv.bin"
+    b.tofile(fin)
+    r = subprocess.run([exe, str(fin), str(fev), str(fad), "2"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    got_eval = np.fromfile(fev).reshape(-1, 37)
+    got_adv = np.fromfile(fad).reshape(-1, 37)
+    ref = b.copy()
+    c = orc.Calculator(P=8, theta=0.5, ncrit=64)
+    c.init(ref)
+    c.calculate()
+    want = c.bodies(len(ref))
+    assert np.array_equal(got_eval[:, 9:12], want[:, 9:12])
+    assert rel(got_eval[:, 15:18], want[:, 15:18]) < 1e-10 and rel(got_eval[:, 0:9], want[:, 0:9]) < 1e-10
+    adv = want.copy()
+    c2 = orc.Calculator(P=8, theta=0.5, ncrit=64)
+    for _ in range(2):
+        c2.advance(adv, 0.01, "rk3")
+    assert rel(got_adv[:, 9:12], adv[:, 9:12]) < 1e-10 and rel(got_adv[:, 12:15], adv[:, 12:15]) < 1e-10

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -194,8 +195,14 @@ int uvlm_create(const uvlm_params *p, uvlm_handle *out)
     set_device(h);
     UVLM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     for (auto &e : h->ev) UVLM_HIP(hipEventCreate(&e));
+    // M2L lane schedule shape: window of W consecutive outputs, K items per lane
+    // (UVLM_M2L_W in {1, 2, 4} overrides the default for tuning sweeps)
     h->sched.W = 2;
     h->sched.K = 3;
+    if (const char *w = std::getenv("UVLM_M2L_W")) {
+      const int v = std::atoi(w);
+      if (v == 1 || v == 2 || v == 4) h->sched.W = v;
+    }
     make_m2l_schedule(p->P, h->sched);
   });
   if (rc != UVLM_OK) {

@@ -78,8 +78,19 @@ __device__ __forceinline__ Sph sph(double dx, double dy, double dz)
   return s;
 }
 
+// 1/k for k = 0..63 (entry 0 unused), so the recurrences below multiply instead of divide.
+__constant__ double c_inv[64] = {
+    0.0,        1.0 / 1,  1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,  1.0 / 6,  1.0 / 7,  1.0 / 8,  1.0 / 9,
+    1.0 / 10, 1.0 / 11, 1.0 / 12, 1.0 / 13, 1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17, 1.0 / 18, 1.0 / 19,
+    1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23, 1.0 / 24, 1.0 / 25, 1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29,
+    1.0 / 30, 1.0 / 31, 1.0 / 32, 1.0 / 33, 1.0 / 34, 1.0 / 35, 1.0 / 36, 1.0 / 37, 1.0 / 38, 1.0 / 39,
+    1.0 / 40, 1.0 / 41, 1.0 / 42, 1.0 / 43, 1.0 / 44, 1.0 / 45, 1.0 / 46, 1.0 / 47, 1.0 / 48, 1.0 / 49,
+    1.0 / 50, 1.0 / 51, 1.0 / 52, 1.0 / 53, 1.0 / 54, 1.0 / 55, 1.0 / 56, 1.0 / 57, 1.0 / 58, 1.0 / 59,
+    1.0 / 60, 1.0 / 61, 1.0 / 62, 1.0 / 63};
+
 // Column m of the regular (singular=false) or singular harmonics, degrees m..P-1, written
-// to Y[tri(n) + m].
+// to Y[tri(n) + m].  Associated Legendre recurrence in n at fixed m:
+// (n-m) P_n^m = x (2n-1) P_{n-1}^m - (n+m-1) P_{n-2}^m, from P_m^m = (-1)^m (2m-1)!! y^m.
 template <bool SINGULAR>
 __device__ void harmonic_column(int P, int m, const Sph &s, double2 *Y)
 {
@@ -90,24 +101,24 @@ __device__ void harmonic_column(int P, int m, const Sph &s, double2 *Y)
   double pmm = 1.0;
   for (int i = 1; i <= m; i++) pmm *= -(double)(2 * i - 1) * s.y;
   // scale at n = m
-  double f;
+  double       f;
+  const double inv = SINGULAR ? 1.0 / s.rho : 0.0;
   if (SINGULAR) {
-    const double inv = 1.0 / s.rho;
-    f                = inv;
+    f = inv;
     for (int i = 0; i < m; i++) f *= -inv;  // (-1)^m / rho^{m+1}
   } else {
     f = 1.0;
-    for (int i = 1; i <= m; i++) f *= -s.rho;  // (-1)^m rho^m
-    for (int i = 2; i <= 2 * m; i++) f /= (double)i;  // / (2m)!
+    for (int i = 1; i <= m; i++) f *= -s.rho * c_inv[2 * i - 1] * c_inv[2 * i];  // (-1)^m rho^m / (2m)!
   }
   double p_prev = 0.0, p = pmm;
   for (int n = m; n < P; n++) {
     if (n > m) {
-      const double pn = (n == m + 1) ? s.x * (2 * m + 1) * p : (s.x * (2 * n - 1) * p - (n + m - 1) * p_prev) / (n - m);
-      p_prev          = p;
-      p               = pn;
-      if (SINGULAR) f *= -(double)(n - m) / s.rho;  // (-1)^n (n-m)! / rho^{n+1}
-      else f *= -s.rho / (double)(n + m);            // (-1)^n rho^n / (n+m)!
+      const double pn = (n == m + 1) ? s.x * (2 * m + 1) * p
+                                     : (s.x * (2 * n - 1) * p - (n + m - 1) * p_prev) * c_inv[n - m];
+      p_prev = p;
+      p      = pn;
+      if (SINGULAR) f *= -(double)(n - m) * inv;  // (-1)^n (n-m)! / rho^{n+1}
+      else f *= -s.rho * c_inv[n + m];            // (-1)^n rho^n / (n+m)!
     }
     const double a = f * p;
     Y[tri(n) + m]  = make_double2(a * e.x, a * e.y);

@@ -9,8 +9,10 @@
 //
 // Layout: one 64-lane workgroup per target leaf.  Lanes = (target t) x (source split s):
 // small leaves use several lanes per target, each taking every S-th source, and the
-// partial sums are combined with cross-lane adds (fixed order, no atomics).  Source
-// leaves of the target's CSR list are staged through LDS 64 bodies at a time.
+// partial sums are combined with cross-lane adds (fixed order, no atomics).  The bodies
+// of all source leaves in the target's CSR list are streamed through LDS in full chunks of
+// 64 across leaf boundaries (prefix sums of the leaf sizes in LDS, binary search per slot),
+// so small leaves do not pay a barrier per leaf.
 //
 // FP64 path: the reference formula (erf, exp, sqrt in double).
 // Mixed path: FP32 arithmetic on coordinates relative to the target leaf centre (the
@@ -65,6 +67,32 @@ __device__ __forceinline__ void accumulate_pair(float *acc, float dx, float dy, 
   }
 }
 
+constexpr int SEG = 256;  // source leaves whose prefix sums sit in LDS at once
+
+// Exclusive prefix sum of v[0..n) (n <= SEG) into pref[0..n]; one wavefront, 4 per lane.
+__device__ __forceinline__ void lds_prefix(int *pref, const int *v, int n)
+{
+  const int lane = threadIdx.x;
+  int       loc[4], tot = 0;
+  for (int k = 0; k < 4; k++) {
+    const int i = 4 * lane + k;
+    loc[k]      = i < n ? v[i] : 0;
+    tot += loc[k];
+  }
+  int incl = tot;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  int run = incl - tot;
+  for (int k = 0; k < 4; k++) {
+    const int i = 4 * lane + k;
+    if (i <= n) pref[i] = run;
+    run += loc[k];
+  }
+  if (lane == 63 && 256 <= n) pref[256] = run;
+}
+
 template <bool MIXED, bool WITH_J>
 __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, int64_t nleaf,
                                             const int32_t *__restrict__ tbody, const int32_t *__restrict__ tnbody,
@@ -77,6 +105,7 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
 {
   __shared__ SrcF64 s64[MIXED ? 1 : WG];
   __shared__ float4 s32[MIXED ? 3 * WG : 1];
+  __shared__ int    pref[SEG + 1], lb[SEG], ln[SEG];
   const int64_t w = blockIdx.x;
   if (w >= nleaf) return;
   const int t = leaves[w], tb = tbody[t], nt = tnbody[t];
@@ -94,12 +123,29 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
     const float   rxi = (float)(xi - cxt), ryi = (float)(yi - cyt), rzi = (float)(zi - czt);
     double        acc[12];
     for (int k = 0; k < 12; k++) acc[k] = 0.0;
-    for (int q = q0; q < q1; q++) {
-      const int sl = src[q], sb = sbody[sl], sn = snbody[sl];
-      for (int j0 = 0; j0 < sn; j0 += WG) {
-        const int cnt = min(WG, sn - j0);
+    for (int g0 = q0; g0 < q1; g0 += SEG) {
+      const int nseg = min(SEG, q1 - g0);
+      __syncthreads();
+      for (int k = lane; k < nseg; k += WG) {
+        const int sl = src[g0 + k];
+        lb[k]        = sbody[sl];
+        ln[k]        = snbody[sl];
+      }
+      __syncthreads();
+      lds_prefix(pref, ln, nseg);
+      __syncthreads();
+      const int total = pref[nseg];
+      for (int j0 = 0; j0 < total; j0 += WG) {
+        const int cnt = min(WG, total - j0);
         if (lane < cnt) {
-          const int64_t bj = sb + j0 + lane;
+          const int g  = j0 + lane;
+          int       lo = 0, hi = nseg - 1;  // last q with pref[q] <= g
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pref[mid] <= g) lo = mid;
+            else hi = mid - 1;
+          }
+          const int64_t bj = lb[lo] + (g - pref[lo]);
           const double  x = sst[(F_X + 0) * scap + bj], y = sst[(F_X + 1) * scap + bj], z = sst[(F_X + 2) * scap + bj];
           const double  sg = sst[F_SIG * scap + bj];
           const double  ax = sst[(F_A + 0) * scap + bj], ay = sst[(F_A + 1) * scap + bj], az = sst[(F_A + 2) * scap + bj];
