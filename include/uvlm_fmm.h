@@ -67,6 +67,8 @@ typedef struct uvlm_timing {
   double eval_ms;     /* whole uvlm_eval */
   int64_t n_bodies, n_cells, n_leaves, n_p2p_pairs, n_m2l_pairs, n_levels;
   int64_t n_interactions; /* sum over P2P pairs of n_target * n_source */
+  double  sensor_ms;      /* last uvlm_sensors_aos: probe tree, lists, M2L, P2P, L2L, L2P */
+  double  adjoint_ms;     /* last uvlm_probe_adjoint, same stages transposed */
 } uvlm_timing;
 
 typedef struct uvlm_handle_s *uvlm_handle;
@@ -131,6 +133,21 @@ int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride);
  * [m][3] = J.at (classic stretching).  Replaces exafmm::direct (traverse.hpp:88-98) with
  * the device P2P formula (multipoleKernels.hpp:223-301); coincident points are skipped. */
 int uvlm_direct(uvlm_handle h, int64_t m, const double *xt, const double *at, double *u, double *J, double *dadt);
+
+/* Adjoint of the probe objective J = sum_p w_p . u(x_p), with u the velocity that
+ * uvlm_sensors_aos computes at probe p from the current particles (the same interaction
+ * lists, transposed).  Outputs, in the current tree order of the particles:
+ *   dJ_dalpha [n][3] = dJ / d alpha_j   and   dJ_dx [n][3] = dJ / d x_j.
+ * No counterpart in the reference (its solver has no adjoint): it is evaluated as one FMM
+ * pass from probe "vortices" w_p onto the particles with the particle's own sigma, whose
+ * field is dJ/d alpha and whose gradient gives dJ/dx = (grad field)^T alpha.  Needs a
+ * uvlm_build; uses none of the forward results. */
+int uvlm_probe_adjoint(uvlm_handle h, int64_t m, const double *probes, const double *weights, double *dJ_dalpha,
+                       double *dJ_dx);
+/* Overwrite positions and strengths [n][3] in the CURRENT tree order without rebuilding
+ * the tree or lists (frozen-tree perturbations, e.g. finite differences).  No counterpart
+ * in the reference. */
+int uvlm_update_bodies_soa(uvlm_handle h, int64_t n, const double *X, const double *alpha);
 
 int uvlm_timings(uvlm_handle h, uvlm_timing *out);
 

@@ -73,6 +73,7 @@ def lib():
             "orc_l2p": (None, [I, P, P, L64, P, P, P]),
             "orc_direct": (None, [L64, P, L64, P, P, P, P, P]),
             "orc_direct_classic": (None, [L64, P, P, L64, P, P, P, P, P]),
+            "orc_direct_tsigma": (None, [L64, P, P, L64, P, P, P, P]),
             "orc_max_threads": (I, []),
             "orc_set_threads": (None, [I]),
         }
@@ -263,6 +264,26 @@ def direct_classic(xt, at, xs, alpha_s, sigma_s):
     dadt = np.zeros((m, 3))
     lib().orc_direct_classic(m, _p(xt), _p(at), xs.shape[0], _p(xs), _p(a), _p(s), _p(u), _p(dadt))
     return u, dadt
+
+
+def probe_adjoint(probes, weights, X, alpha, sigma):
+    """Exact (direct-sum) gradient of J = sum_p weights_p . u(probes_p), u from the particles
+    (X, alpha, sigma): returns dJ/dalpha (n,3), dJ/dX (n,3)."""
+    X, A, S = _c(X), _c(alpha), _c(sigma)
+    Pp = _c(probes)
+    W = _c(np.broadcast_to(weights, Pp.shape))
+    n = X.shape[0]
+    u = np.zeros((n, 3))
+    J = np.zeros((n, 9))
+    lib().orc_direct_tsigma(n, _p(X), _p(S), Pp.shape[0], _p(Pp), _p(W), _p(u), _p(J))
+    gx = np.einsum("iab,ia->ib", J.reshape(-1, 3, 3), A)
+    return u, gx
+
+
+def probe_objective(probes, weights, X, alpha, sigma):
+    """J = sum_p weights_p . u(probes_p) by the direct sum (device P2P formula)."""
+    u = direct(probes, X, alpha, sigma, want_J=False)
+    return float(np.sum(np.broadcast_to(weights, u.shape) * u))
 
 
 def set_threads(n):

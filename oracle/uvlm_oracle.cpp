@@ -1162,6 +1162,27 @@ void orc_direct_classic(int64_t m, const double *xt, const double *at, int64_t n
   }
 }
 
+// Direct sum with the TARGET's core size (xs/as are the sources, st the targets' sigma):
+// the exact adjoint field of the probe objective J = sum_p w_p . u(x_p).  With the probes
+// as sources (strength w_p) and the particles as targets, u = dJ/dalpha_j and
+// J^T alpha_j = dJ/dx_j (no reference counterpart; used to check the GPU adjoint).
+void orc_direct_tsigma(int64_t m, const double *xt, const double *st, int64_t n, const double *xs, const double *as,
+                       double *u, double *J)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < m; i++) {
+    real_t acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    Body   bj;
+    for (int64_t j = 0; j < n; j++) {
+      for (int d = 0; d < 3; d++) { bj.X[d] = xs[3 * j + d]; bj.alpha[d] = as[3 * j + d]; }
+      bj.radius = st[i];
+      p2p_pair_sum(&xt[3 * i], bj, acc);
+    }
+    for (int d = 0; d < 3; d++) u[3 * i + d] = -acc[d];
+    for (int k = 0; k < 9; k++) J[9 * i + k] = -acc[3 + k];
+  }
+}
+
 int orc_max_threads(void) { return omp_get_max_threads(); }
 void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 

@@ -187,3 +187,43 @@ def test_cpp_facade_drop_in(orc, tmp_path):
     for _ in range(2):
         c2.advance(adv, 0.01, "rk3")
     assert rel(got_adv[:, 9:12], adv[:, 9:12]) < 1e-10 and rel(got_adv[:, 12:15], adv[:, 12:15]) < 1e-10
+
+
+def test_probe_adjoint(orc):
+    """Adjoint of J = sum_p v . u(p) (probes from the blob mixture): vs the exact direct-sum
+    adjoint, and vs central finite differences of the FP64 FMM forward objective with the
+    tree and lists frozen."""
+    X, A, S, probes, v, chk = W.blobs(n=30000, n_probe=512, n_check=8)
+    g = FMMCalculator(P=8, theta=0.5, ncrit=64)
+    g.set_arrays(X, A, S)
+    g.build()
+    arr = g.arrays()
+    ga, gx = g.probe_adjoint(probes, v)
+    ea, ex = orc.probe_adjoint(probes, v, arr["X"], arr["alpha"], arr["sigma"])
+    ra, rx = rel(ga, ea), rel(gx, ex)
+    print(f"adjoint vs exact direct: dJ/dalpha {ra:.3e}  dJ/dx {rx:.3e}  t={g.timings()['adjoint_ms']:.3f} ms")
+    assert ra < 2e-2 and rx < 2e-2
+    # frozen-tree central differences of the FMM objective on sampled particles
+    g.calculate()
+    h = 1e-6
+    Xt, At = arr["X"].copy(), arr["alpha"].copy()
+    fd_x, fd_a, an_x, an_a = [], [], [], []
+    for j in chk[:6]:
+        for d in range(3):
+            for which, base in (("x", Xt), ("a", At)):
+                p, m = base.copy(), base.copy()
+                p[j, d] += h
+                m[j, d] -= h
+                g.update_arrays(**({"X": p} if which == "x" else {"alpha": p}))
+                g.calculate()
+                jp = g.probe_objective(probes, v)
+                g.update_arrays(**({"X": m} if which == "x" else {"alpha": m}))
+                g.calculate()
+                jm = g.probe_objective(probes, v)
+                g.update_arrays(X=Xt, alpha=At)
+                (fd_x if which == "x" else fd_a).append((jp - jm) / (2 * h))
+                (an_x if which == "x" else an_a).append((gx if which == "x" else ga)[j, d])
+    print("fd dJ/dx", np.array(fd_x)[:6], "adjoint", np.array(an_x)[:6])
+    ex_, ea_ = rel(np.array(an_x), np.array(fd_x)), rel(np.array(an_a), np.array(fd_a))
+    print(f"adjoint vs FD of the FMM objective: dJ/dx {ex_:.3e}  dJ/dalpha {ea_:.3e}")
+    assert ex_ < 1e-3 and ea_ < 1e-3

@@ -334,3 +334,24 @@ def test_rk_convergence_order(orc, scheme, order):
     e2 = np.linalg.norm(run(16, scheme) - ref)
     observed = math.log2(e1 / e2)
     assert abs(observed - order) < 0.35, (scheme, observed)
+
+
+# ------------------------------------------------------------------- adjoint of the probe objective
+def test_direct_adjoint_matches_finite_differences(orc):
+    """The exact adjoint (direct sum with the target's sigma) against central FD of the
+    forward probe objective J = sum_p v . u(p)."""
+    X, A, S, probes, v, chk = W.blobs(n=3000, n_probe=200, n_check=12)
+    ga, gx = orc.probe_adjoint(probes, v, X, A, S)
+    h = 1e-6
+    for j in chk:
+        for d in range(3):
+            Xp, Xm = X.copy(), X.copy()
+            Xp[j, d] += h
+            Xm[j, d] -= h
+            fd = (orc.probe_objective(probes, v, Xp, A, S) - orc.probe_objective(probes, v, Xm, A, S)) / (2 * h)
+            assert abs(fd - gx[j, d]) < 1e-6 * max(1.0, abs(gx[j]).max()), (j, d, fd, gx[j, d])
+            Ap, Am = A.copy(), A.copy()
+            Ap[j, d] += h
+            Am[j, d] -= h
+            fd = (orc.probe_objective(probes, v, X, Ap, S) - orc.probe_objective(probes, v, X, Am, S)) / (2 * h)
+            assert abs(fd - ga[j, d]) < 1e-7 * max(1.0, abs(ga[j]).max()), (j, d, fd, ga[j, d])

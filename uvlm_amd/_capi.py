@@ -16,7 +16,8 @@ EXPORTS = [
     "uvlm_last_error", "uvlm_abi_version", "uvlm_create", "uvlm_destroy", "uvlm_set_bodies_aos",
     "uvlm_get_bodies_aos", "uvlm_set_bodies_soa", "uvlm_get_bodies_soa", "uvlm_get_permutation",
     "uvlm_build", "uvlm_eval", "uvlm_stage", "uvlm_advance", "uvlm_num_cells", "uvlm_get_tree",
-    "uvlm_get_lists", "uvlm_get_expansions", "uvlm_sensors_aos", "uvlm_direct", "uvlm_timings", "uvlm_sync",
+    "uvlm_get_lists", "uvlm_get_expansions", "uvlm_sensors_aos", "uvlm_direct", "uvlm_probe_adjoint",
+    "uvlm_update_bodies_soa", "uvlm_timings", "uvlm_sync",
 ]
 
 
@@ -37,7 +38,8 @@ class Timing(ctypes.Structure):
                 ("m2l_ms", ctypes.c_double), ("p2p_ms", ctypes.c_double), ("downward_ms", ctypes.c_double),
                 ("eval_ms", ctypes.c_double), ("n_bodies", ctypes.c_int64), ("n_cells", ctypes.c_int64),
                 ("n_leaves", ctypes.c_int64), ("n_p2p_pairs", ctypes.c_int64), ("n_m2l_pairs", ctypes.c_int64),
-                ("n_levels", ctypes.c_int64), ("n_interactions", ctypes.c_int64)]
+                ("n_levels", ctypes.c_int64), ("n_interactions", ctypes.c_int64), ("sensor_ms", ctypes.c_double),
+                ("adjoint_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -73,6 +75,8 @@ def lib():
             "uvlm_get_expansions": (I, [VP, PD, PD]),
             "uvlm_sensors_aos": (I, [VP, VP, I64, I64]),
             "uvlm_direct": (I, [VP, I64, PD, PD, PD, PD, PD]),
+            "uvlm_probe_adjoint": (I, [VP, I64, PD, PD, PD, PD]),
+            "uvlm_update_bodies_soa": (I, [VP, I64, PD, PD]),
             "uvlm_timings": (I, [VP, ctypes.POINTER(Timing)]),
             "uvlm_sync": (I, [VP]),
         }

@@ -39,11 +39,12 @@ struct uvlm_handle_s {
   bool        built = false, evaluated = false;
   hipEvent_t  ev[EV_COUNT]{};
   uvlm_timing tm{};
-  // probe (sensor) evaluation
+  // probe (sensor) evaluation and its adjoint
   Tree    stree;
-  Lists   slists;
-  DevBuf  sst, sst2, Ls, sperm;
+  Lists   slists, alists;
+  DevBuf  sst, sst2, Ls, Ms, sperm, ast;
   int64_t scap = 0;
+  hipEvent_t aev[2]{};
 
   double *F(int f) { return st.as<double>() + (int64_t)f * cap; }
   void    ensure_cap(int64_t n)
@@ -81,6 +82,13 @@ template <class Fn> int guarded(Fn &&fn)
 }
 
 void set_device(uvlm_handle h) { UVLM_HIP(hipSetDevice(h->prm.device)); }
+
+float ev_ms2(uvlm_handle h, int a, int b)
+{
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, h->aev[a], h->aev[b]) != hipSuccess) return 0.f;
+  return ms;
+}
 
 float ev_ms(uvlm_handle h, int a, int b)
 {
@@ -195,13 +203,18 @@ int uvlm_create(const uvlm_params *p, uvlm_handle *out)
     set_device(h);
     UVLM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     for (auto &e : h->ev) UVLM_HIP(hipEventCreate(&e));
+    for (auto &e : h->aev) UVLM_HIP(hipEventCreate(&e));
     // M2L lane schedule shape: window of W consecutive outputs, K items per lane
-    // (UVLM_M2L_W in {1, 2, 4} overrides the default for tuning sweeps)
+    // (UVLM_M2L_W in {1, 2, 4} and UVLM_M2L_K in {2, 3} override the default for tuning sweeps)
     h->sched.W = 2;
     h->sched.K = 3;
     if (const char *w = std::getenv("UVLM_M2L_W")) {
       const int v = std::atoi(w);
       if (v == 1 || v == 2 || v == 4) h->sched.W = v;
+    }
+    if (const char *k = std::getenv("UVLM_M2L_K")) {
+      const int v = std::atoi(k);
+      if ((v == 2 && h->sched.W == 4) || v == 3) h->sched.K = v;
     }
     make_m2l_schedule(p->P, h->sched);
   });
@@ -219,6 +232,8 @@ int uvlm_destroy(uvlm_handle h)
   (void)hipSetDevice(h->prm.device);
   if (h->s) (void)hipStreamSynchronize(h->s);
   for (auto &e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : h->aev)
     if (e) (void)hipEventDestroy(e);
   if (h->s) (void)hipStreamDestroy(h->s);
   delete h;
@@ -459,6 +474,7 @@ int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride)
     h->staging.reserve((size_t)m * stride);
     UVLM_HIP(hipMemcpyAsync(h->staging.p, probes, (size_t)m * stride, hipMemcpyHostToDevice, s));
     aos_to_soa(s, h->staging.p, m, stride / 8, h->sst.as<double>(), sc);
+    UVLM_HIP(hipEventRecord(h->aev[0], s));
     double *sx = h->sst.as<double>();
     build_tree(s, h->sc, sx + F_X * sc, sx + (F_X + 1) * sc, sx + (F_X + 2) * sc, m, h->prm.ncrit, h->stree,
                h->sperm.as<int32_t>());
@@ -474,10 +490,107 @@ int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride)
         h->prm.precision, 1);
     for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls);
     l2p_stretch(s, P, h->stree, Ls, h->sst.as<double>(), sc, h->prm.stretching, h->prm.formulation, 1);
+    UVLM_HIP(hipEventRecord(h->aev[1], s));
     if (stride > 296) UVLM_HIP(hipMemcpyAsync(h->staging.p, probes, (size_t)m * stride, hipMemcpyHostToDevice, s));
     soa_to_aos(s, h->sst.as<double>(), sc, h->staging.p, m, stride / 8);
     UVLM_HIP(hipMemcpyAsync(probes, h->staging.p, (size_t)m * stride, hipMemcpyDeviceToHost, s));
     UVLM_HIP(hipStreamSynchronize(s));
+    h->tm.sensor_ms = ev_ms2(h, 0, 1);
+  });
+}
+
+int uvlm_probe_adjoint(uvlm_handle h, int64_t m, const double *probes, const double *weights, double *dJ_dalpha,
+                       double *dJ_dx)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->built) throw Error(UVLM_ERR_STATE, "uvlm_probe_adjoint needs a uvlm_build");
+    if (m <= 0 || !probes || !weights) throw Error(UVLM_ERR_ARG, "empty or null probes");
+    set_device(h);
+    hipStream_t s = h->s;
+    const int   P = h->prm.P, NT = h->NT;
+    if (m > h->scap) {
+      h->scap = m;
+      h->sst.reserve(sizeof(double) * N_FIELDS * m);
+      h->sst2.reserve(sizeof(double) * N_FIELDS * m);
+      h->sperm.reserve(4 * m);
+    }
+    const int64_t sc = h->scap;
+    // probes as sources: X = probe, alpha = weight, sigma = 1 (the kernel takes the target's)
+    std::vector<double> host((size_t)N_FIELDS * sc, 0.0);
+    for (int64_t i = 0; i < m; i++) {
+      for (int d = 0; d < 3; d++) {
+        host[(F_X + d) * sc + i] = probes[3 * i + d];
+        host[(F_A + d) * sc + i] = weights[3 * i + d];
+      }
+      host[F_SIG * sc + i] = 1.0;
+    }
+    UVLM_HIP(hipMemcpyAsync(h->sst.p, host.data(), 8 * host.size(), hipMemcpyHostToDevice, s));
+    UVLM_HIP(hipStreamSynchronize(s));
+    UVLM_HIP(hipEventRecord(h->aev[0], s));
+    double *sx = h->sst.as<double>();
+    build_tree(s, h->sc, sx + F_X * sc, sx + (F_X + 1) * sc, sx + (F_X + 2) * sc, m, h->prm.ncrit, h->stree,
+               h->sperm.as<int32_t>());
+    gather_fields(s, h->sst.as<double>(), h->sst2.as<double>(), sc, sc, N_FIELDS, h->sperm.as<int32_t>(), m);
+    swap_buf(h->sst, h->sst2);
+    // the forward probe lists, transposed: targets = particle cells, sources = probe cells
+    traverse(s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
+    transpose_lists(s, h->sc, h->stree, h->tree, h->slists, h->alists);
+    // probe multipoles (root included: a probe cell is a source here)
+    h->Ms.reserve(sizeof(double2) * NT * h->stree.C);
+    double2 *Ms = h->Ms.as<double2>();
+    p2m(s, P, h->stree, h->sst.as<double>(), sc, Ms);
+    for (int l = h->stree.nlevels - 2; l >= 0; l--) m2m_level(s, P, h->stree, l, Ms);
+    // adjoint state of the particles: positions, strengths and sigma copied in
+    const int64_t cap = h->cap, N = h->N;
+    h->ast.reserve(sizeof(double) * N_FIELDS * cap);
+    double *a = h->ast.as<double>();
+    zero_doubles(s, a, N_FIELDS * cap);
+    UVLM_HIP(hipMemcpyAsync(a + F_X * cap, h->F(F_X), 8 * 6 * cap, hipMemcpyDeviceToDevice, s));
+    UVLM_HIP(hipMemcpyAsync(a + F_SIG * cap, h->F(F_SIG), 8 * cap, hipMemcpyDeviceToDevice, s));
+    h->L.reserve(sizeof(double2) * NT * h->tree.C);
+    double2 *La = h->L.as<double2>();
+    m2l(s, P, h->tree, h->stree, h->alists, h->sched, Ms, La);
+    zero_doubles(s, (double *)La, 2 * NT);  // the particle root multipole is never formed forward
+    for (int l = 1; l < h->tree.nlevels; l++) l2l_level(s, P, h->tree, l, La);
+    p2p(s, h->tree, h->stree, h->alists, a, cap, h->sst.as<double>(), sc, a, cap, h->prm.precision, 2);
+    l2p_stretch(s, P, h->tree, La, a, cap, UVLM_TRANSPOSED, UVLM_CVPM, 0);
+    UVLM_HIP(hipEventRecord(h->aev[1], s));
+    h->evaluated = false;  // L now holds adjoint locals
+    std::vector<double> buf((size_t)3 * N);
+    auto fetch = [&](int f0, double *out) {
+      if (!out) return;
+      for (int d = 0; d < 3; d++)
+        UVLM_HIP(hipMemcpyAsync(buf.data() + (size_t)d * N, a + (f0 + d) * cap, 8 * N, hipMemcpyDeviceToHost, s));
+      UVLM_HIP(hipStreamSynchronize(s));
+      for (int64_t i = 0; i < N; i++)
+        for (int d = 0; d < 3; d++) out[3 * i + d] = buf[(size_t)d * N + i];
+    };
+    fetch(F_U, dJ_dalpha);
+    fetch(F_DADT, dJ_dx);
+    UVLM_HIP(hipStreamSynchronize(s));
+    h->tm.adjoint_ms = ev_ms2(h, 0, 1);
+  });
+}
+
+int uvlm_update_bodies_soa(uvlm_handle h, int64_t n, const double *X, const double *alpha)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (n != h->N) throw Error(UVLM_ERR_ARG, "size mismatch");
+    set_device(h);
+    std::vector<double> buf((size_t)3 * n);
+    auto put = [&](int f0, const double *in) {
+      if (!in) return;
+      for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) buf[(size_t)d * n + i] = in[3 * i + d];
+      for (int d = 0; d < 3; d++)
+        UVLM_HIP(hipMemcpyAsync(h->F(f0 + d), buf.data() + (size_t)d * n, 8 * n, hipMemcpyHostToDevice, h->s));
+      UVLM_HIP(hipStreamSynchronize(h->s));
+    };
+    put(F_X, X);
+    put(F_A, alpha);
+    h->evaluated = false;
   });
 }
 

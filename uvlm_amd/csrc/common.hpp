@@ -124,6 +124,9 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
 void gather_fields(hipStream_t s, const double *src, double *dst, int64_t src_cap, int64_t dst_cap, int nfields,
                    const int32_t *perm, int64_t N);
 void gather_i32(hipStream_t s, const int32_t *src, int32_t *dst, const int32_t *perm, int64_t N);
+// Lists of (sources as targets): CSR by the cells of `in_sources`, entries = cells of `in_targets`.
+void transpose_lists(hipStream_t s, Scratch &sc, const Tree &in_targets, const Tree &in_sources, const Lists &in,
+                     Lists &out);
 void count_interactions(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, const Lists &l, int64_t *out);
 
 // far.hip

@@ -379,29 +379,49 @@ __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const 
         if (!valid[i]) continue;
         const M2LItem I = it[i];
         for (int n = I.nlo; n <= I.nhi; n++) {
-          const int J = I.j + n;
-          double2   y[W];
+          const int      J  = I.j + n;
+          const double2 *YJ = Yg + tri(J);
+          const double2 *Mn = Mg + 3 * tri(n);
+          double2        y[W];
           // window: y[q] = Y[J, m - k0 - q] at step m; each step shifts by one and loads
-          // y[0], so before m = -n slot i holds Y[J, -n - k0 - 1 - i]
+          // y[0], so before m = -n slot i holds Y[J, -n - k0 - 1 - i] (zero below -J)
           for (int q = 0; q + 1 < W; q++) {
             const int mu = -n - I.k0 - 1 - q;
-            y[q]         = (mu >= -J) ? ylook(Yg, J, mu) : make_double2(0, 0);
+            y[q]         = (mu >= -J) ? conj2(YJ[-mu]) : make_double2(0, 0);
           }
           y[W - 1] = make_double2(0, 0);
-          for (int m = -n; m <= n; m++) {
+          // m < 0: conj(M[n,-m]) Y[J, m-k], no sign (multipoleKernels.hpp:534-539)
+          for (int m = -n; m < 0; m++) {
             for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
-            y[0] = ylook(Yg, J, m - I.k0);
-            double2 mv[3];
-            const int am = m < 0 ? -m : m;
+            y[0] = conj2(YJ[I.k0 - m]);
             for (int d = 0; d < 3; d++) {
-              mv[d] = Mg[3 * (tri(n) + am) + d];
-              if (m < 0) mv[d].y = -mv[d].y;
+              const double2 mv = conj2(Mn[3 * (-m) + d]);
+              for (int q = 0; q < W; q++) cfma(acc[i][q][d], mv, y[q]);
             }
+          }
+          // 0 <= m < k0 <= k: sign (-1)^m for every output of the window (540-546)
+          for (int m = 0; m < I.k0 && m <= n; m++) {
+            for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
+            y[0]            = conj2(YJ[I.k0 - m]);
+            const double sg = sgn(m);
+            for (int d = 0; d < 3; d++) {
+              const double2 mv = make_double2(sg * Mn[3 * m + d].x, sg * Mn[3 * m + d].y);
+              for (int q = 0; q < W; q++) cfma(acc[i][q][d], mv, y[q]);
+            }
+          }
+          // m >= k0: sign (-1)^k when k < m, else (-1)^m
+          for (int m = I.k0; m <= n; m++) {
+            for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
+            y[0] = YJ[m - I.k0];
+            double2 ys[W];
             for (int q = 0; q < W; q++) {
-              const int    k  = I.k0 + q;
-              const double sg = (m < 0) ? 1.0 : ((k < m) ? sgn(k) : sgn(m));
-              const double2 ys = make_double2(y[q].x * sg, y[q].y * sg);
-              for (int d = 0; d < 3; d++) cfma(acc[i][q][d], mv[d], ys);
+              const int k = I.k0 + q;
+              const int e = k < m ? k : m;
+              ys[q]       = (e & 1) ? make_double2(-y[q].x, -y[q].y) : y[q];
+            }
+            for (int d = 0; d < 3; d++) {
+              const double2 mv = Mn[3 * m + d];
+              for (int q = 0; q < W; q++) cfma(acc[i][q][d], mv, ys[q]);
             }
           }
         }
@@ -715,6 +735,7 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
   }
   UVLM_M2L_CASE(1, 3)
   UVLM_M2L_CASE(2, 3)
+  UVLM_M2L_CASE(4, 2)
   UVLM_M2L_CASE(4, 3)
 #undef UVLM_M2L_CASE
   throw Error(UVLM_ERR_INTERNAL, "unsupported M2L schedule shape");

@@ -93,7 +93,10 @@ __device__ __forceinline__ void lds_prefix(int *pref, const int *v, int n)
   if (lane == 63 && 256 <= n) pref[256] = run;
 }
 
-template <bool MIXED, bool WITH_J>
+// TSIG: the core size comes from the TARGET body instead of the source.  That is the
+// transposed near-field kernel used by the adjoint (a particle's own sigma, with the
+// particle now on the target side).
+template <bool MIXED, bool WITH_J, bool TSIG = false>
 __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, int64_t nleaf,
                                             const int32_t *__restrict__ tbody, const int32_t *__restrict__ tnbody,
                                             const double *__restrict__ tcx, const double *__restrict__ tcy,
@@ -121,6 +124,11 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
     const int64_t bi = tb + (valid ? i : 0);
     const double  xi = tst[(F_X + 0) * tcap + bi], yi = tst[(F_X + 1) * tcap + bi], zi = tst[(F_X + 2) * tcap + bi];
     const float   rxi = (float)(xi - cxt), ryi = (float)(yi - cyt), rzi = (float)(zi - czt);
+    // target-sigma constants (TSIG only)
+    const double  tsg = TSIG ? tst[F_SIG * tcap + bi] : 1.0;
+    const double  tqq = 0.5 / (tsg * tsg), tsq = sqrt(tqq);
+    const float   t_q = (float)tqq, t_c1 = (float)(tqq * tsq * INV_4PI), t_c2 = (float)(2.0 * tqq * tqq * tsq * INV_4PI);
+    const double  t_is2 = 1.0 / (tsg * tsg), t_half = 0.5 * t_is2;
     double        acc[12];
     for (int k = 0; k < 12; k++) acc[k] = 0.0;
     for (int g0 = q0; g0 < q1; g0 += SEG) {
@@ -167,7 +175,7 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
             const float4 A = s32[3 * j], B = s32[3 * j + 1], Cc = s32[3 * j + 2];
             const float  dx = rxi - A.x, dy = ryi - A.y, dz = rzi - A.z;
             const float  r2 = dx * dx + dy * dy + dz * dz;
-            const float  xq = r2 * A.w;
+            const float  xq = r2 * (TSIG ? t_q : A.w);
             float        f1, f2;
             if (xq < 1.0f) {
               float g = (float)taylor_coef(NTAYLOR - 1), gp = 0.f;
@@ -176,8 +184,8 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
                 gp = fmaf(gp, xq, g);
                 g  = fmaf(g, xq, (float)taylor_coef(k));
               }
-              f1 = g * B.w;
-              f2 = gp * Cc.x;
+              f1 = g * (TSIG ? t_c1 : B.w);
+              f2 = gp * (TSIG ? t_c2 : Cc.x);
             } else {
               const float ir  = rsqrtf(r2);
               const float s   = sqrtf(xq);
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
             const double  dx = xi - sj.x, dy = yi - sj.y, dz = zi - sj.z;
             const double  R2 = dx * dx + dy * dy + dz * dz;
             if (R2 == 0.0) continue;
-            const double RS     = R2 * sj.halfInvS2;
+            const double RS     = R2 * (TSIG ? t_half : sj.halfInvS2);
             const double sqrtRS = sqrt(RS);
             const double erfS   = erf(sqrtRS);
             const double expm   = exp(-RS);
@@ -212,7 +220,7 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
             acc[1] += f1 * cy;
             acc[2] += f1 * cz;
             if (WITH_J) {
-              const double f2 = (t2 * sj.invS2 - 3.0 * g / R2) * o4pr3;
+              const double f2 = (t2 * (TSIG ? t_is2 : sj.invS2) - 3.0 * g / R2) * o4pr3;
               const double k0 = f2 * cx, k1 = f2 * cy, k2 = f2 * cz;
               acc[3] += k0 * dx;
               acc[4] += k0 * dy + f1 * sj.az;
@@ -319,16 +327,20 @@ void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const
 {
   if (tt.nleaf == 0) return;
   const int nb = (int)tt.nleaf;
-#define UVLM_P2P(MX, WJ)                                                                                          \
-  k_p2p<MX, WJ><<<nb, WG, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(), tt.nbody.as<int32_t>(), \
+#define UVLM_P2P(MX, WJ, TS)                                                                                      \
+  k_p2p<MX, WJ, TS><<<nb, WG, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(), tt.nbody.as<int32_t>(), \
                                   tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),                     \
                                   l.p2p_off.as<int32_t>(), l.p2p_src.as<int32_t>(), stree.body.as<int32_t>(),      \
                                   stree.nbody.as<int32_t>(), tgt, tcap, srcst, scap, out, ocap)
+  // sensor_mode: 0 particles (u and J), 1 probes (u only), 2 adjoint (u and J, target sigma)
   const bool mixed = precision == UVLM_MIXED;
-  if (mixed && !sensor_mode) UVLM_P2P(true, true);
-  else if (mixed) UVLM_P2P(true, false);
-  else if (!sensor_mode) UVLM_P2P(false, true);
-  else UVLM_P2P(false, false);
+  if (sensor_mode == 2) {
+    if (mixed) UVLM_P2P(true, true, true);
+    else UVLM_P2P(false, true, true);
+  } else if (mixed && !sensor_mode) UVLM_P2P(true, true, false);
+  else if (mixed) UVLM_P2P(true, false, false);
+  else if (!sensor_mode) UVLM_P2P(false, true, false);
+  else UVLM_P2P(false, false, false);
 #undef UVLM_P2P
   UVLM_LAUNCH_CHECK();
 }

@@ -401,6 +401,14 @@ __global__ void k_low32_u(int64_t n, const uint64_t *__restrict__ keys, int32_t 
   if (i < n) out[i] = (int32_t)(uint32_t)(keys[i] & 0xffffffffull);
 }
 
+__global__ void k_transpose_keys(int64_t C, const int32_t *__restrict__ off, const int32_t *__restrict__ src,
+                                 uint64_t *__restrict__ keys)
+{
+  const int64_t t = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (t >= C) return;
+  for (int q = off[t]; q < off[t + 1]; q++) keys[q] = ((uint64_t)(uint32_t)src[q] << 32) | (uint64_t)(uint32_t)t;
+}
+
 __global__ void k_count_inter(int64_t C, const int32_t *__restrict__ off, const int32_t *__restrict__ src,
                               const int32_t *__restrict__ tn, const int32_t *__restrict__ sn,
                               unsigned long long *__restrict__ out)
@@ -680,6 +688,33 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
     sc.keys1.reserve(8 * std::max<int64_t>(n, 1));
     if (n > 0) sort_keys_u64(s, sc, raw.as<uint64_t>(), sc.keys1.as<uint64_t>(), n, eb);
     k_csr<<<nblocks(C + 1), BLK, 0, s>>>(C, sc.keys1.as<uint64_t>(), n, off.as<int32_t>());
+    k_low32_u<<<nblocks(n), BLK, 0, s>>>(n, sc.keys1.as<uint64_t>(), src.as<int32_t>());
+    UVLM_LAUNCH_CHECK();
+  }
+}
+
+void transpose_lists(hipStream_t s, Scratch &sc, const Tree &in_targets, const Tree &in_sources, const Lists &in,
+                     Lists &out)
+{
+  const int64_t Cs = in_sources.C;
+  const int     eb = 32 + bits_for(std::max<int64_t>(Cs, 1));
+  out.np2p         = in.np2p;
+  out.nm2l         = in.nm2l;
+  for (int which = 0; which < 2; which++) {
+    const int64_t n    = which == 0 ? in.np2p : in.nm2l;
+    const DevBuf &ioff = which == 0 ? in.p2p_off : in.m2l_off;
+    const DevBuf &isrc = which == 0 ? in.p2p_src : in.m2l_src;
+    DevBuf       &off  = which == 0 ? out.p2p_off : out.m2l_off;
+    DevBuf       &src  = which == 0 ? out.p2p_src : out.m2l_src;
+    off.reserve(4 * (Cs + 1));
+    src.reserve(4 * std::max<int64_t>(n, 1));
+    sc.keys0.reserve(8 * std::max<int64_t>(n, 1));
+    sc.keys1.reserve(8 * std::max<int64_t>(n, 1));
+    k_transpose_keys<<<nblocks(in_targets.C), BLK, 0, s>>>(in_targets.C, ioff.as<int32_t>(), isrc.as<int32_t>(),
+                                                           sc.keys0.as<uint64_t>());
+    UVLM_LAUNCH_CHECK();
+    if (n > 0) sort_keys_u64(s, sc, sc.keys0.as<uint64_t>(), sc.keys1.as<uint64_t>(), n, eb);
+    k_csr<<<nblocks(Cs + 1), BLK, 0, s>>>(Cs, sc.keys1.as<uint64_t>(), n, off.as<int32_t>());
     k_low32_u<<<nblocks(n), BLK, 0, s>>>(n, sc.keys1.as<uint64_t>(), src.as<int32_t>());
     UVLM_LAUNCH_CHECK();
   }

@@ -158,6 +158,29 @@ class FMMCalculator:
                                          _dp(J) if J is not None else None, _dp(D) if D is not None else None))
         return u, J, D
 
+    def update_arrays(self, X=None, alpha=None):
+        """Overwrite positions/strengths in the current tree order, keeping tree and lists."""
+        X = None if X is None else np.ascontiguousarray(X, dtype=np.float64)
+        A = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
+        _capi.check(self.lib.uvlm_update_bodies_soa(self._h, self.n, _dp(X) if X is not None else None,
+                                                    _dp(A) if A is not None else None))
+
+    def probe_adjoint(self, probes, weights):
+        """dJ/dalpha, dJ/dx (tree order) of J = sum_p weights_p . u(probes_p)."""
+        Pp = np.ascontiguousarray(probes, dtype=np.float64)
+        W = np.ascontiguousarray(np.broadcast_to(weights, Pp.shape), dtype=np.float64)
+        ga = np.zeros((self.n, 3))
+        gx = np.zeros((self.n, 3))
+        _capi.check(self.lib.uvlm_probe_adjoint(self._h, Pp.shape[0], _dp(Pp), _dp(W), _dp(ga), _dp(gx)))
+        return ga, gx
+
+    def probe_objective(self, probes, weights):
+        """J = sum_p weights_p . u(probes_p) through getSensorData (needs a calculate())."""
+        from .fmm import make_bodies as _mk
+        s = _mk(probes, np.broadcast_to(weights, np.shape(probes)), np.ones(len(probes)))
+        self.getSensorData(s)
+        return float(np.sum(s[:, BODY_FIELDS["alpha"]] * s[:, BODY_FIELDS["velocity"]]))
+
     # -- inspection -----------------------------------------------------------------
     def cells(self):
         C = ctypes.c_int64()
