@@ -149,6 +149,16 @@ int uvlm_probe_adjoint(uvlm_handle h, int64_t m, const double *probes, const dou
  * in the reference. */
 int uvlm_update_bodies_soa(uvlm_handle h, int64_t n, const double *X, const double *alpha);
 
+/* Pedrizzetti relaxation of the strengths toward the vorticity w = curl u of the last
+ * uvlm_eval: alpha <- (1-f) alpha + f |alpha| w / |w| (bodies with w = 0 unchanged).
+ * Replaces the pseudo-code of src/relaxation.hpp:8-18, which does not compile. */
+int uvlm_relax(uvlm_handle h, double f);
+/* Invariants of the current particle set: out[0..2] linear impulse 1/2 sum x x alpha,
+ * out[3..5] angular impulse 1/3 sum x x (x x alpha), out[6] energy sum u . (x x alpha)
+ * (needs the velocity of a uvlm_eval), out[7..9] total strength sum alpha.  No reference
+ * counterpart (the reference prints only strength extrema, src/main.cpp:219-221). */
+int uvlm_diagnostics(uvlm_handle h, double *out);
+
 int uvlm_timings(uvlm_handle h, uvlm_timing *out);
 
 /* Device synchronisation of the handle's stream. */

@@ -227,3 +227,29 @@ def test_probe_adjoint(orc):
     ex_, ea_ = rel(np.array(an_x), np.array(fd_x)), rel(np.array(an_a), np.array(fd_a))
     print(f"adjoint vs FD of the FMM objective: dJ/dx {ex_:.3e}  dJ/dalpha {ea_:.3e}")
     assert ex_ < 1e-3 and ea_ < 1e-3
+
+
+def test_relaxation_and_diagnostics():
+    """Relaxation invariants (f=0 identity; f=1 aligns alpha with curl u and keeps |alpha|)
+    and diagnostics against their definitions on the downloaded state."""
+    X, A, S = W.scaled("leapfrog", 16384)
+    g = FMMCalculator(P=8, theta=0.5, ncrit=64)
+    g.set_arrays(X, A, S)
+    g.build()
+    g.calculate()
+    arr = g.arrays()
+    dg = g.diagnostics()
+    x, a, u = arr["X"], arr["alpha"], arr["velocity"]
+    c = np.cross(x, a)
+    assert np.allclose(dg["impulse"], 0.5 * c.sum(0), rtol=1e-12, atol=1e-15)
+    assert np.allclose(dg["angular_impulse"], np.cross(x, c).sum(0) / 3, rtol=1e-12, atol=1e-15)
+    assert abs(dg["energy"] - np.einsum("ij,ij->", u, c)) < 1e-12 * abs(dg["energy"])
+    g.relax(0.0)
+    assert np.array_equal(g.arrays()["alpha"], a)
+    g.relax(1.0)
+    a1 = g.arrays()["alpha"]
+    Jm = arr["J"].reshape(-1, 3, 3)
+    w = np.stack([Jm[:, 2, 1] - Jm[:, 1, 2], Jm[:, 0, 2] - Jm[:, 2, 0], Jm[:, 1, 0] - Jm[:, 0, 1]], 1)
+    assert np.allclose(np.linalg.norm(a1, axis=1), np.linalg.norm(a, axis=1), rtol=1e-12)
+    cosang = np.einsum("ij,ij->i", a1, w) / (np.linalg.norm(a1, axis=1) * np.linalg.norm(w, axis=1))
+    assert np.allclose(cosang, 1.0, atol=1e-12)

@@ -17,7 +17,7 @@ EXPORTS = [
     "uvlm_get_bodies_aos", "uvlm_set_bodies_soa", "uvlm_get_bodies_soa", "uvlm_get_permutation",
     "uvlm_build", "uvlm_eval", "uvlm_stage", "uvlm_advance", "uvlm_num_cells", "uvlm_get_tree",
     "uvlm_get_lists", "uvlm_get_expansions", "uvlm_sensors_aos", "uvlm_direct", "uvlm_probe_adjoint",
-    "uvlm_update_bodies_soa", "uvlm_timings", "uvlm_sync",
+    "uvlm_update_bodies_soa", "uvlm_relax", "uvlm_diagnostics", "uvlm_timings", "uvlm_sync",
 ]
 
 
@@ -77,6 +77,8 @@ def lib():
             "uvlm_direct": (I, [VP, I64, PD, PD, PD, PD, PD]),
             "uvlm_probe_adjoint": (I, [VP, I64, PD, PD, PD, PD]),
             "uvlm_update_bodies_soa": (I, [VP, I64, PD, PD]),
+            "uvlm_relax": (I, [VP, D]),
+            "uvlm_diagnostics": (I, [VP, PD]),
             "uvlm_timings": (I, [VP, ctypes.POINTER(Timing)]),
             "uvlm_sync": (I, [VP]),
         }

@@ -6,6 +6,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <vector>
 
 namespace uvlm {
 
@@ -124,6 +125,59 @@ __global__ void k_rk(double *__restrict__ st, int64_t cap, int64_t N, int scheme
   }
 }
 
+// Pedrizzetti relaxation, src/relaxation.hpp:8-18 in 0-based form with J[a][b] = du_a/dx_b:
+// w = curl u = (J21 - J12, J02 - J20, J10 - J01); alpha <- (1-f) alpha + f |alpha| w/|w|.
+// Bodies with |w| == 0 are left unchanged.
+__global__ void k_relax(double *__restrict__ st, int64_t cap, int64_t N, double f)
+{
+  const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (i >= N) return;
+  auto         Jm = [&](int a, int b) { return st[(F_J + 3 * a + b) * cap + i]; };
+  const double w[3] = {Jm(2, 1) - Jm(1, 2), Jm(0, 2) - Jm(2, 0), Jm(1, 0) - Jm(0, 1)};
+  const double nw   = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  if (nw == 0.0) return;
+  double a[3];
+  for (int d = 0; d < 3; d++) a[d] = st[(F_A + d) * cap + i];
+  const double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  for (int d = 0; d < 3; d++) st[(F_A + d) * cap + i] = (1.0 - f) * a[d] + f * na * w[d] / nw;
+}
+
+// Invariants of the particle set, per block partial sums in a fixed order:
+// [0..2] linear impulse 1/2 sum x × alpha, [3..5] angular impulse 1/3 sum x × (x × alpha),
+// [6] kinetic energy sum u . (x × alpha), [7] total strength |sum alpha|^2 (3 sums [7..9]).
+constexpr int NDIAG = 10;
+__global__ void k_diag_partial(const double *__restrict__ st, int64_t cap, int64_t N, double *__restrict__ part)
+{
+  __shared__ double sh[NDIAG][BLK];
+  double v[NDIAG];
+  for (int k = 0; k < NDIAG; k++) v[k] = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLK) {
+    double x[3], a[3], u[3];
+    for (int d = 0; d < 3; d++) {
+      x[d] = st[(F_X + d) * cap + i];
+      a[d] = st[(F_A + d) * cap + i];
+      u[d] = st[(F_U + d) * cap + i];
+    }
+    const double c[3]  = {x[1] * a[2] - x[2] * a[1], x[2] * a[0] - x[0] * a[2], x[0] * a[1] - x[1] * a[0]};
+    const double cc[3] = {x[1] * c[2] - x[2] * c[1], x[2] * c[0] - x[0] * c[2], x[0] * c[1] - x[1] * c[0]};
+    for (int d = 0; d < 3; d++) {
+      v[d] += 0.5 * c[d];
+      v[3 + d] += cc[d] / 3.0;
+      v[7 + d] += a[d];
+    }
+    v[6] += u[0] * c[0] + u[1] * c[1] + u[2] * c[2];
+  }
+  for (int k = 0; k < NDIAG; k++) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < NDIAG; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NDIAG; k++) part[blockIdx.x * NDIAG + k] = sh[k][0];
+}
+
 __global__ void k_reset_sensor(double *__restrict__ st, int64_t cap, int64_t N)
 {
   const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
@@ -154,6 +208,25 @@ void rk_stage(hipStream_t s, double *st, int64_t cap, int64_t N, int scheme, int
 {
   k_rk<<<nblocks(N), BLK, 0, s>>>(st, cap, N, scheme, stage, dt);
   UVLM_LAUNCH_CHECK();
+}
+
+void relax(hipStream_t s, double *st, int64_t cap, int64_t N, double f)
+{
+  k_relax<<<nblocks(N), BLK, 0, s>>>(st, cap, N, f);
+  UVLM_LAUNCH_CHECK();
+}
+
+void diagnostics(hipStream_t s, const double *st, int64_t cap, int64_t N, double *part_dev, double out[10])
+{
+  const int nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, (N + BLK - 1) / BLK));
+  k_diag_partial<<<nb, BLK, 0, s>>>(st, cap, N, part_dev);
+  UVLM_LAUNCH_CHECK();
+  std::vector<double> h((size_t)nb * NDIAG);
+  UVLM_HIP(hipMemcpyAsync(h.data(), part_dev, 8 * h.size(), hipMemcpyDeviceToHost, s));
+  UVLM_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < NDIAG; k++) out[k] = 0.0;
+  for (int b = 0; b < nb; b++)
+    for (int k = 0; k < NDIAG; k++) out[k] += h[(size_t)b * NDIAG + k];
 }
 
 void reset_sensor_rates(hipStream_t s, double *st, int64_t cap, int64_t N)

@@ -573,6 +573,30 @@ int uvlm_probe_adjoint(uvlm_handle h, int64_t m, const double *probes, const dou
   });
 }
 
+int uvlm_relax(uvlm_handle h, double f)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->evaluated) throw Error(UVLM_ERR_STATE, "uvlm_relax needs the J of a uvlm_eval");
+    if (!(f >= 0.0 && f <= 1.0)) throw Error(UVLM_ERR_ARG, "relaxation factor must be in [0, 1]");
+    set_device(h);
+    relax(h->s, h->st.as<double>(), h->cap, h->N, f);
+    UVLM_HIP(hipStreamSynchronize(h->s));
+  });
+}
+
+int uvlm_diagnostics(uvlm_handle h, double *out)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!out) throw Error(UVLM_ERR_ARG, "null output");
+    if (h->N <= 0) throw Error(UVLM_ERR_STATE, "no bodies set");
+    set_device(h);
+    h->sc.partial.reserve(sizeof(double) * 10 * 512);
+    diagnostics(h->s, h->st.as<double>(), h->cap, h->N, h->sc.partial.as<double>(), out);
+  });
+}
+
 int uvlm_update_bodies_soa(uvlm_handle h, int64_t n, const double *X, const double *alpha)
 {
   return guarded([&] {

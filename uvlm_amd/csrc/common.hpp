@@ -160,5 +160,7 @@ void aos_to_soa(hipStream_t s, const void *aos, int64_t n, int64_t stride_double
 void soa_to_aos(hipStream_t s, const double *st, int64_t cap, void *aos, int64_t n, int64_t stride_doubles);
 void rk_stage(hipStream_t s, double *st, int64_t cap, int64_t N, int scheme, int stage, double dt, int store_initial);
 void reset_sensor_rates(hipStream_t s, double *st, int64_t cap, int64_t N);
+void relax(hipStream_t s, double *st, int64_t cap, int64_t N, double f);
+void diagnostics(hipStream_t s, const double *st, int64_t cap, int64_t N, double *part_dev, double out[10]);
 
 }  // namespace uvlm

@@ -181,6 +181,16 @@ class FMMCalculator:
         self.getSensorData(s)
         return float(np.sum(s[:, BODY_FIELDS["alpha"]] * s[:, BODY_FIELDS["velocity"]]))
 
+    def relax(self, f):
+        """Pedrizzetti relaxation of the strengths toward the last eval's vorticity."""
+        _capi.check(self.lib.uvlm_relax(self._h, float(f)))
+
+    def diagnostics(self):
+        """Linear impulse (3), angular impulse (3), energy, total strength (3)."""
+        out = np.zeros(10)
+        _capi.check(self.lib.uvlm_diagnostics(self._h, _dp(out)))
+        return {"impulse": out[0:3], "angular_impulse": out[3:6], "energy": out[6], "total_strength": out[7:10]}
+
     # -- inspection -----------------------------------------------------------------
     def cells(self):
         C = ctypes.c_int64()
