@@ -241,8 +241,9 @@ def test_relaxation_and_diagnostics():
     dg = g.diagnostics()
     x, a, u = arr["X"], arr["alpha"], arr["velocity"]
     c = np.cross(x, a)
-    assert np.allclose(dg["impulse"], 0.5 * c.sum(0), rtol=1e-12, atol=1e-15)
-    assert np.allclose(dg["angular_impulse"], np.cross(x, c).sum(0) / 3, rtol=1e-12, atol=1e-15)
+    I_ref, A_ref = 0.5 * c.sum(0), np.cross(x, c).sum(0) / 3
+    assert np.abs(dg["impulse"] - I_ref).max() < 1e-12 * np.linalg.norm(I_ref)
+    assert np.abs(dg["angular_impulse"] - A_ref).max() < 1e-12 * np.linalg.norm(A_ref)
     assert abs(dg["energy"] - np.einsum("ij,ij->", u, c)) < 1e-12 * abs(dg["energy"])
     g.relax(0.0)
     assert np.array_equal(g.arrays()["alpha"], a)

@@ -144,7 +144,7 @@ __global__ void k_relax(double *__restrict__ st, int64_t cap, int64_t N, double 
 
 // Invariants of the particle set, per block partial sums in a fixed order:
 // [0..2] linear impulse 1/2 sum x × alpha, [3..5] angular impulse 1/3 sum x × (x × alpha),
-// [6] kinetic energy sum u . (x × alpha), [7] total strength |sum alpha|^2 (3 sums [7..9]).
+// [6] kinetic energy sum u . (x × alpha), [7..9] total strength sum alpha.
 constexpr int NDIAG = 10;
 __global__ void k_diag_partial(const double *__restrict__ st, int64_t cap, int64_t N, double *__restrict__ part)
 {
