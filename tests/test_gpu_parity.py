@@ -190,9 +190,11 @@ def test_cpp_facade_drop_in(orc, tmp_path):
 
 
 def test_probe_adjoint(orc):
-    """Adjoint of J = sum_p v . u(p) (probes from the blob mixture): vs the exact direct-sum
-    adjoint, and vs central finite differences of the FP64 FMM forward objective with the
-    tree and lists frozen."""
+    """Adjoint of J = sum_p v . u(p) (probes from the blob mixture): vs central finite
+    differences (h = 1e-6) of the FP64 FMM forward objective with the tree and lists frozen
+    (bar 1e-6; the far-field passes use the forward lists transposed, so the adjoint is the
+    discrete one), and vs the exact direct-sum adjoint (which differs by the FMM's own
+    far-field approximation: singular kernel beyond the MAC)."""
     X, A, S, probes, v, chk = W.blobs(n=30000, n_probe=512, n_check=8)
     g = FMMCalculator(P=8, theta=0.5, ncrit=64)
     g.set_arrays(X, A, S)
@@ -226,7 +228,7 @@ def test_probe_adjoint(orc):
     print("fd dJ/dx", np.array(fd_x)[:6], "adjoint", np.array(an_x)[:6])
     ex_, ea_ = rel(np.array(an_x), np.array(fd_x)), rel(np.array(an_a), np.array(fd_a))
     print(f"adjoint vs FD of the FMM objective: dJ/dx {ex_:.3e}  dJ/dalpha {ea_:.3e}")
-    assert ex_ < 1e-3 and ea_ < 1e-3
+    assert ex_ < 1e-6 and ea_ < 1e-6
 
 
 def test_relaxation_and_diagnostics():
@@ -242,8 +244,9 @@ def test_relaxation_and_diagnostics():
     x, a, u = arr["X"], arr["alpha"], arr["velocity"]
     c = np.cross(x, a)
     I_ref, A_ref = 0.5 * c.sum(0), np.cross(x, c).sum(0) / 3
-    assert np.abs(dg["impulse"] - I_ref).max() < 1e-12 * np.linalg.norm(I_ref)
-    assert np.abs(dg["angular_impulse"] - A_ref).max() < 1e-12 * np.linalg.norm(A_ref)
+    # sums of 16k terms in a different order: agreement to 1e-9 of the vector norm
+    assert np.abs(dg["impulse"] - I_ref).max() < 1e-9 * np.linalg.norm(I_ref)
+    assert np.abs(dg["angular_impulse"] - A_ref).max() < 1e-9 * np.linalg.norm(A_ref)
     assert abs(dg["energy"] - np.einsum("ij,ij->", u, c)) < 1e-12 * abs(dg["energy"])
     g.relax(0.0)
     assert np.array_equal(g.arrays()["alpha"], a)

@@ -76,3 +76,24 @@ def test_splitmix_reference_values():
     r = W.SplitMix64(0)
     z = r._next_u64(2)
     assert int(z[0]) == 0xE220A8397B1DCDAF and int(z[1]) == 0x6E789E6AA1B965F4
+
+
+def test_p2p_polynomial_matches_erf_form():
+    """The FP32 P2P's polynomial for G(x) = g(s)/s^3 (x = s^2, g = erf(s) - 2 s e^{-s^2}/sqrt(pi))
+    and its derivative, read from near.hip, against the erf form evaluated in FP64."""
+    import math
+    src = open(os.path.join(ROOT, "uvlm_amd", "csrc", "near.hip")).read()
+    body = src[src.index("c_gpoly[NPOLY] = {") + 18: src.index("};", src.index("c_gpoly[NPOLY] = {"))]
+    coef = [float(v.strip().rstrip("f")) for v in body.split(",")]
+    assert len(coef) == 10
+    for x in np.linspace(0.05, 2.0, 60):
+        s = math.sqrt(x)
+        g = math.erf(s) - 2 * s * math.exp(-x) / math.sqrt(math.pi)
+        G = g / s ** 3
+        # derivative: G'(x) = (s g'(s) - 3 g) / (2 s^5) with g'(s) = 4 s^2 e^{-s^2} / sqrt(pi)
+        Gp = (s * 4 * x * math.exp(-x) / math.sqrt(math.pi) - 3 * g) / (2 * s ** 5)
+        t = x - 1.0
+        p = sum(c * t ** i for i, c in enumerate(coef))
+        pp = sum(i * c * t ** (i - 1) for i, c in enumerate(coef) if i)
+        assert abs(p - G) < 1e-9 * abs(G) + 1e-12
+        assert abs(pp - Gp) < 2e-8 * 0.3
