@@ -332,6 +332,8 @@ __device__ __forceinline__ M2LItem unpack_item(int32_t v)
   return it;
 }
 
+constexpr int M2L_PF = 12;  // source-multipole entries per lane prefetched into registers (NT <= 192)
+
 template <int W, int K>
 __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const int32_t *__restrict__ off,
                                             const int32_t *__restrict__ src, const double *__restrict__ tx,
@@ -361,19 +363,40 @@ __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const 
       for (int d = 0; d < 3; d++) acc[i][q][d] = make_double2(0, 0);
   const int    s0 = off[t], nsrc = off[t + 1] - s0;
   const double X = tx[t], Yc = ty[t], Z = tz[t];
+  // The group's next source multipole is loaded into registers while the current one is
+  // being used, so its latency overlaps the multiply-adds.
+  double2 pf[M2L_PF];
+#define UVLM_M2L_PREFETCH(SI)                                                                                       \
+  if ((SI) < nsrc) {                                                                                               \
+    const double2 *Ms_ = M + (int64_t)src[s0 + (SI)] * NT;                                                         \
+    _Pragma("unroll") for (int r = 0; r < M2L_PF; r++)                                                             \
+    {                                                                                                              \
+      const int q = sl + 16 * r;                                                                                   \
+      pf[r]       = q < NT ? Ms_[q] : make_double2(0, 0);                                                          \
+    }                                                                                                              \
+  }
+  UVLM_M2L_PREFETCH(g)
   for (int base = 0; base < nsrc; base += 4) {
     const int  si     = base + g;
     const bool active = si < nsrc;
     if (active) {
       const int s = src[s0 + si];
+#pragma unroll
+      for (int r = 0; r < M2L_PF; r++) {
+        const int q = sl + 16 * r;
+        if (q < NT) Mg[q] = pf[r];
+      }
+      if (NT > 16 * M2L_PF) {
+        const double2 *Ms = M + (int64_t)s * NT;
+        for (int q = sl + 16 * M2L_PF; q < NT; q += 16) Mg[q] = Ms[q];
+      }
       if (sl < P) {
         const Sph sp = sph(X - sx[s], Yc - sy[s], Z - sz[s]);
         harmonic_column<true>(P, sl, sp, Yg);
       }
-      const double2 *Ms = M + (int64_t)s * NT;
-      for (int q = sl; q < NT; q += 16) Mg[q] = Ms[q];
     }
     __syncthreads();
+    UVLM_M2L_PREFETCH(si + 4)
     if (active) {
       for (int i = 0; i < K; i++) {
         if (!valid[i]) continue;
@@ -386,11 +409,12 @@ __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const 
           // window: y[q] = Y[J, m - k0 - q] at step m; each step shifts by one and loads
           // y[0], so before m = -n slot i holds Y[J, -n - k0 - 1 - i] (zero below -J)
           for (int q = 0; q + 1 < W; q++) {
-            const int mu = -n - I.k0 - 1 - q;
-            y[q]         = (mu >= -J) ? conj2(YJ[-mu]) : make_double2(0, 0);
+            const int a = n + I.k0 + 1 + q;
+            y[q]        = (a <= J) ? conj2(YJ[a]) : make_double2(0, 0);
           }
           y[W - 1] = make_double2(0, 0);
-          // m < 0: conj(M[n,-m]) Y[J, m-k], no sign (multipoleKernels.hpp:534-539)
+          // m < 0: conj(M[n,-m]) Y[J, m-k], sign +1 (multipoleKernels.hpp:534-539)
+#pragma unroll 2
           for (int m = -n; m < 0; m++) {
             for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
             y[0] = conj2(YJ[I.k0 - m]);
@@ -409,8 +433,8 @@ __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const 
               for (int q = 0; q < W; q++) cfma(acc[i][q][d], mv, y[q]);
             }
           }
-          // m >= k0: sign (-1)^k when k < m, else (-1)^m
-          for (int m = I.k0; m <= n; m++) {
+          // k0 <= m < k0 + W: sign (-1)^k when k < m, else (-1)^m, per window slot
+          for (int m = I.k0; m <= n && m < I.k0 + W; m++) {
             for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
             y[0] = YJ[m - I.k0];
             double2 ys[W];
@@ -424,11 +448,34 @@ __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const 
               for (int q = 0; q < W; q++) cfma(acc[i][q][d], mv, ys[q]);
             }
           }
+          // m >= k0 + W: every slot has k < m, sign (-1)^k = (-1)^k0 (-1)^q.  (-1)^k0 is
+          // applied by flipping the accumulators before and after the loop; (-1)^q is a
+          // compile-time negation.
+          if (I.k0 + W <= n) {
+            const double sk = sgn(I.k0);
+            for (int q = 0; q < W; q++)
+              for (int d = 0; d < 3; d++) acc[i][q][d] = make_double2(sk * acc[i][q][d].x, sk * acc[i][q][d].y);
+#pragma unroll 2
+            for (int m = I.k0 + W; m <= n; m++) {
+              for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
+              y[0] = YJ[m - I.k0];
+              for (int d = 0; d < 3; d++) {
+                const double2 mv = Mn[3 * m + d];
+                for (int q = 0; q < W; q++) {
+                  const double2 yq = (q & 1) ? make_double2(-y[q].x, -y[q].y) : y[q];
+                  cfma(acc[i][q][d], mv, yq);
+                }
+              }
+            }
+            for (int q = 0; q < W; q++)
+              for (int d = 0; d < 3; d++) acc[i][q][d] = make_double2(sk * acc[i][q][d].x, sk * acc[i][q][d].y);
+          }
         }
       }
     }
     __syncthreads();
   }
+#undef UVLM_M2L_PREFETCH
   // combine the 4 source groups (same schedule slot in lanes sl, sl+16, sl+32, sl+48)
   for (int i = 0; i < K; i++)
     for (int q = 0; q < W; q++)
@@ -733,10 +780,10 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
     UVLM_LAUNCH_CHECK();                                                                                           \
     return;                                                                                                        \
   }
-  UVLM_M2L_CASE(1, 3)
+  UVLM_M2L_CASE(1, 4)
+  UVLM_M2L_CASE(2, 2)
   UVLM_M2L_CASE(2, 3)
   UVLM_M2L_CASE(4, 2)
-  UVLM_M2L_CASE(4, 3)
 #undef UVLM_M2L_CASE
   throw Error(UVLM_ERR_INTERNAL, "unsupported M2L schedule shape");
 }
