@@ -205,18 +205,19 @@ int uvlm_create(const uvlm_params *p, uvlm_handle *out)
     for (auto &e : h->ev) UVLM_HIP(hipEventCreate(&e));
     for (auto &e : h->aev) UVLM_HIP(hipEventCreate(&e));
     // M2L lane schedule shape: window of W consecutive outputs, K items per lane
-    // (UVLM_M2L_W in {1, 2, 4} selects the shapes (1,4), (2,3), (4,2); UVLM_M2L_K=2 gives (2,2))
+    // (UVLM_M2L_W in {1, 2} selects the shapes (1,4), (2,3); UVLM_M2L_K=2 gives (2,2); UVLM_M2L_PF=1 prefetches)
     h->sched.W = 2;
     h->sched.K = 3;
     if (const char *w = std::getenv("UVLM_M2L_W")) {
       const int v = std::atoi(w);
-      if (v == 1 || v == 2 || v == 4) h->sched.W = v;
-      h->sched.K = v == 1 ? 4 : v == 4 ? 2 : 3;
+      if (v == 1 || v == 2) h->sched.W = v;
+      h->sched.K = v == 1 ? 4 : 3;
     }
     if (const char *k = std::getenv("UVLM_M2L_K")) {
       const int v = std::atoi(k);
       if (v == 2 && h->sched.W == 2) h->sched.K = 2;
     }
+    if (const char *pf = std::getenv("UVLM_M2L_PF")) h->sched.prefetch = std::atoi(pf) != 0 && h->sched.K != 3;
     make_m2l_schedule(p->P, h->sched);
   });
   if (rc != UVLM_OK) {

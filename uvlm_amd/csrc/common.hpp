@@ -135,6 +135,7 @@ struct M2LSchedule {
   std::vector<int32_t> out_off; // [P(P+1)/2 + 1]
   std::vector<int32_t> out_src; // pieces: (lane*K + item)*W + q
   int K = 0, W = 0;
+  bool prefetch = false;         // next source multipole prefetched into registers
   DevBuf dev;                   // items, out_off, out_src concatenated
   int64_t dev_items = 0, dev_out_off = 0, dev_out_src = 0;
 };

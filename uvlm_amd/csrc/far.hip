@@ -88,6 +88,18 @@ __constant__ double c_inv[64] = {
     1.0 / 50, 1.0 / 51, 1.0 / 52, 1.0 / 53, 1.0 / 54, 1.0 / 55, 1.0 / 56, 1.0 / 57, 1.0 / 58, 1.0 / 59,
     1.0 / 60, 1.0 / 61, 1.0 / 62, 1.0 / 63};
 
+// 1/k! for k = 0..31.
+__constant__ double c_invfact[32] = {
+    1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040, 1.0 / 40320, 1.0 / 362880,
+    1.0 / 3628800, 1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0, 1.0 / 87178291200.0,
+    1.0 / 1307674368000.0, 1.0 / 20922789888000.0, 1.0 / 355687428096000.0, 1.0 / 6402373705728000.0,
+    1.0 / 121645100408832000.0, 1.0 / 2432902008176640000.0, 1.0 / 51090942171709440000.0,
+    1.0 / 1124000727777607680000.0, 1.0 / 25852016738884976640000.0, 1.0 / 620448401733239439360000.0,
+    1.0 / 15511210043330985984000000.0, 1.0 / 403291461126605635584000000.0,
+    1.0 / 10888869450418352160768000000.0, 1.0 / 304888344611713860501504000000.0,
+    1.0 / 8841761993739701954543616000000.0, 1.0 / 265252859812191058636308480000000.0,
+    1.0 / 8222838654177922817725562880000000.0};
+
 // Column m of the regular (singular=false) or singular harmonics, degrees m..P-1, written
 // to Y[tri(n) + m].  Associated Legendre recurrence in n at fixed m:
 // (n-m) P_n^m = x (2n-1) P_{n-1}^m - (n+m-1) P_{n-2}^m, from P_m^m = (-1)^m (2m-1)!! y^m.
@@ -185,13 +197,13 @@ __global__ __launch_bounds__(WG) void k_p2m(int P, int NT, const int32_t *__rest
         for (int t = 1; t <= m; t++) p *= -(double)(2 * t - 1) * s.y;
         double pp = 0.0;
         for (int t = m + 1; t <= n; t++) {
-          const double pn = (t == m + 1) ? s.x * (2 * m + 1) * p : (s.x * (2 * t - 1) * p - (t + m - 1) * pp) / (t - m);
-          pp              = p;
-          p               = pn;
+          const double pn =
+              (t == m + 1) ? s.x * (2 * m + 1) * p : (s.x * (2 * t - 1) * p - (t + m - 1) * pp) * c_inv[t - m];
+          pp = p;
+          p  = pn;
         }
-        double f = 1.0;
+        double f = c_invfact[n + m];
         for (int t = 0; t < n; t++) f *= -s.rho;
-        for (int t = 2; t <= n + m; t++) f /= (double)t;
         const double  v  = f * p;
         const double2 yc = make_double2(v * e.x, -v * e.y);  // conj(Y_n^m) = Y_n^{-m}
         for (int d = 0; d < 3; d++) {
@@ -334,7 +346,7 @@ __device__ __forceinline__ M2LItem unpack_item(int32_t v)
 
 constexpr int M2L_PF = 12;  // source-multipole entries per lane prefetched into registers (NT <= 192)
 
-template <int W, int K>
+template <int W, int K, bool PF>
 __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const int32_t *__restrict__ off,
                                             const int32_t *__restrict__ src, const double *__restrict__ tx,
                                             const double *__restrict__ ty, const double *__restrict__ tz,
@@ -363,13 +375,13 @@ __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const 
       for (int d = 0; d < 3; d++) acc[i][q][d] = make_double2(0, 0);
   const int    s0 = off[t], nsrc = off[t + 1] - s0;
   const double X = tx[t], Yc = ty[t], Z = tz[t];
-  // The group's next source multipole is loaded into registers while the current one is
-  // being used, so its latency overlaps the multiply-adds.
-  double2 pf[M2L_PF];
+  // Optionally, the group's next source multipole is loaded into registers while the
+  // current one is used, so its latency overlaps the multiply-adds.
+  double2 pf[PF ? M2L_PF : 1];
 #define UVLM_M2L_PREFETCH(SI)                                                                                       \
-  if ((SI) < nsrc) {                                                                                               \
+  if (PF && (SI) < nsrc) {                                                                                         \
     const double2 *Ms_ = M + (int64_t)src[s0 + (SI)] * NT;                                                         \
-    _Pragma("unroll") for (int r = 0; r < M2L_PF; r++)                                                             \
+    _Pragma("unroll") for (int r = 0; r < (PF ? M2L_PF : 1); r++)                                                  \
     {                                                                                                              \
       const int q = sl + 16 * r;                                                                                   \
       pf[r]       = q < NT ? Ms_[q] : make_double2(0, 0);                                                          \
@@ -380,15 +392,17 @@ __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const 
     const int  si     = base + g;
     const bool active = si < nsrc;
     if (active) {
-      const int s = src[s0 + si];
+      const int      s  = src[s0 + si];
+      const double2 *Ms = M + (int64_t)s * NT;
+      if (PF) {
 #pragma unroll
-      for (int r = 0; r < M2L_PF; r++) {
-        const int q = sl + 16 * r;
-        if (q < NT) Mg[q] = pf[r];
-      }
-      if (NT > 16 * M2L_PF) {
-        const double2 *Ms = M + (int64_t)s * NT;
+        for (int r = 0; r < (PF ? M2L_PF : 1); r++) {
+          const int q = sl + 16 * r;
+          if (q < NT) Mg[q] = pf[r];
+        }
         for (int q = sl + 16 * M2L_PF; q < NT; q += 16) Mg[q] = Ms[q];
+      } else {
+        for (int q = sl; q < NT; q += 16) Mg[q] = Ms[q];
       }
       if (sl < P) {
         const Sph sp = sph(X - sx[s], Yc - sy[s], Z - sz[s]);
@@ -405,70 +419,47 @@ __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const 
           const int      J  = I.j + n;
           const double2 *YJ = Yg + tri(J);
           const double2 *Mn = Mg + 3 * tri(n);
-          double2        y[W];
-          // window: y[q] = Y[J, m - k0 - q] at step m; each step shifts by one and loads
-          // y[0], so before m = -n slot i holds Y[J, -n - k0 - 1 - i] (zero below -J)
-          for (int q = 0; q + 1 < W; q++) {
-            const int a = n + I.k0 + 1 + q;
-            y[q]        = (a <= J) ? conj2(YJ[a]) : make_double2(0, 0);
+          // Terms m and -m are taken together (multipoleKernels.hpp:534-546):
+          //   conj(M[n,m]) Y[J,-m-k] + s(k,m) M[n,m] Y[J,m-k] = a (Y1 + Y2) + i b (Y2 - Y1)
+          // with M = a + ib, Y1 = Y[J,-m-k], Y2 = s Y[J,m-k], s = (-1)^k if k < m else (-1)^m.
+          // Window slot q holds output k = k0 + q: yA[q] = Y[J, m-k] (slides up with m),
+          // yB[q] = Y[J, -m-k] (slides down); both start at Y[J, -k] for m = 0.
+          double2 yA[W], yB[W];
+          for (int q = 0; q < W; q++) {
+            const int a = min(I.k0 + q, J);
+            yA[q] = yB[q] = conj2(YJ[a]);
           }
-          y[W - 1] = make_double2(0, 0);
-          // m < 0: conj(M[n,-m]) Y[J, m-k], sign +1 (multipoleKernels.hpp:534-539)
+          for (int d = 0; d < 3; d++) {
+            const double2 mv = Mn[d];
+            for (int q = 0; q < W; q++) cfma(acc[i][q][d], mv, yA[q]);
+          }
 #pragma unroll 2
-          for (int m = -n; m < 0; m++) {
-            for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
-            y[0] = conj2(YJ[I.k0 - m]);
-            for (int d = 0; d < 3; d++) {
-              const double2 mv = conj2(Mn[3 * (-m) + d]);
-              for (int q = 0; q < W; q++) cfma(acc[i][q][d], mv, y[q]);
+          for (int m = 1; m <= n; m++) {
+            for (int q = W - 1; q > 0; q--) yA[q] = yA[q - 1];
+            {
+              const int mu = m - I.k0;
+              yA[0]        = mu >= 0 ? YJ[mu] : conj2(YJ[-mu]);
             }
-          }
-          // 0 <= m < k0 <= k: sign (-1)^m for every output of the window (540-546)
-          for (int m = 0; m < I.k0 && m <= n; m++) {
-            for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
-            y[0]            = conj2(YJ[I.k0 - m]);
-            const double sg = sgn(m);
-            for (int d = 0; d < 3; d++) {
-              const double2 mv = make_double2(sg * Mn[3 * m + d].x, sg * Mn[3 * m + d].y);
-              for (int q = 0; q < W; q++) cfma(acc[i][q][d], mv, y[q]);
-            }
-          }
-          // k0 <= m < k0 + W: sign (-1)^k when k < m, else (-1)^m, per window slot
-          for (int m = I.k0; m <= n && m < I.k0 + W; m++) {
-            for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
-            y[0] = YJ[m - I.k0];
-            double2 ys[W];
+            for (int q = 0; q + 1 < W; q++) yB[q] = yB[q + 1];
+            yB[W - 1] = conj2(YJ[min(m + I.k0 + W - 1, J)]);
+            double2 S[W], D[W];
             for (int q = 0; q < W; q++) {
-              const int k = I.k0 + q;
-              const int e = k < m ? k : m;
-              ys[q]       = (e & 1) ? make_double2(-y[q].x, -y[q].y) : y[q];
+              const int     k  = I.k0 + q;
+              const int     e  = k < m ? k : m;
+              const double2 y2 = (e & 1) ? make_double2(-yA[q].x, -yA[q].y) : yA[q];
+              S[q]             = make_double2(y2.x + yB[q].x, y2.y + yB[q].y);
+              D[q]             = make_double2(y2.x - yB[q].x, y2.y - yB[q].y);
             }
             for (int d = 0; d < 3; d++) {
               const double2 mv = Mn[3 * m + d];
-              for (int q = 0; q < W; q++) cfma(acc[i][q][d], mv, ys[q]);
-            }
-          }
-          // m >= k0 + W: every slot has k < m, sign (-1)^k = (-1)^k0 (-1)^q.  (-1)^k0 is
-          // applied by flipping the accumulators before and after the loop; (-1)^q is a
-          // compile-time negation.
-          if (I.k0 + W <= n) {
-            const double sk = sgn(I.k0);
-            for (int q = 0; q < W; q++)
-              for (int d = 0; d < 3; d++) acc[i][q][d] = make_double2(sk * acc[i][q][d].x, sk * acc[i][q][d].y);
-#pragma unroll 2
-            for (int m = I.k0 + W; m <= n; m++) {
-              for (int q = W - 1; q > 0; q--) y[q] = y[q - 1];
-              y[0] = YJ[m - I.k0];
-              for (int d = 0; d < 3; d++) {
-                const double2 mv = Mn[3 * m + d];
-                for (int q = 0; q < W; q++) {
-                  const double2 yq = (q & 1) ? make_double2(-y[q].x, -y[q].y) : y[q];
-                  cfma(acc[i][q][d], mv, yq);
-                }
+              for (int q = 0; q < W; q++) {
+                double2 &a = acc[i][q][d];
+                a.x        = fma(mv.x, S[q].x, a.x);
+                a.x        = fma(-mv.y, D[q].y, a.x);
+                a.y        = fma(mv.x, S[q].y, a.y);
+                a.y        = fma(mv.y, D[q].x, a.y);
               }
             }
-            for (int q = 0; q < W; q++)
-              for (int d = 0; d < 3; d++) acc[i][q][d] = make_double2(sk * acc[i][q][d].x, sk * acc[i][q][d].y);
           }
         }
       }
@@ -538,16 +529,16 @@ __global__ __launch_bounds__(WG) void k_l2p(int P, int NT, const int32_t *__rest
       for (int t = 0; t < mu; t++) e = cmul(e, make_double2(s.cb, s.sb));
       double p = 1.0;
       for (int t = 1; t <= mu; t++) p *= -(double)(2 * t - 1) * s.y;
-      double f = 1.0;
+      double f = c_invfact[2 * mu];
       for (int t = 1; t <= mu; t++) f *= -s.rho;
-      for (int t = 2; t <= 2 * mu; t++) f /= (double)t;
       double pp = 0.0;
       for (int nu = mu; nu < P; nu++) {
         if (nu > mu) {
-          const double pn = (nu == mu + 1) ? s.x * (2 * mu + 1) * p : (s.x * (2 * nu - 1) * p - (nu + mu - 1) * pp) / (nu - mu);
-          pp              = p;
-          p               = pn;
-          f *= -s.rho / (double)(nu + mu);
+          const double pn = (nu == mu + 1) ? s.x * (2 * mu + 1) * p
+                                           : (s.x * (2 * nu - 1) * p - (nu + mu - 1) * pp) * c_inv[nu - mu];
+          pp = p;
+          p  = pn;
+          f *= -s.rho * c_inv[nu + mu];
         }
         const double  v  = f * p;
         const double2 yp = make_double2(v * e.x, v * e.y);  // Y[nu, +mu]
@@ -638,7 +629,8 @@ __global__ void k_zero(double *p, int64_t n)
 }  // namespace
 
 // ---------------------------------------------------------------------------------
-// M2L schedule: items (j, k0, w, nlo, nhi) with cost = (number of m steps) * W, packed
+// M2L schedule: items (j, k0, w, nlo, nhi) with cost = (number of paired m steps, n+1 per
+// source degree n) * W, packed
 // longest-first into 16 lane slots with at most K items each; the item cap is searched
 // to minimise the largest slot load.
 void make_m2l_schedule(int P, M2LSchedule &s)
@@ -655,8 +647,8 @@ void make_m2l_schedule(int P, M2LSchedule &s)
         int       n = 0;
         while (n <= P - 1 - j) {
           int c = 0, lo = n;
-          while (n <= P - 1 - j && (c == 0 || c + W * (2 * n + 1) <= cap)) {
-            c += W * (2 * n + 1);
+          while (n <= P - 1 - j && (c == 0 || c + W * (n + 1) <= cap)) {
+            c += W * (n + 1);
             n++;
           }
           v.push_back({j, k0, w, lo, n - 1, c});
@@ -771,19 +763,20 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
   const size_t lds_part = sizeof(double2) * 16 * sch.K * sch.W * 3;
   const size_t lds      = std::max(lds_main, lds_part);
   const int32_t *d      = sch.dev.as<int32_t>();
-#define UVLM_M2L_CASE(WW, KK)                                                                                      \
-  if (sch.W == WW && sch.K == KK) {                                                                                \
-    k_m2l<WW, KK><<<(int)tt.C, WG, lds, s>>>(P, NT, tt.C, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),       \
+#define UVLM_M2L_CASE(WW, KK, PP)                                                                                  \
+  if (sch.W == WW && sch.K == KK && sch.prefetch == PP) {                                                          \
+    k_m2l<WW, KK, PP><<<(int)tt.C, WG, lds, s>>>(P, NT, tt.C, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),       \
                                              tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),           \
                                              st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, L,     \
                                              d + sch.dev_items, d + sch.dev_out_off, d + sch.dev_out_src);         \
     UVLM_LAUNCH_CHECK();                                                                                           \
     return;                                                                                                        \
   }
-  UVLM_M2L_CASE(1, 4)
-  UVLM_M2L_CASE(2, 2)
-  UVLM_M2L_CASE(2, 3)
-  UVLM_M2L_CASE(4, 2)
+  UVLM_M2L_CASE(1, 4, false)
+  UVLM_M2L_CASE(1, 4, true)
+  UVLM_M2L_CASE(2, 2, false)
+  UVLM_M2L_CASE(2, 2, true)
+  UVLM_M2L_CASE(2, 3, false)
 #undef UVLM_M2L_CASE
   throw Error(UVLM_ERR_INTERNAL, "unsupported M2L schedule shape");
 }
