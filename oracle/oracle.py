@@ -55,6 +55,7 @@ def lib():
             "orc_calculate": (I, [VP]),
             "orc_get_bodies": (I, [VP, P, L64]),
             "orc_advance": (I, [VP, P, L64, D, I]),
+            "orc_stage": (I, [VP, P, L64, D, I, I]),
             "orc_sensor_data": (I, [VP, P, L64]),
             "orc_num_cells": (L64, [VP]),
             "orc_get_cells": (None, [VP, P64, P]),
@@ -150,6 +151,12 @@ class Calculator:
         st = lib().orc_advance(self._h, _p(bodies), bodies.shape[0], dt, SCHEME[scheme])
         if st:
             raise RuntimeError(f"orc_advance failed ({st})")
+
+    def stage(self, bodies, dt, scheme, k):
+        """Stage k of advance() from the full body state in `bodies` (tree order out)."""
+        st = lib().orc_stage(self._h, _p(bodies), bodies.shape[0], dt, SCHEME[scheme], k)
+        if st:
+            raise RuntimeError(f"orc_stage failed ({st})")
 
     def sensor_data(self, sensors):
         st = lib().orc_sensor_data(self._h, _p(sensors), sensors.shape[0])

@@ -929,6 +929,22 @@ struct Calculator {
     return ORC_OK;
   }
 
+  // One stage of advance() from the full body state the caller holds (the *_old fields
+  // included), so a stage can be checked in isolation from the rebuilds before it.
+  int stage(Body *caller, int64_t n, real_t dt_, int scheme, int s)
+  {
+    const int stages = scheme == SCHEME_EULER ? 1 : scheme == SCHEME_RK2 ? 2 : scheme == SCHEME_RK3 ? 3 : 4;
+    if (s < 0 || s >= stages) return ORC_ERR_ARG;
+    dt = dt_;
+    int st = init(caller, n);
+    if (st != ORC_OK) return st;
+    if (s == 0 && (scheme == SCHEME_RK2 || scheme == SCHEME_RK4)) store_initial_state();
+    calculate();
+    step(scheme, s);
+    std::memcpy(caller, bodies.data(), sizeof(Body) * n);
+    return ORC_OK;
+  }
+
   // getSensorData(): fmmCalculator.hpp:117-161, 274-280, 469-491, 726-731.  Uses the
   // multipoles of the last calculate(); permutes the caller's sensors into tree order.
   int sensor_data(Body *caller, int64_t m)
@@ -1020,6 +1036,10 @@ int orc_get_bodies(void *h, Body *out, int64_t n)
 int orc_advance(void *h, Body *bodies, int64_t n, double dt, int scheme)
 {
   return ((Calculator *)h)->advance(bodies, n, dt, scheme);
+}
+int orc_stage(void *h, Body *bodies, int64_t n, double dt, int scheme, int stage)
+{
+  return ((Calculator *)h)->stage(bodies, n, dt, scheme, stage);
 }
 int orc_sensor_data(void *h, Body *sensors, int64_t m) { return ((Calculator *)h)->sensor_data(sensors, m); }
 

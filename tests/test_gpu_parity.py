@@ -96,16 +96,35 @@ def test_stretching_variants(orc, variant):
 
 @pytest.mark.parametrize("scheme", ["euler", "rk2", "rk3", "rk4"])
 def test_advance_matches_oracle(orc, scheme):
+    """Every stage of two time steps, each started from the same full body state on both
+    sides, agrees to 1e-10.  A trajectory over several rebuilds is compared only loosely:
+    a position one ulp apart can fall on the other side of an octant plane or of the
+    multipole acceptance test at a later rebuild, which moves the answer by the FMM
+    truncation error (the oracle alone moves by ~3e-7 when its input is perturbed by
+    1e-15 for this ring), so the trajectory bar is 1e-5."""
     X, A, S = W.scaled("ring", 4096)
-    b_ref = orc.make_bodies(X, A, S)
+    stages = {"euler": 1, "rk2": 2, "rk3": 3, "rk4": 4}[scheme]
     c = orc.Calculator(P=8, theta=0.5, ncrit=64)
     g = FMMCalculator(P=8, theta=0.5, ncrit=64, scheme=scheme)
+    b = make_bodies(X, A, S)
+    for _ in range(2):
+        for k in range(stages):
+            ref = b.copy()
+            c.stage(ref, 0.01, scheme, k)
+            g.init(b)
+            g.calculate()
+            g.stage(k, 0.01)
+            g.copyBodies2Host(b)
+            for sl in (slice(9, 12), slice(12, 15), slice(15, 18), slice(18, 21), slice(21, 24),
+                       slice(24, 27), slice(27, 30), slice(30, 33), 33, 35):
+                assert rel(b[:, sl], ref[:, sl]) < 1e-10, (k, sl)
+    b_ref = orc.make_bodies(X, A, S)
     b_gpu = make_bodies(X, A, S)
     for _ in range(2):
         c.advance(b_ref, 0.01, scheme)
         g.advance(b_gpu, 0.01)
-    for sl in (slice(9, 12), slice(12, 15), slice(21, 24), slice(24, 27), 33):
-        assert rel(b_gpu[:, sl], b_ref[:, sl]) < 1e-10, sl
+    for sl in (slice(9, 12), slice(12, 15)):
+        assert rel(b_gpu[:, sl], b_ref[:, sl]) < 1e-5, sl
 
 
 def test_sensors_match_oracle(orc):

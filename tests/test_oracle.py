@@ -336,6 +336,23 @@ def test_rk_convergence_order(orc, scheme, order):
     assert abs(observed - order) < 0.35, (scheme, observed)
 
 
+@pytest.mark.parametrize("scheme", ["euler", "rk2", "rk3", "rk4"])
+def test_stage_composes_to_advance(orc, scheme):
+    """advance() is its stages run one after another on the full body state: the per-stage
+    entry point reproduces it bit for bit (two steps, 2000 bodies, several leaves)."""
+    X, A, S = W.scaled("cube", 2000)
+    b1 = orc.make_bodies(X, A, S)
+    b2 = b1.copy()
+    c1 = orc.Calculator(P=6, theta=0.5, ncrit=32)
+    c2 = orc.Calculator(P=6, theta=0.5, ncrit=32)
+    stages = {"euler": 1, "rk2": 2, "rk3": 3, "rk4": 4}[scheme]
+    for _ in range(2):
+        c1.advance(b1, 0.01, scheme)
+        for k in range(stages):
+            c2.stage(b2, 0.01, scheme, k)
+    assert np.array_equal(b1, b2)
+
+
 # ------------------------------------------------------------------- adjoint of the probe objective
 def test_direct_adjoint_matches_finite_differences(orc):
     """The exact adjoint (direct sum with the target's sigma) against central FD of the

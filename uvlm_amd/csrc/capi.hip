@@ -154,7 +154,7 @@ void do_eval(uvlm_handle h, bool sync)
   m2l(s, P, t, t, h->lists, h->sched, M, L);
   UVLM_HIP(hipEventRecord(h->ev[EV_M2L], s));
   p2p(s, t, t, h->lists, h->st.as<double>(), h->cap, h->st.as<double>(), h->cap, h->st.as<double>(), h->cap,
-      h->prm.precision, 0);
+      h->prm.precision, 0, h->prm.ncrit);
   UVLM_HIP(hipEventRecord(h->ev[EV_P2P], s));
   for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L);
   l2p_stretch(s, P, t, L, h->st.as<double>(), h->cap, h->prm.stretching, h->prm.formulation, 0);
@@ -489,7 +489,7 @@ int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride)
     reset_sensor_rates(s, h->sst.as<double>(), sc, m);
     m2l(s, P, h->stree, h->tree, h->slists, h->sched, h->M.as<double2>(), Ls);
     p2p(s, h->stree, h->tree, h->slists, h->sst.as<double>(), sc, h->st.as<double>(), h->cap, h->sst.as<double>(), sc,
-        h->prm.precision, 1);
+        h->prm.precision, 1, h->prm.ncrit);
     for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls);
     l2p_stretch(s, P, h->stree, Ls, h->sst.as<double>(), sc, h->prm.stretching, h->prm.formulation, 1);
     UVLM_HIP(hipEventRecord(h->aev[1], s));
@@ -555,7 +555,7 @@ int uvlm_probe_adjoint(uvlm_handle h, int64_t m, const double *probes, const dou
     m2l(s, P, h->tree, h->stree, h->alists, h->sched, Ms, La);
     zero_doubles(s, (double *)La, 2 * NT);  // the particle root multipole is never formed forward
     for (int l = 1; l < h->tree.nlevels; l++) l2l_level(s, P, h->tree, l, La);
-    p2p(s, h->tree, h->stree, h->alists, a, cap, h->sst.as<double>(), sc, a, cap, h->prm.precision, 2);
+    p2p(s, h->tree, h->stree, h->alists, a, cap, h->sst.as<double>(), sc, a, cap, h->prm.precision, 2, h->prm.ncrit);
     l2p_stretch(s, P, h->tree, La, a, cap, UVLM_TRANSPOSED, UVLM_CVPM, 0);
     UVLM_HIP(hipEventRecord(h->aev[1], s));
     h->evaluated = false;  // L now holds adjoint locals

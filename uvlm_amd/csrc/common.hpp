@@ -150,8 +150,10 @@ void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *
                  int formulation, int sensor_mode);
 
 // near.hip
+// max_leaf: an upper bound of the target leaves' body counts (ncrit for a built tree)
 void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const double *tgt, int64_t tcap,
-         const double *src, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode);
+         const double *src, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode,
+         int64_t max_leaf);
 
 void direct(hipStream_t s, int64_t m, const double *xt, const double *at, const double *sst, int64_t scap, int64_t n,
             double *u, double *J, double *dadt);

@@ -97,8 +97,8 @@ __device__ __forceinline__ void lds_prefix(int *pref, const int *v, int n)
 // TSIG: the core size comes from the TARGET body instead of the source.  That is the
 // transposed near-field kernel used by the adjoint (a particle's own sigma, with the
 // particle now on the target side).
-template <bool MIXED, bool WITH_J, bool TSIG = false>
-__global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, int64_t nleaf,
+template <bool MIXED, bool WITH_J, bool TSIG, int BLK>
+__global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves, int64_t nleaf,
                                             const int32_t *__restrict__ tbody, const int32_t *__restrict__ tnbody,
                                             const double *__restrict__ tcx, const double *__restrict__ tcy,
                                             const double *__restrict__ tcz, const int32_t *__restrict__ off,
@@ -107,16 +107,18 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
                                             int64_t tcap, const double *__restrict__ sst, int64_t scap,
                                             double *__restrict__ out, int64_t ocap)
 {
-  __shared__ SrcF64 s64[MIXED ? 1 : WG];
-  __shared__ float4 s32[MIXED ? 3 * WG : 1];
+  constexpr int LGMAX = BLK == 64 ? 6 : 8;
+  __shared__ SrcF64 s64[MIXED ? 1 : BLK];
+  __shared__ float4 s32[MIXED ? 3 * BLK : 1];
   __shared__ int    pref[SEG + 1], lb[SEG], ln[SEG];
+  __shared__ double red[BLK == 64 ? 1 : 12 * BLK];
   const int64_t w = blockIdx.x;
   if (w >= nleaf) return;
   const int t = leaves[w], tb = tbody[t], nt = tnbody[t];
   const int lane = threadIdx.x;
   int       lg   = 0;
-  while ((1 << lg) < nt && lg < 6) lg++;
-  const int    ntp = 1 << lg, S = WG >> lg, ti = lane & (ntp - 1), si = lane >> lg;
+  while ((1 << lg) < nt && lg < LGMAX) lg++;
+  const int    ntp = 1 << lg, S = BLK >> lg, ti = lane & (ntp - 1), si = lane >> lg;
   const double cxt = tcx[t], cyt = tcy[t], czt = tcz[t];
   const int    q0 = off[t], q1 = off[t + 1];
   for (int c0 = 0; c0 < nt; c0 += ntp) {
@@ -135,17 +137,17 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
     for (int g0 = q0; g0 < q1; g0 += SEG) {
       const int nseg = min(SEG, q1 - g0);
       __syncthreads();
-      for (int k = lane; k < nseg; k += WG) {
+      for (int k = lane; k < nseg; k += BLK) {
         const int sl = src[g0 + k];
         lb[k]        = sbody[sl];
         ln[k]        = snbody[sl];
       }
       __syncthreads();
-      lds_prefix(pref, ln, nseg);
+      if (lane < 64) lds_prefix(pref, ln, nseg);
       __syncthreads();
       const int total = pref[nseg];
-      for (int j0 = 0; j0 < total; j0 += WG) {
-        const int cnt = min(WG, total - j0);
+      for (int j0 = 0; j0 < total; j0 += BLK) {
+        const int cnt = min(BLK, total - j0);
         if (lane < cnt) {
           const int g  = j0 + lane;
           int       lo = 0, hi = nseg - 1;  // last q with pref[q] <= g
@@ -239,9 +241,21 @@ __global__ __launch_bounds__(WG) void k_p2p(const int32_t *__restrict__ leaves, 
         __syncthreads();
       }
     }
-    // combine the S source splits (lanes ti + s*ntp)
-    for (int o = ntp; o < WG; o <<= 1)
-      for (int k = 0; k < (WITH_J ? 12 : 3); k++) acc[k] += __shfl_xor(acc[k], o, 64);
+    // combine the S source splits (threads ti + s*ntp) in a fixed order
+    if constexpr (BLK == 64) {
+      for (int o = ntp; o < WG; o <<= 1)
+        for (int k = 0; k < (WITH_J ? 12 : 3); k++) acc[k] += __shfl_xor(acc[k], o, 64);
+    } else if (S > 1) {
+      __syncthreads();
+      for (int k = 0; k < 12; k++) red[k * BLK + lane] = acc[k];
+      __syncthreads();
+      if (si == 0)
+        for (int k = 0; k < (WITH_J ? 12 : 3); k++) {
+          double v = 0.0;
+          for (int q = 0; q < S; q++) v += red[k * BLK + ti + q * ntp];
+          acc[k] = v;
+        }
+    }
     if (si == 0 && valid) {
       for (int d = 0; d < 3; d++) out[(F_U + d) * ocap + bi] = -acc[d];
       if (WITH_J)
@@ -325,17 +339,28 @@ void direct(hipStream_t s, int64_t m, const double *xt, const double *at, const 
 }
 
 void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const double *tgt, int64_t tcap,
-         const double *srcst, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode)
+         const double *srcst, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode,
+         int64_t max_leaf)
 {
   if (tt.nleaf == 0) return;
   const int nb = (int)tt.nleaf;
 #define UVLM_P2P(MX, WJ, TS)                                                                                      \
-  k_p2p<MX, WJ, TS><<<nb, WG, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(), tt.nbody.as<int32_t>(), \
+  if (big)                                                                                                         \
+    k_p2p<MX, WJ, TS, 256><<<nb, 256, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(),           \
+                                              tt.nbody.as<int32_t>(), tt.cx.as<double>(), tt.cy.as<double>(),       \
+                                              tt.cz.as<double>(), l.p2p_off.as<int32_t>(), l.p2p_src.as<int32_t>(), \
+                                              stree.body.as<int32_t>(), stree.nbody.as<int32_t>(), tgt, tcap, srcst,\
+                                              scap, out, ocap);                                                    \
+  else                                                                                                             \
+  k_p2p<MX, WJ, TS, 64><<<nb, WG, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(), tt.nbody.as<int32_t>(), \
                                   tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),                     \
                                   l.p2p_off.as<int32_t>(), l.p2p_src.as<int32_t>(), stree.body.as<int32_t>(),      \
                                   stree.nbody.as<int32_t>(), tgt, tcap, srcst, scap, out, ocap)
-  // sensor_mode: 0 particles (u and J), 1 probes (u only), 2 adjoint (u and J, target sigma)
+  // sensor_mode: 0 particles (u and J), 1 probes (u only), 2 adjoint (u and J, target sigma).
+  // Leaves of more than 64 bodies get 256-thread workgroups, so all their targets share one
+  // pass over the sources.
   const bool mixed = precision == UVLM_MIXED;
+  const bool big   = max_leaf > 64;
   if (sensor_mode == 2) {
     if (mixed) UVLM_P2P(true, true, true);
     else UVLM_P2P(false, true, true);
