@@ -10,7 +10,8 @@ Prints ONE JSON line (rank 0, N=1) with the contract fields plus:
   cpu_baseline -- the oracle (CPU restatement of the reference) timed on the host cores on
                   a bounded sample of the same workload, scaled to ms per 4M-particle eval
   extra        -- FP64-path time, mixed-vs-FP64 difference, error vs an FP64 direct sum on
-                  sampled targets, per-phase times, list sizes
+                  sampled targets, per-phase times, list sizes, and the adjoint/forward time
+                  ratio of the probe objective on 1M particles in three Gaussian blobs
 """
 import argparse
 import json
@@ -40,6 +41,53 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def adjoint_leg(n, theta, ncrit, steps):
+    """Adjoint / forward time of the probe objective J = sum_p v . u(probe_p) on three Gaussian
+    blobs (n particles, 4096 probes, P=8).  Forward = tree + lists + eval + sensor pass (J from
+    scratch); backward = dJ/dalpha and dJ/dx through the adjoint far field and near field on
+    the same tree.  Both are wall-clock on device-resident particles, synchronized."""
+    from uvlm_amd import FMMCalculator, workloads as W
+
+    X, A, S, probes, v, _ = W.blobs(n=n, n_probe=4096, n_check=1)
+    g = FMMCalculator(P=8, theta=theta, ncrit=ncrit, precision="mixed")
+    g.set_arrays(X, A, S)
+
+    def forward():
+        g.build()
+        g.calculate()
+        J = g.probe_objective(probes, v)
+        t = g.timings()
+        return t["tree_ms"] + t["lists_ms"] + t["eval_ms"] + t["sensor_ms"]
+
+    def backward():
+        g.probe_adjoint(probes, v)
+        return g.timings()["adjoint_ms"]
+
+    forward()
+    backward()
+    g.sync()
+    fk = bk = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fk += forward()
+    g.sync()
+    fwd = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bk += backward()
+    g.sync()
+    bwd = (time.perf_counter() - t0) * 1e3 / steps
+    fk /= steps
+    bk /= steps
+    g.close()
+    return {"adjoint_fwd_ratio": round(bk / fk, 3),
+            "adjoint": {"particles": n, "probes": 4096, "P": 8,
+                        "forward_device_ms": round(fk, 3), "backward_device_ms": round(bk, 3),
+                        "forward_wall_ms": round(fwd, 3), "backward_wall_ms": round(bwd, 3),
+                        "note": "ratio = device time of the adjoint pass / device time of tree + lists + eval + "
+                                "sensor pass; wall times include the host copies of J and of the 2x3N gradients"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +101,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=131072, help="particles in the CPU baseline sample (0: skip)")
     ap.add_argument("--err-targets", type=int, default=2048, help="sampled targets for the direct-sum error")
     ap.add_argument("--no-fp64", action="store_true", help="skip timing the FP64 path")
+    ap.add_argument("--adjoint-n", type=int, default=1 << 20,
+                    help="particles in the adjoint leg (three Gaussian blobs, 4096 probes, P=8; 0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
@@ -129,6 +179,10 @@ def main():
         extra["fmm_vs_direct_rel"] = {"u": rel(out["velocity"][idx], ud), "J": rel(out["J"][idx], Jd),
                                       "dadt": rel(out["dadt"][idx], Dd_t), "targets": int(len(idx))}
     g.close()
+
+    if args.adjoint_n > 0:
+        extra.update(adjoint_leg(args.adjoint_n, args.theta, args.ncrit, max(1, min(args.steps, 3))))
+        print(f"[bench] adjoint: {extra['adjoint']}", file=sys.stderr, flush=True)
 
     cpu = None
     if args.cpu_n > 0:

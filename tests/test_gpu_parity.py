@@ -76,6 +76,30 @@ def test_fmm_fp64_parity(orc, name):
     assert rel(L1, L0) < 1e-10
 
 
+@pytest.mark.parametrize("P,S", [(4, 0), (13, 0), (16, 0), (10, 5), (10, 6), (10, 12)])
+def test_fmm_fp64_parity_orders_and_plans(orc, P, S, monkeypatch):
+    """Expansion orders away from the default (P = 13 and 16 need a third coefficient per lane
+    in P2M/M2M/L2L; P = 4 runs with a single home slot) and the M2L plan sizes 5, 6 and 12."""
+    if S:
+        monkeypatch.setenv("UVLM_M2L_SLOTS", str(S))
+    X, A, S_ = W.scaled("cube", 6000)
+    b_ref = orc.make_bodies(X, A, S_)
+    c = orc.Calculator(P=P, theta=0.5, ncrit=32)
+    c.init(b_ref)
+    c.calculate()
+    ref = c.bodies(b_ref.shape[0])
+    g = FMMCalculator(P=P, theta=0.5, ncrit=32)
+    b_gpu = make_bodies(X, A, S_)
+    g.init(b_gpu)
+    g.calculate()
+    out = g.copyBodies2Host()
+    for sl in (slice(15, 18), slice(0, 9), slice(18, 21)):
+        assert rel(out[:, sl], ref[:, sl]) < 1e-10, (P, S, sl)
+    M0, L0 = c.expansions()
+    M1, L1 = g.expansions()
+    assert rel(M1, M0) < 1e-10 and rel(L1, L0) < 1e-10
+
+
 @pytest.mark.parametrize("name", ["ring", "leapfrog", "cube"])
 def test_mixed_vs_fp64(orc, name):
     _, _, ref, out, _, _ = run_pair(orc, name, precision="mixed")

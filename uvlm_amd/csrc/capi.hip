@@ -218,6 +218,9 @@ int uvlm_create(const uvlm_params *p, uvlm_handle *out)
       if (v == 2 && h->sched.W == 2) h->sched.K = 2;
     }
     if (const char *pf = std::getenv("UVLM_M2L_PF")) h->sched.prefetch = std::atoi(pf) != 0 && h->sched.K != 3;
+    // UVLM_M2L_KERNEL=items selects the lane-item kernel; UVLM_M2L_SLOTS sets the plan's slot budget
+    if (const char *kk = std::getenv("UVLM_M2L_KERNEL")) h->sched.kind = std::string(kk) == "items" ? 0 : 1;
+    if (const char *sb = std::getenv("UVLM_M2L_SLOTS")) h->sched.budget = std::atoi(sb);
     make_m2l_schedule(p->P, h->sched);
   });
   if (rc != UVLM_OK) {

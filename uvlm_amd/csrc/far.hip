@@ -15,13 +15,16 @@
 // which computes the same derivatives.
 //
 // Work decomposition (one wavefront = one 64-lane workgroup unless noted):
-//   P2M   : one workgroup per leaf; lane <-> (n,m) coefficient, loop over the leaf's bodies
+//   P2M   : one workgroup per leaf; chunks of bodies: lanes (body, column pair) fill the
+//           harmonics in LDS, then lane <-> (n,m) coefficient sums over the chunk
 //   M2M   : one workgroup per parent cell of a level; lane <-> output (j,k), loop over children
 //   M2L   : one workgroup per target cell; 4 groups of 16 lanes take every 4th source of
-//           the target's list; inside a group each lane runs up to K balanced "items"
-//           (an output row j, a window of W consecutive k, a range of source degrees n);
-//           Y and the source multipole sit in LDS; partial sums are combined in a fixed
-//           order at the end, so results are deterministic and no atomics are used
+//           the target's list; Y and the source multipole sit in LDS.  k_m2l_ns (default):
+//           the lanes of a group walk (n, m) in lock step, each adding the terms of the
+//           outputs bound to its accumulator slots (plan in m2l_plan.hpp).  k_m2l: each lane
+//           runs up to K "items" (an output row j, a window of W consecutive k, a range of
+//           source degrees n).  Partial sums are combined in a fixed order at the end, so
+//           results are deterministic and no atomics are used
 //   L2L   : one workgroup per child cell of a level (gather from the parent)
 //   L2P   : one workgroup per leaf, lane <-> body
 #include "common.hpp"
@@ -33,6 +36,9 @@ namespace uvlm {
 namespace {
 
 constexpr int WG = 64;
+// Expansion coefficients (n, m >= 0) per lane in the lane <-> coefficient kernels:
+// tri(16) = 136 <= QPL * WG.
+constexpr int QPL = 3;
 
 inline int nblocks(int64_t n, int b) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
 
@@ -150,74 +156,53 @@ __global__ __launch_bounds__(WG) void k_p2m(int P, int NT, const int32_t *__rest
                                             const double *__restrict__ cz, const double *__restrict__ st, int64_t cap,
                                             double2 *__restrict__ M)
 {
-  extern __shared__ double smem[];
-  double *sh = smem;  // [WG][8]: rho, x, y, cb, sb, ax, ay, az
+  // Chunks of B bodies.  Phase 1: lanes (body, column pair) write the regular harmonics of
+  // (body - centre) to LDS; lane c of a body takes columns c and P-1-c, so every lane walks
+  // about P+1 degrees.  Phase 2: lane <-> coefficient (n,m) adds alpha conj(Y_n^m) over the
+  // chunk, in body order (multipoleKernels.hpp:305-327).
+  extern __shared__ double2 smq[];
   const int64_t w = blockIdx.x;
   if (w >= nleaf) return;
-  const int c = leaves[w], b0 = cbody[c], nb = cnbody[c];
+  const int    c = leaves[w], b0 = cbody[c], nb = cnbody[c];
   const double X = cx[c], Yc = cy[c], Z = cz[c];
-  const int    lane = threadIdx.x;
-  const int    ncoef = tri(P);
-  double2      acc[3][2];  // up to 2 coefficients per lane (P <= 16 -> 136 coefficients)
-  for (int a = 0; a < 2; a++)
-    for (int d = 0; d < 3; d++) acc[d][a] = make_double2(0, 0);
-  int qn[2], qm[2];
-  for (int a = 0; a < 2; a++) {
-    int q = lane + a * WG;
-    qn[a] = -1;
-    qm[a] = 0;
-    if (q < ncoef) {
-      int n = 0;
-      while (tri(n + 1) <= q) n++;
-      qn[a] = n;
-      qm[a] = q - tri(n);
-    }
-  }
-  for (int c0 = 0; c0 < nb; c0 += WG) {
-    const int i = c0 + lane;
-    if (i < nb) {
+  const int    lane = threadIdx.x, ny = tri(P);
+  const int    C = (P + 1) / 2, B = WG / C;
+  double2     *Yb = smq;                         // [B][ny]
+  double      *al = (double *)(smq + B * ny);    // [B][3]
+  double2      acc[QPL][3];
+  for (int a = 0; a < QPL; a++)
+    for (int d = 0; d < 3; d++) acc[a][d] = make_double2(0, 0);
+  const int pb = lane / C, pc = lane % C;
+  for (int c0 = 0; c0 < nb; c0 += B) {
+    const int i = c0 + pb;
+    if (pb < B && i < nb) {
       const int64_t bi = b0 + i;
-      const Sph     s  = sph(st[(F_X + 0) * cap + bi] - X, st[(F_X + 1) * cap + bi] - Yc, st[(F_X + 2) * cap + bi] - Z);
-      double       *r  = sh + lane * 8;
-      r[0] = s.rho, r[1] = s.x, r[2] = s.y, r[3] = s.cb, r[4] = s.sb;
-      r[5] = st[(F_A + 0) * cap + bi], r[6] = st[(F_A + 1) * cap + bi], r[7] = st[(F_A + 2) * cap + bi];
+      const Sph     sp = sph(st[(F_X + 0) * cap + bi] - X, st[(F_X + 1) * cap + bi] - Yc, st[(F_X + 2) * cap + bi] - Z);
+      harmonic_column<false>(P, pc, sp, Yb + pb * ny);
+      if (P - 1 - pc > pc) harmonic_column<false>(P, P - 1 - pc, sp, Yb + pb * ny);
+      if (pc == 0)
+        for (int d = 0; d < 3; d++) al[pb * 3 + d] = st[(F_A + d) * cap + bi];
     }
     __syncthreads();
-    const int cnt = min(WG, nb - c0);
-    for (int a = 0; a < 2; a++) {
-      const int n = qn[a], m = qm[a];
-      if (n < 0) continue;
+    const int cnt = min(B, nb - c0);
+    for (int a = 0; a < QPL; a++) {
+      const int q = lane + a * WG;
+      if (q >= ny) continue;
       for (int j = 0; j < cnt; j++) {
-        const double *r = sh + j * 8;
-        Sph           s{r[0], r[1], r[2], r[3], r[4]};
-        // Y_n^m directly: e^{imb}, P_n^m by recurrence, (-1)^n rho^n/(n+m)!
-        double2 e = make_double2(1.0, 0.0);
-        for (int t = 0; t < m; t++) e = cmul(e, make_double2(s.cb, s.sb));
-        double p = 1.0;
-        for (int t = 1; t <= m; t++) p *= -(double)(2 * t - 1) * s.y;
-        double pp = 0.0;
-        for (int t = m + 1; t <= n; t++) {
-          const double pn =
-              (t == m + 1) ? s.x * (2 * m + 1) * p : (s.x * (2 * t - 1) * p - (t + m - 1) * pp) * c_inv[t - m];
-          pp = p;
-          p  = pn;
-        }
-        double f = c_invfact[n + m];
-        for (int t = 0; t < n; t++) f *= -s.rho;
-        const double  v  = f * p;
-        const double2 yc = make_double2(v * e.x, -v * e.y);  // conj(Y_n^m) = Y_n^{-m}
+        const double2 y = Yb[j * ny + q];
         for (int d = 0; d < 3; d++) {
-          acc[d][a].x = fma(r[5 + d], yc.x, acc[d][a].x);
-          acc[d][a].y = fma(r[5 + d], yc.y, acc[d][a].y);
+          const double ad = al[j * 3 + d];
+          acc[a][d].x     = fma(ad, y.x, acc[a][d].x);
+          acc[a][d].y     = fma(-ad, y.y, acc[a][d].y);  // conj(Y_n^m) = Y_n^{-m}
         }
       }
     }
     __syncthreads();
   }
-  for (int a = 0; a < 2; a++) {
+  for (int a = 0; a < QPL; a++) {
     const int q = lane + a * WG;
-    if (q < ncoef)
-      for (int d = 0; d < 3; d++) M[(int64_t)c * NT + 3 * q + d] = acc[d][a];
+    if (q < ny)
+      for (int d = 0; d < 3; d++) M[(int64_t)c * NT + 3 * q + d] = acc[a][d];
   }
 }
 
@@ -235,8 +220,8 @@ __global__ __launch_bounds__(WG) void k_m2m(int P, int NT, const int32_t *__rest
   const int p = cells[w], nc = cnchild[p];
   if (nc == 0) return;
   const int lane = threadIdx.x, nout = tri(P);
-  double2   acc[2][3];
-  for (int a = 0; a < 2; a++)
+  double2   acc[QPL][3];
+  for (int a = 0; a < QPL; a++)
     for (int d = 0; d < 3; d++) acc[a][d] = make_double2(0, 0);
   for (int ci = 0; ci < nc; ci++) {
     const int ch = cchild[p] + ci;
@@ -244,7 +229,7 @@ __global__ __launch_bounds__(WG) void k_m2m(int P, int NT, const int32_t *__rest
     if (lane < P) harmonic_column<false>(P, lane, s, Y);
     for (int q = lane; q < NT; q += WG) Mc[q] = M[(int64_t)ch * NT + q];
     __syncthreads();
-    for (int a = 0; a < 2; a++) {
+    for (int a = 0; a < QPL; a++) {
       const int q = lane + a * WG;
       if (q >= nout) continue;
       int j = 0;
@@ -269,7 +254,7 @@ __global__ __launch_bounds__(WG) void k_m2m(int P, int NT, const int32_t *__rest
     }
     __syncthreads();
   }
-  for (int a = 0; a < 2; a++) {
+  for (int a = 0; a < QPL; a++) {
     const int q = lane + a * WG;
     if (q < nout)
       for (int d = 0; d < 3; d++) M[(int64_t)p * NT + 3 * q + d] = acc[a][d];
@@ -294,7 +279,7 @@ __global__ __launch_bounds__(WG) void k_l2l(int P, int NT, const int32_t *__rest
   if (lane < P) harmonic_column<false>(P, lane, s, Y);
   for (int q = lane; q < NT; q += WG) Lp[q] = L[(int64_t)p * NT + q];
   __syncthreads();
-  for (int a = 0; a < 2; a++) {
+  for (int a = 0; a < QPL; a++) {
     const int q = lane + a * WG;
     if (q >= nout) continue;
     int j = 0;
@@ -499,6 +484,144 @@ __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const 
   }
 }
 
+// The n-synchronous M2L (plan in m2l_plan.hpp).  One workgroup (one wavefront) per target
+// cell; its 4 groups of 16 lanes take every 4th source.  All lanes of a group walk the source
+// degrees n and orders m in the same sequence, so M[n,m] is one LDS broadcast per group, and
+// each lane adds the terms of the outputs bound to its S accumulator slots.  Blocks are
+// mapped so that each of the 8 XCDs takes one contiguous range of targets: spatially close
+// targets share most of their sources, and those multipoles then hit the XCD's own L2.
+__device__ __forceinline__ void m2l_terms(double2 (&a)[3], const double2 *__restrict__ YJ,
+                                          const double2 *__restrict__ Mn, int k, int m0, int m1)
+{
+  int m = m0;
+  if (m == 0) {  // m = 0 is its own pair: M[n,0] Y[J,-k], Y[J,-k] = conj(Y[J,k])
+    const double2 y = conj2(YJ[k]);
+    for (int d = 0; d < 3; d++) cfma(a[d], Mn[d], y);
+    m = 1;
+  }
+#pragma unroll 1
+  for (; m < m1; m++) {
+    // conj(M[n,m]) Y[J,-m-k] + s M[n,m] Y[J,m-k] = A (y2 + yB) + i B (y2 - yB), M = A + iB,
+    // y2 = s Y[J,m-k], yB = Y[J,-m-k] = conj(Y[J,m+k]), s = (-1)^min(k,m)
+    // (multipoleKernels.hpp:534-546 with the -m term folded in)
+    const int mu = m - k;
+    double2   y2 = YJ[mu >= 0 ? mu : -mu];
+    if (mu < 0) y2.y = -y2.y;
+    if (min(k, m) & 1) {
+      y2.x = -y2.x;
+      y2.y = -y2.y;
+    }
+    const double2 yb = YJ[m + k];
+    const double2 Sv = make_double2(y2.x + yb.x, y2.y - yb.y);
+    const double2 Dv = make_double2(y2.x - yb.x, y2.y + yb.y);
+    for (int d = 0; d < 3; d++) {
+      const double2 mv = Mn[3 * m + d];
+      a[d].x           = fma(mv.x, Sv.x, a[d].x);
+      a[d].x           = fma(-mv.y, Dv.y, a[d].x);
+      a[d].y           = fma(mv.x, Sv.y, a[d].y);
+      a[d].y           = fma(mv.y, Dv.x, a[d].y);
+    }
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int64_t per_xcd,
+                                               const int32_t *__restrict__ off, const int32_t *__restrict__ src,
+                                               const double *__restrict__ tx, const double *__restrict__ ty,
+                                               const double *__restrict__ tz, const double *__restrict__ sx,
+                                               const double *__restrict__ sy, const double *__restrict__ sz,
+                                               const double2 *__restrict__ M, double2 *__restrict__ L,
+                                               const int32_t *__restrict__ slots, const int32_t *__restrict__ lev,
+                                               const int32_t *__restrict__ out_off,
+                                               const int32_t *__restrict__ out_src)
+{
+  extern __shared__ double2 sm2[];
+  const int64_t b = blockIdx.x;
+  const int64_t t = (b & 7) * per_xcd + (b >> 3);
+  if (t >= ncell) return;
+  const int lane = threadIdx.x, g = lane >> 4, sl = lane & 15;
+  const int ny = tri(P), per = ny + NT;
+  double2  *Yg = sm2 + g * per;
+  double2  *Mg = Yg + ny;
+  // slot s of this lane: packed output (j, k), first order m0 and the degree it serves
+  // (31: home slot); an unused slot gets j = 31, which no degree activates
+  int32_t sv[S];
+  for (int s = 0; s < S; s++) {
+    const int32_t v = slots[sl * S + s];
+    sv[s]           = v >= 0 ? v : (31 | (31 << 15));
+  }
+  double2 acc[S][3];
+  for (int s = 0; s < S; s++)
+    for (int d = 0; d < 3; d++) acc[s][d] = make_double2(0, 0);
+  const int    s0 = off[t], nsrc = off[t + 1] - s0;
+  const double X = tx[t], Yc = ty[t], Z = tz[t];
+  for (int base = 0; base < nsrc; base += 4) {
+    const int  si     = base + g;
+    const bool active = si < nsrc;
+    if (active) {
+      const int      s  = src[s0 + si];
+      const double2 *Ms = M + (int64_t)s * NT;
+      for (int q = sl; q < NT; q += 16) Mg[q] = Ms[q];
+      if (sl < P) {
+        const Sph sp = sph(X - sx[s], Yc - sy[s], Z - sz[s]);
+        harmonic_column<true>(P, sl, sp, Yg);
+      }
+    }
+    __syncthreads();
+    if (active) {
+      for (int n = 0; n < P; n++) {
+        const int      lv = lev[n], cn = lv & 255, hr = lv >> 8;
+        const double2 *Mn = Mg + 3 * tri(n);
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+          const int v = sv[s], sj = v & 31, sk = (v >> 5) & 31, lvl = (v >> 15) & 31;
+          bool      act;
+          int       m0, m1;
+          if (lvl == 31) {  // home slot: every m of an active, unsplit output
+            act = s < hr && sj + n <= P - 1;
+            m0  = 0;
+            m1  = n + 1;
+          } else {  // helper slot of degree lvl: one chunk of the m-range
+            act = lvl == n;
+            m0  = (v >> 10) & 31;
+            m1  = min(m0 + cn, n + 1);
+          }
+          if (act) m2l_terms(acc[s], Yg + tri(sj + n), Mn, sk, m0, m1);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // combine the 4 source groups (same slot in lanes sl, sl+16, sl+32, sl+48)
+  for (int s = 0; s < S; s++)
+    for (int d = 0; d < 3; d++) {
+      double2 &a = acc[s][d];
+      a.x += __shfl_xor(a.x, 16, 64);
+      a.y += __shfl_xor(a.y, 16, 64);
+      a.x += __shfl_xor(a.x, 32, 64);
+      a.y += __shfl_xor(a.y, 32, 64);
+    }
+  double2 *part = sm2;  // [16][S][3]
+  if (g == 0)
+    for (int s = 0; s < S; s++)
+      for (int d = 0; d < 3; d++) part[(sl * S + s) * 3 + d] = acc[s][d];
+  __syncthreads();
+  for (int q = lane; q < ny; q += WG) {
+    int j = 0;
+    while (tri(j + 1) <= q) j++;
+    double2 r[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    for (int e = out_off[q]; e < out_off[q + 1]; e++) {
+      const int pc = out_src[e];
+      for (int d = 0; d < 3; d++) {
+        r[d].x += part[pc * 3 + d].x;
+        r[d].y += part[pc * 3 + d].y;
+      }
+    }
+    const double sj = sgn(j);
+    for (int d = 0; d < 3; d++) L[t * NT + 3 * q + d] = make_double2(sj * r[d].x, sj * r[d].y);
+  }
+}
+
 // ------------------------------------------------------------------ L2P + stretching
 __global__ __launch_bounds__(WG) void k_l2p(int P, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
                                             const int32_t *__restrict__ cbody, const int32_t *__restrict__ cnbody,
@@ -635,6 +758,22 @@ __global__ void k_zero(double *p, int64_t n)
 // to minimise the largest slot load.
 void make_m2l_schedule(int P, M2LSchedule &s)
 {
+  if (s.kind == 1) {
+    if (!make_m2l_plan(P, s.budget, s.plan)) throw Error(UVLM_ERR_INTERNAL, "no M2L plan");
+    const M2LPlan &pl = s.plan;
+    s.dev_items       = 0;
+    s.dev_lev         = pl.slots.size();
+    s.dev_out_off     = s.dev_lev + pl.lev.size();
+    s.dev_out_src     = s.dev_out_off + pl.out_off.size();
+    std::vector<int32_t> all;
+    all.insert(all.end(), pl.slots.begin(), pl.slots.end());
+    all.insert(all.end(), pl.lev.begin(), pl.lev.end());
+    all.insert(all.end(), pl.out_off.begin(), pl.out_off.end());
+    all.insert(all.end(), pl.out_src.begin(), pl.out_src.end());
+    s.dev.reserve(4 * all.size());
+    UVLM_HIP(hipMemcpy(s.dev.p, all.data(), 4 * all.size(), hipMemcpyHostToDevice));
+    return;
+  }
   const int W = s.W, K = s.K, LANES = 16;
   struct It {
     int j, k0, w, nlo, nhi, cost;
@@ -726,9 +865,10 @@ void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, dou
 {
   const int NT = 3 * P * (P + 1) / 2;
   if (t.nleaf == 0) return;
-  k_p2m<<<(int)t.nleaf, WG, WG * 8 * sizeof(double), s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(),
-                                                           t.nbody.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(),
-                                                           t.cz.as<double>(), st, cap, M);
+  const int    B   = WG / ((P + 1) / 2), ny = P * (P + 1) / 2;  // bodies per chunk (k_p2m)
+  const size_t lds = sizeof(double2) * B * ny + sizeof(double) * 3 * B;
+  k_p2m<<<(int)t.nleaf, WG, lds, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(), t.nbody.as<int32_t>(),
+                                      t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(), st, cap, M);
   UVLM_LAUNCH_CHECK();
 }
 
@@ -763,6 +903,30 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
   const size_t lds_part = sizeof(double2) * 16 * sch.K * sch.W * 3;
   const size_t lds      = std::max(lds_main, lds_part);
   const int32_t *d      = sch.dev.as<int32_t>();
+  if (sch.kind == 1) {
+    const int     S       = sch.plan.S;
+    const size_t  lds_ns  = std::max(lds_main, sizeof(double2) * 16 * S * 3);
+    const int64_t per_xcd = (tt.C + 7) / 8;
+    const int     grid    = (int)(8 * per_xcd);
+#define UVLM_M2L_NS(SS)                                                                                            \
+  if (S == SS) {                                                                                                   \
+    k_m2l_ns<SS><<<grid, WG, lds_ns, s>>>(P, NT, tt.C, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),    \
+                                          tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),               \
+                                          st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, L,         \
+                                          d + sch.dev_items, d + sch.dev_lev, d + sch.dev_out_off,                 \
+                                          d + sch.dev_out_src);                                                    \
+    UVLM_LAUNCH_CHECK();                                                                                           \
+    return;                                                                                                        \
+  }
+    UVLM_M2L_NS(4)
+    UVLM_M2L_NS(5)
+    UVLM_M2L_NS(6)
+    UVLM_M2L_NS(8)
+    UVLM_M2L_NS(12)
+    UVLM_M2L_NS(16)
+#undef UVLM_M2L_NS
+    throw Error(UVLM_ERR_INTERNAL, "unsupported M2L plan size");
+  }
 #define UVLM_M2L_CASE(WW, KK, PP)                                                                                  \
   if (sch.W == WW && sch.K == KK && sch.prefetch == PP) {                                                          \
     k_m2l<WW, KK, PP><<<(int)tt.C, WG, lds, s>>>(P, NT, tt.C, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),       \

@@ -1,0 +1,122 @@
+// Host-side work plan of the n-synchronous M2L kernel (k_m2l_ns in far.hip).
+//
+// The M2L sum of one (target, source) pair (multipoleKernels.hpp:514-565) is
+//   L[j,k] += sum_{n <= P-1-j} sum_{|m| <= n} M[n,m] Y[j+n, m-k]
+// over P(P+1)/2 outputs (j, k >= 0).  Terms m and -m are taken together, so output (j,k)
+// at source degree n costs n+1 steps (m = 0..n).  Sixteen lanes share one pair and walk the
+// source degrees n = 0..P-1 in lock step: at each step every lane reads the same M[n,m]
+// (an LDS broadcast) and its own harmonics.  A lane owns a few accumulator slots:
+//   * home slots: output w = tri(j)+k lives in lane w % 16, slot w / 16 and is summed there
+//     over every degree n where it is active (j + n <= P-1) and not split;
+//   * helper slots, one per "split" degree n: when the outputs still active at degree n
+//     leave lanes idle in their last round, the m-range 0..n of each of those outputs is cut
+//     into chunks of equal length, one chunk per lane, in a slot reserved for that degree.
+// Every (output, n, m) term is then owned by exactly one (lane, slot); the kernel adds the
+// slots of an output in the fixed order of `out_src`, so the result is deterministic.
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+namespace uvlm {
+
+struct M2LPlan {
+  static constexpr int LANES = 16;
+  static constexpr int HOME  = 31;  // level field of a home slot
+  int P = 0, S = 0, H = 0;          // slots per lane (kernel template), home slots
+  std::vector<int32_t> slots;       // [LANES][S]: pack_slot(j, k, m0, level) or -1
+  std::vector<int32_t> lev;         // [P]: chunk length | home rounds << 8
+  std::vector<int32_t> out_off;     // [P(P+1)/2 + 1]
+  std::vector<int32_t> out_src;     // pieces of each output: lane * S + slot
+};
+
+inline int32_t pack_slot(int j, int k, int m0, int level) { return j | (k << 5) | (m0 << 10) | (level << 15); }
+inline int     plan_tri(int n) { return n * (n + 1) / 2; }
+
+// Slot counts the kernel is instantiated for.
+inline int plan_template_slots(int need)
+{
+  for (int s : {4, 5, 6, 8, 12, 16})
+    if (need <= s) return s;
+  return -1;
+}
+
+// budget: accumulator slots per lane to aim for (0 = default: 6, or the home slots + 1 when
+// those alone need more; each slot holds 3 complex FP64 sums, and on gfx950 six slots keep
+// the kernel at 3 waves per SIMD).  Returns false if P is outside 1..16.
+inline bool make_m2l_plan(int P, int budget, M2LPlan &pl)
+{
+  if (P < 1 || P > 16) return false;
+  const int L = M2LPlan::LANES, ny = plan_tri(P);
+  const int H = (ny + L - 1) / L;
+  if (budget <= 0) budget = std::max(6, H + 1);
+  budget = std::max(budget, H);
+  const int S = plan_template_slots(budget);
+  if (S < 0 || S < H) return false;
+  budget = std::min(budget, S);
+  // window w = tri(j) + k in (j, k) order: the outputs active at degree n are w < tri(P - n)
+  std::vector<int> wj(ny), wk(ny);
+  for (int j = 0; j < P; j++)
+    for (int k = 0; k <= j; k++) {
+      wj[plan_tri(j) + k] = j;
+      wk[plan_tri(j) + k] = k;
+    }
+  struct Split {
+    int saving, n, cn, c;
+  };
+  std::vector<Split> opts;
+  for (int n = 0; n < P; n++) {
+    const int A = plan_tri(P - n), rem = A % L;
+    if (!rem) continue;
+    const int c0 = std::min(L / rem, n + 1);
+    const int cn = (n + 1 + c0 - 1) / c0;
+    const int c  = (n + 1 + cn - 1) / cn;
+    if (cn < n + 1) opts.push_back({n + 1 - cn, n, cn, c});
+  }
+  std::stable_sort(opts.begin(), opts.end(), [](const Split &a, const Split &b) { return a.saving > b.saving; });
+  std::vector<Split> chosen;
+  for (const Split &o : opts)
+    if (H + (int)chosen.size() < budget) chosen.push_back(o);
+  std::sort(chosen.begin(), chosen.end(), [](const Split &a, const Split &b) { return a.n < b.n; });
+
+  pl.P = P;
+  pl.S = S;
+  pl.H = H;
+  pl.slots.assign((size_t)L * S, -1);
+  pl.lev.assign(P, 0);
+  std::vector<std::vector<int>> pieces(ny);
+  for (int w = 0; w < ny; w++) {
+    const int lane = w % L, slot = w / L;
+    pl.slots[lane * S + slot] = pack_slot(wj[w], wk[w], 0, M2LPlan::HOME);
+    pieces[w].push_back(lane * S + slot);
+  }
+  for (int n = 0; n < P; n++) {
+    const int A = plan_tri(P - n);
+    pl.lev[n]   = (n + 1) | (((A + L - 1) / L) << 8);
+  }
+  for (size_t t = 0; t < chosen.size(); t++) {
+    const Split &o    = chosen[t];
+    const int    A    = plan_tri(P - o.n), full = A / L;
+    const int    slot = H + (int)t;
+    int          lane = 0;
+    for (int w = full * L; w < A; w++)
+      for (int i = 0; i < o.c; i++) {
+        const int m0 = i * o.cn;
+        if (m0 > o.n) break;
+        pl.slots[lane * S + slot] = pack_slot(wj[w], wk[w], m0, o.n);
+        pieces[w].push_back(lane * S + slot);
+        lane++;
+      }
+    pl.lev[o.n] = o.cn | (full << 8);
+  }
+  pl.out_off.assign(ny + 1, 0);
+  pl.out_src.clear();
+  for (int w = 0; w < ny; w++) {
+    std::sort(pieces[w].begin(), pieces[w].end());
+    for (int v : pieces[w]) pl.out_src.push_back(v);
+    pl.out_off[w + 1] = (int)pl.out_src.size();
+  }
+  return true;
+}
+
+}  // namespace uvlm
