@@ -143,6 +143,35 @@ __device__ void harmonic_column(int P, int m, const Sph &s, double2 *Y)
   }
 }
 
+// Column m of the singular harmonics written to the "extended" rows used by k_m2l_ns: row n
+// holds orders -n..n at Yx[n*n + n + mu] with Yx[n, mu] = Y_n^mu for mu >= 0 and
+// Yx[n, -mu] = (-1)^mu conj(Y_n^mu).  With this table the M2L sign rule becomes a per-output
+// factor: (-1)^min(k,m) Y[J, m-k] = (-1)^k Yx[J, m-k].
+__device__ void singular_column_ext(int P, int m, const Sph &s, double2 *Yx)
+{
+  double2 e = make_double2(1.0, 0.0);
+  for (int i = 0; i < m; i++) e = cmul(e, make_double2(s.cb, s.sb));
+  double pmm = 1.0;
+  for (int i = 1; i <= m; i++) pmm *= -(double)(2 * i - 1) * s.y;
+  const double inv = 1.0 / s.rho;
+  double       f   = inv;
+  for (int i = 0; i < m; i++) f *= -inv;
+  const double sm     = (m & 1) ? -1.0 : 1.0;
+  double       p_prev = 0.0, p = pmm;
+  for (int n = m; n < P; n++) {
+    if (n > m) {
+      const double pn = (n == m + 1) ? s.x * (2 * m + 1) * p
+                                     : (s.x * (2 * n - 1) * p - (n + m - 1) * p_prev) * c_inv[n - m];
+      p_prev = p;
+      p      = pn;
+      f *= -(double)(n - m) * inv;
+    }
+    const double a      = f * p;
+    Yx[n * n + n + m]   = make_double2(a * e.x, a * e.y);
+    if (m > 0) Yx[n * n + n - m] = make_double2(sm * a * e.x, -sm * a * e.y);
+  }
+}
+
 // Y[n, mu] for any mu, from the m >= 0 table.
 __device__ __forceinline__ double2 ylook(const double2 *Y, int n, int mu)
 {
@@ -490,30 +519,26 @@ __global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const 
 // each lane adds the terms of the outputs bound to its S accumulator slots.  Blocks are
 // mapped so that each of the 8 XCDs takes one contiguous range of targets: spatially close
 // targets share most of their sources, and those multipoles then hit the XCD's own L2.
-__device__ __forceinline__ void m2l_terms(double2 (&a)[3], const double2 *__restrict__ YJ,
-                                          const double2 *__restrict__ Mn, int k, int m0, int m1)
+__device__ __forceinline__ void m2l_terms(double2 (&a)[3], const double2 *__restrict__ Yrow,
+                                          const double2 *__restrict__ Mn, int k, double sk, int m0, int m1)
 {
-  int m = m0;
-  if (m == 0) {  // m = 0 is its own pair: M[n,0] Y[J,-k], Y[J,-k] = conj(Y[J,k])
-    const double2 y = conj2(YJ[k]);
+  // Sums (-1)^k times the terms of output (j,k) at degree n (multipoleKernels.hpp:534-546),
+  // with Yrow = Yx[J, 0], J = j + n.  Terms m and -m are taken together:
+  //   conj(M[n,m]) Y[J,-m-k] + s M[n,m] Y[J,m-k] = (-1)^k (A S' + i B D'),  M = A + iB,
+  //   S' = Yx[J,m-k] + (-1)^k conj(Y[J,m+k]),  D' = Yx[J,m-k] - (-1)^k conj(Y[J,m+k]).
+  const double2 *Ya = Yrow - k, *Yb = Yrow + k;
+  int            m  = m0;
+  if (m == 0) {  // M[n,0] Y[J,-k] = (-1)^k M[n,0] Yx[J,-k] (Y_J^0 is real)
+    const double2 y = Ya[0];
     for (int d = 0; d < 3; d++) cfma(a[d], Mn[d], y);
     m = 1;
   }
-#pragma unroll 1
+  // unrolled by two, so the LDS reads of step m+1 are in flight during the multiply-adds of m
+#pragma unroll 2
   for (; m < m1; m++) {
-    // conj(M[n,m]) Y[J,-m-k] + s M[n,m] Y[J,m-k] = A (y2 + yB) + i B (y2 - yB), M = A + iB,
-    // y2 = s Y[J,m-k], yB = Y[J,-m-k] = conj(Y[J,m+k]), s = (-1)^min(k,m)
-    // (multipoleKernels.hpp:534-546 with the -m term folded in)
-    const int mu = m - k;
-    double2   y2 = YJ[mu >= 0 ? mu : -mu];
-    if (mu < 0) y2.y = -y2.y;
-    if (min(k, m) & 1) {
-      y2.x = -y2.x;
-      y2.y = -y2.y;
-    }
-    const double2 yb = YJ[m + k];
-    const double2 Sv = make_double2(y2.x + yb.x, y2.y - yb.y);
-    const double2 Dv = make_double2(y2.x - yb.x, y2.y + yb.y);
+    const double2 ya = Ya[m], yb = Yb[m];
+    const double2 Sv = make_double2(fma(sk, yb.x, ya.x), fma(-sk, yb.y, ya.y));
+    const double2 Dv = make_double2(fma(-sk, yb.x, ya.x), fma(sk, yb.y, ya.y));
     for (int d = 0; d < 3; d++) {
       const double2 mv = Mn[3 * m + d];
       a[d].x           = fma(mv.x, Sv.x, a[d].x);
@@ -540,9 +565,9 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
   const int64_t t = (b & 7) * per_xcd + (b >> 3);
   if (t >= ncell) return;
   const int lane = threadIdx.x, g = lane >> 4, sl = lane & 15;
-  const int ny = tri(P), per = ny + NT;
-  double2  *Yg = sm2 + g * per;
-  double2  *Mg = Yg + ny;
+  const int ny = tri(P), per = P * P + NT;
+  double2  *Yx = sm2 + g * per;  // extended singular rows, P*P entries
+  double2  *Mg = Yx + P * P;
   // slot s of this lane: packed output (j, k), first order m0 and the degree it serves
   // (31: home slot); an unused slot gets j = 31, which no degree activates
   int32_t sv[S];
@@ -564,17 +589,22 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
       for (int q = sl; q < NT; q += 16) Mg[q] = Ms[q];
       if (sl < P) {
         const Sph sp = sph(X - sx[s], Yc - sy[s], Z - sz[s]);
-        harmonic_column<true>(P, sl, sp, Yg);
+        singular_column_ext(P, sl, sp, Yx);
       }
     }
     __syncthreads();
     if (active) {
       for (int n = 0; n < P; n++) {
-        const int      lv = lev[n], cn = lv & 255, hr = lv >> 8;
+        const int      lv = lev[n], cn = lv & 255, hr = (lv >> 8) & 255, used = lv >> 16;
         const double2 *Mn = Mg + 3 * tri(n);
 #pragma unroll
         for (int s = 0; s < S; s++) {
-          const int v = sv[s], sj = v & 31, sk = (v >> 5) & 31, lvl = (v >> 15) & 31;
+          if (!((used >> s) & 1)) continue;  // no lane has work in slot s at this degree
+          // decoded here, per degree: the empty asm keeps the compiler from hoisting S copies
+          // of the decoded fields out of the loops, which would cost registers (occupancy)
+          int v = sv[s];
+          asm volatile("" : "+v"(v));
+          const int sj = v & 31, sk = (v >> 5) & 31, lvl = (v >> 15) & 31;
           bool      act;
           int       m0, m1;
           if (lvl == 31) {  // home slot: every m of an active, unsplit output
@@ -586,7 +616,8 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
             m0  = (v >> 10) & 31;
             m1  = min(m0 + cn, n + 1);
           }
-          if (act) m2l_terms(acc[s], Yg + tri(sj + n), Mn, sk, m0, m1);
+          const int J = sj + n;
+          if (act) m2l_terms(acc[s], Yx + J * J + J, Mn, sk, (sk & 1) ? -1.0 : 1.0, m0, m1);
         }
       }
     }
@@ -617,8 +648,8 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
         r[d].y += part[pc * 3 + d].y;
       }
     }
-    const double sj = sgn(j);
-    for (int d = 0; d < 3; d++) L[t * NT + 3 * q + d] = make_double2(sj * r[d].x, sj * r[d].y);
+    const double sjk = sgn(j + q - tri(j));  // (-1)^j of the M2L rule, (-1)^k of the folded rows
+    for (int d = 0; d < 3; d++) L[t * NT + 3 * q + d] = make_double2(sjk * r[d].x, sjk * r[d].y);
   }
 }
 
@@ -905,7 +936,7 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
   const int32_t *d      = sch.dev.as<int32_t>();
   if (sch.kind == 1) {
     const int     S       = sch.plan.S;
-    const size_t  lds_ns  = std::max(lds_main, sizeof(double2) * 16 * S * 3);
+    const size_t  lds_ns  = std::max(sizeof(double2) * 4 * (P * P + NT), sizeof(double2) * 16 * S * 3);
     const int64_t per_xcd = (tt.C + 7) / 8;
     const int     grid    = (int)(8 * per_xcd);
 #define UVLM_M2L_NS(SS)                                                                                            \

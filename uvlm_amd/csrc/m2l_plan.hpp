@@ -3,10 +3,11 @@
 // The M2L sum of one (target, source) pair (multipoleKernels.hpp:514-565) is
 //   L[j,k] += sum_{n <= P-1-j} sum_{|m| <= n} M[n,m] Y[j+n, m-k]
 // over P(P+1)/2 outputs (j, k >= 0).  Terms m and -m are taken together, so output (j,k)
-// at source degree n costs n+1 steps (m = 0..n).  Sixteen lanes share one pair and walk the
-// source degrees n = 0..P-1 in lock step: at each step every lane reads the same M[n,m]
-// (an LDS broadcast) and its own harmonics.  A lane owns a few accumulator slots:
-//   * home slots: output w = tri(j)+k lives in lane w % 16, slot w / 16 and is summed there
+// at source degree n costs n+1 steps (m = 0..n).  A group of G lanes (16 or 32) shares one
+// pair and walks the source degrees n = 0..P-1 in lock step: at each step every lane reads
+// the same M[n,m] (an LDS broadcast) and its own harmonics.  A lane owns a few accumulator
+// slots:
+//   * home slots: output w = tri(j)+k lives in lane w % G, slot w / G and is summed there
 //     over every degree n where it is active (j + n <= P-1) and not split;
 //   * helper slots, one per "split" degree n: when the outputs still active at degree n
 //     leave lanes idle in their last round, the m-range 0..n of each of those outputs is cut
@@ -21,11 +22,10 @@
 namespace uvlm {
 
 struct M2LPlan {
-  static constexpr int LANES = 16;
-  static constexpr int HOME  = 31;  // level field of a home slot
-  int P = 0, S = 0, H = 0;          // slots per lane (kernel template), home slots
-  std::vector<int32_t> slots;       // [LANES][S]: pack_slot(j, k, m0, level) or -1
-  std::vector<int32_t> lev;         // [P]: chunk length | home rounds << 8
+  static constexpr int HOME = 31;   // level field of a home slot
+  int P = 0, S = 0, H = 0, G = 16;  // slots per lane (kernel template), home slots, group lanes
+  std::vector<int32_t> slots;       // [G][S]: pack_slot(j, k, m0, level) or -1
+  std::vector<int32_t> lev;         // [P]: chunk length | home rounds << 8 | active slots << 16
   std::vector<int32_t> out_off;     // [P(P+1)/2 + 1]
   std::vector<int32_t> out_src;     // pieces of each output: lane * S + slot
 };
@@ -41,13 +41,13 @@ inline int plan_template_slots(int need)
   return -1;
 }
 
-// budget: accumulator slots per lane to aim for (0 = default: 6, or the home slots + 1 when
-// those alone need more; each slot holds 3 complex FP64 sums, and on gfx950 six slots keep
-// the kernel at 3 waves per SIMD).  Returns false if P is outside 1..16.
-inline bool make_m2l_plan(int P, int budget, M2LPlan &pl)
+// G: lanes per group (16 or 32).  budget: accumulator slots per lane to aim for (0 = default:
+// 6, or the home slots + 1 when those alone need more; each slot holds 3 complex FP64 sums).
+// Returns false if P is outside 1..16 or G is not 16 or 32.
+inline bool make_m2l_plan(int P, int budget, M2LPlan &pl, int G = 16)
 {
-  if (P < 1 || P > 16) return false;
-  const int L = M2LPlan::LANES, ny = plan_tri(P);
+  if (P < 1 || P > 16 || (G != 16 && G != 32)) return false;
+  const int L = G, ny = plan_tri(P);
   const int H = (ny + L - 1) / L;
   if (budget <= 0) budget = std::max(6, H + 1);
   budget = std::max(budget, H);
@@ -82,6 +82,7 @@ inline bool make_m2l_plan(int P, int budget, M2LPlan &pl)
   pl.P = P;
   pl.S = S;
   pl.H = H;
+  pl.G = G;
   pl.slots.assign((size_t)L * S, -1);
   pl.lev.assign(P, 0);
   std::vector<std::vector<int>> pieces(ny);
@@ -108,6 +109,15 @@ inline bool make_m2l_plan(int P, int budget, M2LPlan &pl)
         lane++;
       }
     pl.lev[o.n] = o.cn | (full << 8);
+  }
+  // slots that any lane uses at degree n (a uniform skip test in the kernel)
+  for (int n = 0; n < P; n++) {
+    const int hr   = (pl.lev[n] >> 8) & 255;
+    int       mask = 0;
+    for (int s = 0; s < H && s < hr; s++) mask |= 1 << s;
+    for (size_t t = 0; t < chosen.size(); t++)
+      if (chosen[t].n == n) mask |= 1 << (H + (int)t);
+    pl.lev[n] |= mask << 16;
   }
   pl.out_off.assign(ny + 1, 0);
   pl.out_src.clear();
