@@ -490,9 +490,9 @@ int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride)
     h->Ls.reserve(sizeof(double2) * h->NT * h->stree.C);
     double2 *Ls = h->Ls.as<double2>();
     reset_sensor_rates(s, h->sst.as<double>(), sc, m);
-    m2l(s, P, h->stree, h->tree, h->slists, h->sched, h->M.as<double2>(), Ls);
+    m2l(s, P, h->stree, h->tree, h->slists, h->sched, h->M.as<double2>(), Ls, &h->sc);
     p2p(s, h->stree, h->tree, h->slists, h->sst.as<double>(), sc, h->st.as<double>(), h->cap, h->sst.as<double>(), sc,
-        h->prm.precision, 1, h->prm.ncrit);
+        h->prm.precision, 1, h->prm.ncrit, &h->sc);
     for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls);
     l2p_stretch(s, P, h->stree, Ls, h->sst.as<double>(), sc, h->prm.stretching, h->prm.formulation, 1);
     UVLM_HIP(hipEventRecord(h->aev[1], s));

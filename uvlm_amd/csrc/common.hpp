@@ -114,6 +114,8 @@ struct Scratch {
   DevBuf rec_cx, rec_cy, rec_cz;
   DevBuf partial, counters, perm;
   DevBuf front0, front1, out_p2p, out_m2l;
+  DevBuf p2p_seg, p2p_part;  // P2P list segments and their partial sums (few-target case)
+  DevBuf m2l_seg, m2l_part;  // the same for M2L
   int64_t rec_cap = 0;
 };
 
@@ -151,7 +153,7 @@ void zero_doubles(hipStream_t s, double *p, int64_t n);
 void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, double2 *M);
 void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M);
 void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, const M2LSchedule &sch, const double2 *M,
-         double2 *L);
+         double2 *L, Scratch *sc = nullptr);
 void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L);
 void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *st, int64_t cap, int stretching,
                  int formulation, int sensor_mode);
@@ -160,7 +162,7 @@ void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *
 // max_leaf: an upper bound of the target leaves' body counts (ncrit for a built tree)
 void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const double *tgt, int64_t tcap,
          const double *src, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode,
-         int64_t max_leaf);
+         int64_t max_leaf, Scratch *sc = nullptr);
 
 void direct(hipStream_t s, int64_t m, const double *xt, const double *at, const double *sst, int64_t scap, int64_t n,
             double *u, double *J, double *dadt);

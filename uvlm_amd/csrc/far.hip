@@ -30,6 +30,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <vector>
 
 namespace uvlm {
 
@@ -549,7 +550,10 @@ __device__ __forceinline__ void m2l_terms(double2 (&a)[3], const double2 *__rest
   }
 }
 
-template <int S>
+// SPLIT: the unit of work is a list segment (seg[3u] = target, [seg[3u+1], seg[3u+2]) = list
+// entries), ncell counts segments, and the segment's local expansion goes to L + u * NT;
+// k_m2l_reduce adds the segments of each target in order.
+template <int S, bool SPLIT = false>
 __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int64_t per_xcd,
                                                const int32_t *__restrict__ off, const int32_t *__restrict__ src,
                                                const double *__restrict__ tx, const double *__restrict__ ty,
@@ -558,12 +562,14 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
                                                const double2 *__restrict__ M, double2 *__restrict__ L,
                                                const int32_t *__restrict__ slots, const int32_t *__restrict__ lev,
                                                const int32_t *__restrict__ out_off,
-                                               const int32_t *__restrict__ out_src)
+                                               const int32_t *__restrict__ out_src,
+                                               const int32_t *__restrict__ seg = nullptr)
 {
   extern __shared__ double2 sm2[];
   const int64_t b = blockIdx.x;
-  const int64_t t = (b & 7) * per_xcd + (b >> 3);
-  if (t >= ncell) return;
+  const int64_t u = (b & 7) * per_xcd + (b >> 3);
+  if (u >= ncell) return;
+  const int64_t t = SPLIT ? seg[3 * u] : u;  // target cell; the output row is u
   const int lane = threadIdx.x, g = lane >> 4, sl = lane & 15;
   const int ny = tri(P), per = P * P + NT;
   double2  *Yx = sm2 + g * per;  // extended singular rows, P*P entries
@@ -578,7 +584,7 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
   double2 acc[S][3];
   for (int s = 0; s < S; s++)
     for (int d = 0; d < 3; d++) acc[s][d] = make_double2(0, 0);
-  const int    s0 = off[t], nsrc = off[t + 1] - s0;
+  const int    s0 = SPLIT ? seg[3 * u + 1] : off[t], nsrc = (SPLIT ? seg[3 * u + 2] : off[t + 1]) - s0;
   const double X = tx[t], Yc = ty[t], Z = tz[t];
   for (int base = 0; base < nsrc; base += 4) {
     const int  si     = base + g;
@@ -649,7 +655,23 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
       }
     }
     const double sjk = sgn(j + q - tri(j));  // (-1)^j of the M2L rule, (-1)^k of the folded rows
-    for (int d = 0; d < 3; d++) L[t * NT + 3 * q + d] = make_double2(sjk * r[d].x, sjk * r[d].y);
+    for (int d = 0; d < 3; d++) L[u * NT + 3 * q + d] = make_double2(sjk * r[d].x, sjk * r[d].y);
+  }
+}
+
+// L of each target = the sum of its segments' expansions, in segment order.
+__global__ __launch_bounds__(WG) void k_m2l_reduce(int NT, int64_t ncell, const int32_t *__restrict__ segoff,
+                                                   const double2 *__restrict__ part, double2 *__restrict__ L)
+{
+  const int64_t t = blockIdx.x;
+  if (t >= ncell) return;
+  for (int q = threadIdx.x; q < NT; q += WG) {
+    double2 r = make_double2(0, 0);
+    for (int u = segoff[t]; u < segoff[t + 1]; u++) {
+      r.x += part[(int64_t)u * NT + q].x;
+      r.y += part[(int64_t)u * NT + q].y;
+    }
+    L[t * NT + q] = r;
   }
 }
 
@@ -925,8 +947,12 @@ void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L)
   UVLM_LAUNCH_CHECK();
 }
 
+// Below this many target cells (probe trees), long lists are cut into segments that run as
+// separate workgroups (see the P2P launcher for the same rule).
+constexpr int64_t M2L_SPLIT_TARGETS = 4096, M2L_SPLIT_SEGMENTS = 8192;
+
 void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, const M2LSchedule &sch, const double2 *M,
-         double2 *L)
+         double2 *L, Scratch *sc)
 {
   const int NT = 3 * P * (P + 1) / 2, ny = P * (P + 1) / 2;
   if (tt.C == 0) return;
@@ -935,18 +961,60 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
   const size_t lds      = std::max(lds_main, lds_part);
   const int32_t *d      = sch.dev.as<int32_t>();
   if (sch.kind == 1) {
-    const int     S       = sch.plan.S;
-    const size_t  lds_ns  = std::max(sizeof(double2) * 4 * (P * P + NT), sizeof(double2) * 16 * S * 3);
-    const int64_t per_xcd = (tt.C + 7) / 8;
+    const int    S      = sch.plan.S;
+    const size_t lds_ns = std::max(sizeof(double2) * 4 * (P * P + NT), sizeof(double2) * 16 * S * 3);
+    // few targets with long lists: one workgroup per list segment, partial expansions reduced
+    const bool     split = sc && tt.C < M2L_SPLIT_TARGETS && l.nm2l >= 4 * tt.C;
+    int64_t        units = tt.C;
+    const int32_t *dseg = nullptr, *dsegoff = nullptr;
+    double2       *out  = L;
+    if (split) {
+      const int64_t        E = std::max<int64_t>(2, (l.nm2l + M2L_SPLIT_SEGMENTS - 1) / M2L_SPLIT_SEGMENTS);
+      std::vector<int32_t> off(tt.C + 1), seg, segoff(tt.C + 1, 0);
+      UVLM_HIP(hipMemcpyAsync(off.data(), l.m2l_off.p, 4 * off.size(), hipMemcpyDeviceToHost, s));
+      UVLM_HIP(hipStreamSynchronize(s));
+      for (int64_t t = 0; t < tt.C; t++) {
+        int32_t e0 = off[t];
+        do {
+          const int32_t e1 = (int32_t)std::min<int64_t>(off[t + 1], e0 + E);
+          seg.push_back((int32_t)t);
+          seg.push_back(e0);
+          seg.push_back(e1);
+          e0 = e1;
+        } while (e0 < off[t + 1]);
+        segoff[t + 1] = (int32_t)(seg.size() / 3);
+      }
+      units = segoff[tt.C];
+      sc->m2l_seg.reserve(4 * (seg.size() + segoff.size()));
+      int32_t *ds = sc->m2l_seg.as<int32_t>();
+      UVLM_HIP(hipMemcpyAsync(ds, seg.data(), 4 * seg.size(), hipMemcpyHostToDevice, s));
+      UVLM_HIP(hipMemcpyAsync(ds + seg.size(), segoff.data(), 4 * segoff.size(), hipMemcpyHostToDevice, s));
+      UVLM_HIP(hipStreamSynchronize(s));
+      dseg    = ds;
+      dsegoff = ds + seg.size();
+      sc->m2l_part.reserve(sizeof(double2) * units * NT);
+      out = sc->m2l_part.as<double2>();
+    }
+    const int64_t per_xcd = (units + 7) / 8;
     const int     grid    = (int)(8 * per_xcd);
 #define UVLM_M2L_NS(SS)                                                                                            \
   if (S == SS) {                                                                                                   \
-    k_m2l_ns<SS><<<grid, WG, lds_ns, s>>>(P, NT, tt.C, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),    \
-                                          tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),               \
-                                          st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, L,         \
-                                          d + sch.dev_items, d + sch.dev_lev, d + sch.dev_out_off,                 \
-                                          d + sch.dev_out_src);                                                    \
+    if (split)                                                                                                     \
+      k_m2l_ns<SS, true><<<grid, WG, lds_ns, s>>>(                                                                 \
+          P, NT, units, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(), tt.cx.as<double>(),            \
+          tt.cy.as<double>(), tt.cz.as<double>(), st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M,   \
+          out, d + sch.dev_items, d + sch.dev_lev, d + sch.dev_out_off, d + sch.dev_out_src, dseg);               \
+    else                                                                                                           \
+      k_m2l_ns<SS><<<grid, WG, lds_ns, s>>>(P, NT, tt.C, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),  \
+                                            tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),             \
+                                            st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, L,       \
+                                            d + sch.dev_items, d + sch.dev_lev, d + sch.dev_out_off,               \
+                                            d + sch.dev_out_src);                                                  \
     UVLM_LAUNCH_CHECK();                                                                                           \
+    if (split) {                                                                                                   \
+      k_m2l_reduce<<<(int)tt.C, WG, 0, s>>>(NT, tt.C, dsegoff, out, L);                                            \
+      UVLM_LAUNCH_CHECK();                                                                                         \
+    }                                                                                                              \
     return;                                                                                                        \
   }
     UVLM_M2L_NS(4)

@@ -27,6 +27,8 @@
 // arithmetic, 1.4e-7 and 1.1e-7 when evaluated by Horner in FP32.
 #include "common.hpp"
 
+#include <vector>
+
 #include <algorithm>
 
 namespace uvlm {
@@ -97,7 +99,10 @@ __device__ __forceinline__ void lds_prefix(int *pref, const int *v, int n)
 // TSIG: the core size comes from the TARGET body instead of the source.  That is the
 // transposed near-field kernel used by the adjoint (a particle's own sigma, with the
 // particle now on the target side).
-template <bool MIXED, bool WITH_J, bool TSIG, int BLK>
+// SPLIT: workgroup b takes the list entries [seg[3b+1], seg[3b+2]) of target leaf seg[3b] and
+// writes its un-negated sums to part + b * 12 * maxnt (field-major, maxnt entries per field);
+// k_p2p_reduce adds the segments of each target in order.
+template <bool MIXED, bool WITH_J, bool TSIG, int BLK, bool SPLIT = false>
 __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves, int64_t nleaf,
                                             const int32_t *__restrict__ tbody, const int32_t *__restrict__ tnbody,
                                             const double *__restrict__ tcx, const double *__restrict__ tcy,
@@ -105,14 +110,16 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
                                             const int32_t *__restrict__ src, const int32_t *__restrict__ sbody,
                                             const int32_t *__restrict__ snbody, const double *__restrict__ tst,
                                             int64_t tcap, const double *__restrict__ sst, int64_t scap,
-                                            double *__restrict__ out, int64_t ocap)
+                                            double *__restrict__ out, int64_t ocap,
+                                            const int32_t *__restrict__ seg = nullptr, double *__restrict__ part = nullptr,
+                                            int maxnt = 0)
 {
   constexpr int LGMAX = BLK == 64 ? 6 : 8;
   __shared__ SrcF64 s64[MIXED ? 1 : BLK];
   __shared__ float4 s32[MIXED ? 3 * BLK : 1];
   __shared__ int    pref[SEG + 1], lb[SEG], ln[SEG];
   __shared__ double red[BLK == 64 ? 1 : 12 * BLK];
-  const int64_t w = blockIdx.x;
+  const int64_t w = SPLIT ? seg[3 * blockIdx.x] : blockIdx.x;
   if (w >= nleaf) return;
   const int t = leaves[w], tb = tbody[t], nt = tnbody[t];
   const int lane = threadIdx.x;
@@ -120,7 +127,7 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
   while ((1 << lg) < nt && lg < LGMAX) lg++;
   const int    ntp = 1 << lg, S = BLK >> lg, ti = lane & (ntp - 1), si = lane >> lg;
   const double cxt = tcx[t], cyt = tcy[t], czt = tcz[t];
-  const int    q0 = off[t], q1 = off[t + 1];
+  const int    q0 = SPLIT ? seg[3 * blockIdx.x + 1] : off[t], q1 = SPLIT ? seg[3 * blockIdx.x + 2] : off[t + 1];
   for (int c0 = 0; c0 < nt; c0 += ntp) {
     const int  i     = c0 + ti;
     const bool valid = i < nt;
@@ -257,9 +264,14 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
         }
     }
     if (si == 0 && valid) {
-      for (int d = 0; d < 3; d++) out[(F_U + d) * ocap + bi] = -acc[d];
-      if (WITH_J)
-        for (int k = 0; k < 9; k++) out[(F_J + k) * ocap + bi] = -acc[3 + k];
+      if (SPLIT) {
+        double *pb = part + (int64_t)blockIdx.x * 12 * maxnt;
+        for (int k = 0; k < (WITH_J ? 12 : 3); k++) pb[k * maxnt + i] = acc[k];
+      } else {
+        for (int d = 0; d < 3; d++) out[(F_U + d) * ocap + bi] = -acc[d];
+        if (WITH_J)
+          for (int k = 0; k < 9; k++) out[(F_J + k) * ocap + bi] = -acc[3 + k];
+      }
     }
   }
 }
@@ -338,12 +350,94 @@ void direct(hipStream_t s, int64_t m, const double *xt, const double *at, const 
   UVLM_LAUNCH_CHECK();
 }
 
+// Sum of the SPLIT partials of each target leaf, segments in order, stored like k_p2p does.
+template <bool WITH_J>
+__global__ __launch_bounds__(WG) void k_p2p_reduce(const int32_t *__restrict__ leaves, int64_t nleaf,
+                                                   const int32_t *__restrict__ tbody,
+                                                   const int32_t *__restrict__ tnbody,
+                                                   const int32_t *__restrict__ segoff, const double *__restrict__ part,
+                                                   int maxnt, double *__restrict__ out, int64_t ocap)
+{
+  const int64_t w = blockIdx.x;
+  if (w >= nleaf) return;
+  const int t = leaves[w], tb = tbody[t], nt = tnbody[t];
+  for (int i = threadIdx.x; i < nt; i += WG) {
+    const int64_t bi = tb + i;
+    for (int k = 0; k < (WITH_J ? 12 : 3); k++) {
+      double v = 0.0;
+      for (int b = segoff[w]; b < segoff[w + 1]; b++) v += part[(int64_t)b * 12 * maxnt + k * maxnt + i];
+      if (k < 3) out[(F_U + k) * ocap + bi] = -v;
+      else out[(F_J + k - 3) * ocap + bi] = -v;
+    }
+  }
+}
+
+// Below this many target leaves (probe trees), long lists are cut into segments of E list
+// entries that run as separate workgroups, so the GPU stays full.
+constexpr int64_t SPLIT_TARGETS = 4096, SPLIT_SEGMENTS = 8192;
+
 void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const double *tgt, int64_t tcap,
          const double *srcst, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode,
-         int64_t max_leaf)
+         int64_t max_leaf, Scratch *sc)
 {
   if (tt.nleaf == 0) return;
   const int nb = (int)tt.nleaf;
+  if (sc && tt.nleaf < SPLIT_TARGETS && l.np2p >= 4 * tt.nleaf) {
+    // segment table on the host: the target tree is small
+    const int64_t        E = std::max<int64_t>(2, (l.np2p + SPLIT_SEGMENTS - 1) / SPLIT_SEGMENTS);
+    std::vector<int32_t> lv(tt.nleaf), off(tt.C + 1);
+    UVLM_HIP(hipMemcpyAsync(lv.data(), tt.leaves.p, 4 * lv.size(), hipMemcpyDeviceToHost, s));
+    UVLM_HIP(hipMemcpyAsync(off.data(), l.p2p_off.p, 4 * off.size(), hipMemcpyDeviceToHost, s));
+    UVLM_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> seg, segoff(tt.nleaf + 1, 0);
+    for (int64_t w = 0; w < tt.nleaf; w++) {
+      const int32_t q0 = off[lv[w]], q1 = off[lv[w] + 1];
+      int32_t       e0 = q0;
+      do {
+        const int32_t e1 = (int32_t)std::min<int64_t>(q1, e0 + E);
+        seg.push_back((int32_t)w);
+        seg.push_back(e0);
+        seg.push_back(e1);
+        e0 = e1;
+      } while (e0 < q1);
+      segoff[w + 1] = (int32_t)(seg.size() / 3);
+    }
+    const int64_t nsg = segoff[tt.nleaf];
+    sc->p2p_seg.reserve(4 * (seg.size() + segoff.size()));
+    int32_t *dseg = sc->p2p_seg.as<int32_t>(), *dsegoff = dseg + seg.size();
+    UVLM_HIP(hipMemcpyAsync(dseg, seg.data(), 4 * seg.size(), hipMemcpyHostToDevice, s));
+    UVLM_HIP(hipMemcpyAsync(dsegoff, segoff.data(), 4 * segoff.size(), hipMemcpyHostToDevice, s));
+    UVLM_HIP(hipStreamSynchronize(s));
+    const int maxnt = (int)max_leaf;
+    sc->p2p_part.reserve(sizeof(double) * nsg * 12 * maxnt);
+    double    *part = sc->p2p_part.as<double>();
+    const bool mixed = precision == UVLM_MIXED, wj = sensor_mode != 1, ts = sensor_mode == 2;
+#define UVLM_P2P_SPLIT(MX, WJ, TS)                                                                                \
+  k_p2p<MX, WJ, TS, 64, true><<<(int)nsg, WG, 0, s>>>(                                                             \
+      tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(), tt.nbody.as<int32_t>(), tt.cx.as<double>(),        \
+      tt.cy.as<double>(), tt.cz.as<double>(), l.p2p_off.as<int32_t>(), l.p2p_src.as<int32_t>(),                   \
+      stree.body.as<int32_t>(), stree.nbody.as<int32_t>(), tgt, tcap, srcst, scap, out, ocap, dseg, part, maxnt)
+    if (ts) {
+      if (mixed) UVLM_P2P_SPLIT(true, true, true);
+      else UVLM_P2P_SPLIT(false, true, true);
+    } else if (wj) {
+      if (mixed) UVLM_P2P_SPLIT(true, true, false);
+      else UVLM_P2P_SPLIT(false, true, false);
+    } else {
+      if (mixed) UVLM_P2P_SPLIT(true, false, false);
+      else UVLM_P2P_SPLIT(false, false, false);
+    }
+#undef UVLM_P2P_SPLIT
+    UVLM_LAUNCH_CHECK();
+    if (wj)
+      k_p2p_reduce<true><<<nb, WG, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(),
+                                           tt.nbody.as<int32_t>(), dsegoff, part, maxnt, out, ocap);
+    else
+      k_p2p_reduce<false><<<nb, WG, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(),
+                                            tt.nbody.as<int32_t>(), dsegoff, part, maxnt, out, ocap);
+    UVLM_LAUNCH_CHECK();
+    return;
+  }
 #define UVLM_P2P(MX, WJ, TS)                                                                                      \
   if (big)                                                                                                         \
     k_p2p<MX, WJ, TS, 256><<<nb, 256, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(),           \
