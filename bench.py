@@ -1,7 +1,9 @@
 """Benchmark: one FMM evaluation (tree + lists + u + stretching) per step on one MI355X.
 
 Workload (default): two coaxial leapfrogging vortex rings, 4,194,304 particles, P=10,
-theta=0.5, ncrit=64, mixed precision (FP32 P2P, FP64 expansions), seeded synthetic data.
+theta=0.5, mixed precision (FP32 P2P, FP64 expansions), seeded synthetic data; the leaf size
+ncrit and the M2L plan size are chosen by uvlm_autotune on this workload (candidates
+64..256), and the ncrit=64 time is reported beside it.
 A step = uvlm_build (tree + interaction lists) + uvlm_eval on device-resident particles.
 
 Prints ONE JSON line (rank 0, N=1) with the contract fields plus:
@@ -35,6 +37,23 @@ def m2l_flop_per_pair(P):
 
 
 P2P_FLOP = 70  # per interaction: 67 add/mul (FMA = 2) + rsqrt + exp + erf
+
+
+def profiled_traffic(workload, kernel_prefix):
+    """HBM bytes per launch of a kernel, from the rocprofv3 FETCH_SIZE / WRITE_SIZE summaries
+    committed under profiles/ for exactly this workload (None if there is none)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.md"))):
+        text = open(path).read()
+        if "```json" not in text:
+            continue
+        blob = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+        if blob.get("workload") != workload:
+            continue
+        for k, v in blob["hbm_bytes_per_launch"].items():
+            if k.startswith(kernel_prefix):
+                return int(v)
+    return None
 
 
 def rel(a, b):
@@ -96,7 +115,9 @@ def main():
     ap.add_argument("--stations", type=int, default=16384, help="azimuthal stations per ring (x128 samples x2 rings)")
     ap.add_argument("--P", type=int, default=10)
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--ncrit", type=int, default=64)
+    ap.add_argument("--ncrit", type=int, default=0,
+                    help="leaf size; 0 (default): chosen by uvlm_autotune among --tune-ncrit on this workload")
+    ap.add_argument("--tune-ncrit", default="64,96,128,160,192,256", help="autotuner candidates")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--cpu-n", type=int, default=131072, help="particles in the CPU baseline sample (0: skip)")
     ap.add_argument("--err-targets", type=int, default=2048, help="sampled targets for the direct-sum error")
@@ -114,8 +135,29 @@ def main():
     X, A, S = W.leapfrog(n_stations=args.stations)
     N = X.shape[0]
 
-    def timed_run(precision, steps, warmup):
-        g = FMMCalculator(P=args.P, theta=args.theta, ncrit=args.ncrit, precision=precision)
+    def tune(precision):
+        """uvlm_autotune over the candidate leaf sizes on this workload and precision."""
+        cands = [int(c) for c in args.tune_ncrit.split(",")]
+        g0 = FMMCalculator(P=args.P, theta=args.theta, ncrit=cands[0], precision=precision)
+        g0.set_arrays(X, A, S)
+        t0 = time.perf_counter()
+        best, best_ms = g0.autotune(ncrits=cands, reps=1)
+        res = {"ncrit": best, "m2l_slots": g0.timings()["m2l_slots"], "candidates": cands,
+               "tune_s": round(time.perf_counter() - t0, 2), "tuned_build_eval_ms": round(best_ms, 3)}
+        g0.close()
+        print(f"[bench] autotune {precision}: {res}", file=sys.stderr, flush=True)
+        return res
+
+    tuned = tune(args.precision) if args.ncrit == 0 else None
+
+    def timed_run(precision, steps, warmup, cfg=None, ncrit=None):
+        """cfg: autotuner result (ncrit + M2L plan size) to run with; else ncrit or --ncrit."""
+        if cfg is not None:
+            os.environ["UVLM_M2L_SLOTS"] = str(cfg["m2l_slots"])
+        else:
+            os.environ.pop("UVLM_M2L_SLOTS", None)
+        nc = cfg["ncrit"] if cfg is not None else (ncrit or args.ncrit)
+        g = FMMCalculator(P=args.P, theta=args.theta, ncrit=nc, precision=precision)
         g.set_arrays(X, A, S)  # inputs resident in HBM before the timed region
         for _ in range(warmup):
             g.build()
@@ -135,21 +177,30 @@ def main():
             phases[k] /= steps
         return g, wall, phases, g.timings()
 
-    g, ms, ph, tm = timed_run(args.precision, args.steps, args.warmup)
-    print(f"[bench] {args.precision}: {ms:.2f} ms/eval  phases={ph}", file=sys.stderr, flush=True)
+    g, ms, ph, tm = timed_run(args.precision, args.steps, args.warmup, cfg=tuned)
+    ncrit_used = tm["ncrit"]
+    print(f"[bench] {args.precision}: {ms:.2f} ms/eval ncrit={ncrit_used} phases={ph}", file=sys.stderr, flush=True)
+    ncrit64 = None
+    if tuned is not None and ncrit_used != 64:  # the leaf size of the configuration as usually quoted
+        g64, ms64n, ph64n, _ = timed_run(args.precision, max(1, min(args.steps, 3)), 1, ncrit=64)
+        g64.close()
+        ncrit64 = {"ms": round(ms64n, 3), "phases_ms": {k: round(v, 3) for k, v in ph64n.items()}}
 
     # roofline of the dominant kernel (M2L or P2P), from the library's HIP events
     m2l_flop = tm["n_m2l_pairs"] * m2l_flop_per_pair(args.P)
     p2p_flop = tm["n_interactions"] * P2P_FLOP
     if ph["m2l_ms"] >= ph["p2p_ms"]:
-        kern, flop, kms, peak = "M2L (FP64)", m2l_flop, ph["m2l_ms"], PEAK["fp64"]
+        kern, key, flop, kms, peak = "M2L (FP64)", "k_m2l_ns", m2l_flop, ph["m2l_ms"], PEAK["fp64"]
     else:
-        kern, flop, kms = "P2P", p2p_flop, ph["p2p_ms"]
+        kern, key, flop, kms = "P2P", "k_p2p", p2p_flop, ph["p2p_ms"]
         peak = PEAK["fp32"] if args.precision == "mixed" else PEAK["fp64"]
     achieved = flop / (kms * 1e-3) / 1e12
+    wl = {"particles": N, "P": args.P, "theta": args.theta, "ncrit": ncrit_used, "precision": args.precision}
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None, "kernel": kern,
-                "roof": "vector ALU (no MFMA in this path; FP64 vector and matrix peaks coincide on MI355X)"}
+                "frac": round(achieved / peak, 4), "traffic": profiled_traffic(wl, key), "kernel": kern,
+                "roof": "vector ALU (no MFMA in this path; FP64 vector and matrix peaks coincide on MI355X)",
+                "traffic_note": "HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in "
+                                "profiles/ for this workload (null if none)"}
 
     extra = {"phases_ms": {k: round(v, 3) for k, v in ph.items()},
              "eval_only_ms": round(ph["eval_ms"], 3), "tree_lists_ms": round(ph["tree_ms"] + ph["lists_ms"], 3),
@@ -158,19 +209,25 @@ def main():
              "interactions_per_s": tm["n_interactions"] / (ph["p2p_ms"] * 1e-3) if ph["p2p_ms"] > 0 else None,
              "m2l_tflops": round(m2l_flop / (ph["m2l_ms"] * 1e-3) / 1e12, 3) if ph["m2l_ms"] > 0 else None,
              "p2p_tflops": round(p2p_flop / (ph["p2p_ms"] * 1e-3) / 1e12, 3) if ph["p2p_ms"] > 0 else None,
-             "adjoint_fwd_ratio": None}
+             "adjoint_fwd_ratio": None, "autotune": tuned, "ncrit64": ncrit64}
 
     # accuracy: FP64 path, mixed-vs-FP64, and both vs an FP64 direct sum on sampled targets
     out = g.arrays()
     if not args.no_fp64 and args.precision == "mixed":
-        g64, ms64, ph64, _ = timed_run("fp64", max(1, min(args.steps, 3)), 1)
-        ref = g64.arrays()
+        # precision: the FP64 path on the same tree and plan as the timed mixed run
+        ga, _, _, _ = timed_run("fp64", 1, 0, cfg=tuned, ncrit=ncrit_used)
+        ref = ga.arrays()
+        ga.close()
+        extra["mixed_vs_fp64_rel"] = {"u": rel(out["velocity"], ref["velocity"]), "J": rel(out["J"], ref["J"]),
+                                      "dadt": rel(out["dadt"], ref["dadt"])}
+        # speed: the FP64 path with its own autotuned leaf size
+        tuned64 = tune("fp64") if tuned is not None else None
+        g64, ms64, ph64, _ = timed_run("fp64", max(1, min(args.steps, 3)), 1, cfg=tuned64)
+        g64.close()
+        extra["fp64_autotune"] = tuned64
         extra["fp64_ms"] = round(ms64, 3)
         extra["fp64_phases_ms"] = {k: round(v, 3) for k, v in ph64.items()}
         extra["fp64_over_mixed"] = round(ms64 / ms, 3)
-        extra["mixed_vs_fp64_rel"] = {"u": rel(out["velocity"], ref["velocity"]), "J": rel(out["J"], ref["J"]),
-                                      "dadt": rel(out["dadt"], ref["dadt"])}
-        g64.close()
     if args.err_targets > 0:
         rng = np.random.default_rng(1003)
         idx = rng.choice(N, size=min(args.err_targets, N), replace=False)
@@ -181,7 +238,7 @@ def main():
     g.close()
 
     if args.adjoint_n > 0:
-        extra.update(adjoint_leg(args.adjoint_n, args.theta, args.ncrit, max(1, min(args.steps, 3))))
+        extra.update(adjoint_leg(args.adjoint_n, args.theta, args.ncrit or 64, max(1, min(args.steps, 3))))
         print(f"[bench] adjoint: {extra['adjoint']}", file=sys.stderr, flush=True)
 
     cpu = None
@@ -191,7 +248,7 @@ def main():
         Xc, Ac, Sc = W.scaled("leapfrog", args.cpu_n)
         threads = O.max_threads()
         b = O.make_bodies(Xc, Ac, Sc)
-        c = O.Calculator(P=args.P, theta=args.theta, ncrit=args.ncrit)
+        c = O.Calculator(P=args.P, theta=args.theta, ncrit=ncrit_used)
         t0 = time.perf_counter()
         c.init(b)
         c.calculate()
@@ -207,8 +264,8 @@ def main():
             "dtype": "f32 P2P + f64 expansions" if args.precision == "mixed" else "f64",
             "data": "synthetic (seeded leapfrogging vortex rings)",
             "config": {"workload": f"leapfrogging vortex rings, {N} particles, P={args.P}, theta={args.theta}, "
-                                   f"ncrit={args.ncrit}, {args.precision}",
-                       "particles": N, "P": args.P, "theta": args.theta, "ncrit": args.ncrit,
+                                   f"ncrit={ncrit_used}{' (autotuned)' if tuned else ''}, {args.precision}",
+                       "particles": N, "P": args.P, "theta": args.theta, "ncrit": ncrit_used,
                        "precision": args.precision, "parallelism": "single GPU"},
             "roofline": roofline, "cpu_baseline": cpu, "extra": extra}
     print(json.dumps(line), flush=True)

@@ -69,6 +69,8 @@ typedef struct uvlm_timing {
   int64_t n_interactions; /* sum over P2P pairs of n_target * n_source */
   double  sensor_ms;      /* last uvlm_sensors_aos: probe tree, lists, M2L, P2P, L2L, L2P */
   double  adjoint_ms;     /* last uvlm_probe_adjoint, same stages transposed */
+  int64_t ncrit;          /* leaf size in use (uvlm_autotune may change it) */
+  int64_t m2l_slots;      /* M2L accumulator slots per lane of the plan in use */
 } uvlm_timing;
 
 typedef struct uvlm_handle_s *uvlm_handle;
@@ -160,6 +162,16 @@ int uvlm_relax(uvlm_handle h, double f);
 int uvlm_diagnostics(uvlm_handle h, double *out);
 
 int uvlm_timings(uvlm_handle h, uvlm_timing *out);
+
+/* Chooses ncrit and the M2L plan size for the bodies currently on the handle by timing
+ * uvlm_build + uvlm_eval: each candidate ncrit[0..n) with each accumulator-slot budget in
+ * {5, 6, 8} (budgets that give the same plan are timed once), one warm-up and `reps` timed
+ * passes each.  The fastest pair is kept on the handle for later builds; *best_ncrit and
+ * *best_ms (mean milliseconds of build + eval) report it.  The bodies are left permuted by
+ * the last trial build, and the handle needs a uvlm_build before its next uvlm_eval.
+ * No counterpart in the reference, where ncrit is a compile-time constant
+ * (src/exafmmTypes.hpp:14). */
+int uvlm_autotune(uvlm_handle h, const int64_t *ncrit, int n, int reps, int64_t *best_ncrit, double *best_ms);
 
 /* Device synchronisation of the handle's stream. */
 int uvlm_sync(uvlm_handle h);

@@ -300,3 +300,26 @@ def test_relaxation_and_diagnostics():
     assert np.allclose(np.linalg.norm(a1, axis=1), np.linalg.norm(a, axis=1), rtol=1e-12)
     cosang = np.einsum("ij,ij->i", a1, w) / (np.linalg.norm(a1, axis=1) * np.linalg.norm(w, axis=1))
     assert np.allclose(cosang, 1.0, atol=1e-12)
+
+
+def test_autotune_keeps_a_candidate_and_the_results_stay_exact(orc):
+    """uvlm_autotune picks one of the candidate leaf sizes; a build + eval with it matches the
+    oracle run at that ncrit to the FP64 bar."""
+    X, A, S = W.scaled("cube", 20000)
+    g = FMMCalculator(P=8, theta=0.5, ncrit=64)
+    g.set_arrays(X, A, S)
+    best, ms = g.autotune(ncrits=(32, 64, 128), reps=1)
+    assert best in (32, 64, 128) and ms > 0
+    b_gpu = make_bodies(X, A, S)
+    g.init(b_gpu)
+    g.calculate()
+    t = g.timings()
+    assert t["ncrit"] == best and t["m2l_slots"] in (5, 6, 8)
+    out = g.copyBodies2Host()
+    b_ref = orc.make_bodies(X, A, S)
+    c = orc.Calculator(P=8, theta=0.5, ncrit=best)
+    c.init(b_ref)
+    c.calculate()
+    ref = c.bodies(len(X))
+    assert rel(out[:, 15:18], ref[:, 15:18]) < 1e-10 and rel(out[:, 0:9], ref[:, 0:9]) < 1e-10
+

@@ -43,6 +43,7 @@ def test_null_arguments_are_errors():
     assert L.uvlm_destroy(None) == 0
     assert L.uvlm_build(None) == 1
     assert L.uvlm_eval(None) == 1
+    assert L.uvlm_autotune(None, None, 0, 1, None, None) == 1
 
 
 def test_body_layout_matches_exafmm_body():

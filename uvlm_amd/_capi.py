@@ -17,7 +17,7 @@ EXPORTS = [
     "uvlm_get_bodies_aos", "uvlm_set_bodies_soa", "uvlm_get_bodies_soa", "uvlm_get_permutation",
     "uvlm_build", "uvlm_eval", "uvlm_stage", "uvlm_advance", "uvlm_num_cells", "uvlm_get_tree",
     "uvlm_get_lists", "uvlm_get_expansions", "uvlm_sensors_aos", "uvlm_direct", "uvlm_probe_adjoint",
-    "uvlm_update_bodies_soa", "uvlm_relax", "uvlm_diagnostics", "uvlm_timings", "uvlm_sync",
+    "uvlm_update_bodies_soa", "uvlm_relax", "uvlm_diagnostics", "uvlm_timings", "uvlm_sync", "uvlm_autotune",
 ]
 
 
@@ -39,7 +39,7 @@ class Timing(ctypes.Structure):
                 ("eval_ms", ctypes.c_double), ("n_bodies", ctypes.c_int64), ("n_cells", ctypes.c_int64),
                 ("n_leaves", ctypes.c_int64), ("n_p2p_pairs", ctypes.c_int64), ("n_m2l_pairs", ctypes.c_int64),
                 ("n_levels", ctypes.c_int64), ("n_interactions", ctypes.c_int64), ("sensor_ms", ctypes.c_double),
-                ("adjoint_ms", ctypes.c_double)]
+                ("adjoint_ms", ctypes.c_double), ("ncrit", ctypes.c_int64), ("m2l_slots", ctypes.c_int64)]
 
 
 _lib = None
@@ -81,6 +81,7 @@ def lib():
             "uvlm_diagnostics": (I, [VP, PD]),
             "uvlm_timings": (I, [VP, ctypes.POINTER(Timing)]),
             "uvlm_sync": (I, [VP]),
+            "uvlm_autotune": (I, [VP, PI64, I, I, PI64, PD]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -204,22 +204,7 @@ int uvlm_create(const uvlm_params *p, uvlm_handle *out)
     UVLM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     for (auto &e : h->ev) UVLM_HIP(hipEventCreate(&e));
     for (auto &e : h->aev) UVLM_HIP(hipEventCreate(&e));
-    // M2L lane schedule shape: window of W consecutive outputs, K items per lane
-    // (UVLM_M2L_W in {1, 2} selects the shapes (1,4), (2,3); UVLM_M2L_K=2 gives (2,2); UVLM_M2L_PF=1 prefetches)
-    h->sched.W = 2;
-    h->sched.K = 3;
-    if (const char *w = std::getenv("UVLM_M2L_W")) {
-      const int v = std::atoi(w);
-      if (v == 1 || v == 2) h->sched.W = v;
-      h->sched.K = v == 1 ? 4 : 3;
-    }
-    if (const char *k = std::getenv("UVLM_M2L_K")) {
-      const int v = std::atoi(k);
-      if (v == 2 && h->sched.W == 2) h->sched.K = 2;
-    }
-    if (const char *pf = std::getenv("UVLM_M2L_PF")) h->sched.prefetch = std::atoi(pf) != 0 && h->sched.K != 3;
-    // UVLM_M2L_KERNEL=items selects the lane-item kernel; UVLM_M2L_SLOTS sets the plan's slot budget
-    if (const char *kk = std::getenv("UVLM_M2L_KERNEL")) h->sched.kind = std::string(kk) == "items" ? 0 : 1;
+    // UVLM_M2L_SLOTS overrides the M2L plan's accumulator-slot budget (uvlm_autotune also sets it)
     if (const char *sb = std::getenv("UVLM_M2L_SLOTS")) h->sched.budget = std::atoi(sb);
     make_m2l_schedule(p->P, h->sched);
   });
@@ -578,6 +563,67 @@ int uvlm_probe_adjoint(uvlm_handle h, int64_t m, const double *probes, const dou
   });
 }
 
+int uvlm_autotune(uvlm_handle h, const int64_t *ncrit, int n, int reps, int64_t *best_ncrit, double *best_ms)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!ncrit || n <= 0 || reps <= 0) throw Error(UVLM_ERR_ARG, "empty candidate list or reps <= 0");
+    for (int i = 0; i < n; i++)
+      if (ncrit[i] < 1) throw Error(UVLM_ERR_ARG, "ncrit candidates must be >= 1");
+    if (h->N <= 0) throw Error(UVLM_ERR_STATE, "no bodies set");
+    set_device(h);
+    const int64_t keep_ncrit = h->prm.ncrit;
+    const int     keep_budget = h->sched.budget;
+    double        best = 1e300;
+    int64_t       bn = keep_ncrit;
+    int           bb = keep_budget;
+    hipEvent_t    e1 = h->ev[EV_DOWN];
+    try {
+      for (int b : {5, 6, 8}) {
+        M2LPlan probe;
+        if (!make_m2l_plan(h->prm.P, b, probe)) continue;
+        if (b != 5) {  // skip budgets whose plan equals the previous one
+          M2LPlan prev;
+          make_m2l_plan(h->prm.P, b == 6 ? 5 : 6, prev);
+          if (prev.S == probe.S && prev.slots == probe.slots && prev.lev == probe.lev) continue;
+        }
+        h->sched.budget = b;
+        make_m2l_schedule(h->prm.P, h->sched);
+        for (int i = 0; i < n; i++) {
+          h->prm.ncrit = ncrit[i];
+          do_build(h);
+          do_eval(h, true);
+          double tot = 0.0;
+          for (int r = 0; r < reps; r++) {
+            do_build(h);
+            do_eval(h, true);
+            UVLM_HIP(hipEventSynchronize(e1));
+            tot += ev_ms(h, EV_B0, EV_DOWN);
+          }
+          const double ms = tot / reps;
+          if (ms < best) {
+            best = ms;
+            bn   = ncrit[i];
+            bb   = b;
+          }
+        }
+      }
+    } catch (...) {
+      h->prm.ncrit    = keep_ncrit;
+      h->sched.budget = keep_budget;
+      make_m2l_schedule(h->prm.P, h->sched);
+      throw;
+    }
+    h->prm.ncrit    = bn;
+    h->sched.budget = bb;
+    make_m2l_schedule(h->prm.P, h->sched);
+    h->built     = false;
+    h->evaluated = false;
+    if (best_ncrit) *best_ncrit = bn;
+    if (best_ms) *best_ms = best;
+  });
+}
+
 int uvlm_relax(uvlm_handle h, double f)
 {
   return guarded([&] {
@@ -650,7 +696,9 @@ int uvlm_direct(uvlm_handle h, int64_t m, const double *xt, const double *at, do
 int uvlm_timings(uvlm_handle h, uvlm_timing *out)
 {
   if (!h || !out) return fail(UVLM_ERR_ARG, "null argument");
-  *out = h->tm;
+  *out           = h->tm;
+  out->ncrit     = h->prm.ncrit;
+  out->m2l_slots = h->sched.plan.S;
   return UVLM_OK;
 }
 

@@ -135,18 +135,10 @@ void count_interactions(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &
 
 // far.hip
 struct M2LSchedule {
-  int kind = 1;                 // 1: n-synchronous plan (k_m2l_ns); 0: lane items (k_m2l)
-  // kind 0
-  std::vector<int32_t> items;   // [16][K] packed (j, k0, w, nlo, nhi) or -1
-  std::vector<int32_t> out_off; // [P(P+1)/2 + 1]
-  std::vector<int32_t> out_src; // pieces: (lane*K + item)*W + q
-  int K = 0, W = 0;
-  bool prefetch = false;         // next source multipole prefetched into registers
-  // kind 1
-  int budget = 0;               // accumulator slots per lane (0: default)
+  int     budget = 0;  // accumulator slots per lane (0: default)
   M2LPlan plan;
-  DevBuf dev;                   // kind 0: items, out_off, out_src; kind 1: slots, lev, out_off, out_src
-  int64_t dev_items = 0, dev_out_off = 0, dev_out_src = 0, dev_lev = 0;
+  DevBuf  dev;         // slots, lev, out_off, out_src
+  int64_t dev_slots = 0, dev_lev = 0, dev_out_off = 0, dev_out_src = 0;
 };
 void make_m2l_schedule(int P, M2LSchedule &s);
 void zero_doubles(hipStream_t s, double *p, int64_t n);

@@ -18,13 +18,12 @@
 //   P2M   : one workgroup per leaf; chunks of bodies: lanes (body, column pair) fill the
 //           harmonics in LDS, then lane <-> (n,m) coefficient sums over the chunk
 //   M2M   : one workgroup per parent cell of a level; lane <-> output (j,k), loop over children
-//   M2L   : one workgroup per target cell; 4 groups of 16 lanes take every 4th source of
-//           the target's list; Y and the source multipole sit in LDS.  k_m2l_ns (default):
-//           the lanes of a group walk (n, m) in lock step, each adding the terms of the
-//           outputs bound to its accumulator slots (plan in m2l_plan.hpp).  k_m2l: each lane
-//           runs up to K "items" (an output row j, a window of W consecutive k, a range of
-//           source degrees n).  Partial sums are combined in a fixed order at the end, so
-//           results are deterministic and no atomics are used
+//   M2L   : one workgroup per target cell (or per list segment when a probe tree has few
+//           cells); 4 groups of 16 lanes take every 4th source of the list; the singular
+//           harmonics and the source multipole sit in LDS; the lanes of a group walk (n, m)
+//           in lock step, each adding the terms of the outputs bound to its accumulator
+//           slots (plan in m2l_plan.hpp).  Partial sums are combined in a fixed order at
+//           the end, so results are deterministic and no atomics are used
 //   L2L   : one workgroup per child cell of a level (gather from the parent)
 //   L2P   : one workgroup per leaf, lane <-> body
 #include "common.hpp"
@@ -341,178 +340,6 @@ __global__ __launch_bounds__(WG) void k_l2l(int P, int NT, const int32_t *__rest
 }
 
 // ------------------------------------------------------------------ M2L
-struct M2LItem {
-  int j, k0, w, nlo, nhi;
-};
-__host__ __device__ inline int32_t pack_item(int j, int k0, int w, int nlo, int nhi)
-{
-  return j | (k0 << 5) | (w << 10) | (nlo << 14) | (nhi << 19) | (1 << 24);
-}
-__device__ __forceinline__ M2LItem unpack_item(int32_t v)
-{
-  M2LItem it;
-  it.j   = v & 31;
-  it.k0  = (v >> 5) & 31;
-  it.w   = (v >> 10) & 15;
-  it.nlo = (v >> 14) & 31;
-  it.nhi = (v >> 19) & 31;
-  return it;
-}
-
-constexpr int M2L_PF = 12;  // source-multipole entries per lane prefetched into registers (NT <= 192)
-
-template <int W, int K, bool PF>
-__global__ __launch_bounds__(WG) void k_m2l(int P, int NT, int64_t ncell, const int32_t *__restrict__ off,
-                                            const int32_t *__restrict__ src, const double *__restrict__ tx,
-                                            const double *__restrict__ ty, const double *__restrict__ tz,
-                                            const double *__restrict__ sx, const double *__restrict__ sy,
-                                            const double *__restrict__ sz, const double2 *__restrict__ M,
-                                            double2 *__restrict__ L, const int32_t *__restrict__ items,
-                                            const int32_t *__restrict__ out_off, const int32_t *__restrict__ out_src)
-{
-  extern __shared__ double2 sm2[];
-  const int64_t t = blockIdx.x;
-  if (t >= ncell) return;
-  const int lane = threadIdx.x, g = lane >> 4, sl = lane & 15;
-  const int ny = tri(P), per = ny + NT;
-  double2  *Yg = sm2 + g * per;
-  double2  *Mg = Yg + ny;
-  M2LItem   it[K];
-  bool      valid[K];
-  for (int i = 0; i < K; i++) {
-    const int32_t v = items[sl * K + i];
-    valid[i]        = v >= 0;
-    it[i]           = unpack_item(valid[i] ? v : 0);
-  }
-  double2 acc[K][W][3];
-  for (int i = 0; i < K; i++)
-    for (int q = 0; q < W; q++)
-      for (int d = 0; d < 3; d++) acc[i][q][d] = make_double2(0, 0);
-  const int    s0 = off[t], nsrc = off[t + 1] - s0;
-  const double X = tx[t], Yc = ty[t], Z = tz[t];
-  // Optionally, the group's next source multipole is loaded into registers while the
-  // current one is used, so its latency overlaps the multiply-adds.
-  double2 pf[PF ? M2L_PF : 1];
-#define UVLM_M2L_PREFETCH(SI)                                                                                       \
-  if (PF && (SI) < nsrc) {                                                                                         \
-    const double2 *Ms_ = M + (int64_t)src[s0 + (SI)] * NT;                                                         \
-    _Pragma("unroll") for (int r = 0; r < (PF ? M2L_PF : 1); r++)                                                  \
-    {                                                                                                              \
-      const int q = sl + 16 * r;                                                                                   \
-      pf[r]       = q < NT ? Ms_[q] : make_double2(0, 0);                                                          \
-    }                                                                                                              \
-  }
-  UVLM_M2L_PREFETCH(g)
-  for (int base = 0; base < nsrc; base += 4) {
-    const int  si     = base + g;
-    const bool active = si < nsrc;
-    if (active) {
-      const int      s  = src[s0 + si];
-      const double2 *Ms = M + (int64_t)s * NT;
-      if (PF) {
-#pragma unroll
-        for (int r = 0; r < (PF ? M2L_PF : 1); r++) {
-          const int q = sl + 16 * r;
-          if (q < NT) Mg[q] = pf[r];
-        }
-        for (int q = sl + 16 * M2L_PF; q < NT; q += 16) Mg[q] = Ms[q];
-      } else {
-        for (int q = sl; q < NT; q += 16) Mg[q] = Ms[q];
-      }
-      if (sl < P) {
-        const Sph sp = sph(X - sx[s], Yc - sy[s], Z - sz[s]);
-        harmonic_column<true>(P, sl, sp, Yg);
-      }
-    }
-    __syncthreads();
-    UVLM_M2L_PREFETCH(si + 4)
-    if (active) {
-      for (int i = 0; i < K; i++) {
-        if (!valid[i]) continue;
-        const M2LItem I = it[i];
-        for (int n = I.nlo; n <= I.nhi; n++) {
-          const int      J  = I.j + n;
-          const double2 *YJ = Yg + tri(J);
-          const double2 *Mn = Mg + 3 * tri(n);
-          // Terms m and -m are taken together (multipoleKernels.hpp:534-546):
-          //   conj(M[n,m]) Y[J,-m-k] + s(k,m) M[n,m] Y[J,m-k] = a (Y1 + Y2) + i b (Y2 - Y1)
-          // with M = a + ib, Y1 = Y[J,-m-k], Y2 = s Y[J,m-k], s = (-1)^k if k < m else (-1)^m.
-          // Window slot q holds output k = k0 + q: yA[q] = Y[J, m-k] (slides up with m),
-          // yB[q] = Y[J, -m-k] (slides down); both start at Y[J, -k] for m = 0.
-          double2 yA[W], yB[W];
-          for (int q = 0; q < W; q++) {
-            const int a = min(I.k0 + q, J);
-            yA[q] = yB[q] = conj2(YJ[a]);
-          }
-          for (int d = 0; d < 3; d++) {
-            const double2 mv = Mn[d];
-            for (int q = 0; q < W; q++) cfma(acc[i][q][d], mv, yA[q]);
-          }
-#pragma unroll 2
-          for (int m = 1; m <= n; m++) {
-            for (int q = W - 1; q > 0; q--) yA[q] = yA[q - 1];
-            {
-              const int mu = m - I.k0;
-              yA[0]        = mu >= 0 ? YJ[mu] : conj2(YJ[-mu]);
-            }
-            for (int q = 0; q + 1 < W; q++) yB[q] = yB[q + 1];
-            yB[W - 1] = conj2(YJ[min(m + I.k0 + W - 1, J)]);
-            double2 S[W], D[W];
-            for (int q = 0; q < W; q++) {
-              const int     k  = I.k0 + q;
-              const int     e  = k < m ? k : m;
-              const double2 y2 = (e & 1) ? make_double2(-yA[q].x, -yA[q].y) : yA[q];
-              S[q]             = make_double2(y2.x + yB[q].x, y2.y + yB[q].y);
-              D[q]             = make_double2(y2.x - yB[q].x, y2.y - yB[q].y);
-            }
-            for (int d = 0; d < 3; d++) {
-              const double2 mv = Mn[3 * m + d];
-              for (int q = 0; q < W; q++) {
-                double2 &a = acc[i][q][d];
-                a.x        = fma(mv.x, S[q].x, a.x);
-                a.x        = fma(-mv.y, D[q].y, a.x);
-                a.y        = fma(mv.x, S[q].y, a.y);
-                a.y        = fma(mv.y, D[q].x, a.y);
-              }
-            }
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
-#undef UVLM_M2L_PREFETCH
-  // combine the 4 source groups (same schedule slot in lanes sl, sl+16, sl+32, sl+48)
-  for (int i = 0; i < K; i++)
-    for (int q = 0; q < W; q++)
-      for (int d = 0; d < 3; d++) {
-        double2 &a = acc[i][q][d];
-        a.x += __shfl_xor(a.x, 16, 64);
-        a.y += __shfl_xor(a.y, 16, 64);
-        a.x += __shfl_xor(a.x, 32, 64);
-        a.y += __shfl_xor(a.y, 32, 64);
-      }
-  double2 *part = sm2;  // [16][K][W][3]
-  if (g == 0)
-    for (int i = 0; i < K; i++)
-      for (int q = 0; q < W; q++)
-        for (int d = 0; d < 3; d++) part[((sl * K + i) * W + q) * 3 + d] = acc[i][q][d];
-  __syncthreads();
-  for (int q = lane; q < ny; q += WG) {
-    int j = 0;
-    while (tri(j + 1) <= q) j++;
-    double2 r[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-    for (int e = out_off[q]; e < out_off[q + 1]; e++) {
-      const int pc = out_src[e];
-      for (int d = 0; d < 3; d++) {
-        r[d].x += part[pc * 3 + d].x;
-        r[d].y += part[pc * 3 + d].y;
-      }
-    }
-    const double sj = sgn(j);
-    for (int d = 0; d < 3; d++) L[t * NT + 3 * q + d] = make_double2(sj * r[d].x, sj * r[d].y);
-  }
-}
 
 // The n-synchronous M2L (plan in m2l_plan.hpp).  One workgroup (one wavefront) per target
 // cell; its 4 groups of 16 lanes take every 4th source.  All lanes of a group walk the source
@@ -811,99 +638,18 @@ __global__ void k_zero(double *p, int64_t n)
 // to minimise the largest slot load.
 void make_m2l_schedule(int P, M2LSchedule &s)
 {
-  if (s.kind == 1) {
-    if (!make_m2l_plan(P, s.budget, s.plan)) throw Error(UVLM_ERR_INTERNAL, "no M2L plan");
-    const M2LPlan &pl = s.plan;
-    s.dev_items       = 0;
-    s.dev_lev         = pl.slots.size();
-    s.dev_out_off     = s.dev_lev + pl.lev.size();
-    s.dev_out_src     = s.dev_out_off + pl.out_off.size();
-    std::vector<int32_t> all;
-    all.insert(all.end(), pl.slots.begin(), pl.slots.end());
-    all.insert(all.end(), pl.lev.begin(), pl.lev.end());
-    all.insert(all.end(), pl.out_off.begin(), pl.out_off.end());
-    all.insert(all.end(), pl.out_src.begin(), pl.out_src.end());
-    s.dev.reserve(4 * all.size());
-    UVLM_HIP(hipMemcpy(s.dev.p, all.data(), 4 * all.size(), hipMemcpyHostToDevice));
-    return;
-  }
-  const int W = s.W, K = s.K, LANES = 16;
-  struct It {
-    int j, k0, w, nlo, nhi, cost;
-  };
-  auto items_for = [&](int cap) {
-    std::vector<It> v;
-    for (int j = 0; j < P; j++)
-      for (int k0 = 0; k0 <= j; k0 += W) {
-        const int w = std::min(W, j + 1 - k0);
-        int       n = 0;
-        while (n <= P - 1 - j) {
-          int c = 0, lo = n;
-          while (n <= P - 1 - j && (c == 0 || c + W * (n + 1) <= cap)) {
-            c += W * (n + 1);
-            n++;
-          }
-          v.push_back({j, k0, w, lo, n - 1, c});
-        }
-      }
-    return v;
-  };
-  int              best_load = 1 << 30;
-  std::vector<It>  best_items;
-  std::vector<int> best_lane;
-  for (int cap = 1; cap <= W * P * P + 1; cap++) {
-    auto v = items_for(cap);
-    std::stable_sort(v.begin(), v.end(), [](const It &a, const It &b) { return a.cost > b.cost; });
-    std::vector<int> load(LANES, 0), cnt(LANES, 0), lane(v.size(), -1);
-    bool             ok = true;
-    for (size_t q = 0; q < v.size(); q++) {
-      int bl = -1;
-      for (int l = 0; l < LANES; l++)
-        if (cnt[l] < K && (bl < 0 || load[l] < load[bl])) bl = l;
-      if (bl < 0) {
-        ok = false;
-        break;
-      }
-      load[bl] += v[q].cost;
-      cnt[bl]++;
-      lane[q] = bl;
-    }
-    if (!ok) continue;
-    const int mx = *std::max_element(load.begin(), load.end());
-    if (mx < best_load) {
-      best_load  = mx;
-      best_items = v;
-      best_lane  = lane;
-    }
-  }
-  if (best_items.empty()) throw Error(UVLM_ERR_INTERNAL, "no M2L schedule");
-  s.items.assign(LANES * K, -1);
-  std::vector<int> cnt(LANES, 0);
-  const int        ny = P * (P + 1) / 2;
-  std::vector<std::vector<int>> pieces(ny);
-  for (size_t q = 0; q < best_items.size(); q++) {
-    const It &I    = best_items[q];
-    const int l    = best_lane[q];
-    const int slot = cnt[l]++;
-    s.items[l * K + slot] = pack_item(I.j, I.k0, I.w, I.nlo, I.nhi);
-    for (int w = 0; w < I.w; w++) pieces[I.j * (I.j + 1) / 2 + I.k0 + w].push_back((l * K + slot) * W + w);
-  }
-  s.out_off.assign(ny + 1, 0);
-  s.out_src.clear();
-  for (int q = 0; q < ny; q++) {
-    std::sort(pieces[q].begin(), pieces[q].end());
-    for (int v : pieces[q]) s.out_src.push_back(v);
-    s.out_off[q + 1] = (int)s.out_src.size();
-  }
-  s.dev_items   = 0;
-  s.dev_out_off = s.items.size();
-  s.dev_out_src = s.dev_out_off + s.out_off.size();
-  const size_t total = s.dev_out_src + s.out_src.size();
-  s.dev.reserve(4 * total);
+  if (!make_m2l_plan(P, s.budget, s.plan)) throw Error(UVLM_ERR_INTERNAL, "no M2L plan");
+  const M2LPlan &pl = s.plan;
+  s.dev_slots       = 0;
+  s.dev_lev         = pl.slots.size();
+  s.dev_out_off     = s.dev_lev + pl.lev.size();
+  s.dev_out_src     = s.dev_out_off + pl.out_off.size();
   std::vector<int32_t> all;
-  all.insert(all.end(), s.items.begin(), s.items.end());
-  all.insert(all.end(), s.out_off.begin(), s.out_off.end());
-  all.insert(all.end(), s.out_src.begin(), s.out_src.end());
+  all.insert(all.end(), pl.slots.begin(), pl.slots.end());
+  all.insert(all.end(), pl.lev.begin(), pl.lev.end());
+  all.insert(all.end(), pl.out_off.begin(), pl.out_off.end());
+  all.insert(all.end(), pl.out_src.begin(), pl.out_src.end());
+  s.dev.reserve(4 * all.size());
   UVLM_HIP(hipMemcpy(s.dev.p, all.data(), 4 * all.size(), hipMemcpyHostToDevice));
 }
 
@@ -954,13 +700,10 @@ constexpr int64_t M2L_SPLIT_TARGETS = 4096, M2L_SPLIT_SEGMENTS = 8192;
 void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, const M2LSchedule &sch, const double2 *M,
          double2 *L, Scratch *sc)
 {
-  const int NT = 3 * P * (P + 1) / 2, ny = P * (P + 1) / 2;
+  const int NT = 3 * P * (P + 1) / 2;
   if (tt.C == 0) return;
-  const size_t lds_main = sizeof(double2) * 4 * (ny + NT);
-  const size_t lds_part = sizeof(double2) * 16 * sch.K * sch.W * 3;
-  const size_t lds      = std::max(lds_main, lds_part);
-  const int32_t *d      = sch.dev.as<int32_t>();
-  if (sch.kind == 1) {
+  const int32_t *d = sch.dev.as<int32_t>();
+  {
     const int    S      = sch.plan.S;
     const size_t lds_ns = std::max(sizeof(double2) * 4 * (P * P + NT), sizeof(double2) * 16 * S * 3);
     // few targets with long lists: one workgroup per list segment, partial expansions reduced
@@ -1003,12 +746,12 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
       k_m2l_ns<SS, true><<<grid, WG, lds_ns, s>>>(                                                                 \
           P, NT, units, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(), tt.cx.as<double>(),            \
           tt.cy.as<double>(), tt.cz.as<double>(), st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M,   \
-          out, d + sch.dev_items, d + sch.dev_lev, d + sch.dev_out_off, d + sch.dev_out_src, dseg);               \
+          out, d + sch.dev_slots, d + sch.dev_lev, d + sch.dev_out_off, d + sch.dev_out_src, dseg);               \
     else                                                                                                           \
       k_m2l_ns<SS><<<grid, WG, lds_ns, s>>>(P, NT, tt.C, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),  \
                                             tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),             \
                                             st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, L,       \
-                                            d + sch.dev_items, d + sch.dev_lev, d + sch.dev_out_off,               \
+                                            d + sch.dev_slots, d + sch.dev_lev, d + sch.dev_out_off,               \
                                             d + sch.dev_out_src);                                                  \
     UVLM_LAUNCH_CHECK();                                                                                           \
     if (split) {                                                                                                   \
@@ -1026,22 +769,6 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
 #undef UVLM_M2L_NS
     throw Error(UVLM_ERR_INTERNAL, "unsupported M2L plan size");
   }
-#define UVLM_M2L_CASE(WW, KK, PP)                                                                                  \
-  if (sch.W == WW && sch.K == KK && sch.prefetch == PP) {                                                          \
-    k_m2l<WW, KK, PP><<<(int)tt.C, WG, lds, s>>>(P, NT, tt.C, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),       \
-                                             tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),           \
-                                             st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, L,     \
-                                             d + sch.dev_items, d + sch.dev_out_off, d + sch.dev_out_src);         \
-    UVLM_LAUNCH_CHECK();                                                                                           \
-    return;                                                                                                        \
-  }
-  UVLM_M2L_CASE(1, 4, false)
-  UVLM_M2L_CASE(1, 4, true)
-  UVLM_M2L_CASE(2, 2, false)
-  UVLM_M2L_CASE(2, 2, true)
-  UVLM_M2L_CASE(2, 3, false)
-#undef UVLM_M2L_CASE
-  throw Error(UVLM_ERR_INTERNAL, "unsupported M2L schedule shape");
 }
 
 void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *st, int64_t cap, int stretching,

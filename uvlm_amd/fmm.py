@@ -221,5 +221,15 @@ class FMMCalculator:
         _capi.check(self.lib.uvlm_timings(self._h, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
 
+    def autotune(self, ncrits=(64, 96, 128, 160, 192, 256), reps=2):
+        """Time build + eval on the bodies already set for each candidate ncrit and M2L plan
+        size; keep the fastest on the handle.  Returns (ncrit, ms).  Needs a build afterwards."""
+        c = np.ascontiguousarray(ncrits, dtype=np.int64)
+        best = ctypes.c_int64()
+        ms = ctypes.c_double()
+        _capi.check(self.lib.uvlm_autotune(self._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(c),
+                                           int(reps), ctypes.byref(best), ctypes.byref(ms)))
+        return int(best.value), float(ms.value)
+
     def sync(self):
         _capi.check(self.lib.uvlm_sync(self._h))
