@@ -16,6 +16,7 @@
 #ifndef UVLM_FMM_CALCULATOR_HPP
 #define UVLM_FMM_CALCULATOR_HPP
 
+#include <array>
 #include <cstddef>
 #include <stdexcept>
 #include <string>
@@ -237,6 +238,36 @@ struct FMMCalculator {
     return t;
   }
   uvlm_handle handle() const { return h_.h; }
+
+  // Calls with no counterpart in the reference class (include/uvlm_fmm.h gives the reasons).
+  // Pedrizzetti relaxation toward curl u of the last calculate(), and the invariants
+  // {impulse[3], angular impulse[3], energy, sum alpha[3]} of the current particles.
+  void relax(exafmm::real_t f) { exafmm::detail::check(uvlm_relax(h_.h, f)); }
+  std::array<double, 10> diagnostics() const
+  {
+    std::array<double, 10> out{};
+    exafmm::detail::check(uvlm_diagnostics(h_.h, out.data()));
+    return out;
+  }
+  // dJ/dalpha and dJ/dx ([n][3] each, tree order) of J = sum_p w_p . u(probe_p).
+  void probeAdjoint(const std::vector<double> &probes, const std::vector<double> &w, std::vector<double> &dJ_dalpha,
+                    std::vector<double> &dJ_dx)
+  {
+    if (probes.size() != w.size() || probes.size() % 3) throw std::runtime_error("probes and w must be [m][3]");
+    dJ_dalpha.assign(3 * n_, 0.0);
+    dJ_dx.assign(3 * n_, 0.0);
+    exafmm::detail::check(uvlm_probe_adjoint(h_.h, (int64_t)(probes.size() / 3), probes.data(), w.data(),
+                                             dJ_dalpha.data(), dJ_dx.data()));
+  }
+  // Leaf size (and M2L plan size) for the bodies last passed to init()/copyBodies2Device();
+  // returns the chosen ncrit.  init() again before calculate().
+  size_t autotune(const std::vector<int64_t> &candidates, int reps = 2)
+  {
+    int64_t best = 0;
+    double  ms   = 0.0;
+    exafmm::detail::check(uvlm_autotune(h_.h, candidates.data(), (int)candidates.size(), reps, &best, &ms));
+    return (size_t)best;
+  }
 
 private:
   static uvlm_params detail_params(int P, double theta, size_t ncrit, Precision precision, int device)

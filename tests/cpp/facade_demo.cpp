@@ -2,6 +2,7 @@
 // driver uses Vortex::FMMCalculator (src/main.cpp:119, 292): instantiate with tags,
 // init + calculate, then advance.  Reads exafmm::Body records from argv[1], writes the
 // bodies after calculate() to argv[2] and after `steps` RK3 steps to argv[3].
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -50,8 +51,14 @@ int main(int argc, char **argv)
     const int steps = std::atoi(argv[4]);
     for (int s = 0; s < steps; s++) fmm.advance(particles, 0.01);
     write_bodies(argv[3], particles);
+    // calls beyond the reference class: invariants and a no-op relaxation (f = 0)
+    fmm.init(particles);
+    fmm.calculate();
+    const std::array<double, 10> inv = fmm.diagnostics();
+    fmm.relax(0.0);
     exafmm::Cells cells = exafmm::buildTree(particles, 64);
-    std::printf("facade ok: %zu bodies, %zu cells, root NBODY %zu\n", particles.size(), cells.size(), cells[0].NBODY);
+    std::printf("facade ok: %zu bodies, %zu cells, root NBODY %zu, energy %.6e\n", particles.size(), cells.size(),
+                cells[0].NBODY, inv[6]);
   } catch (const std::exception &e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;
