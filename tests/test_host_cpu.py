@@ -98,3 +98,30 @@ def test_p2p_polynomial_matches_erf_form():
         pp = sum(i * c * t ** (i - 1) for i, c in enumerate(coef) if i)
         assert abs(p - G) < 1e-9 * abs(G) + 1e-12
         assert abs(pp - Gp) < 2e-8 * 0.3
+
+
+def test_p2p_far_form_matches_erf_form():
+    """The FP32 P2P's form for x = s^2 >= 2, read from near.hip: G = (1 - e (erfcx(s) +
+    2s/sqrt(pi))) w^3 and G' = w^2 (2e/sqrt(pi) - 1.5 G) with w = 1/s, e = e^{-x} and erfcx by
+    the c_far polynomial in w, against the erf form evaluated in FP64."""
+    import math
+    src = open(os.path.join(ROOT, "uvlm_amd", "csrc", "near.hip")).read()
+    body = src[src.index("c_far[NFAR] = {") + 15: src.index("};", src.index("c_far[NFAR] = {"))]
+    coef = [float(v.strip().rstrip("f")) for v in body.split(",")]
+    wc = float(re.search(r"FAR_WC\s*=\s*([0-9.eE+-]+)f", src).group(1))
+    wmin = float(re.search(r"FAR_WMIN\s*=\s*([0-9.eE+-]+)f", src).group(1))
+    assert len(coef) == 8
+    for x in list(np.linspace(2.0, 30.0, 300)) + [50.0, 200.0, 1e4]:
+        s = math.sqrt(x)
+        w = 1 / s
+        e = math.exp(-x)
+        t = max(w, wmin) - wc
+        p = sum(c * t ** i for i, c in enumerate(coef))
+        g = 1 - e * (p + 2 * s / math.sqrt(math.pi))
+        G = g * w ** 3
+        Gp = w * w * (2 * e / math.sqrt(math.pi) - 1.5 * G)
+        gr = math.erf(s) - 2 * s * e / math.sqrt(math.pi)
+        Gr = gr / s ** 3
+        Gpr = (s * 4 * x * e / math.sqrt(math.pi) - 3 * gr) / (2 * s ** 5)
+        assert abs(G - Gr) < 2e-8 * abs(Gr)
+        assert abs(Gp - Gpr) < 5e-8 * abs(Gpr)

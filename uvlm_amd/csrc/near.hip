@@ -25,6 +25,15 @@
 // its degree-9 Chebyshev interpolant (interpolation at the Chebyshev nodes of [0, 2],
 // re-expanded in powers of t = x - 1): max error 8e-11 (G) and 5e-9 (G') in exact
 // arithmetic, 1.4e-7 and 1.1e-7 when evaluated by Horner in FP32.
+// For x >= 2 the same G and G' come from the hardware rsq/exp2 and a short polynomial:
+// with w = x^{-1/2} = 1/s and e = e^{-x},
+//   g = erf(s) - (2/sqrt(pi)) s e = 1 - e (erfcx(s) + (2/sqrt(pi)) s),
+//   G = g w^3,   G' = w^2 ((2/sqrt(pi)) e - (3/2) G)
+// (from G' = (2/sqrt(pi)) s e x^{-3/2} - (3/2) G / x), where erfcx(s) = e^{s^2} erfc(s) is a
+// degree-7 least-squares polynomial in w - 0.4647 on w in [1/4.5, 1/sqrt(2)] (max error
+// 1e-8; beyond s = 4.5 the factor e < 2e-9 makes it irrelevant, so w is clamped there).
+// In FP32 this form is as accurate as erff/expf (4.3e-7 on G, 8.1e-7 on G') for about a
+// third of the instructions.
 #include "common.hpp"
 
 #include <vector>
@@ -45,10 +54,20 @@ __constant__ float c_gpoly[NPOLY] = {4.27593295550599495e-01f,  -2.2628244588289
                                      8.69427232794296300e-05f,  -1.08135023163257920e-05f, 1.22398144588401693e-06f,
                                      -1.21730778572683114e-07f};
 
+// erfcx(1/w) ~ sum_k c_far[k] (w - FAR_WC)^k, w in [FAR_WMIN, 1/sqrt(2)] (see above).
+constexpr int   NFAR     = 8;
+constexpr float FAR_WC   = 0.46466450170438484f;
+constexpr float FAR_WMIN = 0.2222222222222222f;
+__constant__ float c_far[NFAR] = {2.400683016e-01f,  4.403707087e-01f, -1.874301285e-01f, 1.258202083e-02f,
+                                  8.160093427e-02f, -1.016800180e-01f, 5.416155234e-02f,  2.011496387e-02f};
+
 struct SrcF64 {
   double x, y, z, ax, ay, az, invS2, halfInvS2;
 };
 
+// acc[0..2] += f1 c, acc[3..11] += f2 c (x) dX with c = dX x a, and acc[12..14] += f1 a: the
+// f1 eps_{abk} a_k part of J is antisymmetric, so its sum is kept as one vector and folded
+// into J once per chunk (fold_antisym) instead of six multiply-adds per pair.
 __device__ __forceinline__ void accumulate_pair(float *acc, float dx, float dy, float dz, float ax, float ay, float az,
                                                 float f1, float f2, bool withJ)
 {
@@ -59,15 +78,28 @@ __device__ __forceinline__ void accumulate_pair(float *acc, float dx, float dy, 
   if (withJ) {
     const float k0 = f2 * cx, k1 = f2 * cy, k2 = f2 * cz;
     acc[3]  = fmaf(k0, dx, acc[3]);
-    acc[4]  = fmaf(k0, dy, fmaf(f1, az, acc[4]));
-    acc[5]  = fmaf(k0, dz, fmaf(-f1, ay, acc[5]));
-    acc[6]  = fmaf(k1, dx, fmaf(-f1, az, acc[6]));
+    acc[4]  = fmaf(k0, dy, acc[4]);
+    acc[5]  = fmaf(k0, dz, acc[5]);
+    acc[6]  = fmaf(k1, dx, acc[6]);
     acc[7]  = fmaf(k1, dy, acc[7]);
-    acc[8]  = fmaf(k1, dz, fmaf(f1, ax, acc[8]));
-    acc[9]  = fmaf(k2, dx, fmaf(f1, ay, acc[9]));
-    acc[10] = fmaf(k2, dy, fmaf(-f1, ax, acc[10]));
+    acc[8]  = fmaf(k1, dz, acc[8]);
+    acc[9]  = fmaf(k2, dx, acc[9]);
+    acc[10] = fmaf(k2, dy, acc[10]);
     acc[11] = fmaf(k2, dz, acc[11]);
+    acc[12] = fmaf(f1, ax, acc[12]);
+    acc[13] = fmaf(f1, ay, acc[13]);
+    acc[14] = fmaf(f1, az, acc[14]);
   }
+}
+// J[a][b] += eps_{abk} F_k with F = acc[12..14] (row-major J in acc[3..11]).
+__device__ __forceinline__ void fold_antisym(float *acc)
+{
+  acc[4] += acc[14];
+  acc[5] -= acc[13];
+  acc[6] -= acc[14];
+  acc[8] += acc[12];
+  acc[9] += acc[13];
+  acc[10] -= acc[12];
 }
 
 constexpr int SEG = 256;  // source leaves whose prefix sums sit in LDS at once
@@ -179,14 +211,14 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
         }
         __syncthreads();
         if (MIXED) {
-          float fa[12];
-          for (int k = 0; k < 12; k++) fa[k] = 0.f;
+          float fa[15];
+          for (int k = 0; k < 15; k++) fa[k] = 0.f;
           for (int j = si; j < cnt; j += S) {
             const float4 A = s32[3 * j], B = s32[3 * j + 1], Cc = s32[3 * j + 2];
             const float  dx = rxi - A.x, dy = ryi - A.y, dz = rzi - A.z;
             const float  r2 = dx * dx + dy * dy + dz * dz;
             const float  xq = r2 * (TSIG ? t_q : A.w);
-            float        f1, f2;
+            float        G, Gp;  // G(x) and G'(x), x = xq
             if (xq < XPOLY) {
               const float tq = xq - 1.0f;
               float       g = c_gpoly[NPOLY - 1], gp = 0.f;
@@ -195,21 +227,27 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
                 gp = fmaf(gp, tq, g);
                 g  = fmaf(g, tq, c_gpoly[k]);
               }
-              f1 = g * (TSIG ? t_c1 : B.w);
-              f2 = gp * (TSIG ? t_c2 : Cc.x);
+              G  = g;
+              Gp = gp;
             } else {
-              const float ir  = rsqrtf(r2);
-              const float s   = sqrtf(xq);
-              const float t2  = 1.1283791670955126f * s * expf(-xq);
-              const float gg  = erff(s) - t2;
-              const float h   = 2.0f * xq * t2 - 3.0f * gg;
-              const float ir3 = ir * ir * ir * (float)INV_4PI;
-              f1              = gg * ir3;
-              f2              = h * ir3 * ir * ir;
+              const float w = __builtin_amdgcn_rsqf(xq);
+              const float e = __builtin_amdgcn_exp2f(xq * -1.4426950408889634f);
+              const float t = fmaxf(w, FAR_WMIN) - FAR_WC;
+              float       p = c_far[NFAR - 1];
+#pragma unroll
+              for (int k = NFAR - 2; k >= 0; k--) p = fmaf(p, t, c_far[k]);
+              const float g  = fmaf(-e, fmaf(1.1283791670955126f, xq * w, p), 1.0f);
+              const float w2 = w * w;
+              G              = g * w2 * w;
+              Gp             = w2 * fmaf(1.1283791670955126f, e, -1.5f * G);
             }
-            if (r2 == 0.0f) f1 = f2 = 0.f;
+            // coincident pairs are skipped (multipoleKernels.hpp:242): only the f1 alpha terms
+            // of J survive a zero dX, so zeroing f1 is enough
+            const float f1 = r2 == 0.0f ? 0.0f : G * (TSIG ? t_c1 : B.w);
+            const float f2 = Gp * (TSIG ? t_c2 : Cc.x);
             accumulate_pair(fa, dx, dy, dz, B.x, B.y, B.z, f1, f2, WITH_J);
           }
+          if (WITH_J) fold_antisym(fa);
           for (int k = 0; k < (WITH_J ? 12 : 3); k++) acc[k] += (double)fa[k];
         } else {
           for (int j = si; j < cnt; j += S) {
