@@ -4,6 +4,7 @@
 //   exafmm::Body / Bodies / Cell / Cells      (src/exafmmTypes.hpp:18-47, same Body layout)
 //   exafmm::buildTree(Bodies&)                (src/tree.hpp:101-113)
 //   exafmm::direct(Bodies&, Bodies&)          (src/traverse.hpp:88-98)
+//   exafmm::writeVTK(Bodies, path)            (the driver's writeTovtk, src/main.cpp:127-164)
 //   Vortex tags Inviscid/PSE/CSM, Euler/RK2/RK3/RK4, cVPM/rVPM, Transposed/Classic
 //                                             (src/fmmCalculator.hpp:9-23)
 //   Vortex::FMMCalculator<Host, Device, Integration, Viscous, Formulation, Stretching>
@@ -18,6 +19,7 @@
 
 #include <array>
 #include <cstddef>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -144,6 +146,28 @@ inline void direct(Bodies &bodies, Bodies &jbodies, int device = 0)
       bodies[i].velocity[d] += u[3 * i + d];
       bodies[i].dadt[d] += da[3 * i + d];
     }
+}
+// Legacy-ASCII VTK POLYDATA of the particles (positions, alpha, velocity, radius), the file
+// the reference's driver writes per step (src/main.cpp:127-164): same sections, same order,
+// default ostream number formatting.  Host-only; returns false if the file cannot be opened.
+inline bool writeVTK(const Bodies &particles, const std::string &path)
+{
+  std::ofstream file(path);
+  if (!file) return false;
+  const size_t n = particles.size();
+  file << "# vtk DataFile Version 3.0\n" << "vtk output\n" << "ASCII\n" << "DATASET POLYDATA\n";
+  file << "POINTS " << n << " double\n";
+  for (const Body &b : particles) file << b.X[0] << " " << b.X[1] << " " << b.X[2] << "\n";
+  file << "VERTICES " << n << " " << 2 * n << "\n";
+  for (size_t i = 0; i < n; i++) file << "1 " << i << "\n";
+  file << "POINT_DATA " << n << "\n";
+  file << "VECTORS alpha double\n";
+  for (const Body &b : particles) file << b.alpha[0] << " " << b.alpha[1] << " " << b.alpha[2] << "\n";
+  file << "VECTORS velocity double\n";
+  for (const Body &b : particles) file << b.velocity[0] << " " << b.velocity[1] << " " << b.velocity[2] << "\n";
+  file << "SCALARS radius double\n" << "LOOKUP_TABLE default\n";
+  for (const Body &b : particles) file << b.radius << "\n";
+  return (bool)file;
 }
 }  // namespace exafmm
 
