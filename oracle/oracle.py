@@ -27,6 +27,7 @@ BODY_DOUBLES = 37
 
 STRETCH = {"transposed": 0, "classic": 1}
 FORM = {"cvpm": 0, "rvpm": 1}
+VISCOUS = {"inviscid": 0, "pse": 1, "csm": 2}
 SCHEME = {"euler": 0, "rk2": 1, "rk3": 2, "rk4": 3}
 
 _lib = None
@@ -56,6 +57,10 @@ def lib():
             "orc_get_bodies": (I, [VP, P, L64]),
             "orc_advance": (I, [VP, P, L64, D, I]),
             "orc_stage": (I, [VP, P, L64, D, I, I]),
+            "orc_set_viscosity": (None, [VP, I, D]),
+            "orc_viscosity": (I, [VP, D]),
+            "orc_vorticity": (I, [VP, P, L64]),
+            "orc_rbf": (I, [VP, P, L64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), P]),
             "orc_sensor_data": (I, [VP, P, L64]),
             "orc_num_cells": (L64, [VP]),
             "orc_get_cells": (None, [VP, P64, P]),
@@ -157,6 +162,33 @@ class Calculator:
         st = lib().orc_stage(self._h, _p(bodies), bodies.shape[0], dt, SCHEME[scheme], k)
         if st:
             raise RuntimeError(f"orc_stage failed ({st})")
+
+    def set_viscosity(self, scheme, nu):
+        """scheme: "inviscid", "pse" or "csm"; applied after calculate() in advance/stage."""
+        lib().orc_set_viscosity(self._h, VISCOUS[scheme], float(nu))
+
+    def viscosity(self, dt):
+        """calculateViscosity on the current bodies (after calculate())."""
+        st = lib().orc_viscosity(self._h, float(dt))
+        if st:
+            raise RuntimeError(f"orc_viscosity failed ({st})")
+
+    def vorticity(self, sensors):
+        """Gaussian vorticity (negated, into velocity_old) at the probes, near pairs only."""
+        st = lib().orc_vorticity(self._h, _p(sensors), sensors.shape[0])
+        if st:
+            raise RuntimeError(f"orc_vorticity failed ({st})")
+
+    def rbf(self, points):
+        """RBF re-initialisation onto `points` (n, 37); returns (kept, iterations, residuals):
+        points[:kept] hold the kept points in tree order with the fitted alpha."""
+        kept = ctypes.c_int64()
+        it = ctypes.c_int()
+        res = np.zeros(3)
+        st = lib().orc_rbf(self._h, _p(points), points.shape[0], ctypes.byref(kept), ctypes.byref(it), _p(res))
+        if st:
+            raise RuntimeError(f"orc_rbf failed ({st})")
+        return kept.value, it.value, res
 
     def sensor_data(self, sensors):
         st = lib().orc_sensor_data(self._h, _p(sensors), sensors.shape[0])

@@ -690,6 +690,8 @@ struct Calculator {
   int64_t           ncrit;
   int               stretching, formulation;
   real_t            dt = 0;
+  int               viscous = 0;  // 0 Inviscid, 1 PSE, 2 CSM (fmmCalculator.hpp:9-13)
+  real_t            nu      = 0;  // _nu (fmmCalculator.hpp:56; never initialised there)
   std::vector<Body> bodies;
   std::vector<Cell> cells;
   std::vector<Pair> p2p, m2l;
@@ -923,6 +925,7 @@ struct Calculator {
       if (st != ORC_OK) return st;
       if (s == 0 && (scheme == SCHEME_RK2 || scheme == SCHEME_RK4)) store_initial_state();
       calculate();
+      calculate_viscosity();
       step(scheme, s);
       std::memcpy(caller, bodies.data(), sizeof(Body) * n);
     }
@@ -940,8 +943,170 @@ struct Calculator {
     if (st != ORC_OK) return st;
     if (s == 0 && (scheme == SCHEME_RK2 || scheme == SCHEME_RK4)) store_initial_state();
     calculate();
+    calculate_viscosity();
     step(scheme, s);
     std::memcpy(caller, bodies.data(), sizeof(Body) * n);
+    return ORC_OK;
+  }
+
+  // calculateViscosity (fmmCalculator.hpp:713-724) after calculate().
+  //  PSE (multipoleKernels.hpp:392-450, symmetric kernel of Zhao & He): for every P2P pair
+  //  of the particle lists and every target i, the partial sum over the source cell's j
+  //  (R^2 != 0) of (vol_i a_j - vol_j a_i) exp(-R^2 / (2 s_ij^2)) / s_ij^5, s_ij^2 =
+  //  sigma_i^2 + sigma_j^2, vol = 4/3 pi sigma^3, is added to dadt_i times
+  //  nu / (sqrt(2) pi^{3/2}).
+  //  CSM (fmmCalculator.hpp:334-337): drdt_i += (sqrt(sigma_i^2 + 2 nu dt) - sigma_i) / dt.
+  void calculate_viscosity()
+  {
+    if (viscous == 1) {
+      CSR csr;
+      csr.build(cells.size(), p2p);
+      const real_t prefix = nu / std::sqrt(2.0) * (1.0 / M_PI) * (1.0 / std::sqrt(M_PI));
+#pragma omp parallel for schedule(dynamic, 4)
+      for (int64_t t = 0; t < (int64_t)cells.size(); t++) {
+        const Cell &Ct = cells[t];
+        for (int64_t q = csr.off[t]; q < csr.off[t + 1]; q++) {
+          const Cell &Cs = cells[csr.idx[q]];
+          for (int64_t i = Ct.BodyOffset; i < Ct.BodyOffset + Ct.NBODY; i++) {
+            Body        &bi = bodies[i];
+            real_t       sum[3] = {0, 0, 0};
+            const real_t vol_i  = 4. / 3. * M_PI * bi.radius * bi.radius * bi.radius;
+            for (int64_t j = Cs.BodyOffset; j < Cs.BodyOffset + Cs.NBODY; j++) {
+              const Body &bj = bodies[j];
+              real_t      dX[3];
+              for (int d = 0; d < 3; d++) dX[d] = bi.X[d] - bj.X[d];
+              const real_t R2 = dX[0] * dX[0] + dX[1] * dX[1] + dX[2] * dX[2];
+              if (R2 != 0) {
+                const real_t o2    = 1. / (bi.radius * bi.radius + bj.radius * bj.radius);
+                const real_t s5    = o2 * o2 * std::sqrt(o2);
+                const real_t vol_j = 4. / 3. * M_PI * bj.radius * bj.radius * bj.radius;
+                const real_t e     = std::exp(-0.5 * R2 * o2);
+                for (int d = 0; d < 3; d++) sum[d] += (vol_i * bj.alpha[d] - vol_j * bi.alpha[d]) * e * s5;
+              }
+            }
+            for (int d = 0; d < 3; d++) bi.dadt[d] += sum[d] * prefix;
+          }
+        }
+      }
+    } else if (viscous == 2) {
+      for (auto &b : bodies) b.drdt += (std::sqrt(b.radius * b.radius + 2.0 * nu * dt) - b.radius) / dt;
+    }
+  }
+
+  // Gaussian vorticity of the sources at the targets over near (P2P) pairs only
+  // (multipoleKernels.hpp:343-390): out_i -= sum_j alpha_j exp(-R^2 / (2 sigma_j^2)) /
+  // ((2 pi)^{3/2} sigma_j^3), no R^2 test.  `field` selects velocity_old (dual tree) or
+  // velocity (single tree).
+  static void gauss_vorticity(std::vector<Body> &tb, const std::vector<Cell> &tc, const std::vector<Body> &sb,
+                              const std::vector<Cell> &sc, const std::vector<Pair> &pairs, bool to_old)
+  {
+    CSR csr;
+    csr.build(tc.size(), pairs);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t t = 0; t < (int64_t)tc.size(); t++) {
+      const Cell &Ct = tc[t];
+      for (int64_t q = csr.off[t]; q < csr.off[t + 1]; q++) {
+        const Cell &Cs = sc[csr.idx[q]];
+        for (int64_t i = Ct.BodyOffset; i < Ct.BodyOffset + Ct.NBODY; i++) {
+          real_t sum[3] = {0, 0, 0};
+          for (int64_t j = Cs.BodyOffset; j < Cs.BodyOffset + Cs.NBODY; j++) {
+            const Body &bj = sb[j];
+            real_t      dX[3];
+            for (int d = 0; d < 3; d++) dX[d] = tb[i].X[d] - bj.X[d];
+            const real_t R2 = dX[0] * dX[0] + dX[1] * dX[1] + dX[2] * dX[2];
+            const real_t S2 = 2.0 * bj.radius * bj.radius;
+            const real_t z  = std::exp(-R2 / S2) * (1.0 / M_PI) * (1.0 / std::sqrt(M_PI)) / std::sqrt(2.0) / S2 / bj.radius;
+            for (int d = 0; d < 3; d++) sum[d] += bj.alpha[d] * z;
+          }
+          for (int d = 0; d < 3; d++) (to_old ? tb[i].velocity_old[d] : tb[i].velocity[d]) -= sum[d];
+        }
+      }
+    }
+  }
+
+  // calculateVorticityDualTree (fmmCalculator.hpp:746-753) on probes: probe tree, lists
+  // against the particle tree (createSensorTree 117-161), velocity_old reset and filled.
+  int vorticity(Body *caller, int64_t m)
+  {
+    if (!ready) return ORC_ERR_STATE;
+    int st = build_tree(caller, m, ncrit, scells);
+    if (st != ORC_OK) return st;
+    get_list(scells, cells, theta, sp2p, sm2l);
+    sensors.assign(caller, caller + m);
+    for (auto &b : sensors)
+      for (int d = 0; d < 3; d++) b.velocity_old[d] = 0;
+    gauss_vorticity(sensors, scells, bodies, cells, sp2p, true);
+    std::memcpy(caller, sensors.data(), sizeof(Body) * m);
+    return ORC_OK;
+  }
+
+  // RBF re-initialisation (fmmCalculator.hpp:855-917).  The particles must be set with
+  // init().  On return pts[0..*kept) holds the kept points (sensor-tree order) with the
+  // fitted alpha; *iters = CG iterations run, res[3] = last squared residual norms.
+  int rbf(Body *pts, int64_t m, int64_t *kept, int *iters, real_t *res_out)
+  {
+    if (!ready) return ORC_ERR_STATE;
+    const real_t tol = 1e-10, eps = 2.220446049250313e-16;
+    int          st  = vorticity(pts, m);
+    if (st != ORC_OK) return st;
+    std::vector<Body> np;
+    for (int64_t i = 0; i < m; i++)
+      if (!(std::abs(pts[i].velocity_old[0]) < 1e-6 && std::abs(pts[i].velocity_old[1]) < 1e-6 &&
+            std::abs(pts[i].velocity_old[2]) < 1e-6))
+        np.push_back(pts[i]);
+    const int64_t k = np.size();
+    *kept           = k;
+    *iters          = 0;
+    for (int d = 0; d < 3; d++) res_out[d] = 0;
+    if (k == 0) return ORC_OK;
+    st = build_tree(np.data(), k, ncrit, scells);  // createSensorTree(sensorOnly) 164-201
+    if (st != ORC_OK) return st;
+    std::vector<Pair> self_p2p, self_m2l;
+    get_list(scells, scells, theta, self_p2p, self_m2l);
+    auto single = [&]() {  // calculateVorticitySingleTree (788-795): resetSensors, then P2P
+      for (auto &b : np)
+        for (int d = 0; d < 3; d++) b.velocity[d] = b.dadt[d] = 0;
+      gauss_vorticity(np, scells, np, scells, self_p2p, false);
+    };
+    for (auto &b : np) {  // initAlphaValues (765-773)
+      const real_t vol = 4. / 3. * M_PI * b.radius * b.radius * b.radius;
+      for (int d = 0; d < 3; d++) b.alpha[d] = b.velocity_old[d] * vol;
+    }
+    single();
+    real_t res[3] = {0, 0, 0};
+    for (auto &b : np)  // setInitialResidualAndUpdateAlpha (797-819)
+      for (int d = 0; d < 3; d++) {
+        const real_t r = b.velocity_old[d] - b.velocity[d];
+        res[d] += r * r;
+        b.dadt_old[d] = b.alpha[d];
+        b.alpha[d]    = r;
+        b.X_old[d]    = r;
+      }
+    for (int it = 0; it < 100; it++) {
+      single();
+      real_t pAp[3] = {0, 0, 0}, a[3], nr[3] = {0, 0, 0};
+      for (auto &b : np)
+        for (int d = 0; d < 3; d++) pAp[d] += b.alpha[d] * b.velocity[d];
+      for (int d = 0; d < 3; d++) a[d] = (res[d] > eps && std::abs(pAp[d]) > eps) ? res[d] / pAp[d] : 0.0;
+      for (auto &b : np)
+        for (int d = 0; d < 3; d++) {
+          b.dadt_old[d] += a[d] * b.alpha[d];
+          b.X_old[d] -= a[d] * b.velocity[d];
+        }
+      for (auto &b : np)
+        for (int d = 0; d < 3; d++) nr[d] += b.X_old[d] * b.X_old[d];
+      *iters = it + 1;
+      for (int d = 0; d < 3; d++) res_out[d] = nr[d];
+      if (nr[0] < tol && nr[1] < tol && nr[2] < tol) break;
+      real_t beta[3];
+      for (int d = 0; d < 3; d++) beta[d] = (nr[d] > eps && res[d] > eps) ? nr[d] / res[d] : 0.0;
+      for (int d = 0; d < 3; d++) res[d] = nr[d];
+      for (auto &b : np)
+        for (int d = 0; d < 3; d++) b.alpha[d] = b.X_old[d] + beta[d] * b.alpha[d];
+    }
+    for (auto &b : np)  // storeAlphaValues (848-853)
+      for (int d = 0; d < 3; d++) b.alpha[d] = b.dadt_old[d];
+    std::memcpy(pts, np.data(), sizeof(Body) * k);
     return ORC_OK;
   }
 
@@ -1042,6 +1207,24 @@ int orc_stage(void *h, Body *bodies, int64_t n, double dt, int scheme, int stage
   return ((Calculator *)h)->stage(bodies, n, dt, scheme, stage);
 }
 int orc_sensor_data(void *h, Body *sensors, int64_t m) { return ((Calculator *)h)->sensor_data(sensors, m); }
+void orc_set_viscosity(void *h, int scheme, double nu)
+{
+  ((Calculator *)h)->viscous = scheme;
+  ((Calculator *)h)->nu      = nu;
+}
+int orc_viscosity(void *h, double dt)
+{
+  Calculator *c = (Calculator *)h;
+  if (!c->ready) return ORC_ERR_STATE;
+  c->dt = dt;
+  c->calculate_viscosity();
+  return ORC_OK;
+}
+int orc_vorticity(void *h, Body *sensors, int64_t m) { return ((Calculator *)h)->vorticity(sensors, m); }
+int orc_rbf(void *h, Body *pts, int64_t m, int64_t *kept, int *iters, double *res)
+{
+  return ((Calculator *)h)->rbf(pts, m, kept, iters, res);
+}
 
 int64_t orc_num_cells(void *h) { return (int64_t)((Calculator *)h)->cells.size(); }
 // ints: [C][5] = NCHILD, NBODY, ChildOffset, BodyOffset, level; reals: [C][4] = X, R

@@ -372,3 +372,87 @@ def test_direct_adjoint_matches_finite_differences(orc):
             Am[j, d] -= h
             fd = (orc.probe_objective(probes, v, X, Ap, S) - orc.probe_objective(probes, v, X, Am, S)) / (2 * h)
             assert abs(fd - ga[j, d]) < 1e-7 * max(1.0, abs(ga[j]).max()), (j, d, fd, ga[j, d])
+
+
+# ------------------------------------------------------------------- viscous schemes, vorticity, RBF
+def _lattice(L, h):
+    g = (np.arange(-L, L) + 0.5) * h  # even count: no point on the root centre or its axes
+    return np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+
+
+def test_pse_conserves_strength_and_tends_to_the_laplacian(orc):
+    """Symmetric PSE (multipoleKernels.hpp:392-450) on a lattice (spacing h, sigma = h, one leaf so
+    every pair interacts): the exchange sums to zero, and for alpha = omega h^3 of a Gaussian blob
+    of width s it approaches nu V lap(omega) with V = 4/3 pi sigma^3 (the reference's particle
+    volume) up to the O((sigma_ij / s)^2) smoothing error (3.5% at s = 8h)."""
+    h, s, nu = 0.1, 0.8, 0.01
+    X = _lattice(12, h)
+    om = np.exp(-(X ** 2).sum(1) / (2 * s * s))
+    A = np.zeros_like(X)
+    A[:, 2] = om * h ** 3
+    b = orc.make_bodies(X, A, np.full(len(X), h))
+    c = orc.Calculator(P=4, theta=0.5, ncrit=len(X) + 1)
+    c.init(b)
+    c.calculate()
+    base = c.bodies(len(X))[:, 18:21].copy()
+    c.set_viscosity("pse", nu)
+    c.viscosity(0.01)
+    out = c.bodies(len(X))
+    pse = out[:, 18:21] - base
+    assert np.abs(pse.sum(0)).max() < 1e-12 * np.abs(pse).sum()
+    r2 = (out[:, 9:12] ** 2).sum(1)
+    lap = np.exp(-r2 / (2 * s * s)) * (r2 / s ** 4 - 3 / s ** 2)
+    inner = r2 < 0.4 ** 2
+    ratio = pse[inner, 2] / (nu * lap[inner] * 4 / 3 * np.pi * h ** 3)
+    assert np.all(np.abs(ratio - 1) < 0.05), (ratio.min(), ratio.max())
+
+
+def test_gaussian_vorticity_is_a_partition_of_unity(orc):
+    """P2P_vorticity (multipoleKernels.hpp:343-390) of uniform strengths alpha = a h^3 on a
+    lattice reproduces -a (the reference stores the vorticity negated in velocity_old) at
+    interior probes: the normalised Gaussian of width sigma >= h sums to 1 over the lattice."""
+    h = 0.1
+    X = _lattice(10, h)
+    a0 = np.array([1.0, 2.0, -3.0])
+    b = orc.make_bodies(X, np.tile(a0 * h ** 3, (len(X), 1)), np.full(len(X), h))
+    c = orc.Calculator(P=4, theta=0.5, ncrit=len(X) + 1)
+    c.init(b)
+    c.calculate()
+    probes = np.array([[0.013, -0.021, 0.034], [0.2, 0.1, -0.15], [-0.3, 0.25, 0.05]])
+    s = orc.make_bodies(probes, np.zeros_like(probes), np.full(3, 0.05))
+    c.vorticity(s)
+    assert np.abs(s[:, 21:24] + a0).max() < 1e-6
+
+
+def test_csm_grows_the_core_area_by_2_nu_dt(orc):
+    """CSM (fmmCalculator.hpp:334-337) with an Euler step: sigma^2 grows by exactly 2 nu dt."""
+    X, A, S = W.scaled("cube", 500)
+    b = orc.make_bodies(X, A, S)
+    c = orc.Calculator(P=4, theta=0.5, ncrit=64)
+    c.set_viscosity("csm", 0.02)
+    c.advance(b, 0.01, "euler")
+    sig0 = np.full(len(X), S[0])
+    assert np.allclose(b[:, 33] ** 2 - sig0 ** 2, 2 * 0.02 * 0.01, rtol=1e-12, atol=0)
+
+
+def test_rbf_recovers_the_strengths_on_the_particles_own_points(orc):
+    """RBF (fmmCalculator.hpp:855-917): re-initialising onto points identical to the particles
+    (same positions and sigma, so the probe tree equals the particle tree and the dual-tree and
+    single-tree near operators coincide) must return the particles' own strengths.  sigma = h/2
+    keeps the Gaussian matrix well conditioned (13 CG iterations); at sigma = 1.2 h the reference's
+    plain CG stalls at 100 iterations, so the recovery is only as good as the CG tolerance."""
+    h = 0.1
+    X = _lattice(4, h) + np.array([0.004, -0.003, 0.002])
+    rng = np.random.default_rng(3)
+    A = (1.0 + 0.3 * rng.normal(size=X.shape)) * h ** 3
+    b = orc.make_bodies(X, A, np.full(len(X), 0.5 * h))
+    c = orc.Calculator(P=4, theta=0.5, ncrit=64)
+    c.init(b)
+    c.calculate()
+    pts = b.copy()
+    pts[:, 12:15] = 0
+    kept, it, res = c.rbf(pts)
+    assert kept == len(X) and it < 100 and (res < 1e-10).all()
+    got = pts[:kept]
+    assert np.array_equal(got[:, 9:12], b[:, 9:12])  # same points, same tree order
+    assert np.abs(got[:, 12:15] - b[:, 12:15]).max() < 1e-5 * np.abs(b[:, 12:15]).max()
