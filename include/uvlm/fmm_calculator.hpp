@@ -10,9 +10,9 @@
 //   Vortex::FMMCalculator<Host, Device, Integration, Viscous, Formulation, Stretching>
 //     init / calculate / advance / getSensorData / calculateViscosity /
 //     copyBodies2Host / copyBodies2Device     (src/fmmCalculator.hpp:25-1006)
-// Differences: P, theta and ncrit are constructor arguments (the reference has them as
-// constexpr in exafmmTypes.hpp:12-15); Host/Device tags are accepted and ignored; the
-// viscous schemes PSE/CSM are rejected at compile time; failures throw std::runtime_error
+// Differences: P, theta, ncrit and nu are constructor arguments (the reference has the first
+// three as constexpr in exafmmTypes.hpp:12-15 and leaves nu uninitialised); Host/Device tags
+// are accepted and ignored; failures throw std::runtime_error
 // carrying uvlm_last_error().  All compute runs on the GPU through include/uvlm_fmm.h.
 #ifndef UVLM_FMM_CALCULATOR_HPP
 #define UVLM_FMM_CALCULATOR_HPP
@@ -189,8 +189,9 @@ enum class Precision { FP64 = UVLM_FP64, Mixed = UVLM_MIXED };
 template <typename Host, typename Device, typename IntegrationScheme, typename ViscousScheme, typename Formulation,
           typename Stretching>
 struct FMMCalculator {
-  static_assert(std::is_same<ViscousScheme, Inviscid>::value,
-                "only the inviscid scheme is on the GPU path (PSE/CSM are outside it)");
+  static_assert(std::is_same<ViscousScheme, Inviscid>::value || std::is_same<ViscousScheme, PSE>::value ||
+                    std::is_same<ViscousScheme, CSM>::value,
+                "ViscousScheme should be Inviscid, PSE or CSM");
   static_assert(std::is_same<Stretching, Transposed>::value || std::is_same<Stretching, Classic>::value,
                 "Stretching should be either Transposed or Classic");
 
@@ -202,10 +203,16 @@ struct FMMCalculator {
                                                           : UVLM_RK4;
   }
 
+  // nu: the kinematic viscosity of the PSE / CSM schemes (the reference's _nu, which it never
+  // initialises, fmmCalculator.hpp:56); ignored for Inviscid.
   explicit FMMCalculator(int P = 4, double theta = 0.3, size_t ncrit = 128, Precision precision = Precision::FP64,
-                         int device = 0)
+                         int device = 0, exafmm::real_t nu = 0.0)
       : h_(detail_params(P, theta, ncrit, precision, device))
   {
+    const int v = std::is_same<ViscousScheme, PSE>::value   ? UVLM_PSE
+                  : std::is_same<ViscousScheme, CSM>::value ? UVLM_CSM
+                                                            : UVLM_INVISCID;
+    exafmm::detail::check(uvlm_set_viscosity(h_.h, v, nu));
   }
 
   // fmmCalculator.hpp:282-288: tree + lists; permutes `bodies` into tree order.
@@ -218,10 +225,30 @@ struct FMMCalculator {
   // fmmCalculator.hpp:420-467
   void calculate() { exafmm::detail::check(uvlm_eval(h_.h)); }
   // fmmCalculator.hpp:713-724 (inviscid: nothing to do)
-  void calculateViscosity() {}
+  // (PSE: dadt += exchange over the near pairs; CSM: drdt += core spreading over the dt of the
+  //  last advance(), or setTimeStep())
+  void calculateViscosity() { exafmm::detail::check(uvlm_viscosity(h_.h, dt_)); }
+  void setTimeStep(exafmm::real_t dt) { dt_ = dt; }
+  // calculateVorticityDualTree (fmmCalculator.hpp:746-753) at `sensors` (after init()):
+  // velocity_old = -(Gaussian vorticity); the sensors come back in their tree order.
+  void calculateVorticity(exafmm::Bodies &sensors)
+  {
+    exafmm::detail::check(uvlm_vorticity_aos(h_.h, sensors.data(), (int64_t)sensors.size(), sizeof(exafmm::Body)));
+  }
+  // fmmCalculator.hpp:855-917: init(particles), then the strengths of newPoints fitted to the
+  // particles' vorticity; newPoints keeps the points above the vorticity threshold (tree order).
+  void RBF(exafmm::Bodies &particles, exafmm::Bodies &newPoints)
+  {
+    init(particles);
+    int64_t kept = 0;
+    exafmm::detail::check(uvlm_rbf_aos(h_.h, newPoints.data(), (int64_t)newPoints.size(), sizeof(exafmm::Body), &kept,
+                                       nullptr, nullptr));
+    newPoints.resize((size_t)kept);
+  }
   // fmmCalculator.hpp:919-1005: every stage rebuilds the tree; the state stays on the GPU.
   void advance(exafmm::Bodies &bodies, exafmm::real_t dt)
   {
+    dt_ = dt;
     copyBodies2Device(bodies);
     exafmm::detail::check(uvlm_advance(h_.h, dt));
     copyBodies2Host(bodies);
@@ -301,7 +328,8 @@ private:
                                   std::is_same<Formulation, rVPM>::value ? UVLM_RVPM : UVLM_CVPM, scheme(), device);
   }
   exafmm::detail::Handle h_;
-  size_t                 n_ = 0;
+  size_t                 n_  = 0;
+  exafmm::real_t         dt_ = 0.0;
 };
 }  // namespace Vortex
 

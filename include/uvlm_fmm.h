@@ -161,6 +161,31 @@ int uvlm_relax(uvlm_handle h, double f);
  * counterpart (the reference prints only strength extrema, src/main.cpp:219-221). */
 int uvlm_diagnostics(uvlm_handle h, double *out);
 
+/* Viscous schemes (the ViscousScheme tag and _nu of fmmCalculator.hpp:9-13, 56; _nu is never
+ * initialised there).  Once set, uvlm_advance applies the scheme after every stage's eval, as
+ * advance() calls calculateViscosity() (fmmCalculator.hpp:919-1005):
+ *   UVLM_PSE  dadt_i += nu/(sqrt2 pi^1.5) sum_j (vol_i a_j - vol_j a_i) e^{-R^2/(2 s_ij^2)} / s_ij^5
+ *             over the P2P lists (multipoleKernels.hpp:392-450);
+ *   UVLM_CSM  drdt_i += (sqrt(sigma_i^2 + 2 nu dt) - sigma_i) / dt (fmmCalculator.hpp:334-337). */
+enum { UVLM_INVISCID = 0, UVLM_PSE = 1, UVLM_CSM = 2 };
+int uvlm_set_viscosity(uvlm_handle h, int scheme, double nu);
+/* calculateViscosity() (fmmCalculator.hpp:713-724) on the rates of the last uvlm_eval; dt is
+ * the step CSM spreads over. */
+int uvlm_viscosity(uvlm_handle h, double dt);
+/* Gaussian vorticity of the particles at m probes over the near (P2P) pairs of the probe tree
+ * against the particle tree: velocity_old = -sum_j a_j exp(-R^2/(2 sigma_j^2)) /
+ * ((2 pi)^1.5 sigma_j^3) (calculateVorticityDualTree, fmmCalculator.hpp:746-753;
+ * multipoleKernels.hpp:343-370).  Needs a uvlm_build; the probes come back in tree order. */
+int uvlm_vorticity_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride);
+/* RBF re-initialisation (fmmCalculator.hpp:855-917): the particles' vorticity at the m new
+ * points, the points below 1e-6 in every component dropped, and the strengths of the rest fitted
+ * by the reference's conjugate gradients on the Gaussian near-field operator of their own tree
+ * (at most 100 iterations, squared residual < 1e-10 per component).  points[0..*kept) receive
+ * the kept points in tree order with the fitted alpha; *iterations and residuals[3] report the
+ * CG (may be NULL).  Needs a uvlm_build of the particles. */
+int uvlm_rbf_aos(uvlm_handle h, void *points, int64_t m, int64_t stride, int64_t *kept, int *iterations,
+                 double *residuals);
+
 int uvlm_timings(uvlm_handle h, uvlm_timing *out);
 
 /* Chooses ncrit and the M2L plan size for the bodies currently on the handle by timing

@@ -18,6 +18,7 @@ EXPORTS = [
     "uvlm_build", "uvlm_eval", "uvlm_stage", "uvlm_advance", "uvlm_num_cells", "uvlm_get_tree",
     "uvlm_get_lists", "uvlm_get_expansions", "uvlm_sensors_aos", "uvlm_direct", "uvlm_probe_adjoint",
     "uvlm_update_bodies_soa", "uvlm_relax", "uvlm_diagnostics", "uvlm_timings", "uvlm_sync", "uvlm_autotune",
+    "uvlm_set_viscosity", "uvlm_viscosity", "uvlm_vorticity_aos", "uvlm_rbf_aos",
 ]
 
 
@@ -82,6 +83,10 @@ def lib():
             "uvlm_timings": (I, [VP, ctypes.POINTER(Timing)]),
             "uvlm_sync": (I, [VP]),
             "uvlm_autotune": (I, [VP, PI64, I, I, PI64, PD]),
+            "uvlm_set_viscosity": (I, [VP, I, D]),
+            "uvlm_viscosity": (I, [VP, D]),
+            "uvlm_vorticity_aos": (I, [VP, VP, I64, I64]),
+            "uvlm_rbf_aos": (I, [VP, VP, I64, I64, PI64, ctypes.POINTER(ctypes.c_int), PD]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

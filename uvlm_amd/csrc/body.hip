@@ -188,7 +188,112 @@ __global__ void k_reset_sensor(double *__restrict__ st, int64_t cap, int64_t N)
   }
 }
 
+// CSM core spreading (fmmCalculator.hpp:334-337): drdt += (sqrt(sigma^2 + 2 nu dt) - sigma) / dt.
+__global__ void k_csm(double *__restrict__ st, int64_t cap, int64_t N, double nu, double dt)
+{
+  const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (i >= N) return;
+  const double sg = st[F_SIG * cap + i];
+  st[F_DSIG * cap + i] += (sqrt(sg * sg + 2.0 * nu * dt) - sg) / dt;
+}
+
+__global__ void k_zero_fields(double *__restrict__ st, int64_t cap, int64_t N, int f0, int nf)
+{
+  const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (i >= N) return;
+  for (int f = 0; f < nf; f++) st[(f0 + f) * cap + i] = 0.0;
+}
+
+// Per-block partial sums of a_d * b_d (d = 0..2) over fields fa, fb; fixed reduction order.
+__global__ void k_dot3_partial(const double *__restrict__ st, int64_t cap, int64_t N, int fa, int fb,
+                               double *__restrict__ part)
+{
+  __shared__ double sh[3][BLK];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLK)
+    for (int d = 0; d < 3; d++) v[d] += st[(fa + d) * cap + i] * st[(fb + d) * cap + i];
+  for (int d = 0; d < 3; d++) sh[d][threadIdx.x] = v[d];
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int d = 0; d < 3; d++) sh[d][threadIdx.x] += sh[d][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 3; d++) part[blockIdx.x * 3 + d] = sh[d][0];
+}
+
+// RBF conjugate gradients (fmmCalculator.hpp:765-853), fields as in the reference:
+// b = velocity_old (target), Ap = velocity, p = alpha, x = dadt_old, r = X_old.
+// mode 0 initAlphaValues: alpha = velocity_old * 4/3 pi sigma^3
+// mode 1 setInitialResidualAndUpdateAlpha: r = b - velocity; x = alpha; alpha = r; X_old = r
+// mode 2 updateAlphasAndResiduals: x += c_d p_d; r -= c_d Ap_d
+// mode 3 updateP: p = r + c_d p
+// mode 4 storeAlphaValues: alpha = x
+__global__ void k_rbf_vec(double *__restrict__ st, int64_t cap, int64_t N, int mode, double c0, double c1, double c2)
+{
+  const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (i >= N) return;
+  const double c[3] = {c0, c1, c2};
+  for (int d = 0; d < 3; d++) {
+    double &al = st[(F_A + d) * cap + i];
+    double &x  = st[(F_DAOLD + d) * cap + i];
+    double &r  = st[(F_XOLD + d) * cap + i];
+    const double b = st[(F_UOLD + d) * cap + i], Ap = st[(F_U + d) * cap + i];
+    if (mode == 0) {
+      const double sg = st[F_SIG * cap + i];
+      al              = b * (4. / 3. * M_PI * sg * sg * sg);
+    } else if (mode == 1) {
+      const double rr = b - Ap;
+      x               = al;
+      al              = rr;
+      r               = rr;
+    } else if (mode == 2) {
+      x += c[d] * al;
+      r -= c[d] * Ap;
+    } else if (mode == 3) {
+      al = r + c[d] * al;
+    } else {
+      al = x;
+    }
+  }
+}
+
 }  // namespace
+
+void csm(hipStream_t s, double *st, int64_t cap, int64_t N, double nu, double dt)
+{
+  if (N <= 0) return;
+  k_csm<<<nblocks(N), BLK, 0, s>>>(st, cap, N, nu, dt);
+  UVLM_LAUNCH_CHECK();
+}
+
+void zero_fields(hipStream_t s, double *st, int64_t cap, int64_t N, int f0, int nf)
+{
+  if (N <= 0) return;
+  k_zero_fields<<<nblocks(N), BLK, 0, s>>>(st, cap, N, f0, nf);
+  UVLM_LAUNCH_CHECK();
+}
+
+void dot3(hipStream_t s, const double *st, int64_t cap, int64_t N, int fa, int fb, double *part_dev, double out[3])
+{
+  const int nb = (int)std::min<int64_t>(512, std::max<int64_t>(1, (N + BLK - 1) / BLK));
+  k_dot3_partial<<<nb, BLK, 0, s>>>(st, cap, N, fa, fb, part_dev);
+  UVLM_LAUNCH_CHECK();
+  std::vector<double> h((size_t)nb * 3);
+  UVLM_HIP(hipMemcpyAsync(h.data(), part_dev, 8 * h.size(), hipMemcpyDeviceToHost, s));
+  UVLM_HIP(hipStreamSynchronize(s));
+  out[0] = out[1] = out[2] = 0.0;
+  for (int b = 0; b < nb; b++)
+    for (int d = 0; d < 3; d++) out[d] += h[(size_t)b * 3 + d];
+}
+
+void rbf_vec(hipStream_t s, double *st, int64_t cap, int64_t N, int mode, const double c[3])
+{
+  if (N <= 0) return;
+  k_rbf_vec<<<nblocks(N), BLK, 0, s>>>(st, cap, N, mode, c ? c[0] : 0.0, c ? c[1] : 0.0, c ? c[2] : 0.0);
+  UVLM_LAUNCH_CHECK();
+}
 
 void aos_to_soa(hipStream_t s, const void *aos, int64_t n, int64_t stride_doubles, double *st, int64_t cap)
 {

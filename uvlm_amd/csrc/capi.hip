@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -45,6 +46,9 @@ struct uvlm_handle_s {
   DevBuf  sst, sst2, Ls, Ms, sperm, ast;
   int64_t scap = 0;
   hipEvent_t aev[2]{};
+  // viscous scheme applied after each eval of uvlm_advance (UVLM_INVISCID / UVLM_PSE / UVLM_CSM)
+  int    viscous = UVLM_INVISCID;
+  double nu      = 0.0;
 
   double *F(int f) { return st.as<double>() + (int64_t)f * cap; }
   void    ensure_cap(int64_t n)
@@ -168,6 +172,65 @@ void do_eval(uvlm_handle h, bool sync)
     h->tm.downward_ms = ev_ms(h, EV_P2P, EV_DOWN);
     h->tm.eval_ms     = ev_ms(h, EV_E0, EV_DOWN);
   }
+}
+
+// calculateViscosity (fmmCalculator.hpp:713-724) on the results of the last eval.
+void apply_viscosity(uvlm_handle h, double dt)
+{
+  if (h->viscous == UVLM_PSE)
+    gauss_near(h->s, h->tree, h->tree, h->lists, h->st.as<double>(), h->cap, h->st.as<double>(), h->cap,
+               h->st.as<double>(), h->cap, 0, 0, h->nu / std::sqrt(2.0) * INV_PI * INV_SQRTPI);
+  else if (h->viscous == UVLM_CSM)
+    csm(h->s, h->st.as<double>(), h->cap, h->N, h->nu, dt);
+}
+
+// m probe records (AoS, `stride` bytes each) into h->sst, then the probe tree over them (the
+// probes are permuted into its order, like the reference's createSensorTree).
+void upload_probes(uvlm_handle h, const void *probes, int64_t m, int64_t stride)
+{
+  hipStream_t s = h->s;
+  if (m > h->scap) {
+    h->scap = m;
+    h->sst.reserve(sizeof(double) * N_FIELDS * m);
+    h->sst2.reserve(sizeof(double) * N_FIELDS * m);
+    h->sperm.reserve(4 * m);
+  }
+  const int64_t sc = h->scap;
+  h->staging.reserve((size_t)m * stride);
+  UVLM_HIP(hipMemcpyAsync(h->staging.p, probes, (size_t)m * stride, hipMemcpyHostToDevice, s));
+  aos_to_soa(s, h->staging.p, m, stride / 8, h->sst.as<double>(), sc);
+  double *sx = h->sst.as<double>();
+  build_tree(s, h->sc, sx + F_X * sc, sx + (F_X + 1) * sc, sx + (F_X + 2) * sc, m, h->prm.ncrit, h->stree,
+             h->sperm.as<int32_t>());
+  gather_fields(s, h->sst.as<double>(), h->sst2.as<double>(), sc, sc, N_FIELDS, h->sperm.as<int32_t>(), m);
+  swap_buf(h->sst, h->sst2);
+}
+
+// The first m probe records of h->sst as 296-byte exafmm::Body records.
+std::vector<double> download_probes(uvlm_handle h, int64_t m)
+{
+  h->staging.reserve((size_t)m * 296);
+  soa_to_aos(h->s, h->sst.as<double>(), h->scap, h->staging.p, m, N_FIELDS);
+  std::vector<double> out((size_t)m * N_FIELDS);
+  UVLM_HIP(hipMemcpyAsync(out.data(), h->staging.p, 8 * out.size(), hipMemcpyDeviceToHost, h->s));
+  UVLM_HIP(hipStreamSynchronize(h->s));
+  return out;
+}
+
+void copy_out_records(const std::vector<double> &rec, int64_t m, void *dst, int64_t stride)
+{
+  for (int64_t i = 0; i < m; i++) std::memcpy((char *)dst + i * stride, rec.data() + i * N_FIELDS, 296);
+}
+
+// Gaussian vorticity of the particles at the probes already in h->sst (tree order): lists
+// against the particle tree, velocity_old reset and filled (calculateVorticityDualTree,
+// fmmCalculator.hpp:746-753, after the lists of createSensorTree 117-161).
+void probe_vorticity(uvlm_handle h, int64_t m)
+{
+  traverse(h->s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
+  zero_fields(h->s, h->sst.as<double>(), h->scap, m, F_UOLD, 3);
+  gauss_near(h->s, h->stree, h->tree, h->slists, h->sst.as<double>(), h->scap, h->st.as<double>(), h->cap,
+             h->sst.as<double>(), h->scap, 1, F_UOLD, 0.0);
 }
 
 int stages_of(int scheme) { return scheme == UVLM_EULER ? 1 : scheme == UVLM_RK2 ? 2 : scheme == UVLM_RK3 ? 3 : 4; }
@@ -365,6 +428,7 @@ int uvlm_advance(uvlm_handle h, double dt)
     for (int k = 0; k < stages_of(h->prm.scheme); k++) {
       do_build(h);
       do_eval(h, true);
+      apply_viscosity(h, dt);
       rk_stage(h->s, h->st.as<double>(), h->cap, h->N, h->prm.scheme, k, dt, 0);
     }
     UVLM_HIP(hipStreamSynchronize(h->s));
@@ -621,6 +685,106 @@ int uvlm_autotune(uvlm_handle h, const int64_t *ncrit, int n, int reps, int64_t 
     h->evaluated = false;
     if (best_ncrit) *best_ncrit = bn;
     if (best_ms) *best_ms = best;
+  });
+}
+
+int uvlm_set_viscosity(uvlm_handle h, int scheme, double nu)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (scheme != UVLM_INVISCID && scheme != UVLM_PSE && scheme != UVLM_CSM) throw Error(UVLM_ERR_ARG, "bad viscous scheme");
+    if (!(nu >= 0.0) || !std::isfinite(nu)) throw Error(UVLM_ERR_ARG, "nu must be finite and >= 0");
+    h->viscous = scheme;
+    h->nu      = nu;
+  });
+}
+
+int uvlm_viscosity(uvlm_handle h, double dt)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->evaluated) throw Error(UVLM_ERR_STATE, "uvlm_viscosity needs the rates of a uvlm_eval");
+    if (h->viscous == UVLM_CSM && !(dt > 0.0)) throw Error(UVLM_ERR_ARG, "CSM needs dt > 0");
+    set_device(h);
+    apply_viscosity(h, dt);
+    UVLM_HIP(hipStreamSynchronize(h->s));
+  });
+}
+
+int uvlm_vorticity_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->built) throw Error(UVLM_ERR_STATE, "uvlm_vorticity_aos needs a uvlm_build");
+    if (m <= 0 || !probes) throw Error(UVLM_ERR_ARG, "empty or null probes");
+    if (stride < 296 || stride % 8) throw Error(UVLM_ERR_ARG, "stride must be >= 296 and a multiple of 8");
+    set_device(h);
+    upload_probes(h, probes, m, stride);
+    probe_vorticity(h, m);
+    copy_out_records(download_probes(h, m), m, probes, stride);
+  });
+}
+
+int uvlm_rbf_aos(uvlm_handle h, void *points, int64_t m, int64_t stride, int64_t *kept, int *iterations,
+                 double *residuals)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->built) throw Error(UVLM_ERR_STATE, "uvlm_rbf_aos needs a uvlm_build of the particles");
+    if (m <= 0 || !points || !kept) throw Error(UVLM_ERR_ARG, "empty or null points");
+    if (stride < 296 || stride % 8) throw Error(UVLM_ERR_ARG, "stride must be >= 296 and a multiple of 8");
+    set_device(h);
+    hipStream_t s = h->s;
+    // target vorticity at the new points, then drop those below 1e-6 in every component
+    upload_probes(h, points, m, stride);
+    probe_vorticity(h, m);
+    const std::vector<double> all = download_probes(h, m);
+    std::vector<double>       keep;
+    for (int64_t i = 0; i < m; i++) {
+      const double *r = all.data() + i * N_FIELDS;
+      if (!(std::abs(r[F_UOLD]) < 1e-6 && std::abs(r[F_UOLD + 1]) < 1e-6 && std::abs(r[F_UOLD + 2]) < 1e-6))
+        keep.insert(keep.end(), r, r + N_FIELDS);
+    }
+    const int64_t k = (int64_t)(keep.size() / N_FIELDS);
+    *kept           = k;
+    if (iterations) *iterations = 0;
+    if (residuals) residuals[0] = residuals[1] = residuals[2] = 0.0;
+    if (k == 0) return;
+    // tree over the kept points and its self lists (createSensorTree(sensorOnly), 164-201)
+    upload_probes(h, keep.data(), k, 296);
+    traverse(s, h->sc, h->stree, h->stree, h->prm.theta, h->slists);
+    double       *st = h->sst.as<double>();
+    const int64_t sc = h->scap;
+    h->sc.partial.reserve(sizeof(double) * 3 * 512);
+    double *part = h->sc.partial.as<double>();
+    auto single = [&] {  // calculateVorticitySingleTree (788-795): velocity = -A alpha
+      zero_fields(s, st, sc, k, F_U, 3);
+      zero_fields(s, st, sc, k, F_DADT, 3);
+      gauss_near(s, h->stree, h->stree, h->slists, st, sc, st, sc, st, sc, 1, F_U, 0.0);
+    };
+    const double tol = 1e-10, eps = 2.220446049250313e-16;
+    rbf_vec(s, st, sc, k, 0, nullptr);  // initAlphaValues
+    single();
+    rbf_vec(s, st, sc, k, 1, nullptr);  // setInitialResidualAndUpdateAlpha
+    double res[3];
+    dot3(s, st, sc, k, F_XOLD, F_XOLD, part, res);
+    for (int it = 0; it < 100; it++) {
+      single();
+      double pAp[3], a[3], nr[3], beta[3];
+      dot3(s, st, sc, k, F_A, F_U, part, pAp);
+      for (int d = 0; d < 3; d++) a[d] = (res[d] > eps && std::abs(pAp[d]) > eps) ? res[d] / pAp[d] : 0.0;
+      rbf_vec(s, st, sc, k, 2, a);
+      dot3(s, st, sc, k, F_XOLD, F_XOLD, part, nr);
+      if (iterations) *iterations = it + 1;
+      if (residuals)
+        for (int d = 0; d < 3; d++) residuals[d] = nr[d];
+      if (nr[0] < tol && nr[1] < tol && nr[2] < tol) break;
+      for (int d = 0; d < 3; d++) beta[d] = (nr[d] > eps && res[d] > eps) ? nr[d] / res[d] : 0.0;
+      for (int d = 0; d < 3; d++) res[d] = nr[d];
+      rbf_vec(s, st, sc, k, 3, beta);
+    }
+    rbf_vec(s, st, sc, k, 4, nullptr);  // storeAlphaValues
+    copy_out_records(download_probes(h, k), k, points, stride);
   });
 }
 

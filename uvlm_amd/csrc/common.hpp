@@ -156,6 +156,8 @@ void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const
          const double *src, int64_t scap, double *out, int64_t ocap, int precision, int sensor_mode,
          int64_t max_leaf, Scratch *sc = nullptr);
 
+void gauss_near(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const double *tgt, int64_t tcap,
+                const double *srcst, int64_t scap, double *out, int64_t ocap, int mode, int field, double prefix);
 void direct(hipStream_t s, int64_t m, const double *xt, const double *at, const double *sst, int64_t scap, int64_t n,
             double *u, double *J, double *dadt);
 
@@ -166,5 +168,10 @@ void rk_stage(hipStream_t s, double *st, int64_t cap, int64_t N, int scheme, int
 void reset_sensor_rates(hipStream_t s, double *st, int64_t cap, int64_t N);
 void relax(hipStream_t s, double *st, int64_t cap, int64_t N, double f);
 void diagnostics(hipStream_t s, const double *st, int64_t cap, int64_t N, double *part_dev, double out[10]);
+void csm(hipStream_t s, double *st, int64_t cap, int64_t N, double nu, double dt);
+void zero_fields(hipStream_t s, double *st, int64_t cap, int64_t N, int f0, int nf);
+// out[d] = sum_i st[fa+d][i] * st[fb+d][i]; part_dev holds 3 * 512 doubles
+void dot3(hipStream_t s, const double *st, int64_t cap, int64_t N, int fa, int fb, double *part_dev, double out[3]);
+void rbf_vec(hipStream_t s, double *st, int64_t cap, int64_t N, int mode, const double c[3]);
 
 }  // namespace uvlm

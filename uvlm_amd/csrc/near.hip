@@ -410,6 +410,88 @@ __global__ __launch_bounds__(WG) void k_p2p_reduce(const int32_t *__restrict__ l
   }
 }
 
+// Gaussian near-field sums over the P2P lists, FP64 (the viscous and vorticity kernels of
+// multipoleKernels.hpp:343-450).  One 64-lane workgroup per target leaf, lane <-> target;
+// the source bodies of each list entry pass through LDS; sums in list order (deterministic).
+//  GAUSS_PSE : dadt_i += prefix * sum_j (vol_i a_j - vol_j a_i) e^{-R^2 / (2 s_ij^2)} / s_ij^5,
+//              s_ij^2 = sigma_i^2 + sigma_j^2, vol = 4/3 pi sigma^3, pairs with R^2 = 0 skipped
+//  GAUSS_VORT: out[field + d]_i -= sum_j a_j,d e^{-R^2 / (2 sigma_j^2)} / ((2 pi)^{3/2} sigma_j^3)
+enum { GAUSS_PSE = 0, GAUSS_VORT = 1 };
+template <int MODE>
+__global__ __launch_bounds__(WG) void k_gauss(const int32_t *__restrict__ leaves, int64_t nleaf,
+                                              const int32_t *__restrict__ tbody, const int32_t *__restrict__ tnbody,
+                                              const int32_t *__restrict__ off, const int32_t *__restrict__ src,
+                                              const int32_t *__restrict__ sbody, const int32_t *__restrict__ snbody,
+                                              const double *__restrict__ tst, int64_t tcap,
+                                              const double *__restrict__ sst, int64_t scap, double *__restrict__ out,
+                                              int64_t ocap, int field, double prefix)
+{
+  __shared__ double sx[WG], sy[WG], sz[WG], ss[WG], sa0[WG], sa1[WG], sa2[WG];
+  const int64_t w = blockIdx.x;
+  if (w >= nleaf) return;
+  const int t = leaves[w], tb = tbody[t], nt = tnbody[t], lane = threadIdx.x;
+  const int q0 = off[t], q1 = off[t + 1];
+  for (int c0 = 0; c0 < nt; c0 += WG) {
+    const int     i     = c0 + lane;
+    const bool    valid = i < nt;
+    const int64_t bi    = tb + (valid ? i : 0);
+    const double  xi = tst[(F_X + 0) * tcap + bi], yi = tst[(F_X + 1) * tcap + bi], zi = tst[(F_X + 2) * tcap + bi];
+    double        sgi = 0.0, voli = 0.0, ai[3] = {0.0, 0.0, 0.0};
+    if (MODE == GAUSS_PSE) {
+      sgi  = tst[F_SIG * tcap + bi];
+      voli = 4. / 3. * M_PI * sgi * sgi * sgi;
+      for (int d = 0; d < 3; d++) ai[d] = tst[(F_A + d) * tcap + bi];
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int q = q0; q < q1; q++) {
+      const int sl = src[q], b0 = sbody[sl], nb = snbody[sl];
+      for (int j0 = 0; j0 < nb; j0 += WG) {
+        const int cnt = min(WG, nb - j0);
+        __syncthreads();
+        if (lane < cnt) {
+          const int64_t bj = b0 + j0 + lane;
+          sx[lane]         = sst[(F_X + 0) * scap + bj];
+          sy[lane]         = sst[(F_X + 1) * scap + bj];
+          sz[lane]         = sst[(F_X + 2) * scap + bj];
+          ss[lane]         = sst[F_SIG * scap + bj];
+          sa0[lane]        = sst[(F_A + 0) * scap + bj];
+          sa1[lane]        = sst[(F_A + 1) * scap + bj];
+          sa2[lane]        = sst[(F_A + 2) * scap + bj];
+        }
+        __syncthreads();
+        if (!valid) continue;
+        for (int j = 0; j < cnt; j++) {
+          const double dx = xi - sx[j], dy = yi - sy[j], dz = zi - sz[j];
+          const double R2 = dx * dx + dy * dy + dz * dz;
+          const double sj = ss[j];
+          if (MODE == GAUSS_PSE) {
+            if (R2 == 0.0) continue;
+            const double o2   = 1.0 / (sgi * sgi + sj * sj);
+            const double s5   = o2 * o2 * sqrt(o2);
+            const double volj = 4. / 3. * M_PI * sj * sj * sj;
+            const double e    = exp(-0.5 * R2 * o2) * s5;
+            acc[0] += (voli * sa0[j] - volj * ai[0]) * e;
+            acc[1] += (voli * sa1[j] - volj * ai[1]) * e;
+            acc[2] += (voli * sa2[j] - volj * ai[2]) * e;
+          } else {
+            const double S2 = 2.0 * sj * sj;
+            const double z  = exp(-R2 / S2) * (INV_PI * INV_SQRTPI * 0.70710678118654752440) / S2 / sj;
+            acc[0] += sa0[j] * z;
+            acc[1] += sa1[j] * z;
+            acc[2] += sa2[j] * z;
+          }
+        }
+      }
+    }
+    if (valid) {
+      if (MODE == GAUSS_PSE)
+        for (int d = 0; d < 3; d++) out[(F_DADT + d) * ocap + bi] += acc[d] * prefix;
+      else
+        for (int d = 0; d < 3; d++) out[(field + d) * ocap + bi] -= acc[d];
+    }
+  }
+}
+
 // Below this many target leaves (probe trees), long lists are cut into segments of E list
 // entries that run as separate workgroups, so the GPU stays full.
 constexpr int64_t SPLIT_TARGETS = 4096, SPLIT_SEGMENTS = 8192;
@@ -501,6 +583,26 @@ void p2p(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const
   else if (!sensor_mode) UVLM_P2P(false, true, false);
   else UVLM_P2P(false, false, false);
 #undef UVLM_P2P
+  UVLM_LAUNCH_CHECK();
+}
+
+// PSE (mode 0, adds prefix * sum to dadt) or Gaussian vorticity (mode 1, subtracts the sum
+// from fields field..field+2) of the source tree at the target tree's leaves, over the P2P lists.
+void gauss_near(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l, const double *tgt, int64_t tcap,
+                const double *srcst, int64_t scap, double *out, int64_t ocap, int mode, int field, double prefix)
+{
+  if (tt.nleaf == 0) return;
+  const int nb = (int)tt.nleaf;
+  if (mode == 0)
+    k_gauss<GAUSS_PSE><<<nb, WG, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(),
+                                         tt.nbody.as<int32_t>(), l.p2p_off.as<int32_t>(), l.p2p_src.as<int32_t>(),
+                                         stree.body.as<int32_t>(), stree.nbody.as<int32_t>(), tgt, tcap, srcst, scap,
+                                         out, ocap, field, prefix);
+  else
+    k_gauss<GAUSS_VORT><<<nb, WG, 0, s>>>(tt.leaves.as<int32_t>(), tt.nleaf, tt.body.as<int32_t>(),
+                                          tt.nbody.as<int32_t>(), l.p2p_off.as<int32_t>(), l.p2p_src.as<int32_t>(),
+                                          stree.body.as<int32_t>(), stree.nbody.as<int32_t>(), tgt, tcap, srcst, scap,
+                                          out, ocap, field, prefix);
   UVLM_LAUNCH_CHECK();
 }
 

@@ -32,6 +32,7 @@ BODY_BYTES = 8 * BODY_DOUBLES
 PRECISION = {"fp64": 0, "mixed": 1}
 STRETCHING = {"transposed": 0, "classic": 1}
 FORMULATION = {"cvpm": 0, "rvpm": 1}
+VISCOUS = {"inviscid": 0, "pse": 1, "csm": 2}
 SCHEME = {"euler": 0, "rk2": 1, "rk3": 2, "rk4": 3}
 
 UVLMError = _capi.UVLMError
@@ -220,6 +221,33 @@ class FMMCalculator:
         t = _capi.Timing()
         _capi.check(self.lib.uvlm_timings(self._h, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def set_viscosity(self, scheme, nu):
+        """"inviscid", "pse" or "csm" with viscosity nu; uvlm_advance applies it after each eval."""
+        _capi.check(self.lib.uvlm_set_viscosity(self._h, VISCOUS[scheme], float(nu)))
+
+    def viscosity(self, dt):
+        """calculateViscosity on the rates of the last calculate()."""
+        _capi.check(self.lib.uvlm_viscosity(self._h, float(dt)))
+
+    def vorticity(self, probes):
+        """Gaussian vorticity (negated, in velocity_old) of the particles at probe records
+        (m, 37), near pairs only; the probes come back in tree order (in place)."""
+        s = probes
+        assert s.flags.c_contiguous and s.dtype == np.float64 and s.shape[1] == BODY_DOUBLES
+        _capi.check(self.lib.uvlm_vorticity_aos(self._h, _dp(s), s.shape[0], BODY_BYTES))
+
+    def rbf(self, points):
+        """RBF re-initialisation onto point records (m, 37); returns (kept, iterations,
+        residuals) with points[:kept] the kept points (tree order) carrying the fitted alpha."""
+        s = points
+        assert s.flags.c_contiguous and s.dtype == np.float64 and s.shape[1] == BODY_DOUBLES
+        kept = ctypes.c_int64()
+        it = ctypes.c_int()
+        res = np.zeros(3)
+        _capi.check(self.lib.uvlm_rbf_aos(self._h, _dp(s), s.shape[0], BODY_BYTES, ctypes.byref(kept),
+                                          ctypes.byref(it), _dp(res)))
+        return kept.value, it.value, res
 
     def autotune(self, ncrits=(64, 96, 128, 160, 192, 256), reps=2):
         """Time build + eval on the bodies already set for each candidate ncrit and M2L plan
