@@ -1346,6 +1346,83 @@ void orc_direct(int64_t m, const double *xt, int64_t n, const double *xs, const 
   }
 }
 
+// Kernel functions of one pair with the source core sigma: f1 = g / (4 pi R^3) (u = -f1 r x a),
+// f2 = 2 df1/dR^2 = (t2/sigma^2 - 3 g/R^2) / (4 pi R^3) (the J factor of multipoleKernels.hpp:
+// 283-296), f3 = 2 df2/dR^2 = (15 g/R^4 - t2 (5 + 2 s^2) / (sigma^2 R^2)) / (4 pi R^3),
+// s^2 = R^2 / (2 sigma^2), g = erf(s) - t2, t2 = 2 s e^{-s^2} / sqrt(pi).
+static inline void pair_f123(real_t R2, real_t sigma, real_t &f1, real_t &f2, real_t &f3)
+{
+  const real_t S2 = sigma * sigma, s2 = R2 / S2 / 2.0, s = std::sqrt(s2);
+  const real_t t2 = 2.0 / std::sqrt(M_PI) * s * std::exp(-s2);
+  const real_t g  = std::erf(s) - t2;
+  const real_t q  = 0.25 / M_PI / R2 / std::sqrt(R2);
+  f1              = g * q;
+  f2              = (t2 / S2 - 3.0 * g / R2) * q;
+  f3              = (15.0 * g / (R2 * R2) - t2 * (5.0 + 2.0 * s2) / (S2 * R2)) * q;
+}
+
+// Vector-Jacobian product of the direct-sum rates u_i = -sum_j f1 (r_ij x a_j) and transposed
+// stretching dadt_i = J_i^T a_i (r_ij = x_i - x_j, kernel of sigma_j): with cotangents ubar,
+// abar it returns xbar = d/dx and alphabar = d/dalpha of S = sum_i ubar_i.u_i + abar_i.dadt_i.
+// Per pair, with Jbar_i = a_i abar_i^T and e = axial vector of Jbar (e_k = eps_abk Jbar_ab):
+//   S_ij = -a_j . W,   W = f1 (ubar x r + e) + f2 (Jbar r) x r,
+//   dS_ij/dr = -[ f2 (a.(ubar x r + e)) r + f1 (a x ubar) + f3 (a.((Jbar r) x r)) r
+//                 + f2 Jbar^T (r x a) + f2 a x (Jbar r) ],
+// so alphabar_j -= W, xbar_i += dS/dr, xbar_j -= dS/dr; plus alphabar_i += J_i abar_i.
+void orc_direct_vjp(int64_t n, const double *X, const double *A, const double *S, const double *ub, const double *ab,
+                    double *xbar, double *alphabar)
+{
+  std::vector<real_t> J(9 * n), u(3 * n);
+  orc_direct(n, X, n, X, A, S, u.data(), J.data());
+  std::vector<real_t> Jb(9 * n), e(3 * n);
+  for (int64_t i = 0; i < n; i++) {
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) Jb[9 * i + 3 * a + b] = A[3 * i + a] * ab[3 * i + b];
+    const real_t *M = &Jb[9 * i];
+    e[3 * i + 0]    = M[3 * 1 + 2] - M[3 * 2 + 1];
+    e[3 * i + 1]    = M[3 * 2 + 0] - M[3 * 0 + 2];
+    e[3 * i + 2]    = M[3 * 0 + 1] - M[3 * 1 + 0];
+    for (int a = 0; a < 3; a++) {
+      alphabar[3 * i + a] = 0;
+      xbar[3 * i + a]     = 0;
+    }
+  }
+  for (int64_t i = 0; i < n; i++)
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) alphabar[3 * i + a] += J[9 * i + 3 * a + b] * ab[3 * i + b];
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = 0; j < n; j++) {
+      real_t r[3];
+      for (int d = 0; d < 3; d++) r[d] = X[3 * i + d] - X[3 * j + d];
+      const real_t R2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      if (R2 == 0) continue;
+      real_t f1, f2, f3;
+      pair_f123(R2, S[j], f1, f2, f3);
+      const real_t *M = &Jb[9 * i], *uu = &ub[3 * i], *ee = &e[3 * i], *a = &A[3 * j];
+      const real_t  Jr[3]  = {M[0] * r[0] + M[1] * r[1] + M[2] * r[2], M[3] * r[0] + M[4] * r[1] + M[5] * r[2],
+                              M[6] * r[0] + M[7] * r[1] + M[8] * r[2]};
+      const real_t  uxr[3] = {uu[1] * r[2] - uu[2] * r[1], uu[2] * r[0] - uu[0] * r[2], uu[0] * r[1] - uu[1] * r[0]};
+      const real_t  Jrxr[3] = {Jr[1] * r[2] - Jr[2] * r[1], Jr[2] * r[0] - Jr[0] * r[2], Jr[0] * r[1] - Jr[1] * r[0]};
+      real_t        W[3];
+      for (int d = 0; d < 3; d++) W[d] = f1 * (uxr[d] + ee[d]) + f2 * Jrxr[d];
+      const real_t rxa[3] = {r[1] * a[2] - r[2] * a[1], r[2] * a[0] - r[0] * a[2], r[0] * a[1] - r[1] * a[0]};
+      const real_t axu[3] = {a[1] * uu[2] - a[2] * uu[1], a[2] * uu[0] - a[0] * uu[2], a[0] * uu[1] - a[1] * uu[0]};
+      const real_t axJr[3] = {a[1] * Jr[2] - a[2] * Jr[1], a[2] * Jr[0] - a[0] * Jr[2], a[0] * Jr[1] - a[1] * Jr[0]};
+      const real_t c1 = a[0] * (uxr[0] + ee[0]) + a[1] * (uxr[1] + ee[1]) + a[2] * (uxr[2] + ee[2]);
+      const real_t c3 = a[0] * Jrxr[0] + a[1] * Jrxr[1] + a[2] * Jrxr[2];
+      real_t       dS[3];
+      for (int d = 0; d < 3; d++) {
+        const real_t JTrxa = M[0 * 3 + d] * rxa[0] + M[1 * 3 + d] * rxa[1] + M[2 * 3 + d] * rxa[2];
+        dS[d] = -(f2 * c1 * r[d] + f1 * axu[d] + f3 * c3 * r[d] + f2 * JTrxa + f2 * axJr[d]);
+      }
+      for (int d = 0; d < 3; d++) {
+        alphabar[3 * j + d] -= W[d];
+        xbar[3 * i + d] += dS[d];
+        xbar[3 * j + d] -= dS[d];
+      }
+    }
+}
+
 // O(M*N) direct sum with the host formula (multipoleKernels.hpp:174-221): u and classic
 // stretching dadt = J*alpha_i at the targets, with target strengths at_.
 void orc_direct_classic(int64_t m, const double *xt, const double *at, int64_t n, const double *xs, const double *as,

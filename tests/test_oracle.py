@@ -374,6 +374,41 @@ def test_direct_adjoint_matches_finite_differences(orc):
             assert abs(fd - ga[j, d]) < 1e-7 * max(1.0, abs(ga[j]).max()), (j, d, fd, ga[j, d])
 
 
+# ------------------------------------------------------------------- adjoint of the particle rates
+def test_rate_vjp_matches_finite_differences(orc):
+    """direct_vjp (cotangents ubar, abar of u and the transposed stretching J^T alpha) against
+    central FD of S = sum ubar.u + abar.(J^T alpha) evaluated with the direct sum.  A wrong
+    sign, a dropped f3 term or a transposed Jbar each moves the result by O(1)."""
+    rng = np.random.default_rng(0)
+    n = 40
+    X = rng.normal(size=(n, 3)) * 0.3
+    A = rng.normal(size=(n, 3))
+    S = rng.uniform(0.1, 0.3, n)
+    ub = rng.normal(size=(n, 3))
+    ab = rng.normal(size=(n, 3))
+
+    def phi(X, A):
+        u, J = orc.direct(X, X, A, S)
+        dadt = np.einsum("iab,ia->ib", J.reshape(-1, 3, 3), A)
+        return np.sum(ub * u) + np.sum(ab * dadt)
+
+    xb, alb = orc.direct_vjp(X, A, S, ub, ab)
+    h = 1e-6
+    fdx, fda = np.zeros_like(X), np.zeros_like(A)
+    for i in range(n):
+        for d in range(3):
+            Xp, Xm = X.copy(), X.copy()
+            Xp[i, d] += h
+            Xm[i, d] -= h
+            fdx[i, d] = (phi(Xp, A) - phi(Xm, A)) / (2 * h)
+            Ap, Am = A.copy(), A.copy()
+            Ap[i, d] += h
+            Am[i, d] -= h
+            fda[i, d] = (phi(X, Ap) - phi(X, Am)) / (2 * h)
+    assert rel(xb, fdx) < 1e-8
+    assert rel(alb, fda) < 1e-8
+
+
 # ------------------------------------------------------------------- viscous schemes, vorticity, RBF
 def _lattice(L, h):
     g = (np.arange(-L, L) + 0.5) * h  # even count: no point on the root centre or its axes
