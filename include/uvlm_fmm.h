@@ -71,6 +71,7 @@ typedef struct uvlm_timing {
   double  adjoint_ms;     /* last uvlm_probe_adjoint, same stages transposed */
   int64_t ncrit;          /* leaf size in use (uvlm_autotune may change it) */
   int64_t m2l_slots;      /* M2L accumulator slots per lane of the plan in use */
+  double  vjp_ms;         /* last uvlm_eval_vjp: the adjoint pass (forward eval excluded) */
 } uvlm_timing;
 
 typedef struct uvlm_handle_s *uvlm_handle;
@@ -146,6 +147,34 @@ int uvlm_direct(uvlm_handle h, int64_t m, const double *xt, const double *at, do
  * uvlm_build; uses none of the forward results. */
 int uvlm_probe_adjoint(uvlm_handle h, int64_t m, const double *probes, const double *weights, double *dJ_dalpha,
                        double *dJ_dx);
+/* Vector-Jacobian product of one evaluation (cVPM, inviscid; both stretching forms): with
+ * cotangents ubar [n][3] of the velocity and abar [n][3] of dadt, returns
+ *   xbar = dS/dx and alphabar = dS/dalpha of S = sum_i ubar_i . u_i + abar_i . dadt_i,
+ * for the tree and lists of the last uvlm_build held fixed (the exact gradient of the FMM
+ * approximation, expansions included), all in the CURRENT tree order.  Evaluates the rates
+ * first (uvlm_eval), then runs the adjoint in FP64: the P2P pair VJP on the forward and the
+ * transposed P2P lists, the far field of adjoint vortices ubar and vortex dipoles
+ * Jbar = alpha abar^T (classic: abar alpha^T) through P2M, M2M, the transposed M2L list, L2L
+ * and L2P, and a degree-3 L2P of the forward local expansions for the target-side terms.
+ * ubar or abar may be NULL (zero).  No counterpart in the reference (its solver has no
+ * adjoint); replaces differentiating FMMCalculator::calculate (fmmCalculator.hpp:420-467). */
+int uvlm_eval_vjp(uvlm_handle h, int64_t n, const double *ubar, const double *abar, double *xbar, double *alphabar);
+/* Vector-Jacobian product of one uvlm_advance(dt) (the handle's scheme, cVPM, inviscid) from the
+ * current particle state: on entry xbar/alphabar [n][3] hold the cotangent of the state after
+ * the step, on exit that of the state before it.  Both are indexed by the ORIGINAL body order
+ * (the ids uvlm_get_permutation reports), because a step reorders the bodies.  The stage
+ * states are recomputed; each stage's tree is rebuilt as in the forward step and held fixed
+ * in its VJP.  The particle state is left at the step's start (tree not built).  No
+ * counterpart in the reference; differentiates FMMCalculator::advance (fmmCalculator.hpp:
+ * 919-1005). */
+int uvlm_step_vjp(uvlm_handle h, double dt, int64_t n, double *xbar, double *alphabar);
+/* Checkpointed trajectory: uvlm_trajectory_forward advances `steps` steps of dt from the current
+ * state, keeping (X, alpha, sigma) in device memory every `every` steps; uvlm_trajectory_vjp then
+ * maps the cotangent of the final state (xbar/alphabar [n][3], original body order, in place)
+ * to that of the initial state, recomputing each segment from its checkpoint.  The particles
+ * end at the final state again.  No counterpart in the reference. */
+int uvlm_trajectory_forward(uvlm_handle h, int64_t steps, double dt, int64_t every);
+int uvlm_trajectory_vjp(uvlm_handle h, int64_t n, double *xbar, double *alphabar);
 /* Overwrite positions and strengths [n][3] in the CURRENT tree order without rebuilding
  * the tree or lists (frozen-tree perturbations, e.g. finite differences).  No counterpart
  * in the reference. */

@@ -57,7 +57,7 @@ def lib():
             "orc_get_bodies": (I, [VP, P, L64]),
             "orc_advance": (I, [VP, P, L64, D, I]),
             "orc_stage": (I, [VP, P, L64, D, I, I]),
-            "orc_direct_vjp": (None, [L64, P, P, P, P, P, P, P]),
+            "orc_direct_vjp": (None, [L64, P, P, P, P, P, I, P, P]),
             "orc_set_viscosity": (None, [VP, I, D]),
             "orc_viscosity": (I, [VP, D]),
             "orc_vorticity": (I, [VP, P, L64]),
@@ -306,13 +306,13 @@ def direct_classic(xt, at, xs, alpha_s, sigma_s):
     return u, dadt
 
 
-def direct_vjp(X, alpha, sigma, ubar, abar):
-    """Exact VJP of the direct-sum rates (u, transposed stretching dadt = J^T alpha) with
-    cotangents ubar, abar (n, 3): returns (xbar, alphabar)."""
+def direct_vjp(X, alpha, sigma, ubar, abar, stretching="transposed"):
+    """Exact VJP of the direct-sum rates (u, stretching dadt = J^T alpha, or J alpha when
+    stretching == "classic") with cotangents ubar, abar (n, 3): returns (xbar, alphabar)."""
     X, A, S, ub, ab = _c(X), _c(alpha), _c(sigma), _c(ubar), _c(abar)
     n = X.shape[0]
     xb, alb = np.zeros((n, 3)), np.zeros((n, 3))
-    lib().orc_direct_vjp(n, _p(X), _p(A), _p(S), _p(ub), _p(ab), _p(xb), _p(alb))
+    lib().orc_direct_vjp(n, _p(X), _p(A), _p(S), _p(ub), _p(ab), int(stretching == "classic"), _p(xb), _p(alb))
     return xb, alb
 
 

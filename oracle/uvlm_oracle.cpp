@@ -1361,23 +1361,27 @@ static inline void pair_f123(real_t R2, real_t sigma, real_t &f1, real_t &f2, re
   f3              = (15.0 * g / (R2 * R2) - t2 * (5.0 + 2.0 * s2) / (S2 * R2)) * q;
 }
 
-// Vector-Jacobian product of the direct-sum rates u_i = -sum_j f1 (r_ij x a_j) and transposed
-// stretching dadt_i = J_i^T a_i (r_ij = x_i - x_j, kernel of sigma_j): with cotangents ubar,
+// Vector-Jacobian product of the direct-sum rates u_i = -sum_j f1 (r_ij x a_j) and the
+// stretching dadt_i = J_i^T a_i (transposed; classic: J_i a_i) (r_ij = x_i - x_j, kernel of
+// sigma_j): with cotangents ubar,
 // abar it returns xbar = d/dx and alphabar = d/dalpha of S = sum_i ubar_i.u_i + abar_i.dadt_i.
-// Per pair, with Jbar_i = a_i abar_i^T and e = axial vector of Jbar (e_k = eps_abk Jbar_ab):
+// Per pair, with Jbar_i = a_i abar_i^T (classic: abar_i a_i^T) and e = axial vector of Jbar
+// (e_k = eps_abk Jbar_ab):
 //   S_ij = -a_j . W,   W = f1 (ubar x r + e) + f2 (Jbar r) x r,
 //   dS_ij/dr = -[ f2 (a.(ubar x r + e)) r + f1 (a x ubar) + f3 (a.((Jbar r) x r)) r
 //                 + f2 Jbar^T (r x a) + f2 a x (Jbar r) ],
-// so alphabar_j -= W, xbar_i += dS/dr, xbar_j -= dS/dr; plus alphabar_i += J_i abar_i.
+// so alphabar_j -= W, xbar_i += dS/dr, xbar_j -= dS/dr; plus alphabar_i += J_i abar_i
+// (classic: J_i^T abar_i).
 void orc_direct_vjp(int64_t n, const double *X, const double *A, const double *S, const double *ub, const double *ab,
-                    double *xbar, double *alphabar)
+                    int classic, double *xbar, double *alphabar)
 {
   std::vector<real_t> J(9 * n), u(3 * n);
   orc_direct(n, X, n, X, A, S, u.data(), J.data());
   std::vector<real_t> Jb(9 * n), e(3 * n);
   for (int64_t i = 0; i < n; i++) {
     for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++) Jb[9 * i + 3 * a + b] = A[3 * i + a] * ab[3 * i + b];
+      for (int b = 0; b < 3; b++)
+        Jb[9 * i + 3 * a + b] = classic ? ab[3 * i + a] * A[3 * i + b] : A[3 * i + a] * ab[3 * i + b];
     const real_t *M = &Jb[9 * i];
     e[3 * i + 0]    = M[3 * 1 + 2] - M[3 * 2 + 1];
     e[3 * i + 1]    = M[3 * 2 + 0] - M[3 * 0 + 2];
@@ -1389,7 +1393,8 @@ void orc_direct_vjp(int64_t n, const double *X, const double *A, const double *S
   }
   for (int64_t i = 0; i < n; i++)
     for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++) alphabar[3 * i + a] += J[9 * i + 3 * a + b] * ab[3 * i + b];
+      for (int b = 0; b < 3; b++)
+        alphabar[3 * i + a] += (classic ? J[9 * i + 3 * b + a] : J[9 * i + 3 * a + b]) * ab[3 * i + b];
   for (int64_t i = 0; i < n; i++)
     for (int64_t j = 0; j < n; j++) {
       real_t r[3];

@@ -375,10 +375,11 @@ def test_direct_adjoint_matches_finite_differences(orc):
 
 
 # ------------------------------------------------------------------- adjoint of the particle rates
-def test_rate_vjp_matches_finite_differences(orc):
-    """direct_vjp (cotangents ubar, abar of u and the transposed stretching J^T alpha) against
-    central FD of S = sum ubar.u + abar.(J^T alpha) evaluated with the direct sum.  A wrong
-    sign, a dropped f3 term or a transposed Jbar each moves the result by O(1)."""
+@pytest.mark.parametrize("stretching", ["transposed", "classic"])
+def test_rate_vjp_matches_finite_differences(orc, stretching):
+    """direct_vjp (cotangents ubar, abar of u and of the stretching J^T alpha or J alpha) against
+    central FD of S = sum ubar.u + abar.dadt evaluated with the direct sum.  A wrong sign, a
+    dropped f3 term or a transposed Jbar each moves the result by O(1)."""
     rng = np.random.default_rng(0)
     n = 40
     X = rng.normal(size=(n, 3)) * 0.3
@@ -389,10 +390,11 @@ def test_rate_vjp_matches_finite_differences(orc):
 
     def phi(X, A):
         u, J = orc.direct(X, X, A, S)
-        dadt = np.einsum("iab,ia->ib", J.reshape(-1, 3, 3), A)
+        sub = "iab,ib->ia" if stretching == "classic" else "iab,ia->ib"
+        dadt = np.einsum(sub, J.reshape(-1, 3, 3), A)
         return np.sum(ub * u) + np.sum(ab * dadt)
 
-    xb, alb = orc.direct_vjp(X, A, S, ub, ab)
+    xb, alb = orc.direct_vjp(X, A, S, ub, ab, stretching)
     h = 1e-6
     fdx, fda = np.zeros_like(X), np.zeros_like(A)
     for i in range(n):

@@ -18,7 +18,8 @@ EXPORTS = [
     "uvlm_build", "uvlm_eval", "uvlm_stage", "uvlm_advance", "uvlm_num_cells", "uvlm_get_tree",
     "uvlm_get_lists", "uvlm_get_expansions", "uvlm_sensors_aos", "uvlm_direct", "uvlm_probe_adjoint",
     "uvlm_update_bodies_soa", "uvlm_relax", "uvlm_diagnostics", "uvlm_timings", "uvlm_sync", "uvlm_autotune",
-    "uvlm_set_viscosity", "uvlm_viscosity", "uvlm_vorticity_aos", "uvlm_rbf_aos",
+    "uvlm_set_viscosity", "uvlm_viscosity", "uvlm_vorticity_aos", "uvlm_rbf_aos", "uvlm_eval_vjp",
+    "uvlm_step_vjp", "uvlm_trajectory_forward", "uvlm_trajectory_vjp",
 ]
 
 
@@ -40,7 +41,8 @@ class Timing(ctypes.Structure):
                 ("eval_ms", ctypes.c_double), ("n_bodies", ctypes.c_int64), ("n_cells", ctypes.c_int64),
                 ("n_leaves", ctypes.c_int64), ("n_p2p_pairs", ctypes.c_int64), ("n_m2l_pairs", ctypes.c_int64),
                 ("n_levels", ctypes.c_int64), ("n_interactions", ctypes.c_int64), ("sensor_ms", ctypes.c_double),
-                ("adjoint_ms", ctypes.c_double), ("ncrit", ctypes.c_int64), ("m2l_slots", ctypes.c_int64)]
+                ("adjoint_ms", ctypes.c_double), ("ncrit", ctypes.c_int64), ("m2l_slots", ctypes.c_int64),
+                ("vjp_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -87,6 +89,10 @@ def lib():
             "uvlm_viscosity": (I, [VP, D]),
             "uvlm_vorticity_aos": (I, [VP, VP, I64, I64]),
             "uvlm_rbf_aos": (I, [VP, VP, I64, I64, PI64, ctypes.POINTER(ctypes.c_int), PD]),
+            "uvlm_eval_vjp": (I, [VP, I64, PD, PD, PD, PD]),
+            "uvlm_step_vjp": (I, [VP, D, I64, PD, PD]),
+            "uvlm_trajectory_forward": (I, [VP, I64, D, I64]),
+            "uvlm_trajectory_vjp": (I, [VP, I64, PD, PD]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -49,6 +49,14 @@ struct uvlm_handle_s {
   // viscous scheme applied after each eval of uvlm_advance (UVLM_INVISCID / UVLM_PSE / UVLM_CSM)
   int    viscous = UVLM_INVISCID;
   double nu      = 0.0;
+  // adjoint of the rates and of the time step: transposed lists, adjoint fields, stage states
+  // (X, alpha, sigma + original index, per stage), state cotangents in the original order
+  Lists   tlists;
+  DevBuf  vaux, vstage, vorig, vcot, vlam;
+  // trajectory checkpoints (X, alpha, sigma in the original order, every `ck_every` steps)
+  DevBuf  ck;
+  int64_t ck_n = 0, ck_steps = 0, ck_every = 0, ck_N = 0;
+  double  ck_dt = 0.0;
 
   double *F(int f) { return st.as<double>() + (int64_t)f * cap; }
   void    ensure_cap(int64_t n)
@@ -234,6 +242,197 @@ void probe_vorticity(uvlm_handle h, int64_t m)
 }
 
 int stages_of(int scheme) { return scheme == UVLM_EULER ? 1 : scheme == UVLM_RK2 ? 2 : scheme == UVLM_RK3 ? 3 : 4; }
+
+// The stages of one step as a Butcher tableau (y1 = y0 + dt sum_s b_s f_s, stage states
+// Y_s = y0 + dt sum_{r<s} a_sr f_r), derived from the register updates of the rk_stage kernel
+// (fmmCalculator.hpp:511-711): RK2 Heun; RK3 Williamson 2N with A = (0, -5/9, -153/128),
+// B = (1/3, 15/16, 8/15); RK4 as the reference writes it (every intermediate state at dt/2).
+struct Tableau {
+  int    S = 1;
+  double a[4][4]{}, b[4]{};
+};
+Tableau tableau_of(int scheme)
+{
+  Tableau t;
+  t.S = stages_of(scheme);
+  if (scheme == UVLM_EULER) {
+    t.b[0] = 1.0;
+  } else if (scheme == UVLM_RK2) {
+    t.a[1][0] = 1.0;
+    t.b[0] = t.b[1] = 0.5;
+  } else if (scheme == UVLM_RK3) {
+    // z_1 = dt f_0; z_k = A_k z_{k-1} + dt f_{k-1}; y <- y + B_k z_k after each stage
+    const double A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0}, B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+    double       z[3] = {0, 0, 0}, y[3] = {0, 0, 0};  // coefficients of f_0..f_2 (per dt)
+    for (int k = 0; k < 3; k++) {
+      for (int r = 0; r < 3; r++) z[r] *= A[k];
+      z[k] += 1.0;
+      for (int r = 0; r < 3; r++) y[r] += B[k] * z[r];
+      if (k < 2)
+        for (int r = 0; r < 3; r++) t.a[k + 1][r] = y[r];
+    }
+    for (int r = 0; r < 3; r++) t.b[r] = y[r];
+  } else {
+    t.a[1][0] = t.a[2][1] = t.a[3][2] = 0.5;
+    t.b[0] = t.b[3] = 1.0 / 6.0;
+    t.b[1] = t.b[2] = 1.0 / 3.0;
+  }
+  return t;
+}
+
+// Vector-Jacobian product of the last do_eval (forward J in the state, forward L in h->L):
+// on entry the adjoint fields hold ubar (0-2) and abar (15-17) in the device order, on exit
+// xbar (12-14) and alphabar (15-17).  M and L are overwritten by the adjoint expansions.
+void vjp_core(uvlm_handle h)
+{
+  hipStream_t s = h->s;
+  const int   P = h->prm.P, NT = h->NT;
+  const Tree &t = h->tree;
+  const int64_t cap = h->cap, N = h->N;
+  double       *aux = h->vaux.as<double>(), *st = h->st.as<double>();
+  vjp_setup(s, N, st, aux, cap, h->prm.stretching == UVLM_CLASSIC);
+  // target side: far field from the forward local expansions, then the near pairs
+  l2p_adj_tgt(s, P, t, h->L.as<double2>(), st, aux, cap);
+  p2p_adj(s, t, h->lists, 0, st, aux, cap);
+  // source side: adjoint sources up the tree, across the transposed M2L list, down to the bodies
+  transpose_lists(s, h->sc, t, t, h->lists, h->tlists);
+  double2 *M = h->M.as<double2>(), *L = h->L.as<double2>();
+  p2m_adj(s, P, t, st, aux, cap, M);
+  for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M);
+  zero_doubles(s, (double *)M, 2 * NT);
+  m2l(s, P, t, t, h->tlists, h->sched, M, L);
+  for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L);
+  h->ast.reserve(sizeof(double) * N_FIELDS * cap);
+  double *a = h->ast.as<double>();
+  zero_doubles(s, a + F_J * cap, 9 * cap);
+  zero_doubles(s, a + F_U * cap, 3 * cap);
+  UVLM_HIP(hipMemcpyAsync(a + F_X * cap, st + F_X * cap, 8 * 6 * cap, hipMemcpyDeviceToDevice, s));
+  UVLM_HIP(hipMemcpyAsync(a + F_SIG * cap, st + F_SIG * cap, 8 * cap, hipMemcpyDeviceToDevice, s));
+  l2p_stretch(s, P, t, L, a, cap, UVLM_TRANSPOSED, UVLM_CVPM, 0);
+  p2p_adj(s, t, h->tlists, 1, st, aux, cap);
+  vjp_finish(s, N, a, aux, cap);
+  h->evaluated = false;  // M and L hold adjoint expansions
+}
+
+void check_vjp_config(uvlm_handle h)
+{
+  if (h->prm.formulation != UVLM_CVPM) throw Error(UVLM_ERR_ARG, "the rate VJP covers the cVPM formulation");
+  if (h->viscous != UVLM_INVISCID) throw Error(UVLM_ERR_ARG, "the rate VJP covers inviscid steps");
+}
+
+// State of the current bodies (X, alpha, sigma and original index, device order) into slot k
+// of a store of 7-field records, and back.
+void save_state(uvlm_handle h, DevBuf &store, DevBuf &ostore, int k)
+{
+  const int64_t cap = h->cap;
+  double       *dst = store.as<double>() + (int64_t)k * 7 * cap;
+  UVLM_HIP(hipMemcpyAsync(dst, h->F(F_X), 8 * 6 * cap, hipMemcpyDeviceToDevice, h->s));
+  UVLM_HIP(hipMemcpyAsync(dst + 6 * cap, h->F(F_SIG), 8 * cap, hipMemcpyDeviceToDevice, h->s));
+  UVLM_HIP(hipMemcpyAsync(ostore.as<int32_t>() + (int64_t)k * cap, h->orig.p, 4 * cap, hipMemcpyDeviceToDevice, h->s));
+}
+void load_state(uvlm_handle h, const DevBuf &store, const DevBuf &ostore, int k)
+{
+  const int64_t cap = h->cap;
+  const double *src = store.as<double>() + (int64_t)k * 7 * cap;
+  UVLM_HIP(hipMemcpyAsync(h->F(F_X), src, 8 * 6 * cap, hipMemcpyDeviceToDevice, h->s));
+  UVLM_HIP(hipMemcpyAsync(h->F(F_SIG), src + 6 * cap, 8 * cap, hipMemcpyDeviceToDevice, h->s));
+  UVLM_HIP(hipMemcpyAsync(h->orig.p, ostore.as<int32_t>() + (int64_t)k * cap, 4 * cap, hipMemcpyDeviceToDevice, h->s));
+  h->built = h->evaluated = false;
+}
+
+// Cotangent of one step of uvlm_advance: on entry cot (6 fields of cap doubles, original order)
+// holds the cotangent of the state after the step, on exit that of the state before it.  The
+// bodies are left at the step's start.  Stage states are recomputed, and in reverse each stage
+// rebuilds its own tree (the build is deterministic, so it is the forward's tree).
+void step_vjp(uvlm_handle h, double dt, double *cot)
+{
+  hipStream_t   s   = h->s;
+  const int64_t cap = h->cap, N = h->N;
+  const Tableau T   = tableau_of(h->prm.scheme);
+  h->vaux.reserve(sizeof(double) * N_ADJ_FIELDS * cap);
+  h->vstage.reserve(sizeof(double) * 7 * cap * T.S);
+  h->vorig.reserve(sizeof(int32_t) * cap * T.S);
+  h->vcot.reserve(sizeof(double) * 6 * cap * (T.S + 1));
+  // slot s: the state stage s builds and evaluates at (slot 0 = the step's start)
+  save_state(h, h->vstage, h->vorig, 0);
+  for (int k = 0; k + 1 < T.S; k++) {
+    do_build(h);
+    do_eval(h, false);
+    rk_stage(s, h->st.as<double>(), cap, N, h->prm.scheme, k, dt, 0);
+    save_state(h, h->vstage, h->vorig, k + 1);
+  }
+  double *Yb = h->vcot.as<double>();  // Ybar_s at Yb + s * 6 cap; fbar at Yb + S * 6 cap
+  double *fb = Yb + (int64_t)T.S * 6 * cap;
+  for (int k = T.S - 1; k >= 0; k--) {
+    const double *p[6];
+    double        c[6];
+    int           n = 0;
+    p[n]            = cot;
+    c[n++]          = dt * T.b[k];
+    for (int r = k + 1; r < T.S; r++)
+      if (T.a[r][k] != 0.0) {
+        p[n]   = Yb + (int64_t)r * 6 * cap;
+        c[n++] = dt * T.a[r][k];
+      }
+    lincomb(s, 6 * cap, fb, n, p, c);
+    load_state(h, h->vstage, h->vorig, k);
+    do_build(h);
+    do_eval(h, false);
+    cot_gather(s, N, h->orig.as<int32_t>(), fb, cap, h->vaux.as<double>(), cap);
+    vjp_core(h);
+    cot_scatter(s, N, h->orig.as<int32_t>(), h->vaux.as<double>(), cap, Yb + (int64_t)k * 6 * cap, cap);
+  }
+  const double *p[6];
+  double        c[6];
+  p[0] = cot;
+  c[0] = 1.0;
+  for (int k = 0; k < T.S; k++) {
+    p[k + 1] = Yb + (int64_t)k * 6 * cap;
+    c[k + 1] = 1.0;
+  }
+  lincomb(s, 6 * cap, fb, T.S + 1, p, c);
+  UVLM_HIP(hipMemcpyAsync(cot, fb, 8 * 6 * cap, hipMemcpyDeviceToDevice, s));
+  load_state(h, h->vstage, h->vorig, 0);
+}
+
+// The particle state (X, alpha, sigma) as 7 fields of N doubles in the ORIGINAL body order,
+// and back with the original index as the device order.
+void save_state_orig(uvlm_handle h, double *dst) { state_to_orig(h->s, h->N, h->orig.as<int32_t>(), h->st.as<double>(), h->cap, dst); }
+void load_state_orig(uvlm_handle h, const double *src)
+{
+  const int64_t N = h->N;
+  for (int f = 0; f < 6; f++)
+    UVLM_HIP(hipMemcpyAsync(h->F(F_X + f), src + f * N, 8 * N, hipMemcpyDeviceToDevice, h->s));
+  UVLM_HIP(hipMemcpyAsync(h->F(F_SIG), src + 6 * N, 8 * N, hipMemcpyDeviceToDevice, h->s));
+  iota(h->s, h->orig.as<int32_t>(), N);
+  h->built = h->evaluated = false;
+}
+
+// Host AoS (n x 3) pairs <-> 6 SoA fields of `cap` doubles on the device.
+void upload_cot(uvlm_handle h, const double *xb, const double *ab, double *dev)
+{
+  const int64_t       n = h->N, cap = h->cap;
+  std::vector<double> buf((size_t)6 * cap, 0.0);
+  for (int64_t i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      buf[(size_t)d * cap + i]       = xb ? xb[3 * i + d] : 0.0;
+      buf[(size_t)(3 + d) * cap + i] = ab ? ab[3 * i + d] : 0.0;
+    }
+  UVLM_HIP(hipMemcpyAsync(dev, buf.data(), 8 * buf.size(), hipMemcpyHostToDevice, h->s));
+  UVLM_HIP(hipStreamSynchronize(h->s));
+}
+void download_cot(uvlm_handle h, const double *dev, double *xb, double *ab)
+{
+  const int64_t       n = h->N, cap = h->cap;
+  std::vector<double> buf((size_t)6 * cap);
+  UVLM_HIP(hipMemcpyAsync(buf.data(), dev, 8 * buf.size(), hipMemcpyDeviceToHost, h->s));
+  UVLM_HIP(hipStreamSynchronize(h->s));
+  for (int64_t i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      if (xb) xb[3 * i + d] = buf[(size_t)d * cap + i];
+      if (ab) ab[3 * i + d] = buf[(size_t)(3 + d) * cap + i];
+    }
+}
 
 void check_handle(uvlm_handle h)
 {
@@ -809,6 +1008,119 @@ int uvlm_diagnostics(uvlm_handle h, double *out)
     set_device(h);
     h->sc.partial.reserve(sizeof(double) * 10 * 512);
     diagnostics(h->s, h->st.as<double>(), h->cap, h->N, h->sc.partial.as<double>(), out);
+  });
+}
+
+int uvlm_eval_vjp(uvlm_handle h, int64_t n, const double *ubar, const double *abar, double *xbar, double *alphabar)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!h->built) throw Error(UVLM_ERR_STATE, "uvlm_eval_vjp needs a uvlm_build");
+    if (n != h->N) throw Error(UVLM_ERR_ARG, "size mismatch");
+    check_vjp_config(h);
+    set_device(h);
+    const int64_t cap = h->cap;
+    do_eval(h, true);
+    h->vaux.reserve(sizeof(double) * N_ADJ_FIELDS * cap);
+    h->vlam.reserve(sizeof(double) * 6 * cap);
+    upload_cot(h, ubar, abar, h->vlam.as<double>());
+    double *aux = h->vaux.as<double>(), *lam = h->vlam.as<double>();
+    UVLM_HIP(hipMemcpyAsync(aux, lam, 8 * 3 * cap, hipMemcpyDeviceToDevice, h->s));
+    UVLM_HIP(hipMemcpyAsync(aux + 15 * cap, lam + 3 * cap, 8 * 3 * cap, hipMemcpyDeviceToDevice, h->s));
+    UVLM_HIP(hipEventRecord(h->aev[0], h->s));
+    vjp_core(h);
+    UVLM_HIP(hipEventRecord(h->aev[1], h->s));
+    UVLM_HIP(hipMemcpyAsync(lam, aux + 12 * cap, 8 * 6 * cap, hipMemcpyDeviceToDevice, h->s));
+    download_cot(h, lam, xbar, alphabar);
+    h->tm.vjp_ms = ev_ms2(h, 0, 1);
+  });
+}
+
+int uvlm_step_vjp(uvlm_handle h, double dt, int64_t n, double *xbar, double *alphabar)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (n != h->N || !xbar || !alphabar) throw Error(UVLM_ERR_ARG, "size mismatch or null cotangent");
+    check_vjp_config(h);
+    set_device(h);
+    h->vlam.reserve(sizeof(double) * 6 * h->cap);
+    upload_cot(h, xbar, alphabar, h->vlam.as<double>());
+    step_vjp(h, dt, h->vlam.as<double>());
+    download_cot(h, h->vlam.as<double>(), xbar, alphabar);
+  });
+}
+
+int uvlm_trajectory_forward(uvlm_handle h, int64_t steps, double dt, int64_t every)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (h->N <= 0) throw Error(UVLM_ERR_STATE, "no bodies set");
+    if (steps <= 0 || every <= 0) throw Error(UVLM_ERR_ARG, "steps and every must be positive");
+    check_vjp_config(h);
+    set_device(h);
+    hipStream_t   s = h->s;
+    const int64_t N = h->N, cap = h->cap, nck = (steps + every - 1) / every;
+    h->ck.reserve(sizeof(double) * 7 * N * nck);
+    double *ck = h->ck.as<double>();
+    for (int64_t k = 0; k < steps; k++) {
+      if (k % every == 0) {
+        // checkpoint in the original order: scatter positions/strengths by the original index
+        save_state_orig(h, ck + (k / every) * 7 * N);
+      }
+      for (int st = 0; st < stages_of(h->prm.scheme); st++) {
+        do_build(h);
+        do_eval(h, false);
+        rk_stage(s, h->st.as<double>(), cap, N, h->prm.scheme, st, dt, 0);
+      }
+    }
+    UVLM_HIP(hipStreamSynchronize(s));
+    h->evaluated = false;
+    h->ck_n      = nck;
+    h->ck_steps  = steps;
+    h->ck_every  = every;
+    h->ck_N      = N;
+    h->ck_dt     = dt;
+  });
+}
+
+int uvlm_trajectory_vjp(uvlm_handle h, int64_t n, double *xbar, double *alphabar)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (h->ck_n == 0 || h->ck_N != h->N) throw Error(UVLM_ERR_STATE, "uvlm_trajectory_vjp needs a uvlm_trajectory_forward");
+    if (n != h->N || !xbar || !alphabar) throw Error(UVLM_ERR_ARG, "size mismatch or null cotangent");
+    check_vjp_config(h);
+    set_device(h);
+    hipStream_t   s = h->s;
+    const int64_t N = h->N, cap = h->cap, E = h->ck_every;
+    const double  dt = h->ck_dt;
+    // final state (original order) kept to restore it at the end
+    DevBuf fin, seg;
+    fin.reserve(sizeof(double) * 7 * N);
+    save_state_orig(h, fin.as<double>());
+    seg.reserve(sizeof(double) * 7 * N * E);
+    h->vlam.reserve(sizeof(double) * 6 * cap);
+    upload_cot(h, xbar, alphabar, h->vlam.as<double>());
+    for (int64_t c = h->ck_n - 1; c >= 0; c--) {
+      const int64_t k0 = c * E, k1 = std::min(h->ck_steps, k0 + E);
+      // recompute the segment's step starts from its checkpoint
+      load_state_orig(h, h->ck.as<double>() + c * 7 * N);
+      for (int64_t k = k0; k < k1; k++) {
+        save_state_orig(h, seg.as<double>() + (k - k0) * 7 * N);
+        if (k + 1 == k1) break;
+        for (int st = 0; st < stages_of(h->prm.scheme); st++) {
+          do_build(h);
+          do_eval(h, false);
+          rk_stage(s, h->st.as<double>(), cap, N, h->prm.scheme, st, dt, 0);
+        }
+      }
+      for (int64_t k = k1 - 1; k >= k0; k--) {
+        load_state_orig(h, seg.as<double>() + (k - k0) * 7 * N);
+        step_vjp(h, dt, h->vlam.as<double>());
+      }
+    }
+    download_cot(h, h->vlam.as<double>(), xbar, alphabar);
+    load_state_orig(h, fin.as<double>());
   });
 }
 

@@ -147,6 +147,10 @@ void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M);
 void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, const M2LSchedule &sch, const double2 *M,
          double2 *L, Scratch *sc = nullptr);
 void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L);
+// adjoint of the particle rates: vortex + vortex-dipole P2M (aux fields 0..11) and the far-field
+// target-side term (adds into aux fields 12..14)
+void p2m_adj(hipStream_t s, int P, const Tree &t, const double *st, const double *aux, int64_t cap, double2 *M);
+void l2p_adj_tgt(hipStream_t s, int P, const Tree &t, const double2 *L, const double *st, double *aux, int64_t cap);
 void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *st, int64_t cap, int stretching,
                  int formulation, int sensor_mode);
 
@@ -160,6 +164,24 @@ void gauss_near(hipStream_t s, const Tree &tt, const Tree &stree, const Lists &l
                 const double *srcst, int64_t scap, double *out, int64_t ocap, int mode, int field, double prefix);
 void direct(hipStream_t s, int64_t m, const double *xt, const double *at, const double *sst, int64_t scap, int64_t n,
             double *u, double *J, double *dadt);
+
+// adjoint.hip: vector-Jacobian product of the rates; adjoint fields 0-2 ubar, 3-11 Jbar,
+// 12-14 xbar, 15-17 alphabar (abar on entry to vjp_setup)
+constexpr int N_ADJ_FIELDS = 18;
+void vjp_setup(hipStream_t s, int64_t N, const double *st, double *aux, int64_t cap, int classic);
+void vjp_finish(hipStream_t s, int64_t N, const double *ast, double *aux, int64_t cap);
+// side 0: forward P2P list (xbar of targets); side 1: transposed list (xbar, alphabar of sources)
+void p2p_adj(hipStream_t s, const Tree &t, const Lists &l, int side, const double *st, double *aux, int64_t cap);
+// state cotangents in the original body order (6 fields of ccap doubles) <-> adjoint fields
+void cot_gather(hipStream_t s, int64_t N, const int32_t *orig, const double *cot, int64_t ccap, double *aux,
+                int64_t cap);
+void cot_scatter(hipStream_t s, int64_t N, const int32_t *orig, const double *aux, int64_t cap, double *cot,
+                 int64_t ccap);
+// X, alpha (fields F_X..F_A+2) and sigma of the device bodies into 7 fields of N doubles,
+// original order
+void state_to_orig(hipStream_t s, int64_t N, const int32_t *orig, const double *st, int64_t cap, double *dst);
+// out[i] = sum_k c[k] p[k][i] (at most 6 terms)
+void lincomb(hipStream_t s, int64_t n, double *out, int nterms, const double *const *p, const double *c);
 
 // body.hip
 void aos_to_soa(hipStream_t s, const void *aos, int64_t n, int64_t stride_doubles, double *st, int64_t cap);

@@ -339,6 +339,219 @@ __global__ __launch_bounds__(WG) void k_l2l(int P, int NT, const int32_t *__rest
   }
 }
 
+// ------------------------------------------------------------------ adjoint of the particle rates
+// Adjoint sources of the far field (see uvlm_eval_vjp): at each body a vortex ubar (3) and
+// vortex dipoles Jbar (3x3, row d = component, column c = direction), so that
+//   M[n,m][d] += ubar_d conj(Y_n^m(x - X)) + sum_c Jbar_{dc} conj(d_c Y_n^m)(x - X),
+// the derivative taken with respect to the source position.  Regular-harmonic ladder:
+//   d_z Y_n^m = -Y_{n-1}^m,  (d_x + i d_y) Y_n^m = -Y_{n-1}^{m+1},
+//   (d_x - i d_y) Y_n^m = Y_{n-1}^{m-1} (m >= 1);  for m = 0 (Y_n^0 real): d_x Y = -Re Y_{n-1}^1,
+//   d_y Y = -Im Y_{n-1}^1.   Same layout as k_p2m: harmonics of a chunk of bodies in LDS, then
+// lane <-> coefficient.  aux holds ubar at field 0 and Jbar (row-major) at fields 3..11.
+__global__ __launch_bounds__(WG) void k_p2m_adj(int P, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
+                                                const int32_t *__restrict__ cbody, const int32_t *__restrict__ cnbody,
+                                                const double *__restrict__ cx, const double *__restrict__ cy,
+                                                const double *__restrict__ cz, const double *__restrict__ st,
+                                                const double *__restrict__ aux, int64_t cap, double2 *__restrict__ M)
+{
+  extern __shared__ double2 smq[];
+  const int64_t w = blockIdx.x;
+  if (w >= nleaf) return;
+  const int    c = leaves[w], b0 = cbody[c], nb = cnbody[c];
+  const double X = cx[c], Yc = cy[c], Z = cz[c];
+  const int    lane = threadIdx.x, ny = tri(P);
+  const int    C = (P + 1) / 2, B = WG / C;
+  double2     *Yb = smq;                       // [B][ny]
+  double      *wt = (double *)(smq + B * ny);  // [B][12]: ubar, Jbar
+  double2      acc[QPL][3];
+  for (int a = 0; a < QPL; a++)
+    for (int d = 0; d < 3; d++) acc[a][d] = make_double2(0, 0);
+  const int pb = lane / C, pc = lane % C;
+  for (int c0 = 0; c0 < nb; c0 += B) {
+    const int i = c0 + pb;
+    if (pb < B && i < nb) {
+      const int64_t bi = b0 + i;
+      const Sph     sp = sph(st[(F_X + 0) * cap + bi] - X, st[(F_X + 1) * cap + bi] - Yc, st[(F_X + 2) * cap + bi] - Z);
+      harmonic_column<false>(P, pc, sp, Yb + pb * ny);
+      if (P - 1 - pc > pc) harmonic_column<false>(P, P - 1 - pc, sp, Yb + pb * ny);
+      if (pc == 0)
+        for (int f = 0; f < 12; f++) wt[pb * 12 + f] = aux[f * cap + bi];
+    }
+    __syncthreads();
+    const int cnt = min(B, nb - c0);
+    for (int a = 0; a < QPL; a++) {
+      const int q = lane + a * WG;
+      if (q >= ny) continue;
+      int n = 0;
+      while (tri(n + 1) <= q) n++;
+      const int m = q - tri(n);
+      for (int j = 0; j < cnt; j++) {
+        const double2 *Yj = Yb + j * ny;
+        const double2  y  = Yj[q];
+        double2        gx = make_double2(0, 0), gy = make_double2(0, 0), gz = make_double2(0, 0);
+        if (n > 0) {
+          const double2 up = (m + 1 <= n - 1) ? Yj[tri(n - 1) + m + 1] : make_double2(0, 0);
+          if (m >= 1) {
+            const double2 dn = Yj[tri(n - 1) + m - 1];
+            gx = make_double2(0.5 * (dn.x - up.x), 0.5 * (dn.y - up.y));
+            gy = make_double2(-0.5 * (up.y + dn.y), 0.5 * (up.x + dn.x));  // (i/2)(up + dn)
+          } else {
+            gx = make_double2(-up.x, 0.0);
+            gy = make_double2(-up.y, 0.0);
+          }
+          if (m <= n - 1) {
+            const double2 z0 = Yj[tri(n - 1) + m];
+            gz               = make_double2(-z0.x, -z0.y);
+          }
+        }
+        const double *wj = wt + j * 12;
+        for (int d = 0; d < 3; d++) {
+          const double ud = wj[d], jx = wj[3 + 3 * d], jy = wj[4 + 3 * d], jz = wj[5 + 3 * d];
+          // conj of (ubar_d Y + Jbar_dx gx + Jbar_dy gy + Jbar_dz gz)
+          acc[a][d].x += ud * y.x + jx * gx.x + jy * gy.x + jz * gz.x;
+          acc[a][d].y -= ud * y.y + jx * gx.y + jy * gy.y + jz * gz.y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int a = 0; a < QPL; a++) {
+    const int q = lane + a * WG;
+    if (q < ny)
+      for (int d = 0; d < 3; d++) M[(int64_t)c * NT + 3 * q + d] = acc[a][d];
+  }
+}
+
+// Target side of the far-field adjoint: with the forward local expansion L of each leaf, at every
+// body x_i the L2L to the body up to degree 3 gives grad, Hessian and third derivatives of psi;
+// then u = curl psi / (4 pi), J = grad u, and
+//   xbar_i += J^T ubar_i + sum_ab Jbar_ab grad J_ab     (into aux fields 12..14).
+// Third derivatives of Re(w_k o[3,k] Y_3^k(h)) with o[3,k] = A_k + i B_k (w_0 = 1, w_k = 2):
+//   xxx = -3/4 A1 + 1/4 A3, xxy = 1/4 B1 - 1/4 B3, xxz = 1/2 A0 - 1/2 A2, xyy = -1/4 A1 - 1/4 A3,
+//   xyz = 1/2 B2, xzz = A1, yyy = 3/4 B1 + 1/4 B3, yyz = 1/2 A0 + 1/2 A2, yzz = -B1, zzz = -A0.
+__global__ __launch_bounds__(WG) void k_l2p_adj_tgt(int P, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
+                                                    const int32_t *__restrict__ cbody,
+                                                    const int32_t *__restrict__ cnbody, const double *__restrict__ cx,
+                                                    const double *__restrict__ cy, const double *__restrict__ cz,
+                                                    const double2 *__restrict__ L, const double *__restrict__ st,
+                                                    double *__restrict__ aux, int64_t cap)
+{
+  extern __shared__ double2 sm2[];
+  double2      *Lc = sm2;
+  const int64_t w  = blockIdx.x;
+  if (w >= nleaf) return;
+  const int c = leaves[w], b0 = cbody[c], nb = cnbody[c];
+  for (int q = threadIdx.x; q < NT; q += WG) Lc[q] = L[(int64_t)c * NT + q];
+  __syncthreads();
+  const double X = cx[c], Yc = cy[c], Z = cz[c];
+  constexpr int NO = 10;
+  for (int i = threadIdx.x; i < nb; i += WG) {
+    const int64_t bi = b0 + i;
+    const Sph     s  = sph(st[(F_X + 0) * cap + bi] - X, st[(F_X + 1) * cap + bi] - Yc, st[(F_X + 2) * cap + bi] - Z);
+    double2       o[NO][3];
+    for (int a = 0; a < NO; a++)
+      for (int d = 0; d < 3; d++) o[a][d] = make_double2(0, 0);
+    const int OJ[NO] = {0, 1, 1, 2, 2, 2, 3, 3, 3, 3}, OK[NO] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+    for (int mu = 0; mu < P; mu++) {
+      double2 e = make_double2(1.0, 0.0);
+      for (int t = 0; t < mu; t++) e = cmul(e, make_double2(s.cb, s.sb));
+      double p = 1.0;
+      for (int t = 1; t <= mu; t++) p *= -(double)(2 * t - 1) * s.y;
+      double f = c_invfact[2 * mu];
+      for (int t = 1; t <= mu; t++) f *= -s.rho;
+      double pp = 0.0;
+      for (int nu = mu; nu < P; nu++) {
+        if (nu > mu) {
+          const double pn = (nu == mu + 1) ? s.x * (2 * mu + 1) * p
+                                           : (s.x * (2 * nu - 1) * p - (nu + mu - 1) * pp) * c_inv[nu - mu];
+          pp = p;
+          p  = pn;
+          f *= -s.rho * c_inv[nu + mu];
+        }
+        const double  v  = f * p;
+        const double2 yp = make_double2(v * e.x, v * e.y);
+        for (int sgm = 0; sgm < (mu > 0 ? 2 : 1); sgm++) {
+          const int     mup = sgm ? -mu : mu;
+          const double2 yv  = sgm ? conj2(yp) : yp;
+          for (int a = 0; a < NO; a++) {
+            const int j = OJ[a], k = OK[a];
+            const int n = nu + j;
+            if (n >= P) continue;
+            const int m = mup + k;
+            double    sg;
+            double2   lv[3];
+            if (m < 0) {
+              sg = sgn(k);
+              for (int d = 0; d < 3; d++) lv[d] = conj2(Lc[3 * (tri(n) - m) + d]);
+            } else {
+              sg = (m < k) ? sgn(m - k) : 1.0;
+              for (int d = 0; d < 3; d++) lv[d] = Lc[3 * (tri(n) + m) + d];
+            }
+            const double2 ys = make_double2(yv.x * sg, yv.y * sg);
+            for (int d = 0; d < 3; d++) cfma(o[a][d], lv[d], ys);
+          }
+        }
+      }
+    }
+    double H[3][3][3], T[3][3][3][3];
+    for (int d = 0; d < 3; d++) {
+      const double a20 = o[3][d].x, a21 = o[4][d].x, b21 = o[4][d].y, a22 = o[5][d].x, b22 = o[5][d].y;
+      H[d][0][0] = 0.5 * (a22 - a20);
+      H[d][1][1] = -0.5 * (a20 + a22);
+      H[d][2][2] = a20;
+      H[d][0][1] = H[d][1][0] = -0.5 * b22;
+      H[d][0][2] = H[d][2][0] = -a21;
+      H[d][1][2] = H[d][2][1] = b21;
+      const double A0 = o[6][d].x, A1 = o[7][d].x, B1 = o[7][d].y, A2 = o[8][d].x, B2 = o[8][d].y, A3 = o[9][d].x,
+                   B3 = o[9][d].y;
+      const double t[10] = {-0.75 * A1 + 0.25 * A3, 0.25 * B1 - 0.25 * B3, 0.5 * A0 - 0.5 * A2, -0.25 * A1 - 0.25 * A3,
+                            0.5 * B2,               A1,                    0.75 * B1 + 0.25 * B3, 0.5 * A0 + 0.5 * A2,
+                            -B1,                    -A0};
+      // t: xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+          for (int cc = 0; cc < 3; cc++) {
+            const int nx = (a == 0) + (b == 0) + (cc == 0), ny_ = (a == 1) + (b == 1) + (cc == 1);
+            const int nz  = 3 - nx - ny_;
+            int       idx = 0;
+            if (nx == 3) idx = 0;
+            else if (nx == 2 && ny_ == 1) idx = 1;
+            else if (nx == 2) idx = 2;
+            else if (nx == 1 && ny_ == 2) idx = 3;
+            else if (nx == 1 && ny_ == 1) idx = 4;
+            else if (nx == 1) idx = 5;
+            else if (ny_ == 3) idx = 6;
+            else if (ny_ == 2) idx = 7;
+            else if (ny_ == 1) idx = 8;
+            else idx = 9;
+            (void)nz;
+            T[d][a][b][cc] = t[idx];
+          }
+    }
+    // curl structure: u_a = sum eps_{a p q} d_p psi_q
+    const int    P1[3] = {1, 2, 0}, P2[3] = {2, 0, 1};  // u_0 = d1 psi2 - d2 psi1, ...
+    double       Jf[3][3], dJ[3][3][3];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        Jf[a][b] = INV_4PI * (H[P2[a]][P1[a]][b] - H[P1[a]][P2[a]][b]);
+        for (int e2 = 0; e2 < 3; e2++) dJ[a][b][e2] = INV_4PI * (T[P2[a]][P1[a]][b][e2] - T[P1[a]][P2[a]][b][e2]);
+      }
+    double ub[3], Jb[3][3];
+    for (int a = 0; a < 3; a++) {
+      ub[a] = aux[a * cap + bi];
+      for (int b = 0; b < 3; b++) Jb[a][b] = aux[(3 + 3 * a + b) * cap + bi];
+    }
+    for (int e2 = 0; e2 < 3; e2++) {
+      double v = 0.0;
+      for (int a = 0; a < 3; a++) {
+        v += ub[a] * Jf[a][e2];
+        for (int b = 0; b < 3; b++) v += Jb[a][b] * dJ[a][b][e2];
+      }
+      aux[(12 + e2) * cap + bi] += v;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ M2L
 
 // The n-synchronous M2L (plan in m2l_plan.hpp).  One workgroup (one wavefront) per target
@@ -780,6 +993,29 @@ void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *
                                                         t.nbody.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(),
                                                         t.cz.as<double>(), L, st, cap, stretching, formulation,
                                                         sensor_mode);
+  UVLM_LAUNCH_CHECK();
+}
+
+void p2m_adj(hipStream_t s, int P, const Tree &t, const double *st, const double *aux, int64_t cap, double2 *M)
+{
+  const int NT = 3 * P * (P + 1) / 2;
+  if (t.nleaf == 0) return;
+  const int    B   = WG / ((P + 1) / 2), ny = P * (P + 1) / 2;
+  const size_t lds = sizeof(double2) * B * ny + sizeof(double) * 12 * B;
+  k_p2m_adj<<<(int)t.nleaf, WG, lds, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(),
+                                          t.nbody.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(),
+                                          t.cz.as<double>(), st, aux, cap, M);
+  UVLM_LAUNCH_CHECK();
+}
+
+void l2p_adj_tgt(hipStream_t s, int P, const Tree &t, const double2 *L, const double *st, double *aux, int64_t cap)
+{
+  const int NT = 3 * P * (P + 1) / 2;
+  if (t.nleaf == 0) return;
+  k_l2p_adj_tgt<<<(int)t.nleaf, WG, sizeof(double2) * NT, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf,
+                                                               t.body.as<int32_t>(), t.nbody.as<int32_t>(),
+                                                               t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(),
+                                                               L, st, aux, cap);
   UVLM_LAUNCH_CHECK();
 }
 

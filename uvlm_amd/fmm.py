@@ -175,6 +175,35 @@ class FMMCalculator:
         _capi.check(self.lib.uvlm_probe_adjoint(self._h, Pp.shape[0], _dp(Pp), _dp(W), _dp(ga), _dp(gx)))
         return ga, gx
 
+    def eval_vjp(self, ubar=None, abar=None):
+        """(xbar, alphabar), tree order, of S = sum ubar.u + abar.dadt for the current tree held
+        fixed (evaluates the rates first)."""
+        ub = None if ubar is None else np.ascontiguousarray(ubar, dtype=np.float64)
+        ab = None if abar is None else np.ascontiguousarray(abar, dtype=np.float64)
+        xb = np.zeros((self.n, 3))
+        alb = np.zeros((self.n, 3))
+        _capi.check(self.lib.uvlm_eval_vjp(self._h, self.n, _dp(ub) if ub is not None else None,
+                                           _dp(ab) if ab is not None else None, _dp(xb), _dp(alb)))
+        return xb, alb
+
+    def step_vjp(self, dt, xbar, alphabar):
+        """Cotangent of the state before one advance(dt) from that after it (original order)."""
+        xb = np.array(xbar, dtype=np.float64, order="C", copy=True)
+        ab = np.array(alphabar, dtype=np.float64, order="C", copy=True)
+        _capi.check(self.lib.uvlm_step_vjp(self._h, float(dt), self.n, _dp(xb), _dp(ab)))
+        return xb, ab
+
+    def trajectory_forward(self, steps, dt, every=1):
+        """Advance `steps` steps, keeping checkpoints every `every` steps on the device."""
+        _capi.check(self.lib.uvlm_trajectory_forward(self._h, int(steps), float(dt), int(every)))
+
+    def trajectory_vjp(self, xbar, alphabar):
+        """Cotangent of the initial state from that of the final one (original order)."""
+        xb = np.array(xbar, dtype=np.float64, order="C", copy=True)
+        ab = np.array(alphabar, dtype=np.float64, order="C", copy=True)
+        _capi.check(self.lib.uvlm_trajectory_vjp(self._h, self.n, _dp(xb), _dp(ab)))
+        return xb, ab
+
     def probe_objective(self, probes, weights):
         """J = sum_p weights_p . u(probes_p) through getSensorData (needs a calculate())."""
         from .fmm import make_bodies as _mk
