@@ -29,6 +29,7 @@
 // Adjoint fields (SoA, `cap` doubles each): 0-2 ubar, 3-11 Jbar (row-major), 12-14 xbar,
 // 15-17 alphabar.
 #include "common.hpp"
+#include "pair_kernel.hpp"
 
 namespace uvlm {
 
@@ -38,7 +39,7 @@ constexpr int WG = 64;
 
 enum { A_UB = 0, A_JB = 3, A_XB = 12, A_AB = 15 };
 
-__device__ inline void cross(const double a[3], const double b[3], double o[3])
+template <class T> __device__ inline void cross(const T a[3], const T b[3], T o[3])
 {
   o[0] = a[1] * b[2] - a[2] * b[1];
   o[1] = a[2] * b[0] - a[0] * b[2];
@@ -81,75 +82,237 @@ __device__ inline bool pair_adj(const double r[3], const double al[3], double sg
   return true;
 }
 
-// One 64-lane workgroup per leaf of the (single) tree; each lane owns one body of the leaf
-// per chunk, the bodies of the listed leaves stream through LDS in list order (deterministic).
+// Near-field adjoint, one 64-lane workgroup per leaf of the (single) tree.  Like the forward
+// P2P kernel, lanes = (own body) x (split s) so small leaves keep the wavefront busy, the
+// bodies of all listed leaves stream through LDS in full chunks across leaf boundaries, and
+// the splits are combined by cross-lane adds in a fixed order (deterministic, no atomics).
 //  SIDE 0 (forward P2P list, own = target i):            xbar_i += sum_j dS_ij/dr
 //  SIDE 1 (transposed list, own = source j): alphabar_j -= sum_i W_ij, xbar_j -= sum_i dS_ij/dr
-template <int SIDE>
+// Sums over the pairs are arranged so that what belongs to the own body is applied once per
+// chunk:
+//  SIDE 0: sum_j dS_ij = -[X + F1 x ubar + Jbar^T V + A],  X = sum k r, F1 = sum f1 alpha_j,
+//          V = sum f2 (r x alpha_j), A = sum f2 alpha_j x (Jbar r), k = f2 c1 + f3 c3;
+//  SIDE 1: sum_i dS_ij = -[X + sum f2 Jbar_i^T (r x alpha) + alpha x sum (f1 ubar_i + f2 Jbar_i r)].
+// MIXED: FP32 pair arithmetic on coordinates relative to the own leaf centre (formed in FP64
+// while staging), f1, f2, f3 from G, G', G'' (pair_kernel.hpp), FP32 partial sums per chunk
+// added into FP64 accumulators.  Otherwise FP64 with erf and exp (pair_adj).
+template <bool MIXED, int SIDE>
 __global__ __launch_bounds__(WG) void k_p2p_adj(const int32_t *__restrict__ leaves, int64_t nleaf,
                                                 const int32_t *__restrict__ cbody, const int32_t *__restrict__ cnbody,
-                                                const int32_t *__restrict__ off, const int32_t *__restrict__ src,
-                                                const double *__restrict__ st, double *__restrict__ aux, int64_t cap)
+                                                const double *__restrict__ ccx, const double *__restrict__ ccy,
+                                                const double *__restrict__ ccz, const int32_t *__restrict__ off,
+                                                const int32_t *__restrict__ src, const double *__restrict__ st,
+                                                double *__restrict__ aux, int64_t cap)
 {
-  constexpr int NF = SIDE == 0 ? 7 : 15;  // staged fields: x, alpha, sigma | x, ubar, Jbar
-  __shared__ double sh[NF][WG];
+  // staged per other body: MIXED SIDE 0: rel x, q | alpha, c1 | c2, c3; SIDE 1: rel x, ubar, e, Jbar
+  constexpr int NV = SIDE == 0 ? 3 : 5;   // float4 per slot (MIXED)
+  constexpr int ND = SIDE == 0 ? 7 : 15;  // doubles per slot (FP64): x, alpha, sigma | x, ubar, Jbar
+  __shared__ float4 s32[MIXED ? NV * WG : 1];
+  __shared__ double s64[MIXED ? 1 : ND * WG];
+  __shared__ int    pref[SEG + 1], lb[SEG], ln[SEG];
   const int64_t w = blockIdx.x;
   if (w >= nleaf) return;
   const int c = leaves[w], tb = cbody[c], nt = cnbody[c], lane = threadIdx.x;
-  const int q0 = off[c], q1 = off[c + 1];
-  for (int c0 = 0; c0 < nt; c0 += WG) {
-    const int     i     = c0 + lane;
+  int       lg = 0;
+  while ((1 << lg) < nt && lg < 6) lg++;
+  const int    ntp = 1 << lg, S = WG >> lg, ti = lane & (ntp - 1), si = lane >> lg;
+  const double cxt = ccx[c], cyt = ccy[c], czt = ccz[c];
+  const int    q0 = off[c], q1 = off[c + 1];
+  for (int c0 = 0; c0 < nt; c0 += ntp) {
+    const int     i     = c0 + ti;
     const bool    valid = i < nt;
     const int64_t bi    = tb + (valid ? i : 0);
-    double        x[3], al[3] = {0, 0, 0}, sg = 1.0, ub[3] = {0, 0, 0}, Jb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int d = 0; d < 3; d++) x[d] = st[(F_X + d) * cap + bi];
+    const double  xo[3] = {st[(F_X + 0) * cap + bi], st[(F_X + 1) * cap + bi], st[(F_X + 2) * cap + bi]};
+    const float   ro[3] = {(float)(xo[0] - cxt), (float)(xo[1] - cyt), (float)(xo[2] - czt)};
+    // own quantities: SIDE 0 ubar, Jbar, e; SIDE 1 alpha, sigma and its kernel constants
+    double ubd[3] = {0, 0, 0}, Jbd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, ald[3] = {0, 0, 0}, sgd = 1.0;
     if (SIDE == 0) {
-      for (int d = 0; d < 3; d++) ub[d] = aux[(A_UB + d) * cap + bi];
-      for (int k = 0; k < 9; k++) Jb[k] = aux[(A_JB + k) * cap + bi];
+      for (int d = 0; d < 3; d++) ubd[d] = aux[(A_UB + d) * cap + bi];
+      for (int k = 0; k < 9; k++) Jbd[k] = aux[(A_JB + k) * cap + bi];
     } else {
-      for (int d = 0; d < 3; d++) al[d] = st[(F_A + d) * cap + bi];
-      sg = st[F_SIG * cap + bi];
+      for (int d = 0; d < 3; d++) ald[d] = st[(F_A + d) * cap + bi];
+      sgd = st[F_SIG * cap + bi];
     }
+    const double oq = 0.5 / (sgd * sgd), osq = sqrt(oq);
+    const float  o_q = (float)oq, o_c1 = (float)(oq * osq * INV_4PI), o_c2 = (float)(2.0 * oq * oq * osq * INV_4PI),
+                o_c3 = (float)(4.0 * oq * oq * oq * osq * INV_4PI);
+    float ubf[3], Jbf[9], ef[3], alf[3];
+    for (int d = 0; d < 3; d++) {
+      ubf[d] = (float)ubd[d];
+      alf[d] = (float)ald[d];
+    }
+    for (int k = 0; k < 9; k++) Jbf[k] = (float)Jbd[k];
+    ef[0] = Jbf[5] - Jbf[7];
+    ef[1] = Jbf[6] - Jbf[2];
+    ef[2] = Jbf[1] - Jbf[3];
     double ax[3] = {0, 0, 0}, aa[3] = {0, 0, 0};
-    for (int q = q0; q < q1; q++) {
-      const int sl = src[q], b0 = cbody[sl], nb = cnbody[sl];
-      for (int j0 = 0; j0 < nb; j0 += WG) {
-        const int cnt = min(WG, nb - j0);
-        __syncthreads();
+    for (int g0 = q0; g0 < q1; g0 += SEG) {
+      const int nseg = min(SEG, q1 - g0);
+      __syncthreads();
+      for (int k = lane; k < nseg; k += WG) {
+        const int sl = src[g0 + k];
+        lb[k]        = cbody[sl];
+        ln[k]        = cnbody[sl];
+      }
+      __syncthreads();
+      lds_prefix(pref, ln, nseg);
+      __syncthreads();
+      const int total = pref[nseg];
+      for (int j0 = 0; j0 < total; j0 += WG) {
+        const int cnt = min(WG, total - j0);
         if (lane < cnt) {
-          const int64_t bj = b0 + j0 + lane;
-          for (int d = 0; d < 3; d++) sh[d][lane] = st[(F_X + d) * cap + bj];
+          const int g  = j0 + lane;
+          int       lo = 0, hi = nseg - 1;  // last q with pref[q] <= g
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pref[mid] <= g) lo = mid;
+            else hi = mid - 1;
+          }
+          const int64_t bj = lb[lo] + (g - pref[lo]);
+          const double  x = st[(F_X + 0) * cap + bj], y = st[(F_X + 1) * cap + bj], z = st[(F_X + 2) * cap + bj];
           if (SIDE == 0) {
-            for (int d = 0; d < 3; d++) sh[3 + d][lane] = st[(F_A + d) * cap + bj];
-            sh[6][lane] = st[F_SIG * cap + bj];
+            const double sg = st[F_SIG * cap + bj];
+            const double a0 = st[(F_A + 0) * cap + bj], a1 = st[(F_A + 1) * cap + bj], a2 = st[(F_A + 2) * cap + bj];
+            if (MIXED) {
+              const double qq = 0.5 / (sg * sg), sq = sqrt(qq);
+              s32[NV * lane + 0] = make_float4((float)(x - cxt), (float)(y - cyt), (float)(z - czt), (float)qq);
+              s32[NV * lane + 1] = make_float4((float)a0, (float)a1, (float)a2, (float)(qq * sq * INV_4PI));
+              s32[NV * lane + 2] = make_float4((float)(2.0 * qq * qq * sq * INV_4PI),
+                                               (float)(4.0 * qq * qq * qq * sq * INV_4PI), 0.f, 0.f);
+            } else {
+              const double v[7] = {x, y, z, a0, a1, a2, sg};
+              for (int f = 0; f < 7; f++) s64[f * WG + lane] = v[f];
+            }
           } else {
-            for (int f = 0; f < 12; f++) sh[3 + f][lane] = aux[(A_UB + f) * cap + bj];
+            double ub[3], Jb[9];
+            for (int d = 0; d < 3; d++) ub[d] = aux[(A_UB + d) * cap + bj];
+            for (int k = 0; k < 9; k++) Jb[k] = aux[(A_JB + k) * cap + bj];
+            if (MIXED) {
+              s32[NV * lane + 0] = make_float4((float)(x - cxt), (float)(y - cyt), (float)(z - czt), (float)ub[0]);
+              s32[NV * lane + 1] = make_float4((float)ub[1], (float)ub[2], (float)(Jb[5] - Jb[7]), (float)(Jb[6] - Jb[2]));
+              s32[NV * lane + 2] = make_float4((float)(Jb[1] - Jb[3]), (float)Jb[0], (float)Jb[1], (float)Jb[2]);
+              s32[NV * lane + 3] = make_float4((float)Jb[3], (float)Jb[4], (float)Jb[5], (float)Jb[6]);
+              s32[NV * lane + 4] = make_float4((float)Jb[7], (float)Jb[8], 0.f, 0.f);
+            } else {
+              s64[0 * WG + lane] = x;
+              s64[1 * WG + lane] = y;
+              s64[2 * WG + lane] = z;
+              for (int d = 0; d < 3; d++) s64[(3 + d) * WG + lane] = ub[d];
+              for (int k = 0; k < 9; k++) s64[(6 + k) * WG + lane] = Jb[k];
+            }
           }
         }
         __syncthreads();
-        if (!valid) continue;
-        for (int j = 0; j < cnt; j++) {
-          double r[3], W[3], dS[3];
-          if (SIDE == 0) {
-            const double alj[3] = {sh[3][j], sh[4][j], sh[5][j]};
-            for (int d = 0; d < 3; d++) r[d] = x[d] - sh[d][j];
-            if (pair_adj(r, alj, sh[6][j], ub, Jb, W, dS))
-              for (int d = 0; d < 3; d++) ax[d] += dS[d];
-          } else {
-            const double ubi[3] = {sh[3][j], sh[4][j], sh[5][j]};
-            double       Jbi[9];
-            for (int k = 0; k < 9; k++) Jbi[k] = sh[6 + k][j];
-            for (int d = 0; d < 3; d++) r[d] = sh[d][j] - x[d];
-            if (pair_adj(r, al, sg, ubi, Jbi, W, dS))
+        if (MIXED) {
+          // SIDE 0: X, F1, V, A;  SIDE 1: X, JV, UQ, WA
+          float p0[3] = {0, 0, 0}, p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0}, p3[3] = {0, 0, 0};
+          for (int j = si; j < cnt; j += S) {
+            const float4 A0 = s32[NV * j], A1 = s32[NV * j + 1], A2 = s32[NV * j + 2];
+            float        r[3];
+            if (SIDE == 0) {
+              r[0] = ro[0] - A0.x;
+              r[1] = ro[1] - A0.y;
+              r[2] = ro[2] - A0.z;
+            } else {
+              r[0] = A0.x - ro[0];
+              r[1] = A0.y - ro[1];
+              r[2] = A0.z - ro[2];
+            }
+            const float r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            float       G, Gp, Gpp;
+            kernel_g<true>(r2 * (SIDE == 0 ? A0.w : o_q), G, Gp, Gpp);
+            // coincident pairs are skipped (multipoleKernels.hpp:242): every term but the f1 ones
+            // carries r, so zeroing f1 is enough
+            const float f1 = r2 == 0.0f ? 0.0f : G * (SIDE == 0 ? A1.w : o_c1);
+            const float f2 = Gp * (SIDE == 0 ? A2.x : o_c2);
+            const float f3 = Gpp * (SIDE == 0 ? A2.y : o_c3);
+            if (SIDE == 0) {
+              const float al[3] = {A1.x, A1.y, A1.z};
+              float       v[3], Jr[3], aJ[3];
+              cross(r, al, v);
+              for (int d = 0; d < 3; d++) Jr[d] = Jbf[3 * d] * r[0] + Jbf[3 * d + 1] * r[1] + Jbf[3 * d + 2] * r[2];
+              cross(al, Jr, aJ);
+              const float c1 = ubf[0] * v[0] + ubf[1] * v[1] + ubf[2] * v[2] + al[0] * ef[0] + al[1] * ef[1] +
+                               al[2] * ef[2];
+              const float c3 = Jr[0] * v[0] + Jr[1] * v[1] + Jr[2] * v[2];
+              const float k  = f2 * c1 + f3 * c3;
               for (int d = 0; d < 3; d++) {
-                ax[d] -= dS[d];
-                aa[d] -= W[d];
+                p0[d] = fmaf(k, r[d], p0[d]);
+                p1[d] = fmaf(f1, al[d], p1[d]);
+                p2[d] = fmaf(f2, v[d], p2[d]);
+                p3[d] = fmaf(f2, aJ[d], p3[d]);
               }
+            } else {
+              const float4 A3 = s32[NV * j + 3], A4 = s32[NV * j + 4];
+              const float  ub[3] = {A0.w, A1.x, A1.y}, e[3] = {A1.z, A1.w, A2.x};
+              const float  Jb[9] = {A2.y, A2.z, A2.w, A3.x, A3.y, A3.z, A3.w, A4.x, A4.y};
+              float        uxr[3], Jr[3], Jrxr[3], v[3];
+              cross(ub, r, uxr);
+              for (int d = 0; d < 3; d++) Jr[d] = Jb[3 * d] * r[0] + Jb[3 * d + 1] * r[1] + Jb[3 * d + 2] * r[2];
+              cross(Jr, r, Jrxr);
+              cross(r, alf, v);
+              float c1 = 0.f, c3 = 0.f;
+              for (int d = 0; d < 3; d++) {
+                c1 = fmaf(alf[d], uxr[d] + e[d], c1);
+                c3 = fmaf(alf[d], Jrxr[d], c3);
+              }
+              const float k = f2 * c1 + f3 * c3;
+              for (int d = 0; d < 3; d++) {
+                const float JTv = Jb[d] * v[0] + Jb[3 + d] * v[1] + Jb[6 + d] * v[2];
+                p0[d]           = fmaf(k, r[d], p0[d]);
+                p1[d]           = fmaf(f2, JTv, p1[d]);
+                p2[d]           = fmaf(f1, ub[d], fmaf(f2, Jr[d], p2[d]));
+                p3[d]           = fmaf(f1, uxr[d] + e[d], fmaf(f2, Jrxr[d], p3[d]));
+              }
+            }
+          }
+          if (SIDE == 0) {
+            // sum dS = -[X + F1 x ubar + Jbar^T V + A]
+            float fxu[3];
+            cross(p1, ubf, fxu);
+            for (int d = 0; d < 3; d++) {
+              const float JTV = Jbf[d] * p2[0] + Jbf[3 + d] * p2[1] + Jbf[6 + d] * p2[2];
+              ax[d] -= (double)(p0[d] + fxu[d] + JTV + p3[d]);
+            }
+          } else {
+            // xbar_j -= sum dS = X + JV + alpha x UQ;  alphabar_j -= sum W
+            float axq[3];
+            cross(alf, p2, axq);
+            for (int d = 0; d < 3; d++) {
+              ax[d] += (double)(p0[d] + p1[d] + axq[d]);
+              aa[d] -= (double)p3[d];
+            }
+          }
+        } else {
+          for (int j = si; j < cnt; j += S) {
+            double r[3], Wv[3], dS[3];
+            if (SIDE == 0) {
+              const double alj[3] = {s64[3 * WG + j], s64[4 * WG + j], s64[5 * WG + j]};
+              for (int d = 0; d < 3; d++) r[d] = xo[d] - s64[d * WG + j];
+              if (pair_adj(r, alj, s64[6 * WG + j], ubd, Jbd, Wv, dS))
+                for (int d = 0; d < 3; d++) ax[d] += dS[d];
+            } else {
+              const double ubi[3] = {s64[3 * WG + j], s64[4 * WG + j], s64[5 * WG + j]};
+              double       Jbi[9];
+              for (int k = 0; k < 9; k++) Jbi[k] = s64[(6 + k) * WG + j];
+              for (int d = 0; d < 3; d++) r[d] = s64[d * WG + j] - xo[d];
+              if (pair_adj(r, ald, sgd, ubi, Jbi, Wv, dS))
+                for (int d = 0; d < 3; d++) {
+                  ax[d] -= dS[d];
+                  aa[d] -= Wv[d];
+                }
+            }
           }
         }
+        __syncthreads();
       }
     }
-    if (valid) {
+    for (int o = ntp; o < WG; o <<= 1)
+      for (int d = 0; d < 3; d++) {
+        ax[d] += __shfl_xor(ax[d], o, 64);
+        if (SIDE == 1) aa[d] += __shfl_xor(aa[d], o, 64);
+      }
+    if (si == 0 && valid) {
       for (int d = 0; d < 3; d++) aux[(A_XB + d) * cap + bi] += ax[d];
       if (SIDE == 1)
         for (int d = 0; d < 3; d++) aux[(A_AB + d) * cap + bi] += aa[d];
@@ -295,15 +458,21 @@ void vjp_finish(hipStream_t s, int64_t N, const double *ast, double *aux, int64_
   UVLM_LAUNCH_CHECK();
 }
 
-void p2p_adj(hipStream_t s, const Tree &t, const Lists &l, int side, const double *st, double *aux, int64_t cap)
+void p2p_adj(hipStream_t s, const Tree &t, const Lists &l, int side, int mixed, const double *st, double *aux,
+             int64_t cap)
 {
   if (t.nleaf == 0) return;
   const int32_t *lv = t.leaves.as<int32_t>(), *cb = t.body.as<int32_t>(), *cn = t.nbody.as<int32_t>();
   const int32_t *off = l.p2p_off.as<int32_t>(), *src = l.p2p_src.as<int32_t>();
-  if (side == 0)
-    k_p2p_adj<0><<<(int)t.nleaf, WG, 0, s>>>(lv, t.nleaf, cb, cn, off, src, st, aux, cap);
-  else
-    k_p2p_adj<1><<<(int)t.nleaf, WG, 0, s>>>(lv, t.nleaf, cb, cn, off, src, st, aux, cap);
+  const double  *cx = t.cx.as<double>(), *cy = t.cy.as<double>(), *cz = t.cz.as<double>();
+  const int      nb = (int)t.nleaf;
+  if (mixed) {
+    if (side == 0) k_p2p_adj<true, 0><<<nb, WG, 0, s>>>(lv, t.nleaf, cb, cn, cx, cy, cz, off, src, st, aux, cap);
+    else k_p2p_adj<true, 1><<<nb, WG, 0, s>>>(lv, t.nleaf, cb, cn, cx, cy, cz, off, src, st, aux, cap);
+  } else {
+    if (side == 0) k_p2p_adj<false, 0><<<nb, WG, 0, s>>>(lv, t.nleaf, cb, cn, cx, cy, cz, off, src, st, aux, cap);
+    else k_p2p_adj<false, 1><<<nb, WG, 0, s>>>(lv, t.nleaf, cb, cn, cx, cy, cz, off, src, st, aux, cap);
+  }
   UVLM_LAUNCH_CHECK();
 }
 

@@ -293,7 +293,7 @@ void vjp_core(uvlm_handle h)
   vjp_setup(s, N, st, aux, cap, h->prm.stretching == UVLM_CLASSIC);
   // target side: far field from the forward local expansions, then the near pairs
   l2p_adj_tgt(s, P, t, h->L.as<double2>(), st, aux, cap);
-  p2p_adj(s, t, h->lists, 0, st, aux, cap);
+  p2p_adj(s, t, h->lists, 0, h->prm.precision == UVLM_MIXED, st, aux, cap);
   // source side: adjoint sources up the tree, across the transposed M2L list, down to the bodies
   transpose_lists(s, h->sc, t, t, h->lists, h->tlists);
   double2 *M = h->M.as<double2>(), *L = h->L.as<double2>();
@@ -309,7 +309,7 @@ void vjp_core(uvlm_handle h)
   UVLM_HIP(hipMemcpyAsync(a + F_X * cap, st + F_X * cap, 8 * 6 * cap, hipMemcpyDeviceToDevice, s));
   UVLM_HIP(hipMemcpyAsync(a + F_SIG * cap, st + F_SIG * cap, 8 * cap, hipMemcpyDeviceToDevice, s));
   l2p_stretch(s, P, t, L, a, cap, UVLM_TRANSPOSED, UVLM_CVPM, 0);
-  p2p_adj(s, t, h->tlists, 1, st, aux, cap);
+  p2p_adj(s, t, h->tlists, 1, h->prm.precision == UVLM_MIXED, st, aux, cap);
   vjp_finish(s, N, a, aux, cap);
   h->evaluated = false;  // M and L hold adjoint expansions
 }

@@ -171,7 +171,9 @@ constexpr int N_ADJ_FIELDS = 18;
 void vjp_setup(hipStream_t s, int64_t N, const double *st, double *aux, int64_t cap, int classic);
 void vjp_finish(hipStream_t s, int64_t N, const double *ast, double *aux, int64_t cap);
 // side 0: forward P2P list (xbar of targets); side 1: transposed list (xbar, alphabar of sources)
-void p2p_adj(hipStream_t s, const Tree &t, const Lists &l, int side, const double *st, double *aux, int64_t cap);
+// mixed: FP32 pair arithmetic (FP64 accumulation), else FP64
+void p2p_adj(hipStream_t s, const Tree &t, const Lists &l, int side, int mixed, const double *st, double *aux,
+             int64_t cap);
 // state cotangents in the original body order (6 fields of ccap doubles) <-> adjoint fields
 void cot_gather(hipStream_t s, int64_t N, const int32_t *orig, const double *cot, int64_t ccap, double *aux,
                 int64_t cap);

@@ -17,24 +17,9 @@
 // FP64 path: the reference formula (erf, exp, sqrt in double).
 // Mixed path: FP32 arithmetic on coordinates relative to the target leaf centre (the
 // source offsets are formed in FP64 while staging), per-source-leaf FP32 partial sums
-// added into FP64 per-target accumulators.  For s^2 < 2, g/s^3 and its derivative come
-// from a polynomial (no cancellation between erf and t2): with x = s^2,
-//   g / s^3 = G(x) = (2/sqrt(pi)) sum_{i>=0} (-1)^i 2(i+1) / ((2i+3) (i+1)!) x^i,
-//   f1 = G(x) q^{3/2} / (4 pi),  f2 = 2 G'(x) q^{5/2} / (4 pi),  q = 1 / (2 sigma^2),
-// because 2 s^2 t2 - 3 g = s g'(s) - 3 g = 2 s^5 G'(x).  G is approximated on [0, 2] by
-// its degree-9 Chebyshev interpolant (interpolation at the Chebyshev nodes of [0, 2],
-// re-expanded in powers of t = x - 1): max error 8e-11 (G) and 5e-9 (G') in exact
-// arithmetic, 1.4e-7 and 1.1e-7 when evaluated by Horner in FP32.
-// For x >= 2 the same G and G' come from the hardware rsq/exp2 and a short polynomial:
-// with w = x^{-1/2} = 1/s and e = e^{-x},
-//   g = erf(s) - (2/sqrt(pi)) s e = 1 - e (erfcx(s) + (2/sqrt(pi)) s),
-//   G = g w^3,   G' = w^2 ((2/sqrt(pi)) e - (3/2) G)
-// (from G' = (2/sqrt(pi)) s e x^{-3/2} - (3/2) G / x), where erfcx(s) = e^{s^2} erfc(s) is a
-// degree-7 least-squares polynomial in w - 0.4647 on w in [1/4.5, 1/sqrt(2)] (max error
-// 1e-8; beyond s = 4.5 the factor e < 2e-9 makes it irrelevant, so w is clamped there).
-// In FP32 this form is as accurate as erff/expf (4.3e-7 on G, 8.1e-7 on G') for about a
-// third of the instructions.
+// added into FP64 per-target accumulators; g/s^3 and its derivative from pair_kernel.hpp.
 #include "common.hpp"
+#include "pair_kernel.hpp"
 
 #include <vector>
 
@@ -45,21 +30,6 @@ namespace uvlm {
 namespace {
 
 constexpr int WG = 64;
-
-// Chebyshev interpolant of G on x in [0, 2] (see above), coefficients of t^i, t = x - 1.
-constexpr int   NPOLY              = 10;
-constexpr float XPOLY              = 2.0f;
-__constant__ float c_gpoly[NPOLY] = {4.27593295550599495e-01f,  -2.26282445882890593e-01f, 7.52993075578484716e-02f,
-                                     -1.86646103035026413e-02f, 3.70154966273965411e-03f,  -6.12467145832412121e-04f,
-                                     8.69427232794296300e-05f,  -1.08135023163257920e-05f, 1.22398144588401693e-06f,
-                                     -1.21730778572683114e-07f};
-
-// erfcx(1/w) ~ sum_k c_far[k] (w - FAR_WC)^k, w in [FAR_WMIN, 1/sqrt(2)] (see above).
-constexpr int   NFAR     = 8;
-constexpr float FAR_WC   = 0.46466450170438484f;
-constexpr float FAR_WMIN = 0.2222222222222222f;
-__constant__ float c_far[NFAR] = {2.400683016e-01f,  4.403707087e-01f, -1.874301285e-01f, 1.258202083e-02f,
-                                  8.160093427e-02f, -1.016800180e-01f, 5.416155234e-02f,  2.011496387e-02f};
 
 struct SrcF64 {
   double x, y, z, ax, ay, az, invS2, halfInvS2;
@@ -100,32 +70,6 @@ __device__ __forceinline__ void fold_antisym(float *acc)
   acc[8] += acc[12];
   acc[9] += acc[13];
   acc[10] -= acc[12];
-}
-
-constexpr int SEG = 256;  // source leaves whose prefix sums sit in LDS at once
-
-// Exclusive prefix sum of v[0..n) (n <= SEG) into pref[0..n]; one wavefront, 4 per lane.
-__device__ __forceinline__ void lds_prefix(int *pref, const int *v, int n)
-{
-  const int lane = threadIdx.x;
-  int       loc[4], tot = 0;
-  for (int k = 0; k < 4; k++) {
-    const int i = 4 * lane + k;
-    loc[k]      = i < n ? v[i] : 0;
-    tot += loc[k];
-  }
-  int incl = tot;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int u = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += u;
-  }
-  int run = incl - tot;
-  for (int k = 0; k < 4; k++) {
-    const int i = 4 * lane + k;
-    if (i <= n) pref[i] = run;
-    run += loc[k];
-  }
-  if (lane == 63 && 256 <= n) pref[256] = run;
 }
 
 // TSIG: the core size comes from the TARGET body instead of the source.  That is the
