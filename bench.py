@@ -98,13 +98,33 @@ def adjoint_leg(n, theta, ncrit, steps):
     bwd = (time.perf_counter() - t0) * 1e3 / steps
     fk /= steps
     bk /= steps
+    # vector-Jacobian product of the particle rates (u, dadt) on the same tree: the building
+    # block of the time-step and trajectory adjoints
+    rng = np.random.default_rng(0)
+    ub, ab = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    g.build()
+    g.eval_vjp(ub, ab)
+    ev = vj = 0.0
+    for _ in range(steps):
+        g.eval_vjp(ub, ab)
+        t = g.timings()
+        ev += t["eval_ms"]
+        vj += t["vjp_ms"]
+    ev /= steps
+    vj /= steps
     g.close()
     return {"adjoint_fwd_ratio": round(bk / fk, 3),
             "adjoint": {"particles": n, "probes": 4096, "P": 8,
                         "forward_device_ms": round(fk, 3), "backward_device_ms": round(bk, 3),
                         "forward_wall_ms": round(fwd, 3), "backward_wall_ms": round(bwd, 3),
                         "note": "ratio = device time of the adjoint pass / device time of tree + lists + eval + "
-                                "sensor pass; wall times include the host copies of J and of the 2x3N gradients"}}
+                                "sensor pass; wall times include the host copies of J and of the 2x3N gradients",
+                        "rate_vjp": {"eval_device_ms": round(ev, 3), "vjp_device_ms": round(vj, 3),
+                                     "ratio": round(vj / ev, 3),
+                                     "note": "xbar, alphabar of sum ubar.u + abar.dadt for the tree held fixed: "
+                                             "pair VJP on the forward and transposed P2P lists (FP32 pairs, FP64 "
+                                             "sums), adjoint vortex+dipole expansions along the transposed M2L list "
+                                             "(FP64)"}}}
 
 
 def leapfrog_scenario(args, X, A, S, tuned):

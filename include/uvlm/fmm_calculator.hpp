@@ -310,6 +310,35 @@ struct FMMCalculator {
     exafmm::detail::check(uvlm_probe_adjoint(h_.h, (int64_t)(probes.size() / 3), probes.data(), w.data(),
                                              dJ_dalpha.data(), dJ_dx.data()));
   }
+  // xbar, alphabar ([n][3], tree order) of S = sum ubar.u + abar.dadt on the current tree
+  // (calculate() is run first); the tree of the last init() is held fixed.
+  void rateVJP(const std::vector<double> &ubar, const std::vector<double> &abar, std::vector<double> &xbar,
+               std::vector<double> &alphabar)
+  {
+    if (ubar.size() != 3 * n_ || abar.size() != 3 * n_) throw std::runtime_error("ubar and abar must be [n][3]");
+    xbar.assign(3 * n_, 0.0);
+    alphabar.assign(3 * n_, 0.0);
+    exafmm::detail::check(uvlm_eval_vjp(h_.h, (int64_t)n_, ubar.data(), abar.data(), xbar.data(), alphabar.data()));
+  }
+  // One advance(dt) backwards from the device state: cotangents ([n][3], ORIGINAL body order)
+  // of the state after the step in, those of the state before it out.
+  void stepVJP(exafmm::real_t dt, std::vector<double> &xbar, std::vector<double> &alphabar)
+  {
+    if (xbar.size() != 3 * n_ || alphabar.size() != 3 * n_) throw std::runtime_error("cotangents must be [n][3]");
+    exafmm::detail::check(uvlm_step_vjp(h_.h, dt, (int64_t)n_, xbar.data(), alphabar.data()));
+  }
+  // Checkpointed trajectory: `steps` advances of dt from the device state with checkpoints every
+  // `every` steps, then (adjointTrajectory) the cotangent of the final state mapped back to the
+  // initial one, in place (original body order).
+  void forwardTrajectory(size_t steps, exafmm::real_t dt, size_t every)
+  {
+    exafmm::detail::check(uvlm_trajectory_forward(h_.h, (int64_t)steps, dt, (int64_t)every));
+  }
+  void adjointTrajectory(std::vector<double> &xbar, std::vector<double> &alphabar)
+  {
+    if (xbar.size() != 3 * n_ || alphabar.size() != 3 * n_) throw std::runtime_error("cotangents must be [n][3]");
+    exafmm::detail::check(uvlm_trajectory_vjp(h_.h, (int64_t)n_, xbar.data(), alphabar.data()));
+  }
   // Leaf size (and M2L plan size) for the bodies last passed to init()/copyBodies2Device();
   // returns the chosen ncrit.  init() again before calculate().
   size_t autotune(const std::vector<int64_t> &candidates, int reps = 2)
