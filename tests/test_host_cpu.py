@@ -79,11 +79,22 @@ def test_splitmix_reference_values():
     assert int(z[0]) == 0xE220A8397B1DCDAF and int(z[1]) == 0x6E789E6AA1B965F4
 
 
+def _g_series(x, der):
+    """d^der/dx^der of G(x) = (2/sqrt(pi)) sum (-1)^i 2(i+1) / ((2i+3)(i+1)!) x^i (x <= 2)."""
+    import math
+    tot = 0.0
+    for i in range(der, 60):
+        c = 2 / math.sqrt(math.pi) * (-1) ** i * 2 * (i + 1) / ((2 * i + 3) * math.factorial(i + 1))
+        tot += c * math.factorial(i) / math.factorial(i - der) * x ** (i - der)
+    return tot
+
+
 def test_p2p_polynomial_matches_erf_form():
     """The FP32 P2P's polynomial for G(x) = g(s)/s^3 (x = s^2, g = erf(s) - 2 s e^{-s^2}/sqrt(pi))
-    and its derivative, read from near.hip, against the erf form evaluated in FP64."""
+    and its first two derivatives, read from pair_kernel.hpp, against the erf form evaluated in
+    FP64 and (G'') the term-wise second derivative of G's power series."""
     import math
-    src = open(os.path.join(ROOT, "uvlm_amd", "csrc", "near.hip")).read()
+    src = open(os.path.join(ROOT, "uvlm_amd", "csrc", "pair_kernel.hpp")).read()
     body = src[src.index("c_gpoly[NPOLY] = {") + 18: src.index("};", src.index("c_gpoly[NPOLY] = {"))]
     coef = [float(v.strip().rstrip("f")) for v in body.split(",")]
     assert len(coef) == 10
@@ -96,16 +107,21 @@ def test_p2p_polynomial_matches_erf_form():
         t = x - 1.0
         p = sum(c * t ** i for i, c in enumerate(coef))
         pp = sum(i * c * t ** (i - 1) for i, c in enumerate(coef) if i)
+        ppp = sum(i * (i - 1) * c * t ** (i - 2) for i, c in enumerate(coef) if i > 1)
         assert abs(p - G) < 1e-9 * abs(G) + 1e-12
         assert abs(pp - Gp) < 2e-8 * 0.3
+        assert abs(_g_series(x, 1) - Gp) < 1e-12
+        assert abs(ppp - _g_series(x, 2)) < 1e-7
 
 
 def test_p2p_far_form_matches_erf_form():
     """The FP32 P2P's form for x = s^2 >= 2, read from near.hip: G = (1 - e (erfcx(s) +
     2s/sqrt(pi))) w^3 and G' = w^2 (2e/sqrt(pi) - 1.5 G) with w = 1/s, e = e^{-x} and erfcx by
-    the c_far polynomial in w, against the erf form evaluated in FP64."""
+    the c_far polynomial in w, and G'' = -(2/sqrt(pi)) e (w^2 + w^4) - 1.5 w^2 (G' - G w^2), all
+    read from pair_kernel.hpp, against the erf form evaluated in FP64 (G'' by differentiating
+    G' = (2/sqrt(pi)) e / x - 1.5 G / x once more, and by the power series at x = 2)."""
     import math
-    src = open(os.path.join(ROOT, "uvlm_amd", "csrc", "near.hip")).read()
+    src = open(os.path.join(ROOT, "uvlm_amd", "csrc", "pair_kernel.hpp")).read()
     body = src[src.index("c_far[NFAR] = {") + 15: src.index("};", src.index("c_far[NFAR] = {"))]
     coef = [float(v.strip().rstrip("f")) for v in body.split(",")]
     wc = float(re.search(r"FAR_WC\s*=\s*([0-9.eE+-]+)f", src).group(1))
@@ -125,3 +141,8 @@ def test_p2p_far_form_matches_erf_form():
         Gpr = (s * 4 * x * e / math.sqrt(math.pi) - 3 * gr) / (2 * s ** 5)
         assert abs(G - Gr) < 2e-8 * abs(Gr)
         assert abs(Gp - Gpr) < 5e-8 * abs(Gpr)
+        Gpp = -2 / math.sqrt(math.pi) * e * (w ** 2 + w ** 4) - 1.5 * w * w * (Gp - G * w * w)
+        Gppr = -2 / math.sqrt(math.pi) * e * (1 / x + 1 / x ** 2) - 1.5 * Gpr / x + 1.5 * Gr / x ** 2
+        assert abs(Gpp - Gppr) < 2e-7 * abs(Gppr)
+    assert abs(-2 / math.sqrt(math.pi) * math.exp(-2.0) * 0.75 - 1.5 * 0.5 * (_g_series(2.0, 1) - 0.5 * _g_series(2.0, 0))
+               - _g_series(2.0, 2)) < 1e-12
