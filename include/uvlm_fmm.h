@@ -175,6 +175,43 @@ int uvlm_step_vjp(uvlm_handle h, double dt, int64_t n, double *xbar, double *alp
  * end at the final state again.  No counterpart in the reference. */
 int uvlm_trajectory_forward(uvlm_handle h, int64_t steps, double dt, int64_t every);
 int uvlm_trajectory_vjp(uvlm_handle h, int64_t n, double *xbar, double *alphabar);
+/* Append n particles (X [n][3], alpha [n][3], sigma [n]) after the current ones; they take the
+ * next original indices.  The tree is rebuilt by the next build/advance.  Replaces
+ * particles.push_back (main.cpp:288) on a device-resident particle set. */
+int uvlm_append_bodies_soa(uvlm_handle h, int64_t n, const double *X, const double *alpha, const double *sigma);
+
+/* ---- Vortex-lattice wing (src/main.cpp:11-295): the caller that drives the particle wake ----
+ * A lattice of vertices [nchord+1][nspan+1][3] (row 0 the leading edge); nchord = 1 is the
+ * reference's Wing of LE/TE vertex couples (main.cpp:27-74).  Panel p = r * nspan + c carries
+ * the ring of getPanelVortexLine (60-73).  The system A gamma = -U.n - v.n is solved with
+ * A[i][j] = (velocity of ring j at control point i) . n_i (the reference fills AIC(j, i) with this
+ * value, 198-205; on its wing, a lattice of congruent parallelograms, A is symmetric, so the
+ * two agree), v the particle-induced velocity at the control points (getSensorData, 196-197). */
+#define UVLM_WING_MAX_PANELS 2048
+typedef struct uvlm_wing_s *uvlm_wing;
+/* Geometry (calculateTopology, 45-58), ring influences, system matrix and its LU (one workgroup,
+ * partial pivoting, in place of Eigen's fullPivLu, 214) on `device`.  Fails with UVLM_ERR_ARG
+ * where the reference exits: a control point on a vortex line (19-21). */
+int uvlm_wing_create(int device, int64_t nspan, int64_t nchord, const double *vertices, uvlm_wing *out);
+int uvlm_wing_destroy(uvlm_wing w);
+/* Control points [n][3], unit normals [n][3], areas [n], and the LU factors [n][n] (any NULL). */
+int uvlm_wing_get(uvlm_wing w, double *cp, double *normal, double *area, double *aic_lu);
+/* Rigid translation of the wing by d (the matrix and its factors are unchanged). */
+int uvlm_wing_translate(uvlm_wing w, const double d[3]);
+/* Circulations gamma [n] and the Kutta-Joukowski force [3] (223-244: sum over panels of the net
+ * circulation of the leading-edge segment times u x dl, u = rings + U + v at the control point)
+ * at the wing's current position.  particles may be NULL or empty; otherwise they are built and
+ * evaluated at their current state for v.  gamma and force may be NULL. */
+int uvlm_wing_solve(uvlm_wing w, uvlm_handle particles, const double U[3], double *gamma, double force[3]);
+/* Shed one particle per trailing-edge panel from the last solve and append it (246-289).
+ * reference_offset != 0 offsets z by U[1] dt as main.cpp:254 writes it, else by U[2] dt. */
+int uvlm_wing_shed(uvlm_wing w, uvlm_handle particles, const double U[3], double dt, int reference_offset);
+/* One pass of the reference's time loop (main.cpp:168-295): solve, shed, advance the particles
+ * by dt with the handle's scheme (its viscous scheme applied like uvlm_advance), then translate
+ * the wing by -U dt (its vertices are at -dt * step * U at step `step`, 182-184). */
+int uvlm_wing_step(uvlm_wing w, uvlm_handle particles, const double U[3], double dt, int reference_offset,
+                   double *gamma, double force[3]);
+
 /* Overwrite positions and strengths [n][3] in the CURRENT tree order without rebuilding
  * the tree or lists (frozen-tree perturbations, e.g. finite differences).  No counterpart
  * in the reference. */

@@ -1,0 +1,101 @@
+"""The vortex-lattice wing (uvlm_wing_*) and particle appending on the GPU against the oracle's
+restatement of the reference's loop (oracle/vlm.py, main.cpp:168-295) on the same inputs.
+
+Bars: geometry 1e-14; circulations and force of one solve 1e-11 (LU with partial pivoting vs
+LAPACK); the time loop (solve, shed, RK2 advance of the wake, wing moved by -U dt) 1e-10 per
+step while every particle sits in one leaf (the FMM is then the direct sum on both sides), 1e-8
+on a multi-level tree (400 panels); appending = setting the concatenated set, bit for bit.
+"""
+import numpy as np
+import pytest
+
+from uvlm_amd import FMMCalculator, make_bodies
+from uvlm_amd.vlm import Wing, reference_wing
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def _tapered():
+    V = reference_wing(npanels=12, chord=0.5)
+    V = np.stack([V[0], V[0] + 0.45 * (V[1] - V[0]), V[1]])
+    V[:, :, 1] *= (1.0 - 0.3 * np.linspace(0, 1, V.shape[1]))[None, :]
+    return V
+
+
+@pytest.mark.parametrize("which", ["reference40", "tapered12x2"])
+def test_wing_geometry_and_solve(orc, which):
+    from oracle import vlm
+    V = reference_wing(npanels=40) if which == "reference40" else _tapered()
+    U = np.array([0.0, 1.0, 0.0])
+    w, ow = Wing(V), vlm.Wing(V)
+    cp, nrm, area, _ = w.geometry()
+    assert np.abs(cp - ow.cp).max() < 1e-14 and np.abs(nrm - ow.normal).max() < 1e-14
+    assert np.abs(area - ow.area).max() < 1e-14 * ow.area.max()
+    g, F = w.solve(None, U)
+    eg = vlm.solve(ow, U, np.zeros((ow.n, 3)))
+    eF = vlm.force(ow, eg, U, np.zeros((ow.n, 3)))
+    print(f"[{which}] gamma {rel(g, eg):.3e} force {rel(F, eF):.3e}", flush=True)
+    assert rel(g, eg) < 1e-11 and rel(F, eF) < 1e-11
+
+
+def test_wing_rejects_a_control_point_on_a_vortex_line():
+    V = reference_wing(npanels=4)
+    V[1, 2] = V[0, 2]  # a degenerate panel: zero chord at one station
+    with pytest.raises(Exception):
+        Wing(V)
+
+
+def test_append_equals_setting_the_concatenation():
+    rng = np.random.default_rng(4)
+    X, A, S = rng.normal(size=(900, 3)), rng.normal(size=(900, 3)), np.full(900, 0.2)
+    g1 = FMMCalculator(P=6, theta=0.5, ncrit=64)
+    g1.set_arrays(X[:500], A[:500], S[:500])
+    g1.append_arrays(X[500:], A[500:], S[500:])
+    g1.build()
+    g1.calculate()
+    g2 = FMMCalculator(P=6, theta=0.5, ncrit=64)
+    g2.set_arrays(X, A, S)
+    g2.build()
+    g2.calculate()
+    a1, a2 = g1.arrays(), g2.arrays()
+    assert np.array_equal(g1.permutation(), g2.permutation())
+    for k in ("X", "alpha", "velocity", "J", "dadt"):
+        assert np.array_equal(a1[k], a2[k]), k
+
+
+@pytest.mark.parametrize("reference_offset", [True, False])
+def test_wing_loop_matches_oracle_one_leaf(orc, reference_offset):
+    from oracle import vlm
+    V = reference_wing(npanels=6, chord=0.4)
+    V = np.stack([V[0], (V[0] + V[1]) / 2, V[1]])
+    U, dt, steps = np.array([0.0, 1.0, 0.0]), 0.05, 8
+    hist, bodies = vlm.run(V, U, dt, steps, P=6, ncrit=256, scheme="rk2", reference_offset=reference_offset)
+    w = Wing(V)
+    p = FMMCalculator(P=6, theta=0.5, ncrit=256, scheme="rk2")
+    for t in range(steps):
+        g, F = w.step(p, U, dt, reference_offset)
+        eg, eF = hist[t]
+        assert rel(g, eg) < 1e-10 and rel(F, eF) < 1e-10, (t, rel(g, eg), rel(F, eF))
+    arr = p.arrays()
+    assert np.array_equal(p.permutation(), np.arange(p.n))
+    assert rel(arr["X"], bodies[:, 9:12]) < 1e-10 and rel(arr["alpha"], bodies[:, 12:15]) < 1e-10
+    print(f"[loop one leaf, offset {reference_offset}] n={p.n} last force {F}", flush=True)
+
+
+def test_wing_loop_400_panels_matches_oracle(orc):
+    """40 x 10 panels (the configuration's lattice), 5 steps, multi-level particle trees."""
+    from oracle import vlm
+    V = reference_wing(npanels=40, span=4.0, chord=1.0, angle_deg=5.0, nchord=10)
+    U, dt, steps = np.array([0.0, 1.0, 0.0]), 0.02, 5
+    hist, _ = vlm.run(V, U, dt, steps, P=8, theta=0.5, ncrit=16, scheme="rk2")
+    w = Wing(V)
+    p = FMMCalculator(P=8, theta=0.5, ncrit=16, scheme="rk2")
+    for t in range(steps):
+        g, F = w.step(p, U, dt)
+        eg, eF = hist[t]
+        print(f"  step {t}: gamma {rel(g, eg):.3e} force {rel(F, eF):.3e} leaves {p.timings()['n_leaves']}", flush=True)
+        assert rel(g, eg) < 1e-8 and rel(F, eF) < 1e-8

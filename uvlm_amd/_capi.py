@@ -19,7 +19,9 @@ EXPORTS = [
     "uvlm_get_lists", "uvlm_get_expansions", "uvlm_sensors_aos", "uvlm_direct", "uvlm_probe_adjoint",
     "uvlm_update_bodies_soa", "uvlm_relax", "uvlm_diagnostics", "uvlm_timings", "uvlm_sync", "uvlm_autotune",
     "uvlm_set_viscosity", "uvlm_viscosity", "uvlm_vorticity_aos", "uvlm_rbf_aos", "uvlm_eval_vjp",
-    "uvlm_step_vjp", "uvlm_trajectory_forward", "uvlm_trajectory_vjp",
+    "uvlm_step_vjp", "uvlm_trajectory_forward", "uvlm_trajectory_vjp", "uvlm_append_bodies_soa",
+    "uvlm_wing_create", "uvlm_wing_destroy", "uvlm_wing_get", "uvlm_wing_translate", "uvlm_wing_solve",
+    "uvlm_wing_shed", "uvlm_wing_step",
 ]
 
 
@@ -93,6 +95,14 @@ def lib():
             "uvlm_step_vjp": (I, [VP, D, I64, PD, PD]),
             "uvlm_trajectory_forward": (I, [VP, I64, D, I64]),
             "uvlm_trajectory_vjp": (I, [VP, I64, PD, PD]),
+            "uvlm_append_bodies_soa": (I, [VP, I64, PD, PD, PD]),
+            "uvlm_wing_create": (I, [I, I64, I64, PD, ctypes.POINTER(VP)]),
+            "uvlm_wing_destroy": (I, [VP]),
+            "uvlm_wing_get": (I, [VP, PD, PD, PD, PD]),
+            "uvlm_wing_translate": (I, [VP, PD]),
+            "uvlm_wing_solve": (I, [VP, VP, PD, PD, PD]),
+            "uvlm_wing_shed": (I, [VP, VP, PD, D, I]),
+            "uvlm_wing_step": (I, [VP, VP, PD, D, I, PD, PD]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

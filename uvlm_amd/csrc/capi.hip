@@ -71,6 +71,15 @@ struct uvlm_handle_s {
   }
 };
 
+// A vortex-lattice wing on one device (vlm.hip): geometry, ring influences at the control
+// points, the factored system matrix, and the last solve.
+struct uvlm_wing_s {
+  int         device = 0;
+  hipStream_t s      = nullptr;
+  int         nspan = 0, nchord = 0, n = 0;
+  DevBuf      V, cp, nrm, area, B, LU, piv, info, gamma, v, F, shedX, shedA, shedS;
+};
+
 namespace {
 
 int fail(int code, const std::string &msg)
@@ -239,6 +248,83 @@ void probe_vorticity(uvlm_handle h, int64_t m)
   zero_fields(h->s, h->sst.as<double>(), h->scap, m, F_UOLD, 3);
   gauss_near(h->s, h->stree, h->tree, h->slists, h->sst.as<double>(), h->scap, h->st.as<double>(), h->cap,
              h->sst.as<double>(), h->scap, 1, F_UOLD, 0.0);
+}
+
+// Reserve the probe state for m probes.
+void reserve_probes(uvlm_handle h, int64_t m)
+{
+  if (m > h->scap) {
+    h->scap = m;
+    h->sst.reserve(sizeof(double) * N_FIELDS * m);
+    h->sst2.reserve(sizeof(double) * N_FIELDS * m);
+    h->sperm.reserve(4 * m);
+  }
+}
+
+// getSensorData / calculateSensorPoints (fmmCalculator.hpp:469-491, 726-731) on the m probe
+// records already in h->sst: the probe tree (probes permuted into its order, h->sperm), lists
+// against the particle tree, M2L from the particle multipoles, P2P velocity, L2L, L2P.
+void sensor_pass(uvlm_handle h, int64_t m)
+{
+  hipStream_t   s  = h->s;
+  const int64_t sc = h->scap;
+  double       *sx = h->sst.as<double>();
+  build_tree(s, h->sc, sx + F_X * sc, sx + (F_X + 1) * sc, sx + (F_X + 2) * sc, m, h->prm.ncrit, h->stree,
+             h->sperm.as<int32_t>());
+  gather_fields(s, h->sst.as<double>(), h->sst2.as<double>(), sc, sc, N_FIELDS, h->sperm.as<int32_t>(), m);
+  swap_buf(h->sst, h->sst2);
+  traverse(s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
+  const int P = h->prm.P;
+  h->Ls.reserve(sizeof(double2) * h->NT * h->stree.C);
+  double2 *Ls = h->Ls.as<double2>();
+  reset_sensor_rates(s, h->sst.as<double>(), sc, m);
+  m2l(s, P, h->stree, h->tree, h->slists, h->sched, h->M.as<double2>(), Ls, &h->sc);
+  p2p(s, h->stree, h->tree, h->slists, h->sst.as<double>(), sc, h->st.as<double>(), h->cap, h->sst.as<double>(), sc,
+      h->prm.precision, 1, h->prm.ncrit, &h->sc);
+  for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls);
+  l2p_stretch(s, P, h->stree, Ls, h->sst.as<double>(), sc, h->prm.stretching, h->prm.formulation, 1);
+}
+
+// Velocity of the particles (their FMM at the current state: build + eval) at m device points
+// (AoS [m][3]) into out (AoS [m][3], same order).  No particles: zero.
+void velocity_at_points(uvlm_handle h, int64_t m, const double *pts, double *out)
+{
+  if (h->N <= 0) {
+    zero_doubles(h->s, out, 3 * m);
+    return;
+  }
+  do_build(h);
+  do_eval(h, false);
+  reserve_probes(h, m);
+  points_to_soa(h->s, m, pts, h->sst.as<double>(), h->scap);
+  sensor_pass(h, m);
+  probe_velocity_out(h->s, m, h->sperm.as<int32_t>(), h->sst.as<double>(), h->scap, out);
+}
+
+// Append k bodies (device AoS X [k][3], alpha [k][3], sigma [k]) after the current N, growing
+// the state (fields keep their contents) when needed; new bodies get the next original indices.
+void append_bodies_dev(uvlm_handle h, int64_t k, const double *X, const double *A, const double *S)
+{
+  const int64_t N = h->N, want = N + k;
+  if (want > h->cap) {
+    const int64_t ncap = std::max<int64_t>(want, h->cap + h->cap / 2);
+    DevBuf        nst, norig;
+    nst.reserve(sizeof(double) * N_FIELDS * ncap);
+    norig.reserve(4 * ncap);
+    for (int f = 0; f < N_FIELDS && N > 0; f++)
+      UVLM_HIP(hipMemcpyAsync(nst.as<double>() + f * ncap, h->F(f), 8 * N, hipMemcpyDeviceToDevice, h->s));
+    if (N > 0) UVLM_HIP(hipMemcpyAsync(norig.p, h->orig.p, 4 * N, hipMemcpyDeviceToDevice, h->s));
+    UVLM_HIP(hipStreamSynchronize(h->s));
+    swap_buf(h->st, nst);
+    swap_buf(h->orig, norig);
+    h->cap = ncap;
+    h->st2.reserve(sizeof(double) * N_FIELDS * ncap);
+    h->orig2.reserve(4 * ncap);
+    h->perm.reserve(4 * ncap);
+  }
+  append_bodies(h->s, N, k, X, A, S, h->st.as<double>(), h->cap, h->orig.as<int32_t>());
+  h->N     = want;
+  h->built = h->evaluated = false;
 }
 
 int stages_of(int scheme) { return scheme == UVLM_EULER ? 1 : scheme == UVLM_RK2 ? 2 : scheme == UVLM_RK3 ? 3 : 4; }
@@ -717,32 +803,13 @@ int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride)
     if (stride < 296 || stride % 8) throw Error(UVLM_ERR_ARG, "stride must be >= 296 and a multiple of 8");
     set_device(h);
     hipStream_t s = h->s;
-    if (m > h->scap) {
-      h->scap = m;
-      h->sst.reserve(sizeof(double) * N_FIELDS * m);
-      h->sst2.reserve(sizeof(double) * N_FIELDS * m);
-      h->sperm.reserve(4 * m);
-    }
+    reserve_probes(h, m);
     const int64_t sc = h->scap;
     h->staging.reserve((size_t)m * stride);
     UVLM_HIP(hipMemcpyAsync(h->staging.p, probes, (size_t)m * stride, hipMemcpyHostToDevice, s));
     aos_to_soa(s, h->staging.p, m, stride / 8, h->sst.as<double>(), sc);
     UVLM_HIP(hipEventRecord(h->aev[0], s));
-    double *sx = h->sst.as<double>();
-    build_tree(s, h->sc, sx + F_X * sc, sx + (F_X + 1) * sc, sx + (F_X + 2) * sc, m, h->prm.ncrit, h->stree,
-               h->sperm.as<int32_t>());
-    gather_fields(s, h->sst.as<double>(), h->sst2.as<double>(), sc, sc, N_FIELDS, h->sperm.as<int32_t>(), m);
-    swap_buf(h->sst, h->sst2);
-    traverse(s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
-    const int P = h->prm.P;
-    h->Ls.reserve(sizeof(double2) * h->NT * h->stree.C);
-    double2 *Ls = h->Ls.as<double2>();
-    reset_sensor_rates(s, h->sst.as<double>(), sc, m);
-    m2l(s, P, h->stree, h->tree, h->slists, h->sched, h->M.as<double2>(), Ls, &h->sc);
-    p2p(s, h->stree, h->tree, h->slists, h->sst.as<double>(), sc, h->st.as<double>(), h->cap, h->sst.as<double>(), sc,
-        h->prm.precision, 1, h->prm.ncrit, &h->sc);
-    for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls);
-    l2p_stretch(s, P, h->stree, Ls, h->sst.as<double>(), sc, h->prm.stretching, h->prm.formulation, 1);
+    sensor_pass(h, m);
     UVLM_HIP(hipEventRecord(h->aev[1], s));
     if (stride > 296) UVLM_HIP(hipMemcpyAsync(h->staging.p, probes, (size_t)m * stride, hipMemcpyHostToDevice, s));
     soa_to_aos(s, h->sst.as<double>(), sc, h->staging.p, m, stride / 8);
@@ -1121,6 +1188,201 @@ int uvlm_trajectory_vjp(uvlm_handle h, int64_t n, double *xbar, double *alphabar
     }
     download_cot(h, h->vlam.as<double>(), xbar, alphabar);
     load_state_orig(h, fin.as<double>());
+  });
+}
+
+int uvlm_append_bodies_soa(uvlm_handle h, int64_t n, const double *X, const double *alpha, const double *sigma)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (n <= 0 || !X || !alpha || !sigma) throw Error(UVLM_ERR_ARG, "null or empty input");
+    set_device(h);
+    DevBuf buf;
+    buf.reserve(sizeof(double) * 7 * n);
+    double *d = buf.as<double>();
+    UVLM_HIP(hipMemcpyAsync(d, X, 24 * n, hipMemcpyHostToDevice, h->s));
+    UVLM_HIP(hipMemcpyAsync(d + 3 * n, alpha, 24 * n, hipMemcpyHostToDevice, h->s));
+    UVLM_HIP(hipMemcpyAsync(d + 6 * n, sigma, 8 * n, hipMemcpyHostToDevice, h->s));
+    append_bodies_dev(h, n, d, d + 3 * n, d + 6 * n);
+    UVLM_HIP(hipStreamSynchronize(h->s));
+  });
+}
+
+int uvlm_wing_create(int device, int64_t nspan, int64_t nchord, const double *vertices, uvlm_wing *out)
+{
+  uvlm_wing w = nullptr;
+  const int rc = guarded([&] {
+    if (!out || !vertices) throw Error(UVLM_ERR_ARG, "null argument");
+    if (nspan < 1 || nchord < 1 || nspan * nchord > UVLM_WING_MAX_PANELS)
+      throw Error(UVLM_ERR_ARG, "the lattice needs 1 <= nspan * nchord <= " + std::to_string(UVLM_WING_MAX_PANELS) +
+                                    " panels");
+    UVLM_HIP(hipSetDevice(device));
+    w         = new uvlm_wing_s;
+    w->device = device;
+    w->nspan  = (int)nspan;
+    w->nchord = (int)nchord;
+    w->n      = (int)(nspan * nchord);
+    UVLM_HIP(hipStreamCreate(&w->s));
+    const int     n  = w->n;
+    const int64_t nv = (nspan + 1) * (nchord + 1);
+    w->V.reserve(24 * nv);
+    w->cp.reserve(24 * n);
+    w->nrm.reserve(24 * n);
+    w->area.reserve(8 * n);
+    w->B.reserve(24 * (size_t)n * n);
+    w->LU.reserve(8 * (size_t)n * n);
+    w->piv.reserve(4 * n);
+    w->info.reserve(8);
+    w->gamma.reserve(8 * n);
+    w->v.reserve(24 * n);
+    w->F.reserve(24);
+    w->shedX.reserve(24 * nspan);
+    w->shedA.reserve(24 * nspan);
+    w->shedS.reserve(8 * nspan);
+    UVLM_HIP(hipMemcpyAsync(w->V.p, vertices, 24 * nv, hipMemcpyHostToDevice, w->s));
+    // geometry, ring influences at the control points, system matrix, LU (main.cpp:190-214)
+    wing_geom(w->s, n, w->nspan, w->V.as<double>(), w->cp.as<double>(), w->nrm.as<double>(), w->area.as<double>());
+    int *flags = w->info.as<int>();  // [0] singular segments, [1] LU info
+    UVLM_HIP(hipMemsetAsync(flags, 0, 8, w->s));
+    wing_influence(w->s, n, n, w->nspan, w->V.as<double>(), w->cp.as<double>(), w->nrm.as<double>(),
+                   w->B.as<double>(), w->LU.as<double>(), flags);
+    lu_factor(w->s, n, w->LU.as<double>(), w->piv.as<int>(), flags + 1);
+    int hf[2] = {0, 0};
+    UVLM_HIP(hipMemcpyAsync(hf, flags, 8, hipMemcpyDeviceToHost, w->s));
+    UVLM_HIP(hipStreamSynchronize(w->s));
+    if (hf[0]) throw Error(UVLM_ERR_ARG, "a control point on a vortex line or a zero-length vortex line (main.cpp:19-21)");
+    if (hf[1]) throw Error(UVLM_ERR_ARG, "singular system matrix (zero pivot at column " + std::to_string(hf[1] - 1) + ")");
+    *out = w;
+    w    = nullptr;
+  });
+  if (w) {
+    if (w->s) (void)hipStreamDestroy(w->s);
+    delete w;
+  }
+  return rc;
+}
+
+int uvlm_wing_destroy(uvlm_wing w)
+{
+  if (!w) return UVLM_OK;
+  (void)hipSetDevice(w->device);
+  if (w->s) (void)hipStreamSynchronize(w->s);
+  if (w->s) (void)hipStreamDestroy(w->s);
+  delete w;
+  return UVLM_OK;
+}
+
+int uvlm_wing_get(uvlm_wing w, double *cp, double *normal, double *area, double *aic_lu)
+{
+  return guarded([&] {
+    if (!w) throw Error(UVLM_ERR_ARG, "null wing");
+    UVLM_HIP(hipSetDevice(w->device));
+    const int n = w->n;
+    if (cp) UVLM_HIP(hipMemcpyAsync(cp, w->cp.p, 24 * n, hipMemcpyDeviceToHost, w->s));
+    if (normal) UVLM_HIP(hipMemcpyAsync(normal, w->nrm.p, 24 * n, hipMemcpyDeviceToHost, w->s));
+    if (area) UVLM_HIP(hipMemcpyAsync(area, w->area.p, 8 * n, hipMemcpyDeviceToHost, w->s));
+    if (aic_lu) UVLM_HIP(hipMemcpyAsync(aic_lu, w->LU.p, 8 * (size_t)n * n, hipMemcpyDeviceToHost, w->s));
+    UVLM_HIP(hipStreamSynchronize(w->s));
+  });
+}
+
+int uvlm_wing_translate(uvlm_wing w, const double d[3])
+{
+  return guarded([&] {
+    if (!w || !d) throw Error(UVLM_ERR_ARG, "null argument");
+    UVLM_HIP(hipSetDevice(w->device));
+    translate3(w->s, (int64_t)(w->nspan + 1) * (w->nchord + 1), w->V.as<double>(), d);
+    translate3(w->s, w->n, w->cp.as<double>(), d);
+    UVLM_HIP(hipStreamSynchronize(w->s));
+  });
+}
+
+namespace {
+void check_pair(uvlm_wing w, uvlm_handle h)
+{
+  if (!w) throw Error(UVLM_ERR_ARG, "null wing");
+  if (h && h->prm.device != w->device) throw Error(UVLM_ERR_ARG, "wing and particles on different devices");
+}
+
+// main.cpp:196-244 at the wing's current position: particle velocity at the control points,
+// right-hand side, circulations, force.
+void wing_solve(uvlm_wing w, uvlm_handle h, const double U[3])
+{
+  const int n    = w->n;
+  double   *vptr = nullptr;
+  if (h && h->N > 0) {
+    velocity_at_points(h, n, w->cp.as<double>(), w->v.as<double>());
+    UVLM_HIP(hipStreamSynchronize(h->s));
+    vptr = w->v.as<double>();
+  }
+  wing_rhs(w->s, n, w->nrm.as<double>(), U, vptr, w->gamma.as<double>());
+  lu_solve(w->s, n, w->LU.as<double>(), w->piv.as<int>(), w->gamma.as<double>());
+  wing_force(w->s, n, w->nspan, w->V.as<double>(), w->B.as<double>(), w->gamma.as<double>(), U, vptr,
+             w->F.as<double>());
+}
+
+void wing_outputs(uvlm_wing w, double *gamma, double *force)
+{
+  if (gamma) UVLM_HIP(hipMemcpyAsync(gamma, w->gamma.p, 8 * w->n, hipMemcpyDeviceToHost, w->s));
+  if (force) UVLM_HIP(hipMemcpyAsync(force, w->F.p, 24, hipMemcpyDeviceToHost, w->s));
+  UVLM_HIP(hipStreamSynchronize(w->s));
+}
+
+// main.cpp:246-289: one particle per trailing-edge panel, appended to the particles.
+void wing_shed_into(uvlm_wing w, uvlm_handle h, const double U[3], double dt, int ref_offset)
+{
+  wing_shed(w->s, w->nspan, w->nchord, w->V.as<double>(), w->gamma.as<double>(), U, dt, ref_offset,
+            w->shedX.as<double>(), w->shedA.as<double>(), w->shedS.as<double>());
+  UVLM_HIP(hipStreamSynchronize(w->s));
+  append_bodies_dev(h, w->nspan, w->shedX.as<double>(), w->shedA.as<double>(), w->shedS.as<double>());
+  UVLM_HIP(hipStreamSynchronize(h->s));
+}
+}  // namespace
+
+int uvlm_wing_solve(uvlm_wing w, uvlm_handle particles, const double U[3], double *gamma, double force[3])
+{
+  return guarded([&] {
+    check_pair(w, particles);
+    if (!U) throw Error(UVLM_ERR_ARG, "null freestream");
+    UVLM_HIP(hipSetDevice(w->device));
+    wing_solve(w, particles, U);
+    wing_outputs(w, gamma, force);
+  });
+}
+
+int uvlm_wing_shed(uvlm_wing w, uvlm_handle particles, const double U[3], double dt, int reference_offset)
+{
+  return guarded([&] {
+    check_pair(w, particles);
+    if (!particles || !U) throw Error(UVLM_ERR_ARG, "null particles or freestream");
+    UVLM_HIP(hipSetDevice(w->device));
+    wing_shed_into(w, particles, U, dt, reference_offset);
+  });
+}
+
+int uvlm_wing_step(uvlm_wing w, uvlm_handle particles, const double U[3], double dt, int reference_offset,
+                   double *gamma, double force[3])
+{
+  return guarded([&] {
+    check_pair(w, particles);
+    if (!particles || !U) throw Error(UVLM_ERR_ARG, "null particles or freestream");
+    UVLM_HIP(hipSetDevice(w->device));
+    uvlm_handle h = particles;
+    wing_solve(w, h, U);
+    wing_outputs(w, gamma, force);
+    wing_shed_into(w, h, U, dt, reference_offset);
+    for (int k = 0; k < stages_of(h->prm.scheme); k++) {
+      do_build(h);
+      do_eval(h, false);
+      apply_viscosity(h, dt);
+      rk_stage(h->s, h->st.as<double>(), h->cap, h->N, h->prm.scheme, k, dt, 0);
+    }
+    UVLM_HIP(hipStreamSynchronize(h->s));
+    h->evaluated = false;
+    const double d[3] = {-U[0] * dt, -U[1] * dt, -U[2] * dt};
+    translate3(w->s, (int64_t)(w->nspan + 1) * (w->nchord + 1), w->V.as<double>(), d);
+    translate3(w->s, w->n, w->cp.as<double>(), d);
+    UVLM_HIP(hipStreamSynchronize(w->s));
   });
 }
 

@@ -185,6 +185,23 @@ void state_to_orig(hipStream_t s, int64_t N, const int32_t *orig, const double *
 // out[i] = sum_k c[k] p[k][i] (at most 6 terms)
 void lincomb(hipStream_t s, int64_t n, double *out, int nterms, const double *const *p, const double *c);
 
+// vlm.hip: vortex-lattice wing (vertices V AoS [(nchord+1)(nspan+1)][3], n = nspan * nchord panels)
+void wing_geom(hipStream_t s, int n, int nspan, const double *V, double *cp, double *nrm, double *area);
+void wing_influence(hipStream_t s, int m, int n, int nspan, const double *V, const double *pts, const double *nrm,
+                    double *B, double *A, int *singular);
+void lu_factor(hipStream_t s, int n, double *A, int *piv, int *info);
+void lu_solve(hipStream_t s, int n, const double *LU, const int *piv, double *b);
+void wing_rhs(hipStream_t s, int n, const double *nrm, const double U[3], const double *v, double *rhs);
+void wing_force(hipStream_t s, int n, int nspan, const double *V, const double *B, const double *gamma, const double U[3],
+                const double *v, double *F);
+void wing_shed(hipStream_t s, int nspan, int nchord, const double *V, const double *gamma, const double U[3], double dt,
+               int ref_offset, double *X, double *A, double *S);
+void translate3(hipStream_t s, int64_t n, double *p, const double d[3]);
+void points_to_soa(hipStream_t s, int64_t m, const double *pts, double *st, int64_t cap);
+void probe_velocity_out(hipStream_t s, int64_t m, const int32_t *perm, const double *st, int64_t cap, double *out);
+void append_bodies(hipStream_t s, int64_t N, int64_t k, const double *X, const double *A, const double *S, double *st,
+                   int64_t cap, int32_t *orig);
+
 // body.hip
 void aos_to_soa(hipStream_t s, const void *aos, int64_t n, int64_t stride_doubles, double *st, int64_t cap);
 void soa_to_aos(hipStream_t s, const double *st, int64_t cap, void *aos, int64_t n, int64_t stride_doubles);

@@ -1,0 +1,373 @@
+// vlm.hip -- the vortex-lattice wing that drives the particle wake (src/main.cpp:11-295), FP64.
+//
+// Lattice: vertices V[r][c] (r = 0 .. nchord, row 0 the leading edge; c = 0 .. nspan), AoS
+// doubles.  nchord = 1 is the reference's wing (LE_Vertices[c] = V[0][c], TE_Vertices[c] =
+// V[1][c], main.cpp:27-74).  Panel p = r * nspan + c has corners LE_c = V[r][c],
+// LE_c1 = V[r][c+1], TE_c = V[r+1][c], TE_c1 = V[r+1][c+1] and the ring of lines
+// (getPanelVortexLine, main.cpp:60-73) TE_c1->TE_c, TE_c->LE_c, LE_c->LE_c1, LE_c1->TE_c1.
+//
+// Kernels: geometry (calculateTopology, 45-58), ring influences B[t][j] at points (one lane per
+// (point, panel), the segment formula of 11-25), the system matrix A[i][j] = B[i][j] . n_i,
+// a one-workgroup LU with partial pivoting and its solve (the reference's fullPivLu, 214),
+// the right-hand side (207-212), the Kutta-Joukowski force (223-244) and the shed particles
+// (246-289).  Every sum runs in a fixed order (deterministic).
+#include "common.hpp"
+
+namespace uvlm {
+
+namespace {
+
+constexpr int LU_WG = 1024;
+
+struct V3 {
+  double x, y, z;
+};
+__device__ inline V3 ld3(const double *p) { return V3{p[0], p[1], p[2]}; }
+__device__ inline V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline V3 operator*(double s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+__device__ inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline V3 cross(V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+struct Corners {
+  V3 le, le1, te, te1;
+};
+__device__ inline Corners corners(const double *V, int nspan, int p)
+{
+  const int r = p / nspan, c = p % nspan, w = nspan + 1;
+  return Corners{ld3(V + 3 * (r * w + c)), ld3(V + 3 * (r * w + c + 1)), ld3(V + 3 * ((r + 1) * w + c)),
+                 ld3(V + 3 * ((r + 1) * w + c + 1))};
+}
+
+// Unit vortex segment a -> b at point x (main.cpp:11-25); flags the reference's singular case
+// (|r1|, |r2| or |r1 x r2| below 1e-6, where it calls std::exit) and returns zero there.
+__device__ inline V3 segment(V3 a, V3 b, V3 x, int *singular)
+{
+  const V3     r1 = x - a, r2 = x - b, r0 = b - a;
+  const V3     c  = cross(r1, r2);
+  const double n1 = sqrt(dot(r1, r1)), n2 = sqrt(dot(r2, r2)), c2 = dot(c, c);
+  if (n1 < 1e-6 || n2 < 1e-6 || sqrt(c2) < 1e-6) {
+    atomicAdd(singular, 1);
+    return V3{0, 0, 0};
+  }
+  const double k = (r0.x * (r1.x / n1 - r2.x / n2) + r0.y * (r1.y / n1 - r2.y / n2) + r0.z * (r1.z / n1 - r2.z / n2)) /
+                   c2 * INV_4PI;
+  return k * c;
+}
+
+__device__ inline V3 ring(const Corners &q, V3 x, int *singular)
+{
+  return segment(q.te1, q.te, x, singular) + segment(q.te, q.le, x, singular) + segment(q.le, q.le1, x, singular) +
+         segment(q.le1, q.te1, x, singular);
+}
+
+// calculateTopology (main.cpp:45-58): control point, unit normal, area of each panel.
+__global__ void k_wing_geom(int n, int nspan, const double *__restrict__ V, double *__restrict__ cp,
+                            double *__restrict__ nrm, double *__restrict__ area)
+{
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const Corners q = corners(V, nspan, p);
+  const V3      m = 0.25 * (q.le + q.le1 + q.te + q.te1);
+  V3            v = cross(q.le1 - q.te, q.le - q.te1);
+  const double  l = sqrt(dot(v, v));
+  v             = (1.0 / l) * v;
+  cp[3 * p] = m.x, cp[3 * p + 1] = m.y, cp[3 * p + 2] = m.z;
+  nrm[3 * p] = v.x, nrm[3 * p + 1] = v.y, nrm[3 * p + 2] = v.z;
+  area[p] = 0.5 * l;
+}
+
+// B[t][j] (3 doubles) = velocity at pts[t] of ring j with unit circulation; with nrm given (the
+// points are the control points), also A[t][j] = B[t][j] . n_t.
+__global__ void k_wing_influence(int m, int n, int nspan, const double *__restrict__ V, const double *__restrict__ pts,
+                                 const double *__restrict__ nrm, double *__restrict__ B, double *__restrict__ A,
+                                 int *singular)
+{
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= (int64_t)m * n) return;
+  const int t = (int)(e / n), j = (int)(e % n);
+  const V3  u = ring(corners(V, nspan, j), ld3(pts + 3 * t), singular);
+  B[3 * e] = u.x, B[3 * e + 1] = u.y, B[3 * e + 2] = u.z;
+  if (A) A[e] = dot(u, ld3(nrm + 3 * t));
+}
+
+// In-place LU with partial pivoting of the row-major n x n matrix A by one workgroup: pivot
+// = largest |A[i][k]|, i >= k (smallest i on ties); piv[k] = the row swapped with k;
+// *info = 1 + the first k with a zero pivot (0 if none).
+__global__ __launch_bounds__(LU_WG) void k_lu(int n, double *__restrict__ A, int *__restrict__ piv, int *info)
+{
+  __shared__ double sv[LU_WG];
+  __shared__ int    si[LU_WG];
+  const int         tid = threadIdx.x;
+  if (tid == 0) *info = 0;
+  for (int k = 0; k < n; k++) {
+    double bv = -1.0;
+    int    bi = n;
+    for (int i = k + tid; i < n; i += LU_WG) {
+      const double v = fabs(A[(int64_t)i * n + k]);
+      if (v > bv) {
+        bv = v;
+        bi = i;
+      }
+    }
+    sv[tid] = bv;
+    si[tid] = bi;
+    __syncthreads();
+    for (int o = LU_WG / 2; o > 0; o >>= 1) {
+      if (tid < o) {
+        const double v = sv[tid + o];
+        const int    i = si[tid + o];
+        if (v > sv[tid] || (v == sv[tid] && i < si[tid])) {
+          sv[tid] = v;
+          si[tid] = i;
+        }
+      }
+      __syncthreads();
+    }
+    const int p = si[0];
+    __syncthreads();
+    if (tid == 0) piv[k] = p;
+    if (p != k)
+      for (int j = tid; j < n; j += LU_WG) {
+        const double a = A[(int64_t)k * n + j];
+        A[(int64_t)k * n + j] = A[(int64_t)p * n + j];
+        A[(int64_t)p * n + j] = a;
+      }
+    __syncthreads();
+    const double d = A[(int64_t)k * n + k];
+    if (d == 0.0) {
+      if (tid == 0 && *info == 0) *info = k + 1;
+      __syncthreads();
+      continue;
+    }
+    for (int i = k + 1 + tid; i < n; i += LU_WG) A[(int64_t)i * n + k] /= d;
+    __syncthreads();
+    const int r = n - k - 1;
+    for (int64_t e = tid; e < (int64_t)r * r; e += LU_WG) {
+      const int i = k + 1 + (int)(e / r), j = k + 1 + (int)(e % r);
+      A[(int64_t)i * n + j] -= A[(int64_t)i * n + k] * A[(int64_t)k * n + j];
+    }
+    __syncthreads();
+  }
+}
+
+// x = A^{-1} b from k_lu's factors; b is overwritten with x.  One workgroup, columns in order.
+__global__ __launch_bounds__(LU_WG) void k_lu_solve(int n, const double *__restrict__ LU, const int *__restrict__ piv,
+                                                    double *__restrict__ b)
+{
+  const int tid = threadIdx.x;
+  if (tid == 0)
+    for (int k = 0; k < n; k++) {
+      const int p = piv[k];
+      if (p != k) {
+        const double t = b[k];
+        b[k]           = b[p];
+        b[p]           = t;
+      }
+    }
+  __syncthreads();
+  for (int j = 0; j < n; j++) {  // L y = b (unit diagonal)
+    const double y = b[j];
+    for (int i = j + 1 + tid; i < n; i += LU_WG) b[i] -= LU[(int64_t)i * n + j] * y;
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; j--) {  // U x = y
+    __syncthreads();
+    const double x = b[j] / LU[(int64_t)j * n + j];
+    __syncthreads();
+    if (tid == 0) b[j] = x;
+    for (int i = tid; i < j; i += LU_WG) b[i] -= LU[(int64_t)i * n + j] * x;
+    __syncthreads();
+  }
+}
+
+// rhs_i = -U . n_i - v_i . n_i (main.cpp:207-212); v may be null (no particles).
+__global__ void k_wing_rhs(int n, const double *__restrict__ nrm, double Ux, double Uy, double Uz,
+                           const double *__restrict__ v, double *__restrict__ rhs)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const V3 nn = ld3(nrm + 3 * i);
+  double   r  = -(Ux * nn.x + Uy * nn.y + Uz * nn.z);
+  if (v) r -= dot(ld3(v + 3 * i), nn);
+  rhs[i] = r;
+}
+
+// Kutta-Joukowski force (main.cpp:223-244), one workgroup: u_i = sum_j B[i][j] gamma_j + U + v_i
+// at control point i, F = sum_i Gamma_i (u_i x (LE_c1 - LE_c)), Gamma_i = gamma_i minus the
+// circulation of the panel upstream (gamma_i on the first row).  Per-lane partial sums in panel
+// order, then a fixed-order tree over the lanes.
+__global__ __launch_bounds__(256) void k_wing_force(int n, int nspan, const double *__restrict__ V,
+                                                    const double *__restrict__ B, const double *__restrict__ gamma,
+                                                    double Ux, double Uy, double Uz, const double *__restrict__ v,
+                                                    double *__restrict__ F)
+{
+  __shared__ double red[3][256];
+  const int         tid = threadIdx.x;
+  V3                acc{0, 0, 0};
+  for (int i = tid; i < n; i += 256) {
+    V3 u{Ux, Uy, Uz};
+    for (int j = 0; j < n; j++) u = u + gamma[j] * ld3(B + 3 * ((int64_t)i * n + j));
+    if (v) u = u + ld3(v + 3 * i);
+    const Corners q  = corners(V, nspan, i);
+    const double  Gn = gamma[i] - (i >= nspan ? gamma[i - nspan] : 0.0);
+    acc              = acc + Gn * cross(u, q.le1 - q.le);
+  }
+  red[0][tid] = acc.x, red[1][tid] = acc.y, red[2][tid] = acc.z;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o)
+      for (int d = 0; d < 3; d++) red[d][tid] += red[d][tid + o];
+    __syncthreads();
+  }
+  if (tid < 3) F[tid] = red[tid][0];
+}
+
+// Particles shed from the trailing-edge row (main.cpp:246-289), one lane per spanwise panel c:
+// X = midpoint of TE_c1 -> TE_c + U dt (z offset U[1] dt as the reference writes it when
+// ref_offset, else U[2] dt), alpha = (TE_c - TE_c1) g_c + (LE_c - TE_c) (g_c - g_{c-1}) / 2
+// + (TE_c1 - LE_c1) (g_c - g_{c+1}) / 2 with the full g_c at the tips, sigma = 2.5 |TE_c - TE_c1|.
+__global__ void k_wing_shed(int nspan, int nchord, const double *__restrict__ V, const double *__restrict__ gamma,
+                            double Ux, double Uy, double Uz, double dt, int ref_offset, double *__restrict__ X,
+                            double *__restrict__ Al, double *__restrict__ S)
+{
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= nspan) return;
+  const int     p  = (nchord - 1) * nspan + c;
+  const Corners q  = corners(V, nspan, p);
+  const double *g  = gamma + (nchord - 1) * nspan;
+  const V3      d  = q.te - q.te1;
+  const V3      mp = 0.5 * (q.te1 + q.te);
+  X[3 * c]         = mp.x + Ux * dt;
+  X[3 * c + 1]     = mp.y + Uy * dt;
+  X[3 * c + 2]     = mp.z + (ref_offset ? Uy : Uz) * dt;
+  V3 a             = g[c] * d;
+  a                = a + (c > 0 ? 0.5 * (g[c] - g[c - 1]) : g[c]) * (q.le - q.te);
+  a                = a + (c < nspan - 1 ? 0.5 * (g[c] - g[c + 1]) : g[c]) * (q.te1 - q.le1);
+  Al[3 * c] = a.x, Al[3 * c + 1] = a.y, Al[3 * c + 2] = a.z;
+  S[c] = 2.5 * sqrt(dot(d, d));
+}
+
+__global__ void k_translate(int64_t n, double *__restrict__ p, double dx, double dy, double dz)
+{
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i >= n) return;
+  p[3 * i] += dx;
+  p[3 * i + 1] += dy;
+  p[3 * i + 2] += dz;
+}
+
+// Probe records from points: X from pts (AoS [m][3]), every other field zero.
+__global__ void k_points_to_soa(int64_t m, const double *__restrict__ pts, double *__restrict__ st, int64_t cap)
+{
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i >= m) return;
+  for (int f = 0; f < N_FIELDS; f++) st[f * cap + i] = 0.0;
+  for (int d = 0; d < 3; d++) st[(F_X + d) * cap + i] = pts[3 * i + d];
+}
+
+// out[perm[i]] = velocity of probe i (tree order -> the order the points were given in).
+__global__ void k_probe_velocity_out(int64_t m, const int32_t *__restrict__ perm, const double *__restrict__ st,
+                                     int64_t cap, double *__restrict__ out)
+{
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i >= m) return;
+  const int64_t o = perm[i];
+  for (int d = 0; d < 3; d++) out[3 * o + d] = st[(F_U + d) * cap + i];
+}
+
+// New bodies N .. N+k-1: X, alpha, sigma from AoS arrays, every other field zero, original
+// index = position.
+__global__ void k_append_bodies(int64_t N, int64_t k, const double *__restrict__ X, const double *__restrict__ A,
+                                const double *__restrict__ S, double *__restrict__ st, int64_t cap,
+                                int32_t *__restrict__ orig)
+{
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i >= k) return;
+  const int64_t b = N + i;
+  for (int f = 0; f < N_FIELDS; f++) st[f * cap + b] = 0.0;
+  for (int d = 0; d < 3; d++) {
+    st[(F_X + d) * cap + b] = X[3 * i + d];
+    st[(F_A + d) * cap + b] = A[3 * i + d];
+  }
+  st[F_SIG * cap + b] = S[i];
+  orig[b]             = (int32_t)b;
+}
+
+inline int nb256(int64_t n) { return (int)std::max<int64_t>(1, (n + 255) / 256); }
+
+}  // namespace
+
+void wing_geom(hipStream_t s, int n, int nspan, const double *V, double *cp, double *nrm, double *area)
+{
+  k_wing_geom<<<nb256(n), 256, 0, s>>>(n, nspan, V, cp, nrm, area);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_influence(hipStream_t s, int m, int n, int nspan, const double *V, const double *pts, const double *nrm,
+                    double *B, double *A, int *singular)
+{
+  k_wing_influence<<<nb256((int64_t)m * n), 256, 0, s>>>(m, n, nspan, V, pts, nrm, B, A, singular);
+  UVLM_LAUNCH_CHECK();
+}
+
+void lu_factor(hipStream_t s, int n, double *A, int *piv, int *info)
+{
+  k_lu<<<1, LU_WG, 0, s>>>(n, A, piv, info);
+  UVLM_LAUNCH_CHECK();
+}
+
+void lu_solve(hipStream_t s, int n, const double *LU, const int *piv, double *b)
+{
+  k_lu_solve<<<1, LU_WG, 0, s>>>(n, LU, piv, b);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_rhs(hipStream_t s, int n, const double *nrm, const double U[3], const double *v, double *rhs)
+{
+  k_wing_rhs<<<nb256(n), 256, 0, s>>>(n, nrm, U[0], U[1], U[2], v, rhs);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_force(hipStream_t s, int n, int nspan, const double *V, const double *B, const double *gamma, const double U[3],
+                const double *v, double *F)
+{
+  k_wing_force<<<1, 256, 0, s>>>(n, nspan, V, B, gamma, U[0], U[1], U[2], v, F);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_shed(hipStream_t s, int nspan, int nchord, const double *V, const double *gamma, const double U[3], double dt,
+               int ref_offset, double *X, double *A, double *S)
+{
+  k_wing_shed<<<nb256(nspan), 256, 0, s>>>(nspan, nchord, V, gamma, U[0], U[1], U[2], dt, ref_offset, X, A, S);
+  UVLM_LAUNCH_CHECK();
+}
+
+void points_to_soa(hipStream_t s, int64_t m, const double *pts, double *st, int64_t cap)
+{
+  k_points_to_soa<<<nb256(m), 256, 0, s>>>(m, pts, st, cap);
+  UVLM_LAUNCH_CHECK();
+}
+
+void probe_velocity_out(hipStream_t s, int64_t m, const int32_t *perm, const double *st, int64_t cap, double *out)
+{
+  k_probe_velocity_out<<<nb256(m), 256, 0, s>>>(m, perm, st, cap, out);
+  UVLM_LAUNCH_CHECK();
+}
+
+void append_bodies(hipStream_t s, int64_t N, int64_t k, const double *X, const double *A, const double *S, double *st,
+                   int64_t cap, int32_t *orig)
+{
+  if (k <= 0) return;
+  k_append_bodies<<<nb256(k), 256, 0, s>>>(N, k, X, A, S, st, cap, orig);
+  UVLM_LAUNCH_CHECK();
+}
+
+void translate3(hipStream_t s, int64_t n, double *p, const double d[3])
+{
+  if (n <= 0) return;
+  k_translate<<<nb256(n), 256, 0, s>>>(n, p, d[0], d[1], d[2]);
+  UVLM_LAUNCH_CHECK();
+}
+
+}  // namespace uvlm

@@ -96,6 +96,14 @@ class FMMCalculator:
         _capi.check(self.lib.uvlm_set_bodies_soa(self._h, X.shape[0], _dp(X), _dp(A), _dp(S)))
         self.n = X.shape[0]
 
+    def append_arrays(self, X, alpha, sigma):
+        """Append particles after the current ones (next original indices)."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        A = np.ascontiguousarray(alpha, dtype=np.float64)
+        S = np.ascontiguousarray(sigma, dtype=np.float64)
+        _capi.check(self.lib.uvlm_append_bodies_soa(self._h, X.shape[0], _dp(X), _dp(A), _dp(S)))
+        self.n += X.shape[0]
+
     def copyBodies2Host(self, bodies=None):
         out = bodies if bodies is not None else np.zeros((self.n, BODY_DOUBLES))
         assert out.flags.c_contiguous and out.shape == (self.n, BODY_DOUBLES)
