@@ -212,6 +212,19 @@ int uvlm_wing_shed(uvlm_wing w, uvlm_handle particles, const double U[3], double
 int uvlm_wing_step(uvlm_wing w, uvlm_handle particles, const double U[3], double dt, int reference_offset,
                    double *gamma, double force[3]);
 
+/* Adjoint of the lift coefficient through the loop (the build's dC_L adjoint through time; no
+ * counterpart in the reference): from the current particles (n of them) and wing position,
+ * `steps` passes of uvlm_wing_step, then CL = e . F / (|U|^2 S / 2) (S the wing area, e a unit
+ * lift direction) at the final position; returns CL and its gradient with respect to the
+ * particles' positions (xbar [n][3]) and strengths (alphabar [n][3]) at the start, ORIGINAL
+ * order.  The chain: Kutta-Joukowski force -> circulations (A^{-T}) -> the particle velocity at
+ * the control points (probe adjoint) and, back through each step, the RK step VJP of the wake
+ * (uvlm_step_vjp) and the shedding map.  Trees are held fixed per evaluation (each rebuilt as in
+ * the forward pass).  The particles and the wing end at the final state.  cVPM, inviscid. */
+int uvlm_wing_lift_adjoint(uvlm_wing w, uvlm_handle particles, const double U[3], double dt, int64_t steps,
+                           int reference_offset, const double e[3], double *CL, int64_t n, double *xbar,
+                           double *alphabar);
+
 /* Overwrite positions and strengths [n][3] in the CURRENT tree order without rebuilding
  * the tree or lists (frozen-tree perturbations, e.g. finite differences).  No counterpart
  * in the reference. */

@@ -99,3 +99,47 @@ def test_wing_loop_400_panels_matches_oracle(orc):
         eg, eF = hist[t]
         print(f"  step {t}: gamma {rel(g, eg):.3e} force {rel(F, eF):.3e} leaves {p.timings()['n_leaves']}", flush=True)
         assert rel(g, eg) < 1e-8 and rel(F, eF) < 1e-8
+
+
+def test_lift_adjoint_finite_differences():
+    """dC_L(final) / d(wake state at the window start) through 4 loop steps (solve, shed, RK2
+    advance), against central differences of the same loop run from perturbed states (every
+    particle in one leaf, so the FMM is the direct sum and the loop is smooth)."""
+    V = reference_wing(npanels=6, chord=0.4)
+    V = np.stack([V[0], (V[0] + V[1]) / 2, V[1]])
+    U, dt, warm, win = np.array([0.0, 1.0, 0.0]), 0.05, 3, 4
+    w = Wing(V)
+    p = FMMCalculator(P=6, theta=0.5, ncrit=256, scheme="rk2")
+    for _ in range(warm):
+        w.step(p, U, dt)
+    arr, perm = p.arrays(), p.permutation()
+    n = p.n
+    X0, A0, S0 = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    X0[perm], A0[perm], S0[perm] = arr["X"], arr["alpha"], arr["sigma"]
+    area = w.geometry()[2].sum()
+    Vstart = V - warm * dt * U
+
+    def objective(X, A):
+        w2 = Wing(Vstart)
+        p2 = FMMCalculator(P=6, theta=0.5, ncrit=256, scheme="rk2")
+        p2.set_arrays(X, A, S0)
+        for _ in range(win):
+            w2.step(p2, U, dt)
+        _, F = w2.solve(p2, U)
+        return F[2] / (0.5 * U @ U * area)
+
+    CL, xb, ab = w.lift_adjoint(p, U, dt, win)
+    assert abs(CL - objective(X0, A0)) < 1e-12 * abs(CL)
+    h = 1e-6
+    fd, an = [], []
+    for j in (0, 5, n // 2, n - 1):
+        for d in range(3):
+            for which in ("x", "a"):
+                Xp, Ap, Xm, Am = X0.copy(), A0.copy(), X0.copy(), A0.copy()
+                (Xp if which == "x" else Ap)[j, d] += h
+                (Xm if which == "x" else Am)[j, d] -= h
+                fd.append((objective(Xp, Ap) - objective(Xm, Am)) / (2 * h))
+                an.append((xb if which == "x" else ab)[j, d])
+    fd, an = np.array(fd), np.array(an)
+    print(f"[lift adjoint] CL {CL:.6f} n={n} vs FD {rel(an, fd):.3e} |grad| {np.abs(fd).max():.3e}", flush=True)
+    assert rel(an, fd) < 1e-6

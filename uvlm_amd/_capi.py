@@ -21,7 +21,7 @@ EXPORTS = [
     "uvlm_set_viscosity", "uvlm_viscosity", "uvlm_vorticity_aos", "uvlm_rbf_aos", "uvlm_eval_vjp",
     "uvlm_step_vjp", "uvlm_trajectory_forward", "uvlm_trajectory_vjp", "uvlm_append_bodies_soa",
     "uvlm_wing_create", "uvlm_wing_destroy", "uvlm_wing_get", "uvlm_wing_translate", "uvlm_wing_solve",
-    "uvlm_wing_shed", "uvlm_wing_step",
+    "uvlm_wing_shed", "uvlm_wing_step", "uvlm_wing_lift_adjoint",
 ]
 
 
@@ -103,6 +103,7 @@ def lib():
             "uvlm_wing_solve": (I, [VP, VP, PD, PD, PD]),
             "uvlm_wing_shed": (I, [VP, VP, PD, D, I]),
             "uvlm_wing_step": (I, [VP, VP, PD, D, I, PD, PD]),
+            "uvlm_wing_lift_adjoint": (I, [VP, VP, PD, D, I64, I, PD, PD, I64, PD, PD]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

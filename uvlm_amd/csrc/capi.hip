@@ -303,9 +303,11 @@ void velocity_at_points(uvlm_handle h, int64_t m, const double *pts, double *out
 
 // Append k bodies (device AoS X [k][3], alpha [k][3], sigma [k]) after the current N, growing
 // the state (fields keep their contents) when needed; new bodies get the next original indices.
-void append_bodies_dev(uvlm_handle h, int64_t k, const double *X, const double *A, const double *S)
+// Capacity for at least `want` particles; the fields and original indices of the current ones
+// are kept (the field stride changes).
+void grow_particles(uvlm_handle h, int64_t want)
 {
-  const int64_t N = h->N, want = N + k;
+  const int64_t N = h->N;
   if (want > h->cap) {
     const int64_t ncap = std::max<int64_t>(want, h->cap + h->cap / 2);
     DevBuf        nst, norig;
@@ -322,9 +324,55 @@ void append_bodies_dev(uvlm_handle h, int64_t k, const double *X, const double *
     h->orig2.reserve(4 * ncap);
     h->perm.reserve(4 * ncap);
   }
+}
+
+void append_bodies_dev(uvlm_handle h, int64_t k, const double *X, const double *A, const double *S)
+{
+  const int64_t N = h->N, want = N + k;
+  grow_particles(h, want);
   append_bodies(h->s, N, k, X, A, S, h->st.as<double>(), h->cap, h->orig.as<int32_t>());
   h->N     = want;
   h->built = h->evaluated = false;
+}
+
+// Adjoint of the probe objective sum_p w_p . u(x_p) for the m probe "vortices" already in h->sst
+// (X = probe, alpha = w_p, sigma = 1) on the particles' current tree: the forward probe lists
+// transposed, probe multipoles -> particle locals (M2L), L2L, P2P with the particle's sigma, and
+// L2P with transposed stretching.  Leaves dJ/dalpha in h->ast velocity and dJ/dx in h->ast dadt
+// (device order).
+void probe_adjoint_core(uvlm_handle h, int64_t m)
+{
+  hipStream_t   s  = h->s;
+  const int     P  = h->prm.P, NT = h->NT;
+  const int64_t sc = h->scap;
+  double       *sx = h->sst.as<double>();
+  build_tree(s, h->sc, sx + F_X * sc, sx + (F_X + 1) * sc, sx + (F_X + 2) * sc, m, h->prm.ncrit, h->stree,
+             h->sperm.as<int32_t>());
+  gather_fields(s, h->sst.as<double>(), h->sst2.as<double>(), sc, sc, N_FIELDS, h->sperm.as<int32_t>(), m);
+  swap_buf(h->sst, h->sst2);
+  // the forward probe lists, transposed: targets = particle cells, sources = probe cells
+  traverse(s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
+  transpose_lists(s, h->sc, h->stree, h->tree, h->slists, h->alists);
+  // probe multipoles (root included: a probe cell is a source here)
+  h->Ms.reserve(sizeof(double2) * NT * h->stree.C);
+  double2 *Ms = h->Ms.as<double2>();
+  p2m(s, P, h->stree, h->sst.as<double>(), sc, Ms);
+  for (int l = h->stree.nlevels - 2; l >= 0; l--) m2m_level(s, P, h->stree, l, Ms);
+  // adjoint state of the particles: positions, strengths and sigma copied in
+  const int64_t cap = h->cap;
+  h->ast.reserve(sizeof(double) * N_FIELDS * cap);
+  double *a = h->ast.as<double>();
+  zero_doubles(s, a, N_FIELDS * cap);
+  UVLM_HIP(hipMemcpyAsync(a + F_X * cap, h->F(F_X), 8 * 6 * cap, hipMemcpyDeviceToDevice, s));
+  UVLM_HIP(hipMemcpyAsync(a + F_SIG * cap, h->F(F_SIG), 8 * cap, hipMemcpyDeviceToDevice, s));
+  h->L.reserve(sizeof(double2) * NT * h->tree.C);
+  double2 *La = h->L.as<double2>();
+  m2l(s, P, h->tree, h->stree, h->alists, h->sched, Ms, La);
+  zero_doubles(s, (double *)La, 2 * NT);  // the particle root multipole is never formed forward
+  for (int l = 1; l < h->tree.nlevels; l++) l2l_level(s, P, h->tree, l, La);
+  p2p(s, h->tree, h->stree, h->alists, a, cap, h->sst.as<double>(), sc, a, cap, h->prm.precision, 2, h->prm.ncrit);
+  l2p_stretch(s, P, h->tree, La, a, cap, UVLM_TRANSPOSED, UVLM_CVPM, 0);
+  h->evaluated = false;  // L now holds adjoint locals
 }
 
 int stages_of(int scheme) { return scheme == UVLM_EULER ? 1 : scheme == UVLM_RK2 ? 2 : scheme == UVLM_RK3 ? 3 : 4; }
@@ -484,6 +532,17 @@ void step_vjp(uvlm_handle h, double dt, double *cot)
 // The particle state (X, alpha, sigma) as 7 fields of N doubles in the ORIGINAL body order,
 // and back with the original index as the device order.
 void save_state_orig(uvlm_handle h, double *dst) { state_to_orig(h->s, h->N, h->orig.as<int32_t>(), h->st.as<double>(), h->cap, dst); }
+// Like load_state_orig with fields of `stride` doubles, of which the first `count` are loaded.
+void load_state_orig_n(uvlm_handle h, const double *src, int64_t stride, int64_t count)
+{
+  if (count > h->cap) throw Error(UVLM_ERR_INTERNAL, "state larger than the particle capacity");
+  h->N = count;
+  for (int f = 0; f < 6; f++)
+    UVLM_HIP(hipMemcpyAsync(h->F(F_X + f), src + f * stride, 8 * count, hipMemcpyDeviceToDevice, h->s));
+  UVLM_HIP(hipMemcpyAsync(h->F(F_SIG), src + 6 * stride, 8 * count, hipMemcpyDeviceToDevice, h->s));
+  iota(h->s, h->orig.as<int32_t>(), count);
+  h->built = h->evaluated = false;
+}
 void load_state_orig(uvlm_handle h, const double *src)
 {
   const int64_t N = h->N;
@@ -828,13 +887,7 @@ int uvlm_probe_adjoint(uvlm_handle h, int64_t m, const double *probes, const dou
     if (m <= 0 || !probes || !weights) throw Error(UVLM_ERR_ARG, "empty or null probes");
     set_device(h);
     hipStream_t s = h->s;
-    const int   P = h->prm.P, NT = h->NT;
-    if (m > h->scap) {
-      h->scap = m;
-      h->sst.reserve(sizeof(double) * N_FIELDS * m);
-      h->sst2.reserve(sizeof(double) * N_FIELDS * m);
-      h->sperm.reserve(4 * m);
-    }
+    reserve_probes(h, m);
     const int64_t sc = h->scap;
     // probes as sources: X = probe, alpha = weight, sigma = 1 (the kernel takes the target's)
     std::vector<double> host((size_t)N_FIELDS * sc, 0.0);
@@ -848,35 +901,10 @@ int uvlm_probe_adjoint(uvlm_handle h, int64_t m, const double *probes, const dou
     UVLM_HIP(hipMemcpyAsync(h->sst.p, host.data(), 8 * host.size(), hipMemcpyHostToDevice, s));
     UVLM_HIP(hipStreamSynchronize(s));
     UVLM_HIP(hipEventRecord(h->aev[0], s));
-    double *sx = h->sst.as<double>();
-    build_tree(s, h->sc, sx + F_X * sc, sx + (F_X + 1) * sc, sx + (F_X + 2) * sc, m, h->prm.ncrit, h->stree,
-               h->sperm.as<int32_t>());
-    gather_fields(s, h->sst.as<double>(), h->sst2.as<double>(), sc, sc, N_FIELDS, h->sperm.as<int32_t>(), m);
-    swap_buf(h->sst, h->sst2);
-    // the forward probe lists, transposed: targets = particle cells, sources = probe cells
-    traverse(s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
-    transpose_lists(s, h->sc, h->stree, h->tree, h->slists, h->alists);
-    // probe multipoles (root included: a probe cell is a source here)
-    h->Ms.reserve(sizeof(double2) * NT * h->stree.C);
-    double2 *Ms = h->Ms.as<double2>();
-    p2m(s, P, h->stree, h->sst.as<double>(), sc, Ms);
-    for (int l = h->stree.nlevels - 2; l >= 0; l--) m2m_level(s, P, h->stree, l, Ms);
-    // adjoint state of the particles: positions, strengths and sigma copied in
-    const int64_t cap = h->cap, N = h->N;
-    h->ast.reserve(sizeof(double) * N_FIELDS * cap);
-    double *a = h->ast.as<double>();
-    zero_doubles(s, a, N_FIELDS * cap);
-    UVLM_HIP(hipMemcpyAsync(a + F_X * cap, h->F(F_X), 8 * 6 * cap, hipMemcpyDeviceToDevice, s));
-    UVLM_HIP(hipMemcpyAsync(a + F_SIG * cap, h->F(F_SIG), 8 * cap, hipMemcpyDeviceToDevice, s));
-    h->L.reserve(sizeof(double2) * NT * h->tree.C);
-    double2 *La = h->L.as<double2>();
-    m2l(s, P, h->tree, h->stree, h->alists, h->sched, Ms, La);
-    zero_doubles(s, (double *)La, 2 * NT);  // the particle root multipole is never formed forward
-    for (int l = 1; l < h->tree.nlevels; l++) l2l_level(s, P, h->tree, l, La);
-    p2p(s, h->tree, h->stree, h->alists, a, cap, h->sst.as<double>(), sc, a, cap, h->prm.precision, 2, h->prm.ncrit);
-    l2p_stretch(s, P, h->tree, La, a, cap, UVLM_TRANSPOSED, UVLM_CVPM, 0);
+    probe_adjoint_core(h, m);
     UVLM_HIP(hipEventRecord(h->aev[1], s));
-    h->evaluated = false;  // L now holds adjoint locals
+    const int64_t cap = h->cap, N = h->N;
+    double       *a   = h->ast.as<double>();
     std::vector<double> buf((size_t)3 * N);
     auto fetch = [&](int f0, double *out) {
       if (!out) return;
@@ -1383,6 +1411,123 @@ int uvlm_wing_step(uvlm_wing w, uvlm_handle particles, const double U[3], double
     translate3(w->s, (int64_t)(w->nspan + 1) * (w->nchord + 1), w->V.as<double>(), d);
     translate3(w->s, w->n, w->cp.as<double>(), d);
     UVLM_HIP(hipStreamSynchronize(w->s));
+  });
+}
+
+int uvlm_wing_lift_adjoint(uvlm_wing w, uvlm_handle particles, const double U[3], double dt, int64_t steps,
+                           int reference_offset, const double e[3], double *CL, int64_t n, double *xbar,
+                           double *alphabar)
+{
+  return guarded([&] {
+    check_pair(w, particles);
+    if (!particles || !U || !e) throw Error(UVLM_ERR_ARG, "null particles, freestream or direction");
+    if (steps < 0) throw Error(UVLM_ERR_ARG, "negative step count");
+    uvlm_handle h = particles;
+    if (n != h->N) throw Error(UVLM_ERR_ARG, "size mismatch");
+    check_vjp_config(h);
+    UVLM_HIP(hipSetDevice(w->device));
+    hipStream_t   s = h->s, ws = w->s;
+    const int     np = w->n, ns = w->nspan;
+    const int64_t N0 = h->N;
+    // particle capacity for the whole window first, so the cotangent stride stays fixed
+    grow_particles(h, N0 + steps * ns);
+    const int64_t cap = h->cap;
+    // forward: the step-start states y_k and the post-shedding states y_k' in the original order
+    std::vector<DevBuf>  ystart(steps + 1), yshed(steps);
+    std::vector<int64_t> nstart(steps + 1), nshed(steps);
+    auto                 keep = [&](DevBuf &b, int64_t &cnt) {
+      cnt = h->N;
+      b.reserve(sizeof(double) * 7 * std::max<int64_t>(cnt, 1));
+      if (cnt) save_state_orig(h, b.as<double>());
+    };
+    auto restore = [&](const DevBuf &b, int64_t cnt) {
+      if (cnt) load_state_orig_n(h, b.as<double>(), cnt, cnt);
+      else h->N = 0;
+    };
+    keep(ystart[0], nstart[0]);
+    restore(ystart[0], nstart[0]);
+    for (int64_t k = 0; k < steps; k++) {
+      wing_solve(w, h, U);
+      UVLM_HIP(hipStreamSynchronize(ws));
+      wing_shed_into(w, h, U, dt, reference_offset);
+      keep(yshed[k], nshed[k]);
+      restore(yshed[k], nshed[k]);
+      for (int st = 0; st < stages_of(h->prm.scheme); st++) {
+        do_build(h);
+        do_eval(h, false);
+        rk_stage(s, h->st.as<double>(), cap, h->N, h->prm.scheme, st, dt, 0);
+      }
+      const double d[3] = {-U[0] * dt, -U[1] * dt, -U[2] * dt};
+      translate3(ws, (int64_t)(w->nspan + 1) * (w->nchord + 1), w->V.as<double>(), d);
+      translate3(ws, np, w->cp.as<double>(), d);
+      UVLM_HIP(hipStreamSynchronize(ws));
+      keep(ystart[k + 1], nstart[k + 1]);
+      restore(ystart[k + 1], nstart[k + 1]);
+    }
+    // the lift coefficient at the final position
+    wing_solve(w, h, U);
+    std::vector<double> area(np);
+    double              F[3];
+    UVLM_HIP(hipMemcpyAsync(area.data(), w->area.p, 8 * np, hipMemcpyDeviceToHost, ws));
+    UVLM_HIP(hipMemcpyAsync(F, w->F.p, 24, hipMemcpyDeviceToHost, ws));
+    UVLM_HIP(hipStreamSynchronize(ws));
+    double S = 0.0;
+    for (double a : area) S += a;
+    const double q     = 0.5 * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]) * S;
+    const double es[3] = {e[0] / q, e[1] / q, e[2] / q};
+    if (CL) *CL = (e[0] * F[0] + e[1] * F[1] + e[2] * F[2]) / q;
+    // backward
+    DevBuf abuf;
+    abuf.reserve(sizeof(double) * (size_t)(np + 3 * np + np + np));
+    double *c = abuf.as<double>(), *vbar = c + np, *gbar = vbar + 3 * np, *rbar = gbar + np;
+    h->vlam.reserve(sizeof(double) * 6 * cap);
+    double *lam = h->vlam.as<double>();
+    zero_doubles(s, lam, 6 * cap);
+    const double *vN = nstart[steps] ? w->v.as<double>() : nullptr;
+    wing_force_vjp(ws, np, ns, w->V.as<double>(), w->B.as<double>(), w->gamma.as<double>(), U, vN, es, c, vbar, gbar);
+    auto through_solve_and_probes = [&](int64_t npart) {
+      // gbar -> rbar = A^{-T} gbar -> vbar -= rbar n -> particle cotangents (probe adjoint)
+      UVLM_HIP(hipMemcpyAsync(rbar, gbar, 8 * np, hipMemcpyDeviceToDevice, ws));
+      lu_solve_t(ws, np, w->LU.as<double>(), w->piv.as<int>(), rbar);
+      wing_rhs_vjp(ws, np, w->nrm.as<double>(), rbar, vbar);
+      UVLM_HIP(hipStreamSynchronize(ws));
+      if (npart == 0) return;
+      do_build(h);
+      reserve_probes(h, np);
+      probe_sources(s, np, w->cp.as<double>(), vbar, h->sst.as<double>(), h->scap);
+      probe_adjoint_core(h, np);
+      cot_add_probe(s, h->N, h->orig.as<int32_t>(), h->ast.as<double>(), cap, lam, cap);
+      UVLM_HIP(hipStreamSynchronize(s));
+    };
+    restore(ystart[steps], nstart[steps]);
+    through_solve_and_probes(nstart[steps]);
+    for (int64_t k = steps - 1; k >= 0; k--) {
+      const double d[3] = {U[0] * dt, U[1] * dt, U[2] * dt};
+      translate3(ws, (int64_t)(w->nspan + 1) * (w->nchord + 1), w->V.as<double>(), d);
+      translate3(ws, np, w->cp.as<double>(), d);
+      UVLM_HIP(hipStreamSynchronize(ws));
+      restore(yshed[k], nshed[k]);
+      step_vjp(h, dt, lam);
+      UVLM_HIP(hipStreamSynchronize(s));
+      // the shed strengths' cotangents are entries nstart[k] .. nstart[k] + ns - 1
+      wing_shed_vjp(ws, ns, w->nchord, w->V.as<double>(), lam + 3 * cap + nstart[k], cap, gbar);
+      zero_doubles(ws, vbar, 3 * np);
+      UVLM_HIP(hipStreamSynchronize(ws));
+      if (nstart[k]) load_state_orig_n(h, yshed[k].as<double>(), nshed[k], nstart[k]);
+      else h->N = 0;
+      through_solve_and_probes(nstart[k]);
+    }
+    h->N = N0;
+    download_cot(h, lam, xbar, alphabar);
+    // particles and wing back at the final state
+    for (int64_t k = 0; k < steps; k++) {
+      const double d[3] = {-U[0] * dt, -U[1] * dt, -U[2] * dt};
+      translate3(ws, (int64_t)(w->nspan + 1) * (w->nchord + 1), w->V.as<double>(), d);
+      translate3(ws, np, w->cp.as<double>(), d);
+    }
+    UVLM_HIP(hipStreamSynchronize(ws));
+    restore(ystart[steps], nstart[steps]);
+    UVLM_HIP(hipStreamSynchronize(s));
   });
 }
 

@@ -201,6 +201,17 @@ void points_to_soa(hipStream_t s, int64_t m, const double *pts, double *st, int6
 void probe_velocity_out(hipStream_t s, int64_t m, const int32_t *perm, const double *st, int64_t cap, double *out);
 void append_bodies(hipStream_t s, int64_t N, int64_t k, const double *X, const double *A, const double *S, double *st,
                    int64_t cap, int32_t *orig);
+// adjoint of the lift through the loop: force VJP (c [n], vbar [n][3], gbar [n]), A^{-T} solve,
+// rhs and shedding VJPs, probe "vortices", probe-adjoint results into original-order cotangents
+void wing_force_vjp(hipStream_t s, int n, int nspan, const double *V, const double *B, const double *gamma,
+                    const double U[3], const double *v, const double e[3], double *c, double *vbar, double *gbar);
+void lu_solve_t(hipStream_t s, int n, const double *LU, const int *piv, double *b);
+void wing_rhs_vjp(hipStream_t s, int n, const double *nrm, const double *rbar, double *vbar);
+void wing_shed_vjp(hipStream_t s, int nspan, int nchord, const double *V, const double *abar, int64_t stride,
+                   double *gbar);
+void probe_sources(hipStream_t s, int64_t m, const double *pts, const double *wts, double *st, int64_t cap);
+void cot_add_probe(hipStream_t s, int64_t N, const int32_t *orig, const double *ast, int64_t cap, double *cot,
+                   int64_t ccap);
 
 // body.hip
 void aos_to_soa(hipStream_t s, const void *aos, int64_t n, int64_t stride_doubles, double *st, int64_t cap);

@@ -294,6 +294,135 @@ __global__ void k_append_bodies(int64_t N, int64_t k, const double *__restrict__
   orig[b]             = (int32_t)b;
 }
 
+// ---- adjoint of the lift coefficient through the loop ----
+// Force VJP, part 1 (per panel i), for J = e . F with e already scaled: u_i as in k_wing_force,
+// q_i = dl_i x e, c_i = u_i . q_i and w_i = Gamma_i q_i (= dJ/dv_i).
+__global__ void k_wing_force_vjp1(int n, int nspan, const double *__restrict__ V, const double *__restrict__ B,
+                                  const double *__restrict__ gamma, double Ux, double Uy, double Uz,
+                                  const double *__restrict__ v, double ex, double ey, double ez,
+                                  double *__restrict__ c, double *__restrict__ w)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  V3 u{Ux, Uy, Uz};
+  for (int j = 0; j < n; j++) u = u + gamma[j] * ld3(B + 3 * ((int64_t)i * n + j));
+  if (v) u = u + ld3(v + 3 * i);
+  const Corners q  = corners(V, nspan, i);
+  const V3      qe = cross(q.le1 - q.le, V3{ex, ey, ez});
+  const double  Gn = gamma[i] - (i >= nspan ? gamma[i - nspan] : 0.0);
+  c[i]             = dot(u, qe);
+  w[3 * i] = Gn * qe.x, w[3 * i + 1] = Gn * qe.y, w[3 * i + 2] = Gn * qe.z;
+}
+
+// Part 2 (per panel j): dJ/dgamma_j = c_j - c_{j+nspan} (the net circulations) + sum_i w_i . B[i][j].
+__global__ void k_wing_force_vjp2(int n, int nspan, const double *__restrict__ B, const double *__restrict__ c,
+                                  const double *__restrict__ w, double *__restrict__ gbar)
+{
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  double g = c[j] - (j + nspan < n ? c[j + nspan] : 0.0);
+  for (int i = 0; i < n; i++) g += dot(ld3(w + 3 * i), ld3(B + 3 * ((int64_t)i * n + j)));
+  gbar[j] = g;
+}
+
+// y = A^{-T} b from k_lu's factors (P A = L U, so A^T = U^T L^T P): U^T z = b, L^T w = z,
+// y = P^T w; b is overwritten with y.  One workgroup.
+__global__ __launch_bounds__(LU_WG) void k_lu_solve_t(int n, const double *__restrict__ LU, const int *__restrict__ piv,
+                                                      double *__restrict__ b)
+{
+  const int tid = threadIdx.x;
+  for (int j = 0; j < n; j++) {  // U^T z = b: column j of U^T is row j of U
+    __syncthreads();
+    const double z = b[j] / LU[(int64_t)j * n + j];
+    __syncthreads();
+    if (tid == 0) b[j] = z;
+    for (int i = j + 1 + tid; i < n; i += LU_WG) b[i] -= LU[(int64_t)j * n + i] * z;
+  }
+  __syncthreads();
+  for (int j = n - 1; j >= 0; j--) {  // L^T w = z (unit diagonal): row j of L
+    const double x = b[j];
+    for (int i = tid; i < j; i += LU_WG) b[i] -= LU[(int64_t)j * n + i] * x;
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int k = n - 1; k >= 0; k--) {
+      const int p = piv[k];
+      if (p != k) {
+        const double t = b[k];
+        b[k]           = b[p];
+        b[p]           = t;
+      }
+    }
+}
+
+// vbar_i += -rbar_i n_i (rhs_i = -U . n_i - v_i . n_i).
+__global__ void k_wing_rhs_vjp(int n, const double *__restrict__ nrm, const double *__restrict__ rbar,
+                               double *__restrict__ vbar)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  for (int d = 0; d < 3; d++) vbar[3 * i + d] -= rbar[i] * nrm[3 * i + d];
+}
+
+// gbar (all n panels) from the cotangent abar of the shed strengths (k_wing_shed): zero off the
+// trailing-edge row; abar_c(d) = abar[d * stride + c].
+__global__ void k_wing_shed_vjp(int nspan, int nchord, const double *__restrict__ V, const double *__restrict__ abar,
+                                int64_t stride, double *__restrict__ gbar)
+{
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= nspan * nchord) return;
+  const int r0 = (nchord - 1) * nspan;
+  if (c < r0) {
+    gbar[c] = 0.0;
+    return;
+  }
+  const int ci = c - r0;
+  auto      ab = [&](int k) { return V3{abar[k], abar[stride + k], abar[2 * stride + k]}; };
+  auto      cr = [&](int k) { return corners(V, nspan, r0 + k); };
+  const Corners q = cr(ci);
+  const V3      a = ab(ci);
+  double        g = dot(a, q.te - q.te1);
+  g += (ci > 0 ? 0.5 : 1.0) * dot(a, q.le - q.te);
+  g += (ci < nspan - 1 ? 0.5 : 1.0) * dot(a, q.te1 - q.le1);
+  if (ci + 1 < nspan) {
+    const Corners qn = cr(ci + 1);
+    g -= 0.5 * dot(ab(ci + 1), qn.le - qn.te);
+  }
+  if (ci > 0) {
+    const Corners qp = cr(ci - 1);
+    g -= 0.5 * dot(ab(ci - 1), qp.te1 - qp.le1);
+  }
+  gbar[c] = g;
+}
+
+// Probe "vortices" for the probe adjoint: X = pts, alpha = weights, sigma = 1, the rest zero.
+__global__ void k_probe_sources(int64_t m, const double *__restrict__ pts, const double *__restrict__ wts,
+                                double *__restrict__ st, int64_t cap)
+{
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i >= m) return;
+  for (int f = 0; f < N_FIELDS; f++) st[f * cap + i] = 0.0;
+  for (int d = 0; d < 3; d++) {
+    st[(F_X + d) * cap + i] = pts[3 * i + d];
+    st[(F_A + d) * cap + i] = wts[3 * i + d];
+  }
+  st[F_SIG * cap + i] = 1.0;
+}
+
+// cot[d][orig[p]] += ast[F_DADT + d][p], cot[3 + d][orig[p]] += ast[F_U + d][p] (probe adjoint
+// results, device order -> original order).
+__global__ void k_cot_add_probe(int64_t N, const int32_t *__restrict__ orig, const double *__restrict__ ast,
+                                int64_t cap, double *__restrict__ cot, int64_t ccap)
+{
+  const int64_t p = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (p >= N) return;
+  const int64_t o = orig[p];
+  for (int d = 0; d < 3; d++) {
+    cot[d * ccap + o] += ast[(F_DADT + d) * cap + p];
+    cot[(3 + d) * ccap + o] += ast[(F_U + d) * cap + p];
+  }
+}
+
 inline int nb256(int64_t n) { return (int)std::max<int64_t>(1, (n + 255) / 256); }
 
 }  // namespace
@@ -360,6 +489,48 @@ void append_bodies(hipStream_t s, int64_t N, int64_t k, const double *X, const d
 {
   if (k <= 0) return;
   k_append_bodies<<<nb256(k), 256, 0, s>>>(N, k, X, A, S, st, cap, orig);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_force_vjp(hipStream_t s, int n, int nspan, const double *V, const double *B, const double *gamma,
+                    const double U[3], const double *v, const double e[3], double *c, double *vbar, double *gbar)
+{
+  k_wing_force_vjp1<<<nb256(n), 256, 0, s>>>(n, nspan, V, B, gamma, U[0], U[1], U[2], v, e[0], e[1], e[2], c, vbar);
+  UVLM_LAUNCH_CHECK();
+  k_wing_force_vjp2<<<nb256(n), 256, 0, s>>>(n, nspan, B, c, vbar, gbar);
+  UVLM_LAUNCH_CHECK();
+}
+
+void lu_solve_t(hipStream_t s, int n, const double *LU, const int *piv, double *b)
+{
+  k_lu_solve_t<<<1, LU_WG, 0, s>>>(n, LU, piv, b);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_rhs_vjp(hipStream_t s, int n, const double *nrm, const double *rbar, double *vbar)
+{
+  k_wing_rhs_vjp<<<nb256(n), 256, 0, s>>>(n, nrm, rbar, vbar);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_shed_vjp(hipStream_t s, int nspan, int nchord, const double *V, const double *abar, int64_t stride,
+                   double *gbar)
+{
+  k_wing_shed_vjp<<<nb256((int64_t)nspan * nchord), 256, 0, s>>>(nspan, nchord, V, abar, stride, gbar);
+  UVLM_LAUNCH_CHECK();
+}
+
+void probe_sources(hipStream_t s, int64_t m, const double *pts, const double *wts, double *st, int64_t cap)
+{
+  k_probe_sources<<<nb256(m), 256, 0, s>>>(m, pts, wts, st, cap);
+  UVLM_LAUNCH_CHECK();
+}
+
+void cot_add_probe(hipStream_t s, int64_t N, const int32_t *orig, const double *ast, int64_t cap, double *cot,
+                   int64_t ccap)
+{
+  if (N <= 0) return;
+  k_cot_add_probe<<<nb256(N), 256, 0, s>>>(N, orig, ast, cap, cot, ccap);
   UVLM_LAUNCH_CHECK();
 }
 

@@ -79,3 +79,18 @@ class Wing:
                                             _dp(g), _dp(F)))
         particles.n += self.nspan
         return g, F
+
+    def lift_adjoint(self, particles, U, dt, steps, e=(0.0, 0.0, 1.0), reference_offset=True):
+        """CL at the position after `steps` loop steps and its gradient with respect to the
+        particles' positions and strengths now (original order): (CL, xbar, alphabar)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        n = particles.n
+        xb, ab = np.zeros((max(n, 1), 3)), np.zeros((max(n, 1), 3))
+        CL = ctypes.c_double()
+        _capi.check(self.lib.uvlm_wing_lift_adjoint(self._w, particles._h, _dp(U), float(dt), int(steps),
+                                                    int(bool(reference_offset)), _dp(e),
+                                                    ctypes.cast(ctypes.byref(CL), ctypes.POINTER(ctypes.c_double)),
+                                                    n, _dp(xb), _dp(ab)))
+        particles.n += steps * self.nspan
+        return CL.value, xb[:n], ab[:n]
