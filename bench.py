@@ -189,6 +189,72 @@ def leapfrog_scenario(args, X, A, S, tuned):
     print(json.dumps(line), flush=True)
 
 
+def vlm_run(steps, window, precision, log=True):
+    """The wing-and-wake loop (the reference's main.cpp:168-295 time loop): a 40 x 10 vortex
+    lattice (span 10, chord 1, 5 degrees, freestream (0, 1, 0)) solved every step against the
+    FMM velocity of its particle wake at the control points, 40 particles shed per step, the
+    wake advanced by RK2 with dt = 0.02 (P = 8, theta = 0.5, ncrit 64).  After steps - window
+    steps, the lift-coefficient adjoint runs over the last window (its forward steps recomputed
+    inside).  Returns the measurements as a dict."""
+    from uvlm_amd import FMMCalculator
+    from uvlm_amd.vlm import Wing, reference_wing
+
+    V = reference_wing(npanels=40, span=10.0, chord=1.0, angle_deg=5.0, nchord=10)
+    U, dt = np.array([0.0, 1.0, 0.0]), 0.02
+    win = min(window, steps)
+    w = Wing(V)
+    p = FMMCalculator(P=8, theta=0.5, ncrit=64, precision=precision, scheme="rk2")
+    area = float(w.geometry()[2].sum())
+    step_ms, cl = [], []
+    t_all = time.perf_counter()
+    for k in range(steps - win):
+        t0 = time.perf_counter()
+        _, F = w.step(p, U, dt)
+        p.sync()
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+        cl.append(F[2] / (0.5 * area))
+        if log and (k + 1) % 200 == 0:
+            print(f"[vlm] step {k + 1}: {step_ms[-1]:.2f} ms, {p.n} particles, CL {cl[-1]:.5f}", file=sys.stderr,
+                  flush=True)
+    n0 = p.n
+    t0 = time.perf_counter()
+    CL, xb, ab = w.lift_adjoint(p, U, dt, win)
+    p.sync()
+    adj_s = time.perf_counter() - t0
+    fwd_ms = float(np.mean(step_ms[-win:])) if step_ms else float("nan")
+    out = {"steps": steps, "panels": w.n, "particles_final": p.n,
+           "step_ms": {"mean": round(float(np.mean(step_ms)), 3) if step_ms else None,
+                       "last": round(step_ms[-1], 3) if step_ms else None},
+           "CL_final": CL, "CL_history_every_100": [round(c, 6) for c in cl[::100]],
+           "lift_adjoint": {"window_steps": win, "particles_at_window_start": n0, "wall_s": round(adj_s, 3),
+                            "ratio_to_forward_window": round(adj_s * 1e3 / (win * fwd_ms), 3)
+                            if step_ms and win else None,
+                            "grad_norm_x": float(np.linalg.norm(xb)), "grad_norm_alpha": float(np.linalg.norm(ab)),
+                            "note": "wall time of uvlm_wing_lift_adjoint: forward window recomputed with states kept "
+                                    "in HBM, then force VJP, A^-T solves, probe adjoints and RK2 step VJPs back to "
+                                    "the window start"},
+           "wall_s": round(time.perf_counter() - t_all, 2)}
+    w.close()
+    p.close()
+    return out
+
+
+def vlm_scenario(args):
+    """`--vlm-steps` steps of the wing-and-wake loop (vlm_run) as one JSON line (ms per step)."""
+    r = vlm_run(args.vlm_steps, args.vlm_window, args.precision)
+    line = {"metric": "ms per wing + wake loop step (VLM solve + FMM sensor eval + RK2 wake step)",
+            "value": r["step_ms"]["mean"], "unit": "ms", "n_gpus": 1, "steps": args.vlm_steps, "warmup": 0,
+            "ms_per_step": r["step_ms"]["mean"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 P2P + f64 expansions, f64 VLM" if args.precision == "mixed" else "f64",
+            "data": "synthetic (40 x 10 panel wing, wake grown from nothing)",
+            "config": {"workload": "vortex-lattice wing 40x10 panels + particle wake, RK2 dt=0.02, P=8, theta=0.5, "
+                                   f"ncrit=64, {args.vlm_steps} steps, lift adjoint over the last "
+                                   f"{r['lift_adjoint']['window_steps']}",
+                       "panels": r["panels"], "particles_final": r["particles_final"]},
+            "extra": r}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +272,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip timing the FP64 path")
     ap.add_argument("--scenario-steps", type=int, default=0,
                     help="instead of the eval benchmark: this many RK3 steps of the two-ring time integration")
+    ap.add_argument("--vlm-steps", type=int, default=0,
+                    help="instead of the eval benchmark: this many steps of the wing + wake loop (40 x 10 panels)")
+    ap.add_argument("--vlm-window", type=int, default=50, help="steps of the lift adjoint at the end of --vlm-steps")
+    ap.add_argument("--vlm-leg-steps", type=int, default=250,
+                    help="wing + wake loop steps reported beside the eval benchmark (0: skip)")
     ap.add_argument("--adjoint-n", type=int, default=1 << 20,
                     help="particles in the adjoint leg (three Gaussian blobs, 4096 probes, P=8; 0: skip)")
     args = ap.parse_args()
@@ -213,6 +284,10 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.vlm_steps > 0:
+        vlm_scenario(args)
+        return
 
     from uvlm_amd import FMMCalculator, workloads as W
 
@@ -327,6 +402,11 @@ def main():
     if args.adjoint_n > 0:
         extra.update(adjoint_leg(args.adjoint_n, args.theta, args.ncrit or 64, max(1, min(args.steps, 3))))
         print(f"[bench] adjoint: {extra['adjoint']}", file=sys.stderr, flush=True)
+
+    if args.vlm_leg_steps > 0:
+        extra["wing_wake"] = vlm_run(args.vlm_leg_steps, min(args.vlm_window, args.vlm_leg_steps), args.precision,
+                                     log=False)
+        print(f"[bench] wing + wake: {extra['wing_wake']}", file=sys.stderr, flush=True)
 
     cpu = None
     if args.cpu_n > 0:

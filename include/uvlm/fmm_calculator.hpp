@@ -266,7 +266,10 @@ struct FMMCalculator {
   }
   void copyBodies2Host(exafmm::Bodies &bodies)
   {
-    if (bodies.size() != n_) bodies.resize(n_);
+    int64_t n = 0;
+    exafmm::detail::check(uvlm_num_bodies(h_.h, &n));  // the wing's shedding appends on the device
+    n_ = (size_t)n;
+    bodies.resize(n_);
     exafmm::detail::check(uvlm_get_bodies_aos(h_.h, bodies.data(), (int64_t)n_, sizeof(exafmm::Body)));
   }
   // Interaction lists as (target cell, source cell) pairs (the reference keeps them in

@@ -175,6 +175,8 @@ int uvlm_step_vjp(uvlm_handle h, double dt, int64_t n, double *xbar, double *alp
  * end at the final state again.  No counterpart in the reference. */
 int uvlm_trajectory_forward(uvlm_handle h, int64_t steps, double dt, int64_t every);
 int uvlm_trajectory_vjp(uvlm_handle h, int64_t n, double *xbar, double *alphabar);
+/* Number of particles on the handle (it grows with uvlm_append_bodies_soa and shedding). */
+int uvlm_num_bodies(uvlm_handle h, int64_t *n);
 /* Append n particles (X [n][3], alpha [n][3], sigma [n]) after the current ones; they take the
  * next original indices.  The tree is rebuilt by the next build/advance.  Replaces
  * particles.push_back (main.cpp:288) on a device-resident particle set. */

@@ -143,3 +143,25 @@ def test_lift_adjoint_finite_differences():
     fd, an = np.array(fd), np.array(an)
     print(f"[lift adjoint] CL {CL:.6f} n={n} vs FD {rel(an, fd):.3e} |grad| {np.abs(fd).max():.3e}", flush=True)
     assert rel(an, fd) < 1e-6
+
+
+def test_cpp_wing_facade_runs_the_reference_loop(orc, tmp_path):
+    """tests/cpp/wing_demo.cpp: the driver's loop written against include/uvlm/vlm_wing.hpp and
+    the FMMCalculator facade (RK2, P = 6, ncrit 256) gives the oracle's forces and wake."""
+    import os
+    import subprocess
+    from oracle import vlm
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "uvlm_amd", "lib",
+                       "uvlm_wing_demo")
+    out = tmp_path / "wing.bin"
+    steps, npanels = 6, 6
+    r = subprocess.run([exe, str(npanels), str(steps), str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    raw = np.fromfile(out, dtype=np.float64)
+    forces = raw[:3 * steps].reshape(steps, 3)
+    bodies = raw[3 * steps:].reshape(-1, 37)
+    V = reference_wing(npanels=npanels, chord=0.4)
+    hist, eb = vlm.run(V, [0.0, 1.0, 0.0], 0.05, steps, P=6, ncrit=256, scheme="rk2")
+    assert rel(forces, np.array([h[1] for h in hist])) < 1e-10
+    assert bodies.shape == eb.shape and rel(bodies[:, 9:15], eb[:, 9:15]) < 1e-10

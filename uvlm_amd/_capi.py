@@ -21,7 +21,7 @@ EXPORTS = [
     "uvlm_set_viscosity", "uvlm_viscosity", "uvlm_vorticity_aos", "uvlm_rbf_aos", "uvlm_eval_vjp",
     "uvlm_step_vjp", "uvlm_trajectory_forward", "uvlm_trajectory_vjp", "uvlm_append_bodies_soa",
     "uvlm_wing_create", "uvlm_wing_destroy", "uvlm_wing_get", "uvlm_wing_translate", "uvlm_wing_solve",
-    "uvlm_wing_shed", "uvlm_wing_step", "uvlm_wing_lift_adjoint",
+    "uvlm_wing_shed", "uvlm_wing_step", "uvlm_wing_lift_adjoint", "uvlm_num_bodies",
 ]
 
 
@@ -96,6 +96,7 @@ def lib():
             "uvlm_trajectory_forward": (I, [VP, I64, D, I64]),
             "uvlm_trajectory_vjp": (I, [VP, I64, PD, PD]),
             "uvlm_append_bodies_soa": (I, [VP, I64, PD, PD, PD]),
+            "uvlm_num_bodies": (I, [VP, PI64]),
             "uvlm_wing_create": (I, [I, I64, I64, PD, ctypes.POINTER(VP)]),
             "uvlm_wing_destroy": (I, [VP]),
             "uvlm_wing_get": (I, [VP, PD, PD, PD, PD]),

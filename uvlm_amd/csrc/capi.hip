@@ -780,6 +780,13 @@ int uvlm_advance(uvlm_handle h, double dt)
   });
 }
 
+int uvlm_num_bodies(uvlm_handle h, int64_t *n)
+{
+  if (!h || !n) return fail(UVLM_ERR_ARG, "null argument");
+  *n = h->N;
+  return UVLM_OK;
+}
+
 int uvlm_num_cells(uvlm_handle h, int64_t *n_cells)
 {
   return guarded([&] {
