@@ -3,8 +3,8 @@
   * near field only (one leaf holds every particle): the GPU VJP against the oracle's exact
     direct-sum VJP, itself pinned by central differences, both stretching forms, bar 1e-10;
   * far field included (multi-level tree, P = 8): against central differences (h = 1e-6) of the
-    FP64 GPU forward with the tree and lists frozen, so the expansions' own truncation is part
-    of the differentiated function; bar 1e-6;
+    FP64 GPU forward with the tree and lists frozen, on 64 sampled particles, so the expansions'
+    own truncation is part of the differentiated function; bar 1e-6;
   * time steps (Euler, RK2, RK3, RK4) and a checkpointed RK2 trajectory on one leaf, against
     central differences of uvlm_advance, bar 1e-7.
 """
@@ -72,7 +72,7 @@ def test_eval_vjp_frozen_tree_finite_differences(orc, stretching):
     X0, A0 = arr["X"].copy(), arr["alpha"].copy()
     h = 1e-6
     fd, an = {"x": [], "a": []}, {"x": [], "a": []}
-    for j in rng.choice(n, 6, replace=False):
+    for j in rng.choice(n, 64, replace=False):
         for d in range(3):
             for which, base, grad in (("x", X0, xb), ("a", A0, alb)):
                 p, m = base.copy(), base.copy()

@@ -411,6 +411,59 @@ def test_rate_vjp_matches_finite_differences(orc, stretching):
     assert rel(alb, fda) < 1e-8
 
 
+def _rates_complex(X, A, S, classic):
+    """Direct-sum u and dadt of the device P2P formula (multipoleKernels.hpp:223-301) in complex
+    arithmetic for complex-step differentiation: the first-order extension
+    erf(a + ib) = erf(a) + ib (2/sqrt(pi)) e^{-a^2} in the style of complexify.hpp's cplx."""
+    from scipy.special import erf
+    n = X.shape[0]
+    u = np.zeros((n, 3), dtype=complex)
+    J = np.zeros((n, 3, 3), dtype=complex)
+    for i in range(n):
+        r = X[i][None, :] - X
+        R2 = np.sum(r * r, axis=1)
+        keep = np.abs(R2) > 0
+        r, R2, a, sg = r[keep], R2[keep], A[keep], S[keep]
+        s = np.sqrt(R2 / (2 * sg * sg))
+        t2 = 2 / np.sqrt(np.pi) * s * np.exp(-s * s)
+        g = erf(s.real) + 1j * s.imag * 2 / np.sqrt(np.pi) * np.exp(-s.real ** 2) - t2
+        q = 1 / (4 * np.pi * R2 * np.sqrt(R2))
+        f1, f2 = g * q, (t2 / (sg * sg) - 3 * g / R2) * q
+        c = np.cross(r, a)
+        u[i] = -np.sum(f1[:, None] * c, axis=0)
+        J[i] = -np.einsum("p,pa,pb->ab", f2, c, r)
+        eps = np.zeros((3, 3, 3))
+        eps[0, 1, 2] = eps[1, 2, 0] = eps[2, 0, 1] = 1
+        eps[0, 2, 1] = eps[2, 1, 0] = eps[1, 0, 2] = -1
+        J[i] -= np.einsum("abk,p,pk->ab", eps, f1, a)
+    dadt = np.einsum("iab,ib->ia", J, A) if classic else np.einsum("iab,ia->ib", J, A)
+    return u, dadt
+
+
+@pytest.mark.parametrize("stretching", ["transposed", "classic"])
+def test_rate_vjp_matches_complex_step(orc, stretching):
+    """direct_vjp against complex-step derivatives (h = 1e-30, no subtraction error) of the same
+    objective: agreement to 1e-11, beyond what central differences can pin."""
+    rng = np.random.default_rng(7)
+    n = 24
+    X = rng.normal(size=(n, 3)) * 0.3
+    A = rng.normal(size=(n, 3))
+    S = rng.uniform(0.1, 0.3, n)
+    ub = rng.normal(size=(n, 3))
+    ab = rng.normal(size=(n, 3))
+    xb, alb = orc.direct_vjp(X, A, S, ub, ab, stretching)
+    h = 1e-30
+    for i in range(0, n, 5):
+        for d in range(3):
+            for which in ("x", "a"):
+                Xc, Ac = X.astype(complex), A.astype(complex)
+                (Xc if which == "x" else Ac)[i, d] += 1j * h
+                u, dadt = _rates_complex(Xc, Ac, S, stretching == "classic")
+                cs = (np.sum(ub * u) + np.sum(ab * dadt)).imag / h
+                an = (xb if which == "x" else alb)[i, d]
+                assert abs(cs - an) < 1e-11 * max(1.0, abs(an)), (i, d, which, cs, an)
+
+
 # ------------------------------------------------------------------- viscous schemes, vorticity, RBF
 def _lattice(L, h):
     g = (np.arange(-L, L) + 0.5) * h  # even count: no point on the root centre or its axes
