@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--vlm-window", type=int, default=50, help="steps of the lift adjoint at the end of --vlm-steps")
     ap.add_argument("--vlm-leg-steps", type=int, default=250,
                     help="wing + wake loop steps reported beside the eval benchmark (0: skip)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint leg (three Gaussian blobs) and print its measurements")
     ap.add_argument("--adjoint-n", type=int, default=1 << 20,
                     help="particles in the adjoint leg (three Gaussian blobs, 4096 probes, P=8; 0: skip)")
     args = ap.parse_args()
@@ -287,6 +289,9 @@ def main():
 
     if args.vlm_steps > 0:
         vlm_scenario(args)
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(args.adjoint_n, args.theta, args.ncrit or 64, max(1, args.steps))), flush=True)
         return
 
     from uvlm_amd import FMMCalculator, workloads as W

@@ -52,6 +52,7 @@ struct uvlm_handle_s {
   // adjoint of the rates and of the time step: transposed lists, adjoint fields, stage states
   // (X, alpha, sigma + original index, per stage), state cotangents in the original order
   Lists   tlists;
+  bool    tlists_ok = false;  // tlists is the transpose of the current build's lists
   DevBuf  vaux, vstage, vorig, vcot, vlam;
   // trajectory checkpoints (X, alpha, sigma in the original order, every `ck_every` steps)
   DevBuf  ck;
@@ -77,7 +78,7 @@ struct uvlm_wing_s {
   int         device = 0;
   hipStream_t s      = nullptr;
   int         nspan = 0, nchord = 0, n = 0;
-  DevBuf      V, cp, nrm, area, B, LU, piv, info, gamma, v, F, shedX, shedA, shedS;
+  DevBuf      V, cp, nrm, area, B, LU, Ainv, piv, info, rhs, gamma, v, u, F, shedX, shedA, shedS;
 };
 
 namespace {
@@ -156,6 +157,7 @@ void do_build(uvlm_handle h)
   h->tm.n_interactions = h->lists.n_interactions;
   h->built             = true;
   h->evaluated         = false;
+  h->tlists_ok         = false;
 }
 
 void do_eval(uvlm_handle h, bool sync)
@@ -429,7 +431,10 @@ void vjp_core(uvlm_handle h)
   l2p_adj_tgt(s, P, t, h->L.as<double2>(), st, aux, cap);
   p2p_adj(s, t, h->lists, 0, h->prm.precision == UVLM_MIXED, st, aux, cap);
   // source side: adjoint sources up the tree, across the transposed M2L list, down to the bodies
-  transpose_lists(s, h->sc, t, t, h->lists, h->tlists);
+  if (!h->tlists_ok) {
+    transpose_lists(s, h->sc, t, t, h->lists, h->tlists);
+    h->tlists_ok = true;
+  }
   double2 *M = h->M.as<double2>(), *L = h->L.as<double2>();
   p2m_adj(s, P, t, st, aux, cap, M);
   for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M);
@@ -1266,6 +1271,9 @@ int uvlm_wing_create(int device, int64_t nspan, int64_t nchord, const double *ve
     w->area.reserve(8 * n);
     w->B.reserve(24 * (size_t)n * n);
     w->LU.reserve(8 * (size_t)n * n);
+    w->Ainv.reserve(8 * (size_t)n * n);
+    w->rhs.reserve(8 * n);
+    w->u.reserve(24 * n);
     w->piv.reserve(4 * n);
     w->info.reserve(8);
     w->gamma.reserve(8 * n);
@@ -1287,6 +1295,10 @@ int uvlm_wing_create(int device, int64_t nspan, int64_t nchord, const double *ve
     UVLM_HIP(hipStreamSynchronize(w->s));
     if (hf[0]) throw Error(UVLM_ERR_ARG, "a control point on a vortex line or a zero-length vortex line (main.cpp:19-21)");
     if (hf[1]) throw Error(UVLM_ERR_ARG, "singular system matrix (zero pivot at column " + std::to_string(hf[1] - 1) + ")");
+    // the matrix is fixed for the wing (rigid motion only): its inverse turns every solve of
+    // the loop, and of the adjoint, into one product
+    lu_inverse(w->s, n, w->LU.as<double>(), w->piv.as<int>(), w->Ainv.as<double>());
+    UVLM_HIP(hipStreamSynchronize(w->s));
     *out = w;
     w    = nullptr;
   });
@@ -1350,10 +1362,10 @@ void wing_solve(uvlm_wing w, uvlm_handle h, const double U[3])
     UVLM_HIP(hipStreamSynchronize(h->s));
     vptr = w->v.as<double>();
   }
-  wing_rhs(w->s, n, w->nrm.as<double>(), U, vptr, w->gamma.as<double>());
-  lu_solve(w->s, n, w->LU.as<double>(), w->piv.as<int>(), w->gamma.as<double>());
-  wing_force(w->s, n, w->nspan, w->V.as<double>(), w->B.as<double>(), w->gamma.as<double>(), U, vptr,
-             w->F.as<double>());
+  wing_rhs(w->s, n, w->nrm.as<double>(), U, vptr, w->rhs.as<double>());
+  matvec_inv(w->s, n, w->Ainv.as<double>(), w->rhs.as<double>(), w->gamma.as<double>());
+  wing_velocity(w->s, n, w->B.as<double>(), w->gamma.as<double>(), U, vptr, w->u.as<double>());
+  wing_force(w->s, n, w->nspan, w->V.as<double>(), w->u.as<double>(), w->gamma.as<double>(), w->F.as<double>());
 }
 
 void wing_outputs(uvlm_wing w, double *gamma, double *force)
@@ -1490,12 +1502,11 @@ int uvlm_wing_lift_adjoint(uvlm_wing w, uvlm_handle particles, const double U[3]
     h->vlam.reserve(sizeof(double) * 6 * cap);
     double *lam = h->vlam.as<double>();
     zero_doubles(s, lam, 6 * cap);
-    const double *vN = nstart[steps] ? w->v.as<double>() : nullptr;
-    wing_force_vjp(ws, np, ns, w->V.as<double>(), w->B.as<double>(), w->gamma.as<double>(), U, vN, es, c, vbar, gbar);
+    wing_force_vjp(ws, np, ns, w->V.as<double>(), w->B.as<double>(), w->u.as<double>(), w->gamma.as<double>(), es, c,
+                   vbar, gbar);
     auto through_solve_and_probes = [&](int64_t npart) {
       // gbar -> rbar = A^{-T} gbar -> vbar -= rbar n -> particle cotangents (probe adjoint)
-      UVLM_HIP(hipMemcpyAsync(rbar, gbar, 8 * np, hipMemcpyDeviceToDevice, ws));
-      lu_solve_t(ws, np, w->LU.as<double>(), w->piv.as<int>(), rbar);
+      matvec_inv_t(ws, np, w->Ainv.as<double>(), gbar, rbar);
       wing_rhs_vjp(ws, np, w->nrm.as<double>(), rbar, vbar);
       UVLM_HIP(hipStreamSynchronize(ws));
       if (npart == 0) return;

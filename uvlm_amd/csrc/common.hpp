@@ -190,10 +190,15 @@ void wing_geom(hipStream_t s, int n, int nspan, const double *V, double *cp, dou
 void wing_influence(hipStream_t s, int m, int n, int nspan, const double *V, const double *pts, const double *nrm,
                     double *B, double *A, int *singular);
 void lu_factor(hipStream_t s, int n, double *A, int *piv, int *info);
-void lu_solve(hipStream_t s, int n, const double *LU, const int *piv, double *b);
+// A^{-1} column-major from the factors (one workgroup per column) and products with it
+void lu_inverse(hipStream_t s, int n, const double *LU, const int *piv, double *Xc);
+void matvec_inv(hipStream_t s, int n, const double *Xc, const double *b, double *out);
+void matvec_inv_t(hipStream_t s, int n, const double *Xc, const double *g, double *out);
+// u_i = U + v_i + sum_j B[i][j] gamma_j at the control points (v may be null)
+void wing_velocity(hipStream_t s, int n, const double *B, const double *gamma, const double U[3], const double *v,
+                   double *u);
 void wing_rhs(hipStream_t s, int n, const double *nrm, const double U[3], const double *v, double *rhs);
-void wing_force(hipStream_t s, int n, int nspan, const double *V, const double *B, const double *gamma, const double U[3],
-                const double *v, double *F);
+void wing_force(hipStream_t s, int n, int nspan, const double *V, const double *u, const double *gamma, double *F);
 void wing_shed(hipStream_t s, int nspan, int nchord, const double *V, const double *gamma, const double U[3], double dt,
                int ref_offset, double *X, double *A, double *S);
 void translate3(hipStream_t s, int64_t n, double *p, const double d[3]);
@@ -201,11 +206,10 @@ void points_to_soa(hipStream_t s, int64_t m, const double *pts, double *st, int6
 void probe_velocity_out(hipStream_t s, int64_t m, const int32_t *perm, const double *st, int64_t cap, double *out);
 void append_bodies(hipStream_t s, int64_t N, int64_t k, const double *X, const double *A, const double *S, double *st,
                    int64_t cap, int32_t *orig);
-// adjoint of the lift through the loop: force VJP (c [n], vbar [n][3], gbar [n]), A^{-T} solve,
+// adjoint of the lift through the loop: force VJP (c [n], vbar [n][3], gbar [n]),
 // rhs and shedding VJPs, probe "vortices", probe-adjoint results into original-order cotangents
-void wing_force_vjp(hipStream_t s, int n, int nspan, const double *V, const double *B, const double *gamma,
-                    const double U[3], const double *v, const double e[3], double *c, double *vbar, double *gbar);
-void lu_solve_t(hipStream_t s, int n, const double *LU, const int *piv, double *b);
+void wing_force_vjp(hipStream_t s, int n, int nspan, const double *V, const double *B, const double *u,
+                    const double *gamma, const double e[3], double *c, double *vbar, double *gbar);
 void wing_rhs_vjp(hipStream_t s, int n, const double *nrm, const double *rbar, double *vbar);
 void wing_shed_vjp(hipStream_t s, int nspan, int nchord, const double *V, const double *abar, int64_t stride,
                    double *gbar);

@@ -65,16 +65,35 @@ __global__ void k_bounds_partial(const double *__restrict__ x, const double *__r
     for (int d = 0; d < 6; d++) part[blockIdx.x * 6 + d] = s[d][0];
 }
 
-// X0 and R0 exactly as tree.hpp:19-25 computes them from Xmin/Xmax.
-__global__ void k_bounds_final(const double *part, int nb, double *X0R0)
+// X0 and R0 exactly as tree.hpp:19-25 computes them from Xmin/Xmax (one workgroup of BLK;
+// min and max are exact in any order, so the parallel reduction gives the serial result).
+__global__ __launch_bounds__(BLK) void k_bounds_final(const double *part, int nb, double *X0R0)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ double s[6][BLK];
   double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < nb; b++)
+  for (int b = threadIdx.x; b < nb; b += BLK)
     for (int d = 0; d < 3; d++) {
       mn[d] = fmin(part[b * 6 + d], mn[d]);
       mx[d] = fmax(part[b * 6 + 3 + d], mx[d]);
     }
+  for (int d = 0; d < 3; d++) {
+    s[d][threadIdx.x]     = mn[d];
+    s[3 + d][threadIdx.x] = mx[d];
+  }
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int d = 0; d < 3; d++) {
+        s[d][threadIdx.x]     = fmin(s[d][threadIdx.x + w], s[d][threadIdx.x]);
+        s[3 + d][threadIdx.x] = fmax(s[3 + d][threadIdx.x + w], s[3 + d][threadIdx.x]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  for (int d = 0; d < 3; d++) {
+    mn[d] = s[d][0];
+    mx[d] = s[3 + d][0];
+  }
   double X0[3], R0 = 0;
   for (int d = 0; d < 3; d++) X0[d] = (mx[d] + mn[d]) / 2;
   for (int d = 0; d < 3; d++) {
@@ -401,12 +420,20 @@ __global__ void k_low32_u(int64_t n, const uint64_t *__restrict__ keys, int32_t 
   if (i < n) out[i] = (int32_t)(uint32_t)(keys[i] & 0xffffffffull);
 }
 
-__global__ void k_transpose_keys(int64_t C, const int32_t *__restrict__ off, const int32_t *__restrict__ src,
-                                 uint64_t *__restrict__ keys)
+// One lane per list entry q: its row t (the last t with off[t] <= q) by binary search, key
+// (source, target) for the transposed CSR.
+__global__ void k_transpose_keys(int64_t C, int64_t n, const int32_t *__restrict__ off,
+                                 const int32_t *__restrict__ src, uint64_t *__restrict__ keys)
 {
-  const int64_t t = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (t >= C) return;
-  for (int q = off[t]; q < off[t + 1]; q++) keys[q] = ((uint64_t)(uint32_t)src[q] << 32) | (uint64_t)(uint32_t)t;
+  const int64_t q = blockIdx.x * (int64_t)BLK + threadIdx.x;
+  if (q >= n) return;
+  int64_t lo = 0, hi = C - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= q) lo = mid;
+    else hi = mid - 1;
+  }
+  keys[q] = ((uint64_t)(uint32_t)src[q] << 32) | (uint64_t)(uint32_t)lo;
 }
 
 __global__ void k_count_inter(int64_t C, const int32_t *__restrict__ off, const int32_t *__restrict__ src,
@@ -469,7 +496,7 @@ void build_tree(hipStream_t s, Scratch &sc, const double *x, const double *y, co
   sc.partial.reserve(sizeof(double) * 6 * nb);
   tree.X0R0.reserve(sizeof(double) * 4);
   k_bounds_partial<<<nb, BLK, 0, s>>>(x, y, z, N, sc.partial.as<double>());
-  k_bounds_final<<<1, 64, 0, s>>>(sc.partial.as<double>(), nb, tree.X0R0.as<double>());
+  k_bounds_final<<<1, BLK, 0, s>>>(sc.partial.as<double>(), nb, tree.X0R0.as<double>());
   UVLM_LAUNCH_CHECK();
   // 2-3. keys and stable sort
   sc.keys0.reserve(8 * N);
@@ -710,9 +737,11 @@ void transpose_lists(hipStream_t s, Scratch &sc, const Tree &in_targets, const T
     src.reserve(4 * std::max<int64_t>(n, 1));
     sc.keys0.reserve(8 * std::max<int64_t>(n, 1));
     sc.keys1.reserve(8 * std::max<int64_t>(n, 1));
-    k_transpose_keys<<<nblocks(in_targets.C), BLK, 0, s>>>(in_targets.C, ioff.as<int32_t>(), isrc.as<int32_t>(),
-                                                           sc.keys0.as<uint64_t>());
-    UVLM_LAUNCH_CHECK();
+    if (n > 0) {
+      k_transpose_keys<<<nblocks(n), BLK, 0, s>>>(in_targets.C, n, ioff.as<int32_t>(), isrc.as<int32_t>(),
+                                                  sc.keys0.as<uint64_t>());
+      UVLM_LAUNCH_CHECK();
+    }
     if (n > 0) sort_keys_u64(s, sc, sc.keys0.as<uint64_t>(), sc.keys1.as<uint64_t>(), n, eb);
     k_csr<<<nblocks(Cs + 1), BLK, 0, s>>>(Cs, sc.keys1.as<uint64_t>(), n, off.as<int32_t>());
     k_low32_u<<<nblocks(n), BLK, 0, s>>>(n, sc.keys1.as<uint64_t>(), src.as<int32_t>());

@@ -8,7 +8,8 @@
 //
 // Kernels: geometry (calculateTopology, 45-58), ring influences B[t][j] at points (one lane per
 // (point, panel), the segment formula of 11-25), the system matrix A[i][j] = B[i][j] . n_i,
-// a one-workgroup LU with partial pivoting and its solve (the reference's fullPivLu, 214),
+// a one-workgroup LU with partial pivoting and the inverse from it (in place of the reference's
+// fullPivLu solve, 214; the matrix is fixed under the loop's rigid motion),
 // the right-hand side (207-212), the Kutta-Joukowski force (223-244) and the shed particles
 // (246-289).  Every sum runs in a fixed order (deterministic).
 #include "common.hpp"
@@ -151,11 +152,17 @@ __global__ __launch_bounds__(LU_WG) void k_lu(int n, double *__restrict__ A, int
   }
 }
 
-// x = A^{-1} b from k_lu's factors; b is overwritten with x.  One workgroup, columns in order.
-__global__ __launch_bounds__(LU_WG) void k_lu_solve(int n, const double *__restrict__ LU, const int *__restrict__ piv,
-                                                    double *__restrict__ b)
+// Columns of A^{-1} from k_lu's factors, one workgroup per column c: e_c permuted by the pivots,
+// then the unit-lower and the upper triangular solves; Xc[c * n + i] = (A^{-1})[i][c]
+// (column-major).
+__global__ __launch_bounds__(256) void k_lu_inverse(int n, const double *__restrict__ LU, const int *__restrict__ piv,
+                                                    double *__restrict__ Xc)
 {
-  const int tid = threadIdx.x;
+  const int c = blockIdx.x, tid = threadIdx.x;
+  if (c >= n) return;
+  double *b = Xc + (int64_t)c * n;
+  for (int i = tid; i < n; i += 256) b[i] = (i == c) ? 1.0 : 0.0;
+  __syncthreads();
   if (tid == 0)
     for (int k = 0; k < n; k++) {
       const int p = piv[k];
@@ -166,18 +173,64 @@ __global__ __launch_bounds__(LU_WG) void k_lu_solve(int n, const double *__restr
       }
     }
   __syncthreads();
-  for (int j = 0; j < n; j++) {  // L y = b (unit diagonal)
+  for (int j = 0; j < n; j++) {
     const double y = b[j];
-    for (int i = j + 1 + tid; i < n; i += LU_WG) b[i] -= LU[(int64_t)i * n + j] * y;
+    for (int i = j + 1 + tid; i < n; i += 256) b[i] -= LU[(int64_t)i * n + j] * y;
     __syncthreads();
   }
-  for (int j = n - 1; j >= 0; j--) {  // U x = y
+  for (int j = n - 1; j >= 0; j--) {
     __syncthreads();
     const double x = b[j] / LU[(int64_t)j * n + j];
     __syncthreads();
     if (tid == 0) b[j] = x;
-    for (int i = tid; i < j; i += LU_WG) b[i] -= LU[(int64_t)i * n + j] * x;
+    for (int i = tid; i < j; i += 256) b[i] -= LU[(int64_t)i * n + j] * x;
     __syncthreads();
+  }
+}
+
+// out_i = sum_c Xc[c * n + i] b_c (= A^{-1} b), one lane per row, columns in order.
+__global__ void k_matvec_inv(int n, const double *__restrict__ Xc, const double *__restrict__ b,
+                             double *__restrict__ out)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double r = 0.0;
+  for (int c = 0; c < n; c++) r += Xc[(int64_t)c * n + i] * b[c];
+  out[i] = r;
+}
+
+// out_c = sum_i Xc[c * n + i] g_i (= A^{-T} g), one wavefront per c: lanes over i, then a
+// fixed-order butterfly.
+__global__ __launch_bounds__(64) void k_matvec_inv_t(int n, const double *__restrict__ Xc,
+                                                     const double *__restrict__ g, double *__restrict__ out)
+{
+  const int c = blockIdx.x, lane = threadIdx.x;
+  if (c >= n) return;
+  double r = 0.0;
+  for (int i = lane; i < n; i += 64) r += Xc[(int64_t)c * n + i] * g[i];
+  for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+  if (lane == 0) out[c] = r;
+}
+
+// u_i = U + v_i + sum_j B[i][j] gamma_j: the velocity at control point i (main.cpp:229-239), one
+// wavefront per i, lanes over j, fixed-order butterfly.
+__global__ __launch_bounds__(64) void k_wing_velocity(int n, const double *__restrict__ B,
+                                                      const double *__restrict__ gamma, double Ux, double Uy,
+                                                      double Uz, const double *__restrict__ v, double *__restrict__ u)
+{
+  const int i = blockIdx.x, lane = threadIdx.x;
+  if (i >= n) return;
+  V3 a{0, 0, 0};
+  for (int j = lane; j < n; j += 64) a = a + gamma[j] * ld3(B + 3 * ((int64_t)i * n + j));
+  for (int o = 32; o > 0; o >>= 1) {
+    a.x += __shfl_xor(a.x, o, 64);
+    a.y += __shfl_xor(a.y, o, 64);
+    a.z += __shfl_xor(a.z, o, 64);
+  }
+  if (lane == 0) {
+    a = a + V3{Ux, Uy, Uz};
+    if (v) a = a + ld3(v + 3 * i);
+    u[3 * i] = a.x, u[3 * i + 1] = a.y, u[3 * i + 2] = a.z;
   }
 }
 
@@ -198,17 +251,14 @@ __global__ void k_wing_rhs(int n, const double *__restrict__ nrm, double Ux, dou
 // circulation of the panel upstream (gamma_i on the first row).  Per-lane partial sums in panel
 // order, then a fixed-order tree over the lanes.
 __global__ __launch_bounds__(256) void k_wing_force(int n, int nspan, const double *__restrict__ V,
-                                                    const double *__restrict__ B, const double *__restrict__ gamma,
-                                                    double Ux, double Uy, double Uz, const double *__restrict__ v,
+                                                    const double *__restrict__ uc, const double *__restrict__ gamma,
                                                     double *__restrict__ F)
 {
   __shared__ double red[3][256];
   const int         tid = threadIdx.x;
   V3                acc{0, 0, 0};
   for (int i = tid; i < n; i += 256) {
-    V3 u{Ux, Uy, Uz};
-    for (int j = 0; j < n; j++) u = u + gamma[j] * ld3(B + 3 * ((int64_t)i * n + j));
-    if (v) u = u + ld3(v + 3 * i);
+    const V3      u  = ld3(uc + 3 * i);
     const Corners q  = corners(V, nspan, i);
     const double  Gn = gamma[i] - (i >= nspan ? gamma[i - nspan] : 0.0);
     acc              = acc + Gn * cross(u, q.le1 - q.le);
@@ -297,16 +347,13 @@ __global__ void k_append_bodies(int64_t N, int64_t k, const double *__restrict__
 // ---- adjoint of the lift coefficient through the loop ----
 // Force VJP, part 1 (per panel i), for J = e . F with e already scaled: u_i as in k_wing_force,
 // q_i = dl_i x e, c_i = u_i . q_i and w_i = Gamma_i q_i (= dJ/dv_i).
-__global__ void k_wing_force_vjp1(int n, int nspan, const double *__restrict__ V, const double *__restrict__ B,
-                                  const double *__restrict__ gamma, double Ux, double Uy, double Uz,
-                                  const double *__restrict__ v, double ex, double ey, double ez,
+__global__ void k_wing_force_vjp1(int n, int nspan, const double *__restrict__ V, const double *__restrict__ uc,
+                                  const double *__restrict__ gamma, double ex, double ey, double ez,
                                   double *__restrict__ c, double *__restrict__ w)
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  V3 u{Ux, Uy, Uz};
-  for (int j = 0; j < n; j++) u = u + gamma[j] * ld3(B + 3 * ((int64_t)i * n + j));
-  if (v) u = u + ld3(v + 3 * i);
+  const V3      u  = ld3(uc + 3 * i);
   const Corners q  = corners(V, nspan, i);
   const V3      qe = cross(q.le1 - q.le, V3{ex, ey, ez});
   const double  Gn = gamma[i] - (i >= nspan ? gamma[i - nspan] : 0.0);
@@ -315,44 +362,16 @@ __global__ void k_wing_force_vjp1(int n, int nspan, const double *__restrict__ V
 }
 
 // Part 2 (per panel j): dJ/dgamma_j = c_j - c_{j+nspan} (the net circulations) + sum_i w_i . B[i][j].
-__global__ void k_wing_force_vjp2(int n, int nspan, const double *__restrict__ B, const double *__restrict__ c,
-                                  const double *__restrict__ w, double *__restrict__ gbar)
+__global__ __launch_bounds__(64) void k_wing_force_vjp2(int n, int nspan, const double *__restrict__ B,
+                                                        const double *__restrict__ c, const double *__restrict__ w,
+                                                        double *__restrict__ gbar)
 {
-  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.x, lane = threadIdx.x;
   if (j >= n) return;
-  double g = c[j] - (j + nspan < n ? c[j + nspan] : 0.0);
-  for (int i = 0; i < n; i++) g += dot(ld3(w + 3 * i), ld3(B + 3 * ((int64_t)i * n + j)));
-  gbar[j] = g;
-}
-
-// y = A^{-T} b from k_lu's factors (P A = L U, so A^T = U^T L^T P): U^T z = b, L^T w = z,
-// y = P^T w; b is overwritten with y.  One workgroup.
-__global__ __launch_bounds__(LU_WG) void k_lu_solve_t(int n, const double *__restrict__ LU, const int *__restrict__ piv,
-                                                      double *__restrict__ b)
-{
-  const int tid = threadIdx.x;
-  for (int j = 0; j < n; j++) {  // U^T z = b: column j of U^T is row j of U
-    __syncthreads();
-    const double z = b[j] / LU[(int64_t)j * n + j];
-    __syncthreads();
-    if (tid == 0) b[j] = z;
-    for (int i = j + 1 + tid; i < n; i += LU_WG) b[i] -= LU[(int64_t)j * n + i] * z;
-  }
-  __syncthreads();
-  for (int j = n - 1; j >= 0; j--) {  // L^T w = z (unit diagonal): row j of L
-    const double x = b[j];
-    for (int i = tid; i < j; i += LU_WG) b[i] -= LU[(int64_t)j * n + i] * x;
-    __syncthreads();
-  }
-  if (tid == 0)
-    for (int k = n - 1; k >= 0; k--) {
-      const int p = piv[k];
-      if (p != k) {
-        const double t = b[k];
-        b[k]           = b[p];
-        b[p]           = t;
-      }
-    }
+  double g = 0.0;
+  for (int i = lane; i < n; i += 64) g += dot(ld3(w + 3 * i), ld3(B + 3 * ((int64_t)i * n + j)));
+  for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);
+  if (lane == 0) gbar[j] = g + c[j] - (j + nspan < n ? c[j + nspan] : 0.0);
 }
 
 // vbar_i += -rbar_i n_i (rhs_i = -U . n_i - v_i . n_i).
@@ -446,9 +465,28 @@ void lu_factor(hipStream_t s, int n, double *A, int *piv, int *info)
   UVLM_LAUNCH_CHECK();
 }
 
-void lu_solve(hipStream_t s, int n, const double *LU, const int *piv, double *b)
+void lu_inverse(hipStream_t s, int n, const double *LU, const int *piv, double *Xc)
 {
-  k_lu_solve<<<1, LU_WG, 0, s>>>(n, LU, piv, b);
+  k_lu_inverse<<<n, 256, 0, s>>>(n, LU, piv, Xc);
+  UVLM_LAUNCH_CHECK();
+}
+
+void matvec_inv(hipStream_t s, int n, const double *Xc, const double *b, double *out)
+{
+  k_matvec_inv<<<nb256(n), 256, 0, s>>>(n, Xc, b, out);
+  UVLM_LAUNCH_CHECK();
+}
+
+void matvec_inv_t(hipStream_t s, int n, const double *Xc, const double *g, double *out)
+{
+  k_matvec_inv_t<<<n, 64, 0, s>>>(n, Xc, g, out);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_velocity(hipStream_t s, int n, const double *B, const double *gamma, const double U[3], const double *v,
+                   double *u)
+{
+  k_wing_velocity<<<n, 64, 0, s>>>(n, B, gamma, U[0], U[1], U[2], v, u);
   UVLM_LAUNCH_CHECK();
 }
 
@@ -458,10 +496,9 @@ void wing_rhs(hipStream_t s, int n, const double *nrm, const double U[3], const 
   UVLM_LAUNCH_CHECK();
 }
 
-void wing_force(hipStream_t s, int n, int nspan, const double *V, const double *B, const double *gamma, const double U[3],
-                const double *v, double *F)
+void wing_force(hipStream_t s, int n, int nspan, const double *V, const double *u, const double *gamma, double *F)
 {
-  k_wing_force<<<1, 256, 0, s>>>(n, nspan, V, B, gamma, U[0], U[1], U[2], v, F);
+  k_wing_force<<<1, 256, 0, s>>>(n, nspan, V, u, gamma, F);
   UVLM_LAUNCH_CHECK();
 }
 
@@ -492,18 +529,12 @@ void append_bodies(hipStream_t s, int64_t N, int64_t k, const double *X, const d
   UVLM_LAUNCH_CHECK();
 }
 
-void wing_force_vjp(hipStream_t s, int n, int nspan, const double *V, const double *B, const double *gamma,
-                    const double U[3], const double *v, const double e[3], double *c, double *vbar, double *gbar)
+void wing_force_vjp(hipStream_t s, int n, int nspan, const double *V, const double *B, const double *u,
+                    const double *gamma, const double e[3], double *c, double *vbar, double *gbar)
 {
-  k_wing_force_vjp1<<<nb256(n), 256, 0, s>>>(n, nspan, V, B, gamma, U[0], U[1], U[2], v, e[0], e[1], e[2], c, vbar);
+  k_wing_force_vjp1<<<nb256(n), 256, 0, s>>>(n, nspan, V, u, gamma, e[0], e[1], e[2], c, vbar);
   UVLM_LAUNCH_CHECK();
-  k_wing_force_vjp2<<<nb256(n), 256, 0, s>>>(n, nspan, B, c, vbar, gbar);
-  UVLM_LAUNCH_CHECK();
-}
-
-void lu_solve_t(hipStream_t s, int n, const double *LU, const int *piv, double *b)
-{
-  k_lu_solve_t<<<1, LU_WG, 0, s>>>(n, LU, piv, b);
+  k_wing_force_vjp2<<<n, 64, 0, s>>>(n, nspan, B, c, vbar, gbar);
   UVLM_LAUNCH_CHECK();
 }
 
