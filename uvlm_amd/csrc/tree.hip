@@ -129,6 +129,42 @@ __global__ void k_keys(const double *__restrict__ x, const double *__restrict__ 
   idx[i]  = (uint32_t)i;
 }
 
+// The root record (begin 0, end N, no parent, rank 0, centre X0) and the error flag, on the
+// device: no host round trip.
+__global__ void k_root_record(int32_t n, const double *__restrict__ X0R0, int32_t *rb, int32_t *re, int32_t *rp,
+                              int32_t *rr, double *cx, double *cy, double *cz, int32_t *err)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  rb[0] = 0;
+  re[0] = n;
+  rp[0] = -1;
+  rr[0] = 0;
+  cx[0] = X0R0[0];
+  cy[0] = X0R0[1];
+  cz[0] = X0R0[2];
+  *err  = 0;
+}
+
+__global__ void k_set_counters(unsigned long long *c, unsigned long long a, unsigned long long b,
+                               unsigned long long d)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  c[0] = a;
+  c[1] = b;
+  c[2] = d;
+}
+
+// The three numbers the host needs after counting a level: last exclusive-scan entry, last
+// child count, error flag.
+__global__ void k_level_summary(const int32_t *__restrict__ scan_last, const int32_t *__restrict__ nchild_last,
+                                const int32_t *__restrict__ err, int32_t *__restrict__ out)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  out[0] = *scan_last;
+  out[1] = *nchild_last;
+  out[2] = *err;
+}
+
 __device__ __forceinline__ int digit_at(uint64_t key, int l) { return (int)((key >> (3 * (KEY_LEVELS - 1 - l))) & 7); }
 
 __device__ __forceinline__ void octant_bounds(const uint64_t *__restrict__ keys, int b, int e, int l, int bnd[9])
@@ -533,18 +569,11 @@ void build_tree(hipStream_t s, Scratch &sc, const double *x, const double *y, co
   };
   ensure_rec(std::max<int64_t>(1024, 8 * (N / std::max<int64_t>(1, ncrit) + 1)), 0);
   sc.counters.reserve(64);
-  int32_t *err = sc.counters.as<int32_t>();
-  UVLM_HIP(hipMemsetAsync(err, 0, 4, s));
-  {
-    const int32_t zero = 0, n32 = (int32_t)N, par = -1;
-    h2d(s, sc.rec_begin.p, &zero, 4);
-    h2d(s, sc.rec_end.p, &n32, 4);
-    h2d(s, sc.rec_parent.p, &par, 4);
-    h2d(s, sc.rec_rank.p, &zero, 4);
-    UVLM_HIP(hipMemcpyAsync(sc.rec_cx.p, tree.X0R0.p, 8, hipMemcpyDeviceToDevice, s));
-    UVLM_HIP(hipMemcpyAsync(sc.rec_cy.p, tree.X0R0.as<double>() + 1, 8, hipMemcpyDeviceToDevice, s));
-    UVLM_HIP(hipMemcpyAsync(sc.rec_cz.p, tree.X0R0.as<double>() + 2, 8, hipMemcpyDeviceToDevice, s));
-  }
+  int32_t *err = sc.counters.as<int32_t>(), *summary = err + 8;
+  k_root_record<<<1, 64, 0, s>>>((int32_t)N, tree.X0R0.as<double>(), sc.rec_begin.as<int32_t>(),
+                                 sc.rec_end.as<int32_t>(), sc.rec_parent.as<int32_t>(), sc.rec_rank.as<int32_t>(),
+                                 sc.rec_cx.as<double>(), sc.rec_cy.as<double>(), sc.rec_cz.as<double>(), err);
+  UVLM_LAUNCH_CHECK();
   std::vector<int64_t> off = {0, 1};
   for (int l = 0;; l++) {
     const int64_t o = off[l], cnt = off[l + 1] - off[l];
@@ -552,12 +581,13 @@ void build_tree(hipStream_t s, Scratch &sc, const double *x, const double *y, co
                                                   sc.rec_end.as<int32_t>(), keys, sc.rec_nchild.as<int32_t>(), err);
     UVLM_LAUNCH_CHECK();
     excl_scan_i32(s, sc, sc.rec_nchild.as<int32_t>() + o, sc.rec_scan.as<int32_t>(), cnt);
-    int32_t last[2], e = 0;
-    to_host(s, &last[0], sc.rec_scan.as<int32_t>() + cnt - 1);
-    to_host(s, &last[1], sc.rec_nchild.as<int32_t>() + o + cnt - 1);
-    to_host(s, &e, err);
-    if (e) throw Error(UVLM_ERR_DEPTH, "more than ncrit coincident bodies: octree deeper than 21 levels");
-    const int64_t total = (int64_t)last[0] + last[1];
+    k_level_summary<<<1, 64, 0, s>>>(sc.rec_scan.as<int32_t>() + cnt - 1, sc.rec_nchild.as<int32_t>() + o + cnt - 1,
+                                     err, summary);
+    UVLM_LAUNCH_CHECK();
+    int32_t sm3[3] = {0, 0, 0};
+    to_host(s, sm3, summary, 3);
+    if (sm3[2]) throw Error(UVLM_ERR_DEPTH, "more than ncrit coincident bodies: octree deeper than 21 levels");
+    const int64_t total = (int64_t)sm3[0] + sm3[1];
     if (total == 0) break;
     ensure_rec(off[l + 1] + total + 1, off[l + 1]);
     k_write_children<<<nblocks(cnt), BLK, 0, s>>>(
@@ -673,7 +703,7 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
   while (nf > 0) {
     unsigned long long start[3] = {0, cnt[1], cnt[2]};
     for (;;) {
-      h2d(s, dcnt, start, sizeof(start));
+      k_set_counters<<<1, 64, 0, s>>>(dcnt, start[0], start[1], start[2]);
       k_traverse<<<nblocks(nf), BLK, 0, s>>>(sc.front0.as<int32_t>(), nf, T, S, theta, sc.front1.as<int32_t>(), cap_f,
                                              sc.out_p2p.as<uint64_t>(), cap_p, sc.out_m2l.as<uint64_t>(), cap_m, dcnt);
       UVLM_LAUNCH_CHECK();
