@@ -197,6 +197,29 @@ def test_trajectory_vjp_finite_differences():
     assert rel(ax, fx) < 1e-7 and rel(aa, fa) < 1e-7
 
 
+@pytest.mark.parametrize("scheme", ["rk3", "rk4"])
+def test_trajectory_vjp_other_schemes(scheme):
+    """Checkpointed trajectories of RK3 and RK4 (every = 3 over 7 steps: a short last segment)
+    equal the composed step VJPs."""
+    n, dt, steps = 120, 0.02, 7
+    X, A, S = _blobs(n, seed=13)
+    A = A * 0.05
+    rng = np.random.default_rng(10)
+    cx, ca = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    g = FMMCalculator(P=6, theta=0.5, ncrit=256, scheme=scheme)
+    g.set_arrays(X, A, S)
+    g.trajectory_forward(steps, dt, every=3)
+    xb, ab = g.trajectory_vjp(cx, ca)
+    states = [(X, A)]
+    for _ in range(steps - 1):
+        states.append(_advance_orig(g, *states[-1], S, dt))
+    lx, la = cx, ca
+    for Xk, Ak in reversed(states):
+        g.set_arrays(Xk, Ak, S)
+        lx, la = g.step_vjp(dt, lx, la)
+    assert rel(xb, lx) < 1e-13 and rel(ab, la) < 1e-13
+
+
 def test_vjp_rejects_unsupported_configurations():
     X, A, S = _blobs(200)
     g = FMMCalculator(P=4, theta=0.5, ncrit=64, formulation="rvpm")

@@ -165,3 +165,60 @@ def test_cpp_wing_facade_runs_the_reference_loop(orc, tmp_path):
     hist, eb = vlm.run(V, [0.0, 1.0, 0.0], 0.05, steps, P=6, ncrit=256, scheme="rk2")
     assert rel(forces, np.array([h[1] for h in hist])) < 1e-10
     assert bodies.shape == eb.shape and rel(bodies[:, 9:15], eb[:, 9:15]) < 1e-10
+
+
+def test_wing_single_panel_and_empty_window(orc):
+    """Edge cases: one spanwise panel (both tip terms on the same shed particle) against the
+    oracle's solve and shedding, and a lift adjoint window started with no particles (no gradient
+    entries, CL as the loop gives it).  A single particle is not run through the oracle's FMM:
+    it sits on its own cell centre, where the reference's cart2sph divides by zero."""
+    from oracle import vlm
+    U, dt = np.array([0.0, 1.0, 0.0]), 0.05
+    V1 = reference_wing(npanels=1, chord=0.5)
+    w, ow = Wing(V1), vlm.Wing(V1)
+    g, F = w.solve(None, U)
+    eg = vlm.solve(ow, U, np.zeros((1, 3)))
+    assert rel(g, eg) < 1e-14 and rel(F, vlm.force(ow, eg, U, np.zeros((1, 3)))) < 1e-14
+    p = FMMCalculator(P=6, theta=0.5, ncrit=256, scheme="rk2")
+    w.shed(p, U, dt)
+    X, A, S = vlm.shed(ow, eg, U, dt)
+    arr = p.arrays()
+    assert p.n == 1 and rel(arr["X"], X) < 1e-15 and rel(arr["alpha"], A) < 1e-14 and rel(arr["sigma"], S) < 1e-15
+    # an even panel count: no particle lands on the root centre of the oracle's trees
+    V = reference_wing(npanels=4, chord=0.5)
+    hist, _ = vlm.run(V, U, dt, 4, P=6, ncrit=256, scheme="rk2")
+    w2 = Wing(V)
+    p2 = FMMCalculator(P=6, theta=0.5, ncrit=256, scheme="rk2")
+    CL, xb, ab = w2.lift_adjoint(p2, U, dt, 3)
+    area = w2.geometry()[2].sum()
+    assert xb.shape == (0, 3) and ab.shape == (0, 3) and p2.n == 12
+    assert abs(CL - hist[3][1][2] / (0.5 * area)) < 1e-11 * abs(CL)
+
+
+def test_lift_adjoint_mixed_matches_fp64():
+    """The lift adjoint with FP32 pair arithmetic against the FP64 one (same loop, one leaf)."""
+    V = reference_wing(npanels=6, chord=0.4)
+    V = np.stack([V[0], (V[0] + V[1]) / 2, V[1]])
+    U, dt = np.array([0.0, 1.0, 0.0]), 0.05
+    out = {}
+    for prec in ("fp64", "mixed"):
+        w = Wing(V)
+        p = FMMCalculator(P=6, theta=0.5, ncrit=256, scheme="rk2", precision=prec)
+        for _ in range(3):
+            w.step(p, U, dt)
+        out[prec] = w.lift_adjoint(p, U, dt, 4)
+    CLd, xd, ad = out["fp64"]
+    CLm, xm, am = out["mixed"]
+    print(f"[lift adjoint mixed] CL {abs(CLm - CLd) / abs(CLd):.2e} x {rel(xm, xd):.2e} a {rel(am, ad):.2e}", flush=True)
+    assert abs(CLm - CLd) < 1e-6 * abs(CLd) and rel(xm, xd) < 1e-5 and rel(am, ad) < 1e-5
+
+
+def test_wing_rejects_bad_inputs():
+    with pytest.raises(Exception):
+        Wing(np.zeros((2, 1, 3)))  # no panel
+    V = reference_wing(npanels=6)
+    w = Wing(V)
+    p = FMMCalculator(P=6, theta=0.5, ncrit=64, formulation="rvpm")
+    w.step(p, [0.0, 1.0, 0.0], 0.05)  # the forward loop runs with rVPM ...
+    with pytest.raises(Exception):
+        w.lift_adjoint(p, [0.0, 1.0, 0.0], 0.05, 2)  # ... its adjoint covers cVPM only
