@@ -16,12 +16,23 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "common.hpp"
 
 using namespace uvlm;
 
 namespace {
 thread_local std::string g_last_error;
+
+// roctx range with the label the reference gives the same work (Kokkos::Profiling regions of
+// fmmCalculator.hpp), so rocprofv3 --marker-trace shows the reference's phases.
+struct Region {
+  explicit Region(const char *name) { roctxRangePushA(name); }
+  ~Region() { roctxRangePop(); }
+  Region(const Region &)            = delete;
+  Region &operator=(const Region &) = delete;
+};
 
 enum Ev { EV_B0, EV_TREE, EV_LISTS, EV_E0, EV_UP, EV_M2L, EV_P2P, EV_DOWN, EV_COUNT };
 }  // namespace
@@ -133,6 +144,7 @@ void iota(hipStream_t s, int32_t *p, int64_t n)
 
 void do_build(uvlm_handle h)
 {
+  Region      region("Tree Construction on CPU");  // fmmCalculator.hpp:73
   if (h->N <= 0) throw Error(UVLM_ERR_STATE, "no bodies set");
   hipStream_t s = h->s;
   UVLM_HIP(hipEventRecord(h->ev[EV_B0], s));
@@ -162,6 +174,7 @@ void do_build(uvlm_handle h)
 
 void do_eval(uvlm_handle h, bool sync)
 {
+  Region      region("Calculation of FMM");  // fmmCalculator.hpp:422
   if (!h->built) throw Error(UVLM_ERR_STATE, "uvlm_eval before uvlm_build");
   hipStream_t s = h->s;
   const int   P = h->prm.P, NT = h->NT;
@@ -246,6 +259,7 @@ void copy_out_records(const std::vector<double> &rec, int64_t m, void *dst, int6
 // fmmCalculator.hpp:746-753, after the lists of createSensorTree 117-161).
 void probe_vorticity(uvlm_handle h, int64_t m)
 {
+  Region region("Calculation of Vorticity");  // fmmCalculator.hpp:735
   traverse(h->s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
   zero_fields(h->s, h->sst.as<double>(), h->scap, m, F_UOLD, 3);
   gauss_near(h->s, h->stree, h->tree, h->slists, h->sst.as<double>(), h->scap, h->st.as<double>(), h->cap,
@@ -268,6 +282,7 @@ void reserve_probes(uvlm_handle h, int64_t m)
 // against the particle tree, M2L from the particle multipoles, P2P velocity, L2L, L2P.
 void sensor_pass(uvlm_handle h, int64_t m)
 {
+  Region        region("Calculation of Sensors");  // fmmCalculator.hpp:471
   hipStream_t   s  = h->s;
   const int64_t sc = h->scap;
   double       *sx = h->sst.as<double>();
@@ -644,6 +659,7 @@ int uvlm_destroy(uvlm_handle h)
 
 int uvlm_set_bodies_aos(uvlm_handle h, const void *bodies, int64_t n, int64_t stride)
 {
+  Region region("copyBodies2Device");  // fmmCalculator.hpp:216
   return guarded([&] {
     check_handle(h);
     if (n <= 0 || !bodies) throw Error(UVLM_ERR_ARG, "empty or null bodies");
@@ -662,6 +678,7 @@ int uvlm_set_bodies_aos(uvlm_handle h, const void *bodies, int64_t n, int64_t st
 
 int uvlm_get_bodies_aos(uvlm_handle h, void *bodies, int64_t n, int64_t stride)
 {
+  Region region("copyBodies2Host");  // fmmCalculator.hpp:237
   return guarded([&] {
     check_handle(h);
     if (n != h->N || !bodies) throw Error(UVLM_ERR_ARG, "size mismatch");
@@ -740,6 +757,7 @@ int uvlm_get_permutation(uvlm_handle h, int64_t n, int64_t *perm)
 
 int uvlm_build(uvlm_handle h)
 {
+  Region region("Initialization of FMM");  // fmmCalculator.hpp:284
   return guarded([&] {
     check_handle(h);
     set_device(h);
