@@ -90,7 +90,6 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
                                             const int32_t *__restrict__ seg = nullptr, double *__restrict__ part = nullptr,
                                             int maxnt = 0)
 {
-  constexpr int LGMAX = BLK == 64 ? 6 : 8;
   __shared__ SrcF64 s64[MIXED ? 1 : BLK];
   __shared__ float4 s32[MIXED ? 3 * BLK : 1];
   __shared__ int    pref[SEG + 1], lb[SEG], ln[SEG];
@@ -99,17 +98,19 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
   if (w >= nleaf) return;
   const int t = leaves[w], tb = tbody[t], nt = tnbody[t];
   const int lane = threadIdx.x;
-  int       lg   = 0;
-  while ((1 << lg) < nt && lg < LGMAX) lg++;
-  const int    ntp = 1 << lg, S = BLK >> lg, ti = lane & (ntp - 1), si = lane >> lg;
+  // lanes = (target ti) x (source split si): ntp = the leaf's size (at most BLK), S = BLK / ntp
+  // splits; the BLK - ntp * S lanes left over take no sources
+  const int    ntp = max(1, min(nt, BLK)), S = BLK / ntp, ti = lane % ntp, si = lane / ntp;
+  const bool   lane_on = si < S;
   const double cxt = tcx[t], cyt = tcy[t], czt = tcz[t];
   const int    q0 = SPLIT ? seg[3 * blockIdx.x + 1] : off[t], q1 = SPLIT ? seg[3 * blockIdx.x + 2] : off[t + 1];
   for (int c0 = 0; c0 < nt; c0 += ntp) {
     const int  i     = c0 + ti;
-    const bool valid = i < nt;
+    const bool valid = lane_on && i < nt;
     const int64_t bi = tb + (valid ? i : 0);
     const double  xi = tst[(F_X + 0) * tcap + bi], yi = tst[(F_X + 1) * tcap + bi], zi = tst[(F_X + 2) * tcap + bi];
     const float   rxi = (float)(xi - cxt), ryi = (float)(yi - cyt), rzi = (float)(zi - czt);
+    const int     jfirst = lane_on ? si : BLK;  // lanes without a split skip the pair loops
     // target-sigma constants (TSIG only)
     const double  tsg = TSIG ? tst[F_SIG * tcap + bi] : 1.0;
     const double  tqq = 0.5 / (tsg * tsg), tsq = sqrt(tqq);
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
         if (MIXED) {
           float fa[15];
           for (int k = 0; k < 15; k++) fa[k] = 0.f;
-          for (int j = si; j < cnt; j += S) {
+          for (int j = jfirst; j < cnt; j += S) {
             const float4 A = s32[3 * j], B = s32[3 * j + 1], Cc = s32[3 * j + 2];
             const float  dx = rxi - A.x, dy = ryi - A.y, dz = rzi - A.z;
             const float  r2 = dx * dx + dy * dy + dz * dz;
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
           if (WITH_J) fold_antisym(fa);
           for (int k = 0; k < (WITH_J ? 12 : 3); k++) acc[k] += (double)fa[k];
         } else {
-          for (int j = si; j < cnt; j += S) {
+          for (int j = jfirst; j < cnt; j += S) {
             const SrcF64 &sj = s64[j];
             const double  dx = xi - sj.x, dy = yi - sj.y, dz = zi - sj.z;
             const double  R2 = dx * dx + dy * dy + dz * dz;
@@ -230,10 +231,14 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
         __syncthreads();
       }
     }
-    // combine the S source splits (threads ti + s*ntp) in a fixed order
+    // combine the S source splits (threads ti + q*ntp) in split order
     if constexpr (BLK == 64) {
-      for (int o = ntp; o < WG; o <<= 1)
-        for (int k = 0; k < (WITH_J ? 12 : 3); k++) acc[k] += __shfl_xor(acc[k], o, 64);
+      if (S > 1)
+        for (int k = 0; k < (WITH_J ? 12 : 3); k++) {
+          double v = acc[k];
+          for (int q = 1; q < S; q++) v += __shfl(acc[k], ti + q * ntp, 64);
+          acc[k] = v;
+        }
     } else if (S > 1) {
       __syncthreads();
       for (int k = 0; k < 12; k++) red[k * BLK + lane] = acc[k];
