@@ -113,14 +113,15 @@ __global__ __launch_bounds__(WG) void k_p2p_adj(const int32_t *__restrict__ leav
   const int64_t w = blockIdx.x;
   if (w >= nleaf) return;
   const int c = leaves[w], tb = cbody[c], nt = cnbody[c], lane = threadIdx.x;
-  int       lg = 0;
-  while ((1 << lg) < nt && lg < 6) lg++;
-  const int    ntp = 1 << lg, S = WG >> lg, ti = lane & (ntp - 1), si = lane >> lg;
+  // lanes = (own body ti) x (split si), ntp = the leaf's size (at most WG), S = WG / ntp splits
+  const int    ntp = max(1, min(nt, WG)), S = WG / ntp, ti = lane % ntp, si = lane / ntp;
+  const bool   lane_on = si < S;
   const double cxt = ccx[c], cyt = ccy[c], czt = ccz[c];
   const int    q0 = off[c], q1 = off[c + 1];
   for (int c0 = 0; c0 < nt; c0 += ntp) {
     const int     i     = c0 + ti;
-    const bool    valid = i < nt;
+    const bool    valid = lane_on && i < nt;
+    const int     jfirst = lane_on ? si : WG;  // lanes without a split skip the pair loops
     const int64_t bi    = tb + (valid ? i : 0);
     const double  xo[3] = {st[(F_X + 0) * cap + bi], st[(F_X + 1) * cap + bi], st[(F_X + 2) * cap + bi]};
     const float   ro[3] = {(float)(xo[0] - cxt), (float)(xo[1] - cyt), (float)(xo[2] - czt)};
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(WG) void k_p2p_adj(const int32_t *__restrict__ leav
         if (MIXED) {
           // SIDE 0: X, F1, V, A;  SIDE 1: X, JV, UQ, WA
           float p0[3] = {0, 0, 0}, p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0}, p3[3] = {0, 0, 0};
-          for (int j = si; j < cnt; j += S) {
+          for (int j = jfirst; j < cnt; j += S) {
             const float4 A0 = s32[NV * j], A1 = s32[NV * j + 1], A2 = s32[NV * j + 2];
             float        r[3];
             if (SIDE == 0) {
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(WG) void k_p2p_adj(const int32_t *__restrict__ leav
             }
           }
         } else {
-          for (int j = si; j < cnt; j += S) {
+          for (int j = jfirst; j < cnt; j += S) {
             double r[3], Wv[3], dS[3];
             if (SIDE == 0) {
               const double alj[3] = {s64[3 * WG + j], s64[4 * WG + j], s64[5 * WG + j]};
@@ -307,11 +308,16 @@ __global__ __launch_bounds__(WG) void k_p2p_adj(const int32_t *__restrict__ leav
         __syncthreads();
       }
     }
-    for (int o = ntp; o < WG; o <<= 1)
-      for (int d = 0; d < 3; d++) {
-        ax[d] += __shfl_xor(ax[d], o, 64);
-        if (SIDE == 1) aa[d] += __shfl_xor(aa[d], o, 64);
+    // combine the splits (lanes ti + q * ntp) in split order
+    for (int d = 0; d < 3; d++) {
+      double vx = ax[d], va = aa[d];
+      for (int q = 1; q < S; q++) {
+        vx += __shfl(ax[d], ti + q * ntp, 64);
+        if (SIDE == 1) va += __shfl(aa[d], ti + q * ntp, 64);
       }
+      ax[d] = vx;
+      aa[d] = va;
+    }
     if (si == 0 && valid) {
       for (int d = 0; d < 3; d++) aux[(A_XB + d) * cap + bi] += ax[d];
       if (SIDE == 1)
