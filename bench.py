@@ -265,7 +265,7 @@ def main():
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--ncrit", type=int, default=0,
                     help="leaf size; 0 (default): chosen by uvlm_autotune among --tune-ncrit on this workload")
-    ap.add_argument("--tune-ncrit", default="64,96,128,160,192,256", help="autotuner candidates")
+    ap.add_argument("--tune-ncrit", default="64,96,128,160,192,224,256", help="autotuner candidates")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--cpu-n", type=int, default=131072, help="particles in the CPU baseline sample (0: skip)")
     ap.add_argument("--err-targets", type=int, default=2048, help="sampled targets for the direct-sum error")
