@@ -340,13 +340,14 @@ __global__ void k_low32(int64_t N, const uint64_t *__restrict__ ck, int32_t *__r
   if (i < N) perm[i] = (int32_t)(uint32_t)(ck[i] & 0xffffffffull);
 }
 
+// One field per grid row (blockIdx.y), so the whole state moves with full memory parallelism.
 __global__ void k_gather_fields(const double *__restrict__ src, double *__restrict__ dst, int64_t scap, int64_t dcap,
-                                int nfields, const int32_t *__restrict__ perm, int64_t N)
+                                const int32_t *__restrict__ perm, int64_t N)
 {
   const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
   if (i >= N) return;
-  const int64_t p = perm[i];
-  for (int f = 0; f < nfields; f++) dst[f * dcap + i] = src[f * scap + p];
+  const int64_t f = blockIdx.y;
+  dst[f * dcap + i] = src[f * scap + perm[i]];
 }
 
 __global__ void k_gather_i32(const int32_t *__restrict__ src, int32_t *__restrict__ dst,
@@ -654,7 +655,8 @@ void build_tree(hipStream_t s, Scratch &sc, const double *x, const double *y, co
 void gather_fields(hipStream_t s, const double *src, double *dst, int64_t src_cap, int64_t dst_cap, int nfields,
                    const int32_t *perm, int64_t N)
 {
-  k_gather_fields<<<nblocks(N), BLK, 0, s>>>(src, dst, src_cap, dst_cap, nfields, perm, N);
+  if (N <= 0 || nfields <= 0) return;
+  k_gather_fields<<<dim3(nblocks(N), nfields), BLK, 0, s>>>(src, dst, src_cap, dst_cap, perm, N);
   UVLM_LAUNCH_CHECK();
 }
 
