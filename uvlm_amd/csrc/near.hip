@@ -79,7 +79,7 @@ __device__ __forceinline__ void fold_antisym(float *acc)
 // writes its un-negated sums to part + b * 12 * maxnt (field-major, maxnt entries per field);
 // k_p2p_reduce adds the segments of each target in order.
 template <bool MIXED, bool WITH_J, bool TSIG, int BLK, bool SPLIT = false>
-__global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves, int64_t nleaf,
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MIXED ? 4 : 1))) void k_p2p(const int32_t *__restrict__ leaves, int64_t nleaf,
                                             const int32_t *__restrict__ tbody, const int32_t *__restrict__ tnbody,
                                             const double *__restrict__ tcx, const double *__restrict__ tcy,
                                             const double *__restrict__ tcz, const int32_t *__restrict__ off,
@@ -90,10 +90,15 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
                                             const int32_t *__restrict__ seg = nullptr, double *__restrict__ part = nullptr,
                                             int maxnt = 0)
 {
-  __shared__ SrcF64 s64[MIXED ? 1 : BLK];
-  __shared__ float4 s32[MIXED ? 3 * BLK : 1];
-  __shared__ int    pref[SEG + 1], lb[SEG], ln[SEG];
-  __shared__ double red[BLK == 64 ? 1 : 12 * BLK];
+  // One LDS buffer holds the staged sources during the pair loops and the split partial sums
+  // after them: mixed sources take two float4 per entry for 2 BLK entries (BLK staged, the rest
+  // zero padding), FP64 sources one SrcF64 per entry, the partials 12 doubles per lane.
+  constexpr int SMEM = BLK == 64 ? 64 * BLK : 96 * BLK;
+  __shared__ __align__(16) unsigned char smem[SMEM];
+  __shared__ int pref[SEG + 1], lb[SEG], ln[SEG];
+  float4 *s32 = reinterpret_cast<float4 *>(smem);
+  SrcF64 *s64 = reinterpret_cast<SrcF64 *>(smem);
+  double *red = reinterpret_cast<double *>(smem);
   const int64_t w = SPLIT ? seg[3 * blockIdx.x] : blockIdx.x;
   if (w >= nleaf) return;
   const int t = leaves[w], tb = tbody[t], nt = tnbody[t];
@@ -110,7 +115,7 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
     const int64_t bi = tb + (valid ? i : 0);
     const double  xi = tst[(F_X + 0) * tcap + bi], yi = tst[(F_X + 1) * tcap + bi], zi = tst[(F_X + 2) * tcap + bi];
     const float   rxi = (float)(xi - cxt), ryi = (float)(yi - cyt), rzi = (float)(zi - czt);
-    const int     jfirst = lane_on ? si : BLK;  // lanes without a split skip the pair loops
+    const int     jfirst = lane_on ? si : BLK;  // lanes without a split skip the FP64 pair loop
     // target-sigma constants (TSIG only)
     const double  tsg = TSIG ? tst[F_SIG * tcap + bi] : 1.0;
     const double  tqq = 0.5 / (tsg * tsg), tsq = sqrt(tqq);
@@ -145,52 +150,51 @@ __global__ __launch_bounds__(BLK) void k_p2p(const int32_t *__restrict__ leaves,
           const double  sg = sst[F_SIG * scap + bj];
           const double  ax = sst[(F_A + 0) * scap + bj], ay = sst[(F_A + 1) * scap + bj], az = sst[(F_A + 2) * scap + bj];
           if (MIXED) {
-            const double qq = 0.5 / (sg * sg), sq = sqrt(qq);
-            s32[3 * lane + 0] = make_float4((float)(x - cxt), (float)(y - cyt), (float)(z - czt), (float)qq);
-            s32[3 * lane + 1] = make_float4((float)ax, (float)ay, (float)az, (float)(qq * sq * INV_4PI));
-            s32[3 * lane + 2] = make_float4((float)(2.0 * qq * qq * sq * INV_4PI), 0.f, 0.f, 0.f);
+            // (offset, q) and (alpha c1, 2 q) with q = 1 / (2 sigma^2), c1 = q^{3/2} / (4 pi): then
+            // f1 = G(x) and f2 = 2 q G'(x) for the scaled strength.  TSIG keeps alpha as is.
+            const double qq = 0.5 / (sg * sg), c1 = TSIG ? 1.0 : qq * sqrt(qq) * INV_4PI;
+            s32[2 * lane + 0] = make_float4((float)(x - cxt), (float)(y - cyt), (float)(z - czt), (float)qq);
+            s32[2 * lane + 1] = make_float4((float)(ax * c1), (float)(ay * c1), (float)(az * c1), (float)(2.0 * qq));
           } else {
             const double is2 = 1.0 / (sg * sg);
             s64[lane]        = SrcF64{x, y, z, ax, ay, az, is2, 0.5 * is2};
           }
         }
+        if (MIXED) {
+          // zero entries (no strength) past the chunk: every lane runs the same trip count
+          const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (lane >= cnt) s32[2 * lane] = s32[2 * lane + 1] = zero;
+          s32[2 * (BLK + lane)] = s32[2 * (BLK + lane) + 1] = zero;
+        }
         __syncthreads();
         if (MIXED) {
-          float fa[15];
+          // lane (ti, si) takes entries si, si + S, ... two per step; T2 * S <= 2 BLK entries
+          // are read (S <= BLK), those from cnt on are the zero padding
+          float        fa[15];
+          const int    T2 = 2 * ((cnt + 2 * S - 1) / (2 * S));
+          const float4 *ps = s32 + 2 * (lane_on ? si : 0);
           for (int k = 0; k < 15; k++) fa[k] = 0.f;
-          for (int j = jfirst; j < cnt; j += S) {
-            const float4 A = s32[3 * j], B = s32[3 * j + 1], Cc = s32[3 * j + 2];
-            const float  dx = rxi - A.x, dy = ryi - A.y, dz = rzi - A.z;
-            const float  r2 = dx * dx + dy * dy + dz * dz;
-            const float  xq = r2 * (TSIG ? t_q : A.w);
-            float        G, Gp;  // G(x) and G'(x), x = xq
-            if (xq < XPOLY) {
-              const float tq = xq - 1.0f;
-              float       g = c_gpoly[NPOLY - 1], gp = 0.f;
-#pragma unroll
-              for (int k = NPOLY - 2; k >= 0; k--) {
-                gp = fmaf(gp, tq, g);
-                g  = fmaf(g, tq, c_gpoly[k]);
-              }
-              G  = g;
-              Gp = gp;
-            } else {
-              const float w = __builtin_amdgcn_rsqf(xq);
-              const float e = __builtin_amdgcn_exp2f(xq * -1.4426950408889634f);
-              const float t = fmaxf(w, FAR_WMIN) - FAR_WC;
-              float       p = c_far[NFAR - 1];
-#pragma unroll
-              for (int k = NFAR - 2; k >= 0; k--) p = fmaf(p, t, c_far[k]);
-              const float g  = fmaf(-e, fmaf(1.1283791670955126f, xq * w, p), 1.0f);
-              const float w2 = w * w;
-              G              = g * w2 * w;
-              Gp             = w2 * fmaf(1.1283791670955126f, e, -1.5f * G);
+          for (int k = 0; k < T2; k += 2, ps += 4 * S) {
+            const float4 A0 = ps[0], B0 = ps[1], A1 = ps[2 * S], B1 = ps[2 * S + 1];
+            const float  dx0 = rxi - A0.x, dy0 = ryi - A0.y, dz0 = rzi - A0.z;
+            const float  dx1 = rxi - A1.x, dy1 = ryi - A1.y, dz1 = rzi - A1.z;
+            const float  r20 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0, r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+            const float  xq0 = r20 * (TSIG ? t_q : A0.w), xq1 = r21 * (TSIG ? t_q : A1.w);
+            float        G0, Gp0, G1, Gp1;  // G(x) and G'(x) of both pairs
+            g_near(xq0, xq1, G0, Gp0, G1, Gp1);
+            if (fmaxf(xq0, xq1) >= XPOLY) {  // lanes with a far pair take both far forms
+              float Gf0, Gpf0, Gf1, Gpf1;
+              g_far(xq0, Gf0, Gpf0);
+              g_far(xq1, Gf1, Gpf1);
+              if (xq0 >= XPOLY) G0 = Gf0, Gp0 = Gpf0;
+              if (xq1 >= XPOLY) G1 = Gf1, Gp1 = Gpf1;
             }
             // coincident pairs are skipped (multipoleKernels.hpp:242): only the f1 alpha terms
             // of J survive a zero dX, so zeroing f1 is enough
-            const float f1 = r2 == 0.0f ? 0.0f : G * (TSIG ? t_c1 : B.w);
-            const float f2 = Gp * (TSIG ? t_c2 : Cc.x);
-            accumulate_pair(fa, dx, dy, dz, B.x, B.y, B.z, f1, f2, WITH_J);
+            const float f10 = r20 == 0.0f ? 0.0f : (TSIG ? G0 * t_c1 : G0);
+            const float f11 = r21 == 0.0f ? 0.0f : (TSIG ? G1 * t_c1 : G1);
+            accumulate_pair(fa, dx0, dy0, dz0, B0.x, B0.y, B0.z, f10, Gp0 * (TSIG ? t_c2 : B0.w), WITH_J);
+            accumulate_pair(fa, dx1, dy1, dz1, B1.x, B1.y, B1.z, f11, Gp1 * (TSIG ? t_c2 : B1.w), WITH_J);
           }
           if (WITH_J) fold_antisym(fa);
           for (int k = 0; k < (WITH_J ? 12 : 3); k++) acc[k] += (double)fa[k];

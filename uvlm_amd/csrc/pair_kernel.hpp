@@ -74,6 +74,36 @@ __device__ __forceinline__ void kernel_g(float xq, float &G, float &Gp, float &G
   }
 }
 
+// G, G' at two points x0, x1 < XPOLY by the Chebyshev interpolant (both chains interleaved).
+__device__ __forceinline__ void g_near(float x0, float x1, float &G0, float &Gp0, float &G1, float &Gp1)
+{
+  const float t0 = x0 - 1.0f, t1 = x1 - 1.0f;
+  float       g0 = c_gpoly[NPOLY - 1], gp0 = 0.f, g1 = g0, gp1 = 0.f;
+#pragma unroll
+  for (int k = NPOLY - 2; k >= 0; k--) {
+    gp0 = fmaf(gp0, t0, g0);
+    gp1 = fmaf(gp1, t1, g1);
+    g0  = fmaf(g0, t0, c_gpoly[k]);
+    g1  = fmaf(g1, t1, c_gpoly[k]);
+  }
+  G0 = g0, Gp0 = gp0, G1 = g1, Gp1 = gp1;
+}
+
+// G, G' at x >= XPOLY from rsq/exp2 and the erfcx polynomial (the x >= XPOLY branch of kernel_g).
+__device__ __forceinline__ void g_far(float xq, float &G, float &Gp)
+{
+  const float w = __builtin_amdgcn_rsqf(xq);
+  const float e = __builtin_amdgcn_exp2f(xq * -1.4426950408889634f);
+  const float t = fmaxf(w, FAR_WMIN) - FAR_WC;
+  float       p = c_far[NFAR - 1];
+#pragma unroll
+  for (int k = NFAR - 2; k >= 0; k--) p = fmaf(p, t, c_far[k]);
+  const float g  = fmaf(-e, fmaf(1.1283791670955126f, xq * w, p), 1.0f);
+  const float w2 = w * w;
+  G              = g * w2 * w;
+  Gp             = w2 * fmaf(1.1283791670955126f, e, -1.5f * G);
+}
+
 constexpr int SEG = 256;  // source leaves whose prefix sums sit in LDS at once
 
 // Exclusive prefix sum of v[0..n) (n <= SEG) into pref[0..n]; one wavefront, 4 per lane.
