@@ -114,6 +114,35 @@ def test_p2p_polynomial_matches_erf_form():
         assert abs(ppp - _g_series(x, 2)) < 1e-7
 
 
+def test_p2p_degree8_polynomial_matches_erf_form():
+    """The forward P2P's degree-8 polynomial for G and G' (c_gpoly8 in pair_kernel.hpp), evaluated
+    as the kernel does (Horner in FP32, chains started after their first step), against the erf
+    form in FP64 and (small x, where the erf form cancels) G's power series."""
+    import math
+    src = open(os.path.join(ROOT, "uvlm_amd", "csrc", "pair_kernel.hpp")).read()
+    key = "c_gpoly8[NPOLY8] = {"
+    body = src[src.index(key) + len(key): src.index("};", src.index(key))]
+    coef = np.array([float(v.strip().rstrip("f")) for v in body.split(",")], dtype=np.float32)
+    assert len(coef) == 9
+    f32 = np.float32
+    for x in np.linspace(0.0, 2.0, 201):
+        t = f32(f32(x) - f32(1.0))
+        gp = coef[8]
+        g = f32(gp * t + coef[7])
+        for k in range(6, -1, -1):
+            gp = f32(gp * t + g)
+            g = f32(g * t + coef[k])
+        if x < 0.05:
+            G, Gp = _g_series(x, 0), _g_series(x, 1)
+        else:
+            s = math.sqrt(x)
+            gr = math.erf(s) - 2 * s * math.exp(-x) / math.sqrt(math.pi)
+            G = gr / s ** 3
+            Gp = (s * 4 * x * math.exp(-x) / math.sqrt(math.pi) - 3 * gr) / (2 * s ** 5)
+        assert abs(float(g) - G) < 2e-7 * abs(G)
+        assert abs(float(gp) - Gp) < 1.2e-7
+
+
 def test_p2p_far_form_matches_erf_form():
     """The FP32 P2P's form for x = s^2 >= 2, read from near.hip: G = (1 - e (erfcx(s) +
     2s/sqrt(pi))) w^3 and G' = w^2 (2e/sqrt(pi) - 1.5 G) with w = 1/s, e = e^{-x} and erfcx by

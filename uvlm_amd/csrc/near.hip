@@ -35,9 +35,11 @@ struct SrcF64 {
   double x, y, z, ax, ay, az, invS2, halfInvS2;
 };
 
-// acc[0..2] += f1 c, acc[3..11] += f2 c (x) dX with c = dX x a, and acc[12..14] += f1 a: the
+// acc[0..2] += f1 c, acc[3..10] += f2 c (x) dX with c = dX x a, and acc[12..14] += f1 a: the
 // f1 eps_{abk} a_k part of J is antisymmetric, so its sum is kept as one vector and folded
-// into J once per chunk (fold_antisym) instead of six multiply-adds per pair.
+// into J once per chunk (fold_antisym) instead of six multiply-adds per pair.  c is
+// perpendicular to dX, so c (x) dX is traceless and its zz entry (acc[11]) is formed from the
+// xx and yy sums in fold_antisym.
 __device__ __forceinline__ void accumulate_pair(float *acc, float dx, float dy, float dz, float ax, float ay, float az,
                                                 float f1, float f2, bool withJ)
 {
@@ -55,15 +57,16 @@ __device__ __forceinline__ void accumulate_pair(float *acc, float dx, float dy, 
     acc[8]  = fmaf(k1, dz, acc[8]);
     acc[9]  = fmaf(k2, dx, acc[9]);
     acc[10] = fmaf(k2, dy, acc[10]);
-    acc[11] = fmaf(k2, dz, acc[11]);
     acc[12] = fmaf(f1, ax, acc[12]);
     acc[13] = fmaf(f1, ay, acc[13]);
     acc[14] = fmaf(f1, az, acc[14]);
   }
 }
-// J[a][b] += eps_{abk} F_k with F = acc[12..14] (row-major J in acc[3..11]).
+// J[2][2] = -(J[0][0] + J[1][1]) and J[a][b] += eps_{abk} F_k with F = acc[12..14] (row-major J
+// in acc[3..11]).
 __device__ __forceinline__ void fold_antisym(float *acc)
 {
+  acc[11] = -(acc[3] + acc[7]);
   acc[4] += acc[14];
   acc[5] -= acc[13];
   acc[6] -= acc[14];
@@ -179,15 +182,16 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MIXED ? 4 :
             const float  dx0 = rxi - A0.x, dy0 = ryi - A0.y, dz0 = rzi - A0.z;
             const float  dx1 = rxi - A1.x, dy1 = ryi - A1.y, dz1 = rzi - A1.z;
             const float  r20 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0, r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-            const float  xq0 = r20 * (TSIG ? t_q : A0.w), xq1 = r21 * (TSIG ? t_q : A1.w);
+            const float  q0 = TSIG ? t_q : A0.w, q1 = TSIG ? t_q : A1.w;
+            const float  tq0 = fmaf(r20, q0, -1.0f), tq1 = fmaf(r21, q1, -1.0f);  // x - 1
             float        G0, Gp0, G1, Gp1;  // G(x) and G'(x) of both pairs
-            g_near(xq0, xq1, G0, Gp0, G1, Gp1);
-            if (fmaxf(xq0, xq1) >= XPOLY) {  // lanes with a far pair take both far forms
+            g_near(tq0, tq1, G0, Gp0, G1, Gp1);
+            if (fmaxf(tq0, tq1) >= XPOLY - 1.0f) {  // lanes with a far pair take both far forms
               float Gf0, Gpf0, Gf1, Gpf1;
-              g_far(xq0, Gf0, Gpf0);
-              g_far(xq1, Gf1, Gpf1);
-              if (xq0 >= XPOLY) G0 = Gf0, Gp0 = Gpf0;
-              if (xq1 >= XPOLY) G1 = Gf1, Gp1 = Gpf1;
+              g_far(r20 * q0, Gf0, Gpf0);
+              g_far(r21 * q1, Gf1, Gpf1);
+              if (tq0 >= XPOLY - 1.0f) G0 = Gf0, Gp0 = Gpf0;
+              if (tq1 >= XPOLY - 1.0f) G1 = Gf1, Gp1 = Gpf1;
             }
             // coincident pairs are skipped (multipoleKernels.hpp:242): only the f1 alpha terms
             // of J survive a zero dX, so zeroing f1 is enough

@@ -9,7 +9,8 @@
 //   f3 = 2 df2/dR^2       = 4 G''(x) q^{7/2} / (4 pi),      q = 1 / (2 sigma^2),
 // because 2 s^2 t2 - 3 g = s g'(s) - 3 g = 2 s^5 G'(x), and
 //   G(x) = (2/sqrt(pi)) sum_{i>=0} (-1)^i 2(i+1) / ((2i+3) (i+1)!) x^i.
-// For x < 2, G is approximated on [0, 2] by its degree-9 Chebyshev interpolant (interpolation
+// For x < 2, G is approximated on [0, 2] by its degree-9 Chebyshev interpolant (kernel_g, which
+// also needs G''; the forward kernels use a degree-8 one, c_gpoly8 below) (interpolation
 // at the Chebyshev nodes of [0, 2], re-expanded in powers of t = x - 1), with no cancellation
 // between erf and t2: max error 8e-11 (G), 5e-9 (G') and 7.6e-8 (G'', |G''| <= 0.33) in exact
 // arithmetic, 1.4e-7 and 1.1e-7 (G, G') when evaluated by Horner in FP32.
@@ -35,6 +36,14 @@ __constant__ float c_gpoly[NPOLY] = {4.27593295550599495e-01f,  -2.2628244588289
                                      -1.86646103035026413e-02f, 3.70154966273965411e-03f,  -6.12467145832412121e-04f,
                                      8.69427232794296300e-05f,  -1.08135023163257920e-05f, 1.22398144588401693e-06f,
                                      -1.21730778572683114e-07f};
+
+// The forward near-field kernels need G and G' only: the degree-8 interpolant at the Chebyshev
+// nodes of [0, 2] (coefficients of t^i, t = x - 1) is accurate to 5e-10 (G) and 4e-8 (G') in
+// exact arithmetic and to 7e-8 and 8e-8 when evaluated by Horner in FP32, as the degree-9 one.
+constexpr int   NPOLY8             = 9;
+__constant__ float c_gpoly8[NPOLY8] = {4.27593295529120287e-01f,  -2.26282441609536422e-01f, 7.52993082456607232e-02f,
+                                       -1.86646673148319273e-02f, 3.70154622223736220e-03f,  -6.12261825120658876e-04f,
+                                       8.69482296212105497e-05f,  -1.10873394265394848e-05f, 1.22122772547046681e-06f};
 
 // erfcx(1/w) ~ sum_k c_far[k] (w - FAR_WC)^k, w in [FAR_WMIN, 1/sqrt(2)].
 constexpr int   NFAR     = 8;
@@ -74,17 +83,18 @@ __device__ __forceinline__ void kernel_g(float xq, float &G, float &Gp, float &G
   }
 }
 
-// G, G' at two points x0, x1 < XPOLY by the Chebyshev interpolant (both chains interleaved).
-__device__ __forceinline__ void g_near(float x0, float x1, float &G0, float &Gp0, float &G1, float &Gp1)
+// G, G' at t0 = x0 - 1 and t1 = x1 - 1 (x < XPOLY) by the degree-8 interpolant, both Horner
+// chains interleaved; the chains start after their first step (G' = c8, G = c8 t + c7).
+__device__ __forceinline__ void g_near(float t0, float t1, float &G0, float &Gp0, float &G1, float &Gp1)
 {
-  const float t0 = x0 - 1.0f, t1 = x1 - 1.0f;
-  float       g0 = c_gpoly[NPOLY - 1], gp0 = 0.f, g1 = g0, gp1 = 0.f;
+  float gp0 = c_gpoly8[NPOLY8 - 1], gp1 = gp0;
+  float g0 = fmaf(gp0, t0, c_gpoly8[NPOLY8 - 2]), g1 = fmaf(gp1, t1, c_gpoly8[NPOLY8 - 2]);
 #pragma unroll
-  for (int k = NPOLY - 2; k >= 0; k--) {
+  for (int k = NPOLY8 - 3; k >= 0; k--) {
     gp0 = fmaf(gp0, t0, g0);
     gp1 = fmaf(gp1, t1, g1);
-    g0  = fmaf(g0, t0, c_gpoly[k]);
-    g1  = fmaf(g1, t1, c_gpoly[k]);
+    g0  = fmaf(g0, t0, c_gpoly8[k]);
+    g1  = fmaf(g1, t1, c_gpoly8[k]);
   }
   G0 = g0, Gp0 = gp0, G1 = g1, Gp1 = gp1;
 }
