@@ -82,7 +82,7 @@ __device__ __forceinline__ void fold_antisym(float *acc)
 // writes its un-negated sums to part + b * 12 * maxnt (field-major, maxnt entries per field);
 // k_p2p_reduce adds the segments of each target in order.
 template <bool MIXED, bool WITH_J, bool TSIG, int BLK, bool SPLIT = false>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MIXED ? 4 : 1))) void k_p2p(const int32_t *__restrict__ leaves, int64_t nleaf,
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 1))) void k_p2p(const int32_t *__restrict__ leaves, int64_t nleaf,
                                             const int32_t *__restrict__ tbody, const int32_t *__restrict__ tnbody,
                                             const double *__restrict__ tcx, const double *__restrict__ tcy,
                                             const double *__restrict__ tcz, const int32_t *__restrict__ off,
