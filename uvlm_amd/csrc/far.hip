@@ -60,26 +60,29 @@ __device__ __forceinline__ int     tri(int n) { return n * (n + 1) / 2; }
 // Spherical angles of dX as the quantities the harmonics need: rho, x = cos(a), y = sin(a),
 // e^{ib}.  cos/sin of the polar angle and of the azimuth come from the Cartesian
 // components; at rho == 0 the reference's cart2sph gives a = 0, b = atan2(0,0) = 0.
+// rho, 1/rho, cos/sin of the polar angle (x, y) and of the azimuth (cb, sb) of (dx, dy, dz),
+// from two reciprocal square roots and products (no divisions).
 struct Sph {
-  double rho, x, y, cb, sb;
+  double rho, inv, x, y, cb, sb;
 };
 __device__ __forceinline__ Sph sph(double dx, double dy, double dz)
 {
   Sph          s;
-  const double rxy2 = dx * dx + dy * dy;
-  s.rho             = sqrt(rxy2 + dz * dz);
-  if (s.rho == 0.0) {
-    s.x = 1.0, s.y = 0.0, s.cb = 1.0, s.sb = 0.0;
+  const double rxy2 = dx * dx + dy * dy, r2 = rxy2 + dz * dz;
+  if (r2 == 0.0) {
+    s.rho = 0.0, s.inv = 0.0, s.x = 1.0, s.y = 0.0, s.cb = 1.0, s.sb = 0.0;
     return s;
   }
-  const double rxy = sqrt(rxy2);
-  s.x              = dz / s.rho;
-  s.y              = rxy / s.rho;
-  if (rxy == 0.0) {
-    s.cb = 1.0, s.sb = 0.0;
+  s.inv            = rsqrt(r2);
+  s.rho            = r2 * s.inv;
+  s.x              = dz * s.inv;
+  if (rxy2 == 0.0) {
+    s.y = 0.0, s.cb = 1.0, s.sb = 0.0;
   } else {
-    s.cb = dx / rxy;
-    s.sb = dy / rxy;
+    const double ixy = rsqrt(rxy2);
+    s.y              = rxy2 * ixy * s.inv;
+    s.cb             = dx * ixy;
+    s.sb             = dy * ixy;
   }
   return s;
 }
@@ -120,7 +123,7 @@ __device__ void harmonic_column(int P, int m, const Sph &s, double2 *Y)
   for (int i = 1; i <= m; i++) pmm *= -(double)(2 * i - 1) * s.y;
   // scale at n = m
   double       f;
-  const double inv = SINGULAR ? 1.0 / s.rho : 0.0;
+  const double inv = SINGULAR ? s.inv : 0.0;
   if (SINGULAR) {
     f = inv;
     for (int i = 0; i < m; i++) f *= -inv;  // (-1)^m / rho^{m+1}
@@ -153,7 +156,7 @@ __device__ void singular_column_ext(int P, int m, const Sph &s, double2 *Yx)
   for (int i = 0; i < m; i++) e = cmul(e, make_double2(s.cb, s.sb));
   double pmm = 1.0;
   for (int i = 1; i <= m; i++) pmm *= -(double)(2 * i - 1) * s.y;
-  const double inv = 1.0 / s.rho;
+  const double inv = s.inv;
   double       f   = inv;
   for (int i = 0; i < m; i++) f *= -inv;
   const double sm     = (m & 1) ? -1.0 : 1.0;
@@ -626,17 +629,26 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
     for (int d = 0; d < 3; d++) acc[s][d] = make_double2(0, 0);
   const int    s0 = SPLIT ? seg[3 * u + 1] : off[t], nsrc = (SPLIT ? seg[3 * u + 2] : off[t + 1]) - s0;
   const double X = tx[t], Yc = ty[t], Z = tz[t];
+  // the next batch's source centre and the index of the one after it are loaded a batch ahead,
+  // so their latency overlaps the current batch's multiply-adds
+  int    s_cur = g < nsrc ? src[s0 + g] : 0, s_nxt = g + 4 < nsrc ? src[s0 + g + 4] : 0;
+  double cx_cur = 0.0, cy_cur = 0.0, cz_cur = 0.0;
+  if (g < nsrc) cx_cur = sx[s_cur], cy_cur = sy[s_cur], cz_cur = sz[s_cur];
   for (int base = 0; base < nsrc; base += 4) {
     const int  si     = base + g;
     const bool active = si < nsrc;
     if (active) {
-      const int      s  = src[s0 + si];
-      const double2 *Ms = M + (int64_t)s * NT;
+      const double2 *Ms = M + (int64_t)s_cur * NT;
       for (int q = sl; q < NT; q += 16) Mg[q] = Ms[q];
       if (sl < P) {
-        const Sph sp = sph(X - sx[s], Yc - sy[s], Z - sz[s]);
+        const Sph sp = sph(X - cx_cur, Yc - cy_cur, Z - cz_cur);
         singular_column_ext(P, sl, sp, Yx);
       }
+    }
+    if (si + 4 < nsrc) {
+      s_cur  = s_nxt;
+      cx_cur = sx[s_cur], cy_cur = sy[s_cur], cz_cur = sz[s_cur];
+      if (si + 8 < nsrc) s_nxt = src[s0 + si + 8];
     }
     __syncthreads();
     if (active) {
