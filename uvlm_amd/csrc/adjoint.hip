@@ -114,7 +114,7 @@ __global__ __launch_bounds__(WG) void k_p2p_adj(const int32_t *__restrict__ leav
   if (w >= nleaf) return;
   const int c = leaves[w], tb = cbody[c], nt = cnbody[c], lane = threadIdx.x;
   // lanes = (own body ti) x (split si), ntp = the leaf's size (at most WG), S = WG / ntp splits
-  const int    ntp = max(1, min(nt, WG)), S = WG / ntp, ti = lane % ntp, si = lane / ntp;
+  const int    ntp = near_targets_per_pass<WG>(nt), S = WG / ntp, ti = lane % ntp, si = lane / ntp;
   const bool   lane_on = si < S;
   const double cxt = ccx[c], cyt = ccy[c], czt = ccz[c];
   const int    q0 = off[c], q1 = off[c + 1];

@@ -106,9 +106,9 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MIXED ? 5 :
   if (w >= nleaf) return;
   const int t = leaves[w], tb = tbody[t], nt = tnbody[t];
   const int lane = threadIdx.x;
-  // lanes = (target ti) x (source split si): ntp = the leaf's size (at most BLK), S = BLK / ntp
-  // splits; the BLK - ntp * S lanes left over take no sources
-  const int    ntp = max(1, min(nt, BLK)), S = BLK / ntp, ti = lane % ntp, si = lane / ntp;
+  // lanes = (target ti) x (source split si): ntp targets per pass (near_targets_per_pass), S = BLK /
+  // ntp splits; the BLK - ntp * S lanes left over take no sources
+  const int    ntp = near_targets_per_pass<BLK>(nt), S = BLK / ntp, ti = lane % ntp, si = lane / ntp;
   const bool   lane_on = si < S;
   const double cxt = tcx[t], cyt = tcy[t], czt = tcz[t];
   const int    q0 = SPLIT ? seg[3 * blockIdx.x + 1] : off[t], q1 = SPLIT ? seg[3 * blockIdx.x + 2] : off[t + 1];

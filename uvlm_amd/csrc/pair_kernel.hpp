@@ -114,6 +114,27 @@ __device__ __forceinline__ void g_far(float xq, float &G, float &Gp)
   Gp             = w2 * fmaf(1.1283791670955126f, e, -1.5f * G);
 }
 
+// Targets per pass of a near-field workgroup of BLK lanes over a leaf of nt targets: lanes are
+// (target, source split) with S = BLK / ntp splits, and the leaf takes ceil(nt / ntp) passes over
+// its source list.  With BLK = 256 a pass costs about 1/S of the list per lane plus the staging
+// (about 1/128), so ntp = ceil(nt / np) with np in 1..8 minimising np (1/S + 1/128); e.g. nt = 200
+// takes 4 passes of 50 targets and 5 splits (98% of the lanes busy) instead of one pass with 56
+// idle lanes.  With BLK = 64 a 64-source chunk is too short to pay for its staging again: one pass.
+template <int BLK>
+__device__ __forceinline__ int near_targets_per_pass(int nt)
+{
+  int best = max(1, min(nt, BLK));
+  if (BLK <= 64) return best;
+  float cost = 3.0e38f;
+  for (int np = 1; np <= 8; np++) {  // np = 1 gives best, with cost 1/S + 1/128
+    const int ntp = max(1, (nt + np - 1) / np);
+    if (ntp > BLK) continue;
+    const float c = np * (1.0f / (float)(BLK / ntp) + 1.0f / 128.0f);
+    if (c < cost) cost = c, best = ntp;
+  }
+  return best;
+}
+
 constexpr int SEG = 256;  // source leaves whose prefix sums sit in LDS at once
 
 // Exclusive prefix sum of v[0..n) (n <= SEG) into pref[0..n]; one wavefront, 4 per lane.
