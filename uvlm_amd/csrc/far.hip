@@ -239,57 +239,82 @@ __global__ __launch_bounds__(WG) void k_p2m(int P, int NT, const int32_t *__rest
 }
 
 // ------------------------------------------------------------------ M2M
-__global__ __launch_bounds__(WG) void k_m2m(int P, int NT, const int32_t *__restrict__ cells, int64_t ncell,
-                                            const int32_t *__restrict__ cnchild, const int32_t *__restrict__ cchild,
-                                            const double *__restrict__ cx, const double *__restrict__ cy,
-                                            const double *__restrict__ cz, double2 *__restrict__ M)
+// One workgroup of nwv wavefronts per parent cell (nwv = 8, or 4 when 8 LDS slices would pass
+// 64 KB): wavefront c translates children c, c + nwv, ... (their harmonics and multipoles in its
+// own LDS slice, lane <-> output coefficient (n,m)), so the children are translated side by side
+// instead of one after another; the wavefronts' sums are then added in wavefront order through LDS.
+constexpr int M2M_MAXWV = 8;
+__global__ __launch_bounds__(M2M_MAXWV * WG) void k_m2m(int P, int NT, const int32_t *__restrict__ cells,
+                                                        int64_t ncell, const int32_t *__restrict__ cnchild,
+                                                        const int32_t *__restrict__ cchild,
+                                                        const double *__restrict__ cx, const double *__restrict__ cy,
+                                                        const double *__restrict__ cz, double2 *__restrict__ M)
 {
   extern __shared__ double2 sm2[];
-  double2      *Y  = sm2;             // tri(P)
-  double2      *Mc = sm2 + tri(P);    // NT
-  const int64_t w  = blockIdx.x;
+  const int64_t w = blockIdx.x;
   if (w >= ncell) return;
   const int p = cells[w], nc = cnchild[p];
   if (nc == 0) return;
-  const int lane = threadIdx.x, nout = tri(P);
+  const int nwv = blockDim.x / WG, wv = threadIdx.x / WG, lane = threadIdx.x % WG, nout = tri(P);
+  double2  *Y  = sm2 + wv * (tri(P) + NT);  // tri(P)
+  double2  *Mc = Y + tri(P);                // NT
   double2   acc[QPL][3];
   for (int a = 0; a < QPL; a++)
     for (int d = 0; d < 3; d++) acc[a][d] = make_double2(0, 0);
-  for (int ci = 0; ci < nc; ci++) {
-    const int ch = cchild[p] + ci;
-    const Sph s  = sph(cx[p] - cx[ch], cy[p] - cy[ch], cz[p] - cz[ch]);
-    if (lane < P) harmonic_column<false>(P, lane, s, Y);
-    for (int q = lane; q < NT; q += WG) Mc[q] = M[(int64_t)ch * NT + q];
-    __syncthreads();
-    for (int a = 0; a < QPL; a++) {
-      const int q = lane + a * WG;
-      if (q >= nout) continue;
-      int j = 0;
-      while (tri(j + 1) <= q) j++;
-      const int k = q - tri(j);
-      for (int n = 0; n <= j; n++) {
-        for (int m = max(-n, -j + k + n); m <= min(k - 1, n); m++) {
-          const int     jnkms = tri(j - n) + k - m;
-          const double2 y     = ylook(Y, n, -m);
-          const double  sg    = (m >= 0 ? 1.0 : sgn(m)) * sgn(n);
-          const double2 ys    = make_double2(y.x * sg, y.y * sg);
-          for (int d = 0; d < 3; d++) cfma(acc[a][d], Mc[3 * jnkms + d], ys);
-        }
-        for (int m = k; m <= min(n, j + k - n); m++) {
-          const int     jnkms = tri(j - n) - k + m;
-          const double2 y     = ylook(Y, n, -m);
-          const double  sg    = sgn(k + n + m);
-          const double2 ys    = make_double2(y.x * sg, y.y * sg);
-          for (int d = 0; d < 3; d++) cfma(acc[a][d], conj2(Mc[3 * jnkms + d]), ys);
-        }
-      }
+  for (int c0 = 0; c0 < nc; c0 += nwv) {
+    const int  ci = c0 + wv;
+    const bool on = ci < nc;  // uniform over the wavefront
+    if (on) {
+      const int ch = cchild[p] + ci;
+      const Sph s  = sph(cx[p] - cx[ch], cy[p] - cy[ch], cz[p] - cz[ch]);
+      if (lane < P) harmonic_column<false>(P, lane, s, Y);
+      for (int q = lane; q < NT; q += WG) Mc[q] = M[(int64_t)ch * NT + q];
     }
     __syncthreads();
+    if (on)
+      for (int a = 0; a < QPL; a++) {
+        const int q = lane + a * WG;
+        if (q >= nout) continue;
+        int j = 0;
+        while (tri(j + 1) <= q) j++;
+        const int k = q - tri(j);
+        for (int n = 0; n <= j; n++) {
+          for (int m = max(-n, -j + k + n); m <= min(k - 1, n); m++) {
+            const int     jnkms = tri(j - n) + k - m;
+            const double2 y     = ylook(Y, n, -m);
+            const double  sg    = (m >= 0 ? 1.0 : sgn(m)) * sgn(n);
+            const double2 ys    = make_double2(y.x * sg, y.y * sg);
+            for (int d = 0; d < 3; d++) cfma(acc[a][d], Mc[3 * jnkms + d], ys);
+          }
+          for (int m = k; m <= min(n, j + k - n); m++) {
+            const int     jnkms = tri(j - n) - k + m;
+            const double2 y     = ylook(Y, n, -m);
+            const double  sg    = sgn(k + n + m);
+            const double2 ys    = make_double2(y.x * sg, y.y * sg);
+            for (int d = 0; d < 3; d++) cfma(acc[a][d], conj2(Mc[3 * jnkms + d]), ys);
+          }
+        }
+      }
+    __syncthreads();
   }
-  for (int a = 0; a < QPL; a++) {
+  // sum over the wavefronts in order, one coefficient slot a at a time: part[wv][lane][3]
+  double2  *part = sm2;
+  const int nw   = min(nwv, nc);
+  for (int a = 0; a < QPL && a * WG < nout; a++) {
+    if (wv < nw)
+      for (int d = 0; d < 3; d++) part[(wv * WG + lane) * 3 + d] = acc[a][d];
+    __syncthreads();
     const int q = lane + a * WG;
-    if (q < nout)
-      for (int d = 0; d < 3; d++) M[(int64_t)p * NT + 3 * q + d] = acc[a][d];
+    if (wv == 0 && q < nout)
+      for (int d = 0; d < 3; d++) {
+        double2 r = part[lane * 3 + d];
+        for (int v = 1; v < nw; v++) {
+          r.x += part[(v * WG + lane) * 3 + d].x;
+          r.y += part[(v * WG + lane) * 3 + d].y;
+        }
+        M[(int64_t)p * NT + 3 * q + d] = r;
+      }
+    __syncthreads();
   }
 }
 
@@ -901,8 +926,11 @@ void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M)
   const int     NT = 3 * P * (P + 1) / 2;
   const int64_t o = t.lvl_off[level], n = t.lvl_off[level + 1] - o;
   if (n <= 0) return;
-  const size_t lds = sizeof(double2) * (P * (P + 1) / 2 + NT);
-  k_m2m<<<(int)n, WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.nchild.as<int32_t>(), t.child.as<int32_t>(),
+  // wavefronts per workgroup: 8 LDS slices of (harmonics + multipole) up to 64 KB, else 4
+  const size_t slice = sizeof(double2) * (P * (P + 1) / 2 + NT);
+  const int    nwv   = slice * M2M_MAXWV <= 65536 ? M2M_MAXWV : M2M_MAXWV / 2;
+  const size_t lds   = std::max(slice * nwv, sizeof(double2) * 3 * WG * nwv);
+  k_m2m<<<(int)n, nwv * WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.nchild.as<int32_t>(), t.child.as<int32_t>(),
                                 t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(), M);
   UVLM_LAUNCH_CHECK();
 }
