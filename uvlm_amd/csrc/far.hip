@@ -205,15 +205,24 @@ __global__ __launch_bounds__(WG) void k_p2m(int P, int NT, const int32_t *__rest
   for (int a = 0; a < QPL; a++)
     for (int d = 0; d < 3; d++) acc[a][d] = make_double2(0, 0);
   const int pb = lane / C, pc = lane % C;
+  // the body of the next chunk is loaded during the current chunk's sums
+  double xb[3] = {0, 0, 0}, ab[3] = {0, 0, 0};
+  if (pb < B && pb < nb) {
+    const int64_t bi = b0 + pb;
+    for (int d = 0; d < 3; d++) xb[d] = st[(F_X + d) * cap + bi], ab[d] = st[(F_A + d) * cap + bi];
+  }
   for (int c0 = 0; c0 < nb; c0 += B) {
     const int i = c0 + pb;
     if (pb < B && i < nb) {
-      const int64_t bi = b0 + i;
-      const Sph     sp = sph(st[(F_X + 0) * cap + bi] - X, st[(F_X + 1) * cap + bi] - Yc, st[(F_X + 2) * cap + bi] - Z);
+      const Sph sp = sph(xb[0] - X, xb[1] - Yc, xb[2] - Z);
       harmonic_column<false>(P, pc, sp, Yb + pb * ny);
       if (P - 1 - pc > pc) harmonic_column<false>(P, P - 1 - pc, sp, Yb + pb * ny);
       if (pc == 0)
-        for (int d = 0; d < 3; d++) al[pb * 3 + d] = st[(F_A + d) * cap + bi];
+        for (int d = 0; d < 3; d++) al[pb * 3 + d] = ab[d];
+    }
+    if (pb < B && i + B < nb) {
+      const int64_t bi = b0 + i + B;
+      for (int d = 0; d < 3; d++) xb[d] = st[(F_X + d) * cap + bi], ab[d] = st[(F_A + d) * cap + bi];
     }
     __syncthreads();
     const int cnt = min(B, nb - c0);
