@@ -1,0 +1,70 @@
+// Exhaustive host check of the M2L work plan (uvlm_amd/csrc/m2l_plan.hpp) for G = 16 and 32
+// lanes per group, P = 1..16 and every slot budget: every (output j,k; degree n; order m) term
+// of the truncated M2L sum is owned by exactly one (lane, slot), every harmonic and multipole
+// index the kernel forms stays inside its row, and the per-output piece lists name each active
+// (lane, slot) once.  Exit status 0 when all plans pass.
+#include "../../uvlm_amd/csrc/m2l_plan.hpp"
+
+#include <cstdio>
+using namespace uvlm;
+int main()
+{
+  int bad = 0;
+  for (int G : {16, 32})
+  for (int P = 1; P <= 16; P++)
+    for (int budget : {0, 4, 5, 6, 8, 9, 12, 16}) {
+      M2LPlan pl;
+      if (!make_m2l_plan(P, budget, pl, G)) { printf("G=%d P=%d budget=%d: no plan\n", G, P, budget); continue; }
+      const int ny = plan_tri(P), S = pl.S, NT = 3 * ny;
+      // count[w][n][m]
+      std::vector<int> cnt(ny * 16 * 16, 0);
+      long cost = 0;
+      std::vector<int> owner_ok(G * S, 0);
+      for (int n = 0; n < P; n++) {
+        const int lv = pl.lev[n], cn = lv & 255, hr = (lv >> 8) & 255, used = lv >> 16;
+        for (int s = 0; s < S; s++) {
+          if (!((used >> s) & 1)) continue;
+          int mx = 0;
+          for (int l = 0; l < G; l++) {
+            const int v = pl.slots[l * S + s];
+            if (v < 0) continue;
+            const int j = v & 31, k = (v >> 5) & 31, m0 = (v >> 10) & 31, lvl = (v >> 15) & 31;
+            bool act; int a, b;
+            if (lvl == M2LPlan::HOME) { act = s < hr && j + n <= P - 1; a = 0; b = n + 1; }
+            else { act = lvl == n; a = m0; b = std::min(m0 + cn, n + 1); }
+            if (!act) continue;
+            const int J = j + n;
+            if (J > P - 1 || k > j) { printf("P=%d bad J/k\n", P); bad++; }
+            for (int m = a; m < b; m++) {
+              const int mu = m - k < 0 ? k - m : m - k;
+              if (mu > J || m + k > J || 3 * (plan_tri(n) + m) + 2 >= NT) { printf("P=%d oob\n", P); bad++; }
+              cnt[(plan_tri(j) + k) * 256 + n * 16 + m]++;
+            }
+            mx = std::max(mx, b - a);
+          }
+          cost += mx;
+        }
+      }
+      for (int j = 0; j < P; j++)
+        for (int k = 0; k <= j; k++)
+          for (int n = 0; n + j <= P - 1; n++)
+            for (int m = 0; m <= n; m++)
+              if (cnt[(plan_tri(j) + k) * 256 + n * 16 + m] != 1) {
+                printf("P=%d budget=%d: term (%d,%d,%d,%d) count %d\n", P, budget, j, k, n, m,
+                       cnt[(plan_tri(j) + k) * 256 + n * 16 + m]);
+                bad++;
+              }
+      // pieces: every (lane, slot) that is ever active is listed under its output exactly once
+      for (int w = 0; w < ny; w++)
+        for (int e = pl.out_off[w]; e < pl.out_off[w + 1]; e++) {
+          const int pc = pl.out_src[e], v = pl.slots[pc];
+          if (pc < 0 || pc >= G * S || v < 0 || plan_tri(v & 31) + ((v >> 5) & 31) != w) { printf("bad piece\n"); bad++; }
+          owner_ok[pc]++;
+        }
+      for (int pc = 0; pc < G * S; pc++)
+        if ((pl.slots[pc] >= 0) != (owner_ok[pc] == 1)) { printf("P=%d piece %d listed %d\n", P, pc, owner_ok[pc]); bad++; }
+      (void)cost;
+    }
+  printf("%s\n", bad ? "FAILED" : "all terms owned exactly once, in bounds");
+  return bad != 0;
+}

@@ -175,3 +175,16 @@ def test_p2p_far_form_matches_erf_form():
         assert abs(Gpp - Gppr) < 2e-7 * abs(Gppr)
     assert abs(-2 / math.sqrt(math.pi) * math.exp(-2.0) * 0.75 - 1.5 * 0.5 * (_g_series(2.0, 1) - 0.5 * _g_series(2.0, 0))
                - _g_series(2.0, 2)) < 1e-12
+
+
+def test_m2l_plan_owns_every_term_once(tmp_path):
+    """tests/cpp/m2l_plan_check.cpp, compiled with g++ and run: the M2L work plan for every P in
+    1..16, group width 16/32 and slot budget owns each truncated M2L term exactly once and keeps
+    every index in bounds."""
+    import subprocess
+    exe = str(tmp_path / "m2l_plan_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(ROOT, "tests", "cpp", "m2l_plan_check.cpp")],
+                   check=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "all terms owned exactly once, in bounds" in r.stdout
