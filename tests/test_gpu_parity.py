@@ -63,6 +63,28 @@ def test_tree_and_lists_bitexact(orc, name):
     assert len(np.unique(rp, axis=0)) == len(rp) and len(np.unique(rm, axis=0)) == len(rm)
 
 
+def test_tree_rebuilds_on_one_handle_stay_bitexact(orc):
+    """A handle sorts only the key digits its previous tree used (plus one level); a much deeper
+    tree on the next build (uniform cube -> clustered blobs) must fall back to the full sort, and
+    every rebuild must still match the oracle's tree, permutation and lists bit for bit."""
+    g = FMMCalculator(P=8, theta=0.5, ncrit=64)
+    for name in ("cube", "blobs", "cube", "leapfrog", "blobs"):
+        X, A, S = CASES[name]()
+        b_ref = orc.make_bodies(X, A, S)
+        c = orc.Calculator(P=8, theta=0.5, ncrit=64)
+        c.init(b_ref)
+        b_gpu = make_bodies(X, A, S)
+        g.init(b_gpu)
+        ri, rr = c.cells()
+        gi, gr = g.cells()
+        print(f"[rebuild {name}] cells={len(ri)} levels={int(ri[:, 4].max()) + 1}", flush=True)
+        assert np.array_equal(b_gpu, b_ref), name
+        assert np.array_equal(ri, gi) and np.array_equal(rr, gr), name
+        rp, rm = c.pairs()
+        gp, gm = g.lists()
+        assert np.array_equal(np.unique(rp, axis=0), gp) and np.array_equal(np.unique(rm, axis=0), gm), name
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_fmm_fp64_parity(orc, name):
     c, g, ref, out, _, _ = run_pair(orc, name)

@@ -92,6 +92,7 @@ inline void swap_buf(DevBuf &a, DevBuf &b)
 struct Tree {
   int64_t C = 0, nleaf = 0;
   int     nlevels = 0;
+  int     sort_digits = 21;       // key digits the next build of this tree sorts (its last depth + 1)
   std::vector<int64_t> lvl_off;   // host: level l cells are lvl_cells[lvl_off[l] .. lvl_off[l+1])
   DevBuf nchild, nbody, child, body, level, parent;  // int32 [C]
   DevBuf cx, cy, cz, R;                              // double [C]

@@ -9,14 +9,17 @@
 //   2. a 63-bit key per body: the octant digits of the 21 levels below the root, each
 //      digit (x>cx)+2(y>cy)+4(z>cz) against the FP64 centre recurrence
 //      c_{l+1} = c_l + (R0/2^(l+1))*(+-1) (tree.hpp:57, 86-92)
-//   3. radix sort of (key, index): stable, so equal keys keep the input order
+//   3. radix sort of (key, index): stable, so equal keys keep the input order.  Only the top
+//      digits the tree can use are sorted: the previous build's depth plus one level; a cell
+//      that must split below them flags the build, which then sorts all 21 digits again
 //   4. cells level by level: a cell splits iff NBODY > ncrit (tree.hpp:43); its children are
 //      the non-empty digit ranges, found by binary search in the sorted keys
 //   5. cell numbering of the reference: root 0, child groups appended in DFS pre-order of
 //      their parents (tree.hpp:81-96), i.e. ChildOffset(p) = 1 + sum of NCHILD over internal
 //      cells before p in pre-order; pre-order = order by (BodyOffset, level)
 //   6. inside a leaf the bodies keep their input order (the reference stops sorting at the
-//      leaf), so a second sort on (leaf begin, input index) fixes the final permutation.
+//      leaf), so a second, stable sort of the input indices by their leaf's first body fixes the
+//      final permutation.
 // Traversal (semantics of src/traverse.hpp:32-85): breadth-first over (target, source)
 // pairs with the reference's split rule and MAC; outputs are appended with one atomic per
 // wavefront and category, then sorted by (target, source).
@@ -182,7 +185,8 @@ __device__ __forceinline__ void octant_bounds(const uint64_t *__restrict__ keys,
   }
 }
 
-__global__ void k_count_children(int64_t off, int64_t cnt, int l, int64_t ncrit, const int32_t *__restrict__ rb,
+// err bit 0: deeper than the key holds; bit 1: a split needs a digit below the dlim sorted ones.
+__global__ void k_count_children(int64_t off, int64_t cnt, int l, int64_t ncrit, int dlim, const int32_t *__restrict__ rb,
                                  const int32_t *__restrict__ re, const uint64_t *__restrict__ keys,
                                  int32_t *__restrict__ nchild, int32_t *__restrict__ err)
 {
@@ -194,6 +198,8 @@ __global__ void k_count_children(int64_t off, int64_t cnt, int l, int64_t ncrit,
   if (e - b > ncrit) {
     if (l >= KEY_LEVELS) {
       atomicOr(err, 1);
+    } else if (l >= dlim) {
+      atomicOr(err, 2);
     } else {
       int bnd[9];
       octant_bounds(keys, b, e, l, bnd);
@@ -322,22 +328,22 @@ __global__ void k_scatter_cells(int64_t C, LevelOffsets lo, const int32_t *__res
   if (nc == 0) leaves[k - pfx[k]] = g;
 }
 
+// lb[input index] = first body of its leaf (one wavefront per leaf; lanes stride over its bodies)
 __global__ void k_leaf_begin(int64_t nleafrec, const int32_t *__restrict__ leaves, const int32_t *__restrict__ c_body,
                              const int32_t *__restrict__ c_nbody, const uint32_t *__restrict__ idx,
-                             uint64_t *__restrict__ ck)
+                             uint32_t *__restrict__ lb)
 {
-  // one wavefront per leaf; lanes stride over its bodies
   const int64_t w    = (blockIdx.x * (int64_t)BLK + threadIdx.x) >> 6;
   const int     lane = threadIdx.x & 63;
   if (w >= nleafrec) return;
   const int g = leaves[w], b = c_body[g], n = c_nbody[g];
-  for (int i = lane; i < n; i += 64) ck[b + i] = ((uint64_t)(uint32_t)b << 32) | (uint64_t)idx[b + i];
+  for (int i = lane; i < n; i += 64) lb[idx[b + i]] = (uint32_t)b;
 }
 
-__global__ void k_low32(int64_t N, const uint64_t *__restrict__ ck, int32_t *__restrict__ perm)
+__global__ void k_iota_u32(int64_t N, uint32_t *__restrict__ v)
 {
   const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (i < N) perm[i] = (int32_t)(uint32_t)(ck[i] & 0xffffffffull);
+  if (i < N) v[i] = (uint32_t)i;
 }
 
 // One field per grid row (blockIdx.y), so the whole state moves with full memory parallelism.
@@ -543,62 +549,79 @@ void build_tree(hipStream_t s, Scratch &sc, const double *x, const double *y, co
   k_keys<<<nblocks(N), BLK, 0, s>>>(x, y, z, N, tree.X0R0.as<double>(), sc.keys0.as<uint64_t>(),
                                      sc.idx0.as<uint32_t>());
   UVLM_LAUNCH_CHECK();
-  {
-    size_t bytes = 0;
-    UVLM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, sc.keys0.as<uint64_t>(), sc.keys1.as<uint64_t>(),
-                                       sc.idx0.as<uint32_t>(), sc.idx1.as<uint32_t>(), (size_t)N, 0,
-                                       3 * KEY_LEVELS, s));
-    sc.sort_tmp.reserve(bytes);
-    UVLM_HIP(rocprim::radix_sort_pairs(sc.sort_tmp.p, bytes, sc.keys0.as<uint64_t>(), sc.keys1.as<uint64_t>(),
-                                       sc.idx0.as<uint32_t>(), sc.idx1.as<uint32_t>(), (size_t)N, 0,
-                                       3 * KEY_LEVELS, s));
-  }
   const uint64_t *keys = sc.keys1.as<uint64_t>();
   const uint32_t *idx  = sc.idx1.as<uint32_t>();
-
-  // 4. records level by level
-  auto ensure_rec = [&](int64_t want, int64_t used) {
-    if (want <= sc.rec_cap) return;
-    int64_t cap = std::max<int64_t>(want, 2 * sc.rec_cap);
-    cap         = std::max<int64_t>(cap, 1024);
-    for (DevBuf *b : {&sc.rec_begin, &sc.rec_end, &sc.rec_parent, &sc.rec_rank, &sc.rec_nchild})
-      b->reserve(4 * cap, used > 0, s);
-    for (DevBuf *b : {&sc.rec_cx, &sc.rec_cy, &sc.rec_cz}) b->reserve(8 * cap, used > 0, s);
-    for (DevBuf *b : {&sc.rec_scan, &sc.rec_pfx, &sc.rec_pre, &sc.rec_gidx, &sc.rec_childoff})
-      b->reserve(4 * (cap + 1), used > 0, s);
-    sc.rec_cap = cap;
-  };
-  ensure_rec(std::max<int64_t>(1024, 8 * (N / std::max<int64_t>(1, ncrit) + 1)), 0);
-  sc.counters.reserve(64);
-  int32_t *err = sc.counters.as<int32_t>(), *summary = err + 8;
-  k_root_record<<<1, 64, 0, s>>>((int32_t)N, tree.X0R0.as<double>(), sc.rec_begin.as<int32_t>(),
-                                 sc.rec_end.as<int32_t>(), sc.rec_parent.as<int32_t>(), sc.rec_rank.as<int32_t>(),
-                                 sc.rec_cx.as<double>(), sc.rec_cy.as<double>(), sc.rec_cz.as<double>(), err);
-  UVLM_LAUNCH_CHECK();
-  std::vector<int64_t> off = {0, 1};
-  for (int l = 0;; l++) {
-    const int64_t o = off[l], cnt = off[l + 1] - off[l];
-    k_count_children<<<nblocks(cnt), BLK, 0, s>>>(o, cnt, l, ncrit, sc.rec_begin.as<int32_t>(),
-                                                  sc.rec_end.as<int32_t>(), keys, sc.rec_nchild.as<int32_t>(), err);
+  std::vector<int64_t> off;
+  int                  dlim = std::min(KEY_LEVELS, std::max(1, tree.sort_digits));
+  for (;;) {
+    // stable sort of (key, index) on the top dlim digits
+    {
+      size_t    bytes = 0;
+      const int b0    = 3 * (KEY_LEVELS - dlim);
+      UVLM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, sc.keys0.as<uint64_t>(), sc.keys1.as<uint64_t>(),
+                                         sc.idx0.as<uint32_t>(), sc.idx1.as<uint32_t>(), (size_t)N, b0,
+                                         3 * KEY_LEVELS, s));
+      sc.sort_tmp.reserve(bytes);
+      UVLM_HIP(rocprim::radix_sort_pairs(sc.sort_tmp.p, bytes, sc.keys0.as<uint64_t>(), sc.keys1.as<uint64_t>(),
+                                         sc.idx0.as<uint32_t>(), sc.idx1.as<uint32_t>(), (size_t)N, b0,
+                                         3 * KEY_LEVELS, s));
+    }
+    // 4. records level by level
+    auto ensure_rec = [&](int64_t want, int64_t used) {
+      if (want <= sc.rec_cap) return;
+      int64_t cap = std::max<int64_t>(want, 2 * sc.rec_cap);
+      cap         = std::max<int64_t>(cap, 1024);
+      for (DevBuf *b : {&sc.rec_begin, &sc.rec_end, &sc.rec_parent, &sc.rec_rank, &sc.rec_nchild})
+        b->reserve(4 * cap, used > 0, s);
+      for (DevBuf *b : {&sc.rec_cx, &sc.rec_cy, &sc.rec_cz}) b->reserve(8 * cap, used > 0, s);
+      for (DevBuf *b : {&sc.rec_scan, &sc.rec_pfx, &sc.rec_pre, &sc.rec_gidx, &sc.rec_childoff})
+        b->reserve(4 * (cap + 1), used > 0, s);
+      sc.rec_cap = cap;
+    };
+    ensure_rec(std::max<int64_t>(1024, 8 * (N / std::max<int64_t>(1, ncrit) + 1)), 0);
+    sc.counters.reserve(64);
+    int32_t *err = sc.counters.as<int32_t>(), *summary = err + 8;
+    k_root_record<<<1, 64, 0, s>>>((int32_t)N, tree.X0R0.as<double>(), sc.rec_begin.as<int32_t>(),
+                                   sc.rec_end.as<int32_t>(), sc.rec_parent.as<int32_t>(), sc.rec_rank.as<int32_t>(),
+                                   sc.rec_cx.as<double>(), sc.rec_cy.as<double>(), sc.rec_cz.as<double>(), err);
     UVLM_LAUNCH_CHECK();
-    excl_scan_i32(s, sc, sc.rec_nchild.as<int32_t>() + o, sc.rec_scan.as<int32_t>(), cnt);
-    k_level_summary<<<1, 64, 0, s>>>(sc.rec_scan.as<int32_t>() + cnt - 1, sc.rec_nchild.as<int32_t>() + o + cnt - 1,
-                                     err, summary);
-    UVLM_LAUNCH_CHECK();
-    int32_t sm3[3] = {0, 0, 0};
-    to_host(s, sm3, summary, 3);
-    if (sm3[2]) throw Error(UVLM_ERR_DEPTH, "more than ncrit coincident bodies: octree deeper than 21 levels");
-    const int64_t total = (int64_t)sm3[0] + sm3[1];
-    if (total == 0) break;
-    ensure_rec(off[l + 1] + total + 1, off[l + 1]);
-    k_write_children<<<nblocks(cnt), BLK, 0, s>>>(
-        o, cnt, l, sc.rec_begin.as<int32_t>(), sc.rec_end.as<int32_t>(), keys, sc.rec_nchild.as<int32_t>(),
-        sc.rec_scan.as<int32_t>(), off[l + 1], tree.X0R0.as<double>(), sc.rec_begin.as<int32_t>(),
-        sc.rec_end.as<int32_t>(), sc.rec_parent.as<int32_t>(), sc.rec_rank.as<int32_t>(), sc.rec_cx.as<double>(),
-        sc.rec_cy.as<double>(), sc.rec_cz.as<double>());
-    UVLM_LAUNCH_CHECK();
-    off.push_back(off[l + 1] + total);
+    off = {0, 1};
+    bool deeper = false;  // a cell must split below the sorted digits
+    for (int l = 0;; l++) {
+      const int64_t o = off[l], cnt = off[l + 1] - off[l];
+      k_count_children<<<nblocks(cnt), BLK, 0, s>>>(o, cnt, l, ncrit, dlim, sc.rec_begin.as<int32_t>(),
+                                                    sc.rec_end.as<int32_t>(), keys, sc.rec_nchild.as<int32_t>(), err);
+      UVLM_LAUNCH_CHECK();
+      excl_scan_i32(s, sc, sc.rec_nchild.as<int32_t>() + o, sc.rec_scan.as<int32_t>(), cnt);
+      k_level_summary<<<1, 64, 0, s>>>(sc.rec_scan.as<int32_t>() + cnt - 1, sc.rec_nchild.as<int32_t>() + o + cnt - 1,
+                                       err, summary);
+      UVLM_LAUNCH_CHECK();
+      int32_t sm3[3] = {0, 0, 0};
+      to_host(s, sm3, summary, 3);
+      if (sm3[2] & 1) throw Error(UVLM_ERR_DEPTH, "more than ncrit coincident bodies: octree deeper than 21 levels");
+      if (sm3[2] & 2) {
+        deeper = true;
+        break;
+      }
+      const int64_t total = (int64_t)sm3[0] + sm3[1];
+      if (total == 0) break;
+      ensure_rec(off[l + 1] + total + 1, off[l + 1]);
+      k_write_children<<<nblocks(cnt), BLK, 0, s>>>(
+          o, cnt, l, sc.rec_begin.as<int32_t>(), sc.rec_end.as<int32_t>(), keys, sc.rec_nchild.as<int32_t>(),
+          sc.rec_scan.as<int32_t>(), off[l + 1], tree.X0R0.as<double>(), sc.rec_begin.as<int32_t>(),
+          sc.rec_end.as<int32_t>(), sc.rec_parent.as<int32_t>(), sc.rec_rank.as<int32_t>(), sc.rec_cx.as<double>(),
+          sc.rec_cy.as<double>(), sc.rec_cz.as<double>());
+      UVLM_LAUNCH_CHECK();
+      off.push_back(off[l + 1] + total);
+    }
+    if (deeper) {  // sort all digits and build again
+      dlim = KEY_LEVELS;
+      continue;
+    }
+    break;
   }
+  tree.sort_digits = (int)off.size() - 1;  // digits 0 .. nlev-1: this tree's splits plus one spare level
+
   const int64_t C    = off.back();
   const int     nlev = (int)off.size() - 1;
   if (nlev > KEY_LEVELS + 1) throw Error(UVLM_ERR_INTERNAL, "level count");
@@ -642,14 +665,20 @@ void build_tree(hipStream_t s, Scratch &sc, const double *x, const double *y, co
       tree.lvl_cells.as<int32_t>(), tree.leaves.as<int32_t>());
   UVLM_LAUNCH_CHECK();
 
-  // 6. final permutation: sort (leaf begin, input index)
-  uint64_t *ck = sc.keys0.as<uint64_t>();
+  // 6. final permutation: the input indices 0..N-1, stably sorted by their leaf's first body
+  uint32_t *lb = sc.keys0.as<uint32_t>(), *lb_sorted = sc.keys1.as<uint32_t>(), *iota = sc.idx0.as<uint32_t>();
   k_leaf_begin<<<nblocks(tree.nleaf * 64), BLK, 0, s>>>(tree.nleaf, tree.leaves.as<int32_t>(),
-                                                        tree.body.as<int32_t>(), tree.nbody.as<int32_t>(), idx, ck);
+                                                        tree.body.as<int32_t>(), tree.nbody.as<int32_t>(), idx, lb);
+  k_iota_u32<<<nblocks(N), BLK, 0, s>>>(N, iota);
   UVLM_LAUNCH_CHECK();
-  sort_keys_u64(s, sc, ck, sc.keys1.as<uint64_t>(), N, 32 + bits_for(N));
-  k_low32<<<nblocks(N), BLK, 0, s>>>(N, sc.keys1.as<uint64_t>(), perm_out);
-  UVLM_LAUNCH_CHECK();
+  {
+    size_t bytes = 0;
+    UVLM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, lb, lb_sorted, iota, (uint32_t *)perm_out, (size_t)N, 0,
+                                       bits_for(N), s));
+    sc.sort_tmp.reserve(bytes);
+    UVLM_HIP(rocprim::radix_sort_pairs(sc.sort_tmp.p, bytes, lb, lb_sorted, iota, (uint32_t *)perm_out, (size_t)N, 0,
+                                       bits_for(N), s));
+  }
 }
 
 void gather_fields(hipStream_t s, const double *src, double *dst, int64_t src_cap, int64_t dst_cap, int nfields,
