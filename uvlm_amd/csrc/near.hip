@@ -7,17 +7,19 @@
 //   u -= f1 (dX x a_j),   J[a][b] -= f2 (dX x a_j)_a dX_b + f1 eps_{abk} a_j,k.
 // Only the source core size sigma_j enters (multipoleKernels.hpp:243).
 //
-// Layout: one 64-lane workgroup per target leaf.  Lanes = (target t) x (source split s):
-// small leaves use several lanes per target, each taking every S-th source, and the
-// partial sums are combined with cross-lane adds (fixed order, no atomics).  The bodies
-// of all source leaves in the target's CSR list are streamed through LDS in full chunks of
-// 64 across leaf boundaries (prefix sums of the leaf sizes in LDS, binary search per slot),
-// so small leaves do not pay a barrier per leaf.
+// Layout: one workgroup of BLK lanes per target leaf (64, or 256 when leaves may hold more than
+// 64 bodies).  Lanes = (target t) x (source split s): a pass takes ntp of the leaf's targets
+// (near_targets_per_pass) and S = BLK / ntp splits, each taking every S-th source, and the
+// partial sums are combined in a fixed order (cross-lane adds or LDS, no atomics).  The bodies
+// of all source leaves in the target's CSR list are streamed through LDS in full chunks of BLK
+// across leaf boundaries (prefix sums of the leaf sizes in LDS, binary search per slot), so
+// small leaves do not pay a barrier per leaf.
 //
 // FP64 path: the reference formula (erf, exp, sqrt in double).
-// Mixed path: FP32 arithmetic on coordinates relative to the target leaf centre (the
-// source offsets are formed in FP64 while staging), per-source-leaf FP32 partial sums
-// added into FP64 per-target accumulators; g/s^3 and its derivative from pair_kernel.hpp.
+// Mixed path: FP32 arithmetic on coordinates relative to the target leaf centre (the source
+// offsets are formed in FP64 while staging), two sources per step over zero-padded chunks (one
+// trip count for all lanes), per-chunk FP32 partial sums added into FP64 per-target
+// accumulators; G = g/s^3 and its derivative from pair_kernel.hpp.
 #include "common.hpp"
 #include "pair_kernel.hpp"
 
