@@ -384,7 +384,8 @@ __device__ __forceinline__ int wave_excl_scan(int v, int &total)
 // One BFS step.  counters: [0] next frontier, [1] P2P, [2] M2L (64-bit).
 __global__ void k_traverse(const int32_t *__restrict__ ft, int64_t nf, TreeView T, TreeView S, double theta,
                            int32_t *__restrict__ nxt, int64_t cap_f, uint64_t *__restrict__ p2p, int64_t cap_p,
-                           uint64_t *__restrict__ m2l, int64_t cap_m, unsigned long long *__restrict__ counters)
+                           uint64_t *__restrict__ m2l, int64_t cap_m, unsigned long long *__restrict__ counters,
+                           int sbits)
 {
   const int64_t i      = blockIdx.x * (int64_t)BLK + threadIdx.x;
   int           cls[8];  // 0 push, 1 P2P, 2 M2L
@@ -426,7 +427,7 @@ __global__ void k_traverse(const int32_t *__restrict__ ft, int64_t nf, TreeView 
   bp = __shfl(bp, 0, 64);
   bm = __shfl(bm, 0, 64);
   for (int c = 0; c < nout; c++) {
-    const uint64_t key = ((uint64_t)(uint32_t)ct[c] << 32) | (uint64_t)(uint32_t)cs[c];
+    const uint64_t key = ((uint64_t)(uint32_t)ct[c] << sbits) | (uint64_t)(uint32_t)cs[c];
     if (cls[c] == 0) {
       const int64_t q = (int64_t)bf + of++;
       if (q < cap_f) {
@@ -443,11 +444,12 @@ __global__ void k_traverse(const int32_t *__restrict__ ft, int64_t nf, TreeView 
   }
 }
 
-__global__ void k_csr(int64_t C, const uint64_t *__restrict__ keys, int64_t n, int32_t *__restrict__ off)
+// List keys are (row << lbits) | entry, sorted: off[c] = the first key of row c.
+__global__ void k_csr(int64_t C, const uint64_t *__restrict__ keys, int64_t n, int lbits, int32_t *__restrict__ off)
 {
   const int64_t c = blockIdx.x * (int64_t)BLK + threadIdx.x;
   if (c > C) return;
-  const uint64_t want = (uint64_t)c << 32;
+  const uint64_t want = (uint64_t)c << lbits;
   int64_t        lo = 0, hi = n;
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
@@ -457,16 +459,16 @@ __global__ void k_csr(int64_t C, const uint64_t *__restrict__ keys, int64_t n, i
   off[c] = (int32_t)lo;
 }
 
-__global__ void k_low32_u(int64_t n, const uint64_t *__restrict__ keys, int32_t *__restrict__ out)
+__global__ void k_low_bits(int64_t n, const uint64_t *__restrict__ keys, int lbits, int32_t *__restrict__ out)
 {
   const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
-  if (i < n) out[i] = (int32_t)(uint32_t)(keys[i] & 0xffffffffull);
+  if (i < n) out[i] = (int32_t)(uint32_t)(keys[i] & ((1ull << lbits) - 1));
 }
 
 // One lane per list entry q: its row t (the last t with off[t] <= q) by binary search, key
 // (source, target) for the transposed CSR.
 __global__ void k_transpose_keys(int64_t C, int64_t n, const int32_t *__restrict__ off,
-                                 const int32_t *__restrict__ src, uint64_t *__restrict__ keys)
+                                 const int32_t *__restrict__ src, int tbits, uint64_t *__restrict__ keys)
 {
   const int64_t q = blockIdx.x * (int64_t)BLK + threadIdx.x;
   if (q >= n) return;
@@ -476,7 +478,7 @@ __global__ void k_transpose_keys(int64_t C, int64_t n, const int32_t *__restrict
     if (off[mid] <= q) lo = mid;
     else hi = mid - 1;
   }
-  keys[q] = ((uint64_t)(uint32_t)src[q] << 32) | (uint64_t)(uint32_t)lo;
+  keys[q] = ((uint64_t)(uint32_t)src[q] << tbits) | (uint64_t)(uint32_t)lo;
 }
 
 __global__ void k_count_inter(int64_t C, const int32_t *__restrict__ off, const int32_t *__restrict__ src,
@@ -704,7 +706,8 @@ static TreeView view(const Tree &t)
 
 void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double theta, Lists &lists)
 {
-  TreeView T = view(tt), S = view(st);
+  TreeView  T = view(tt), S = view(st);
+  const int sbits = bits_for(std::max<int64_t>(st.C, 1));  // list keys: (target << sbits) | source
   int32_t  root_nchild[2];
   to_host(s, &root_nchild[0], tt.nchild.p);
   to_host(s, &root_nchild[1], st.nchild.p);
@@ -736,7 +739,8 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
     for (;;) {
       k_set_counters<<<1, 64, 0, s>>>(dcnt, start[0], start[1], start[2]);
       k_traverse<<<nblocks(nf), BLK, 0, s>>>(sc.front0.as<int32_t>(), nf, T, S, theta, sc.front1.as<int32_t>(), cap_f,
-                                             sc.out_p2p.as<uint64_t>(), cap_p, sc.out_m2l.as<uint64_t>(), cap_m, dcnt);
+                                             sc.out_p2p.as<uint64_t>(), cap_p, sc.out_m2l.as<uint64_t>(), cap_m, dcnt,
+                                             sbits);
       UVLM_LAUNCH_CHECK();
       to_host(s, cnt, dcnt, 3);
       bool redo = false;
@@ -765,7 +769,7 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
   lists.np2p = (int64_t)cnt[1];
   lists.nm2l = (int64_t)cnt[2];
   const int64_t C  = tt.C;
-  const int     eb = 32 + bits_for(std::max<int64_t>(tt.C, 1));
+  const int     eb = sbits + bits_for(std::max<int64_t>(tt.C, 1));
   for (int which = 0; which < 2; which++) {
     const int64_t n   = which == 0 ? lists.np2p : lists.nm2l;
     DevBuf       &raw = which == 0 ? sc.out_p2p : sc.out_m2l;
@@ -775,8 +779,8 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
     src.reserve(4 * std::max<int64_t>(n, 1));
     sc.keys1.reserve(8 * std::max<int64_t>(n, 1));
     if (n > 0) sort_keys_u64(s, sc, raw.as<uint64_t>(), sc.keys1.as<uint64_t>(), n, eb);
-    k_csr<<<nblocks(C + 1), BLK, 0, s>>>(C, sc.keys1.as<uint64_t>(), n, off.as<int32_t>());
-    k_low32_u<<<nblocks(n), BLK, 0, s>>>(n, sc.keys1.as<uint64_t>(), src.as<int32_t>());
+    k_csr<<<nblocks(C + 1), BLK, 0, s>>>(C, sc.keys1.as<uint64_t>(), n, sbits, off.as<int32_t>());
+    k_low_bits<<<nblocks(n), BLK, 0, s>>>(n, sc.keys1.as<uint64_t>(), sbits, src.as<int32_t>());
     UVLM_LAUNCH_CHECK();
   }
 }
@@ -784,8 +788,9 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
 void transpose_lists(hipStream_t s, Scratch &sc, const Tree &in_targets, const Tree &in_sources, const Lists &in,
                      Lists &out)
 {
-  const int64_t Cs = in_sources.C;
-  const int     eb = 32 + bits_for(std::max<int64_t>(Cs, 1));
+  const int64_t Cs    = in_sources.C;
+  const int     tbits = bits_for(std::max<int64_t>(in_targets.C, 1));  // entry (target cell) bits of a key
+  const int     eb    = tbits + bits_for(std::max<int64_t>(Cs, 1));
   out.np2p         = in.np2p;
   out.nm2l         = in.nm2l;
   for (int which = 0; which < 2; which++) {
@@ -799,13 +804,13 @@ void transpose_lists(hipStream_t s, Scratch &sc, const Tree &in_targets, const T
     sc.keys0.reserve(8 * std::max<int64_t>(n, 1));
     sc.keys1.reserve(8 * std::max<int64_t>(n, 1));
     if (n > 0) {
-      k_transpose_keys<<<nblocks(n), BLK, 0, s>>>(in_targets.C, n, ioff.as<int32_t>(), isrc.as<int32_t>(),
+      k_transpose_keys<<<nblocks(n), BLK, 0, s>>>(in_targets.C, n, ioff.as<int32_t>(), isrc.as<int32_t>(), tbits,
                                                   sc.keys0.as<uint64_t>());
       UVLM_LAUNCH_CHECK();
     }
     if (n > 0) sort_keys_u64(s, sc, sc.keys0.as<uint64_t>(), sc.keys1.as<uint64_t>(), n, eb);
-    k_csr<<<nblocks(Cs + 1), BLK, 0, s>>>(Cs, sc.keys1.as<uint64_t>(), n, off.as<int32_t>());
-    k_low32_u<<<nblocks(n), BLK, 0, s>>>(n, sc.keys1.as<uint64_t>(), src.as<int32_t>());
+    k_csr<<<nblocks(Cs + 1), BLK, 0, s>>>(Cs, sc.keys1.as<uint64_t>(), n, tbits, off.as<int32_t>());
+    k_low_bits<<<nblocks(n), BLK, 0, s>>>(n, sc.keys1.as<uint64_t>(), tbits, src.as<int32_t>());
     UVLM_LAUNCH_CHECK();
   }
 }
