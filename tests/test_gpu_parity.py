@@ -85,6 +85,29 @@ def test_tree_rebuilds_on_one_handle_stay_bitexact(orc):
         assert np.array_equal(np.unique(rp, axis=0), gp) and np.array_equal(np.unique(rm, axis=0), gm), name
 
 
+def test_lists_regrow_from_tiny_buffers(orc, monkeypatch):
+    """The traversal runs its steps in batches without the host and, when a frontier or list
+    buffer overflows, grows it and traverses again: with 64-entry first buffers (several regrowths)
+    the lists and the FMM result still match the oracle exactly."""
+    monkeypatch.setenv("UVLM_LIST_CAPACITY", "64")
+    X, A, S = CASES["blobs"]()
+    P, th, nc = PARAMS["blobs"]
+    b_ref = orc.make_bodies(X, A, S)
+    c = orc.Calculator(P=P, theta=th, ncrit=nc)
+    c.init(b_ref)
+    c.calculate()
+    ref = c.bodies(b_ref.shape[0])
+    g = FMMCalculator(P=P, theta=th, ncrit=nc)
+    b_gpu = make_bodies(X, A, S)
+    g.init(b_gpu)
+    rp, rm = c.pairs()
+    gp, gm = g.lists()
+    assert np.array_equal(np.unique(rp, axis=0), gp) and np.array_equal(np.unique(rm, axis=0), gm)
+    g.calculate()
+    out = g.copyBodies2Host()
+    assert rel(out[:, 15:18], ref[:, 15:18]) < 1e-10
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_fmm_fp64_parity(orc, name):
     c, g, ref, out, _, _ = run_pair(orc, name)

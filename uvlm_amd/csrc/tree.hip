@@ -23,6 +23,7 @@
 // Traversal (semantics of src/traverse.hpp:32-85): breadth-first over (target, source)
 // pairs with the reference's split rule and MAC; outputs are appended with one atomic per
 // wavefront and category, then sorted by (target, source).
+#include <cstdlib>
 #include <cstring>
 
 #include "common.hpp"
@@ -146,15 +147,6 @@ __global__ void k_root_record(int32_t n, const double *__restrict__ X0R0, int32_
   cy[0] = X0R0[1];
   cz[0] = X0R0[2];
   *err  = 0;
-}
-
-__global__ void k_set_counters(unsigned long long *c, unsigned long long a, unsigned long long b,
-                               unsigned long long d)
-{
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  c[0] = a;
-  c[1] = b;
-  c[2] = d;
 }
 
 // The three numbers the host needs after counting a level: last exclusive-scan entry, last
@@ -382,12 +374,11 @@ __device__ __forceinline__ int wave_excl_scan(int v, int &total)
 }
 
 // One BFS step.  counters: [0] next frontier, [1] P2P, [2] M2L (64-bit).
-__global__ void k_traverse(const int32_t *__restrict__ ft, int64_t nf, TreeView T, TreeView S, double theta,
-                           int32_t *__restrict__ nxt, int64_t cap_f, uint64_t *__restrict__ p2p, int64_t cap_p,
-                           uint64_t *__restrict__ m2l, int64_t cap_m, unsigned long long *__restrict__ counters,
-                           int sbits)
+__device__ __forceinline__ void traverse_round(const int32_t *__restrict__ ft, int64_t nf, int64_t i, TreeView T,
+                                               TreeView S, double theta, int32_t *__restrict__ nxt, int64_t cap_f,
+                                               uint64_t *__restrict__ p2p, int64_t cap_p, uint64_t *__restrict__ m2l,
+                                               int64_t cap_m, unsigned long long *__restrict__ counters, int sbits)
 {
-  const int64_t i      = blockIdx.x * (int64_t)BLK + threadIdx.x;
   int           cls[8];  // 0 push, 1 P2P, 2 M2L
   int           ct[8], cs[8];
   int           nout = 0, npush = 0, np2p = 0, nm2l = 0;
@@ -442,6 +433,35 @@ __global__ void k_traverse(const int32_t *__restrict__ ft, int64_t nf, TreeView 
       if (q < cap_m) m2l[q] = key;
     }
   }
+}
+
+// One breadth-first step.  counters: [0] next frontier size, [1] P2P and [2] M2L list sizes so far,
+// [3] this step's frontier size (read here, so a batch of steps runs without the host).  The
+// blocks stride over the frontier; every lane of a block takes part in each round.
+__global__ void k_traverse(const int32_t *__restrict__ ft, TreeView T, TreeView S, double theta,
+                           int32_t *__restrict__ nxt, int64_t cap_f, uint64_t *__restrict__ p2p, int64_t cap_p,
+                           uint64_t *__restrict__ m2l, int64_t cap_m, unsigned long long *__restrict__ counters,
+                           int sbits)
+{
+  const int64_t nf = (int64_t)counters[3];
+  for (int64_t base = blockIdx.x * (int64_t)BLK; base < nf; base += (int64_t)gridDim.x * BLK)
+    traverse_round(ft, nf, base + threadIdx.x, T, S, theta, nxt, cap_f, p2p, cap_p, m2l, cap_m, counters, sbits);
+}
+
+// End of a step: the next frontier becomes the current one.  counters[4] (sticky) flags a buffer
+// that overflowed (1 frontier, 2 P2P, 4 M2L); the traversal then stops (frontier 0) and the host
+// starts it again with larger buffers.  counters[5] keeps the largest frontier seen.
+__global__ void k_step_end(unsigned long long *__restrict__ c, int64_t cap_f, int64_t cap_p, int64_t cap_m)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long f = c[4];
+  if ((int64_t)c[0] > cap_f) f |= 1;
+  if ((int64_t)c[1] > cap_p) f |= 2;
+  if ((int64_t)c[2] > cap_m) f |= 4;
+  c[5] = c[0] > c[5] ? c[0] : c[5];
+  c[4] = f;
+  c[3] = f ? 0 : c[0];
+  c[0] = 0;
 }
 
 // List keys are (row << lbits) | entry, sorted: off[c] = the first key of row c.
@@ -714,57 +734,59 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
   int64_t cap_f = std::max<int64_t>(4096, 8 * (tt.C + st.C));
   int64_t cap_p = std::max<int64_t>(4096, 32 * tt.nleaf);
   int64_t cap_m = std::max<int64_t>(4096, 64 * tt.C);
+  if (const char *e = std::getenv("UVLM_LIST_CAPACITY")) {  // first sizes of fresh buffers (tests of the regrowth)
+    const long long v = std::atoll(e);
+    if (v > 0) cap_f = cap_p = cap_m = v;
+  }
   sc.front0.reserve(8 * cap_f);
   sc.front1.reserve(8 * cap_f);
   sc.out_p2p.reserve(8 * cap_p);
   sc.out_m2l.reserve(8 * cap_m);
-  cap_f = sc.front0.bytes / 8;
+  cap_f = std::min(sc.front0.bytes, sc.front1.bytes) / 8;
   cap_p = sc.out_p2p.bytes / 8;
   cap_m = sc.out_m2l.bytes / 8;
-  unsigned long long cnt[3] = {0, 0, 0};
+  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
   sc.counters.reserve(64);
   unsigned long long *dcnt = (unsigned long long *)sc.counters.p;
-  int64_t nf = 0;
   if (root_nchild[0] == 0 && root_nchild[1] == 0) {
     const uint64_t k0 = 0;
     h2d(s, sc.out_p2p.p, &k0, 8);
     cnt[1] = 1;
   } else {
-    const int32_t pair[2] = {0, 0};
-    h2d(s, sc.front0.p, pair, 8);
-    nf = 1;
-  }
-  while (nf > 0) {
-    unsigned long long start[3] = {0, cnt[1], cnt[2]};
+    // steps run in batches of STEPS without the host; a batch ends with one read of the counters
+    constexpr int STEPS = 6;
     for (;;) {
-      k_set_counters<<<1, 64, 0, s>>>(dcnt, start[0], start[1], start[2]);
-      k_traverse<<<nblocks(nf), BLK, 0, s>>>(sc.front0.as<int32_t>(), nf, T, S, theta, sc.front1.as<int32_t>(), cap_f,
-                                             sc.out_p2p.as<uint64_t>(), cap_p, sc.out_m2l.as<uint64_t>(), cap_m, dcnt,
-                                             sbits);
-      UVLM_LAUNCH_CHECK();
-      to_host(s, cnt, dcnt, 3);
-      bool redo = false;
-      if ((int64_t)cnt[0] > cap_f) {
-        sc.front1.reserve(8 * cnt[0]);
-        cap_f = sc.front1.bytes / 8;
-        redo  = true;
+      const int32_t            pair[2] = {0, 0};
+      const unsigned long long init[6] = {0, 0, 0, 1, 0, 0};
+      h2d(s, sc.front0.p, pair, 8);
+      h2d(s, dcnt, init, sizeof(init));
+      const int grid = nblocks(std::min<int64_t>(cap_f, 2048 * (int64_t)BLK));
+      int       par  = 0;
+      for (;;) {
+        for (int k = 0; k < STEPS; k++, par ^= 1) {
+          DevBuf &fin = par ? sc.front1 : sc.front0, &fout = par ? sc.front0 : sc.front1;
+          k_traverse<<<grid, BLK, 0, s>>>(fin.as<int32_t>(), T, S, theta, fout.as<int32_t>(), cap_f,
+                                          sc.out_p2p.as<uint64_t>(), cap_p, sc.out_m2l.as<uint64_t>(), cap_m, dcnt,
+                                          sbits);
+          k_step_end<<<1, 64, 0, s>>>(dcnt, cap_f, cap_p, cap_m);
+          UVLM_LAUNCH_CHECK();
+        }
+        to_host(s, cnt, dcnt, 6);
+        if (cnt[4] || cnt[3] == 0) break;
       }
-      if ((int64_t)cnt[1] > cap_p) {
-        sc.out_p2p.reserve(8 * cnt[1], true, s);
-        cap_p = sc.out_p2p.bytes / 8;
-        redo  = true;
+      if (!cnt[4]) break;
+      // a buffer overflowed: grow what did (at least doubled) and traverse again
+      if (cnt[4] & 1) {
+        const int64_t want = std::max<int64_t>(2 * cap_f, 2 * (int64_t)cnt[5]);
+        sc.front0.reserve(8 * want);
+        sc.front1.reserve(8 * want);
       }
-      if ((int64_t)cnt[2] > cap_m) {
-        sc.out_m2l.reserve(8 * cnt[2], true, s);
-        cap_m = sc.out_m2l.bytes / 8;
-        redo  = true;
-      }
-      if (!redo) break;
+      if (cnt[4] & 2) sc.out_p2p.reserve(8 * std::max<int64_t>(2 * cap_p, 2 * (int64_t)cnt[1]));
+      if (cnt[4] & 4) sc.out_m2l.reserve(8 * std::max<int64_t>(2 * cap_m, 2 * (int64_t)cnt[2]));
+      cap_f = std::min(sc.front0.bytes, sc.front1.bytes) / 8;
+      cap_p = sc.out_p2p.bytes / 8;
+      cap_m = sc.out_m2l.bytes / 8;
     }
-    nf = (int64_t)cnt[0];
-    swap_buf(sc.front0, sc.front1);
-    if (nf > (int64_t)(sc.front1.bytes / 8)) sc.front1.reserve(8 * nf);
-    cap_f = std::min<int64_t>(sc.front0.bytes, sc.front1.bytes) / 8;
   }
   lists.np2p = (int64_t)cnt[1];
   lists.nm2l = (int64_t)cnt[2];
