@@ -85,6 +85,31 @@ def test_tree_rebuilds_on_one_handle_stay_bitexact(orc):
         assert np.array_equal(np.unique(rp, axis=0), gp) and np.array_equal(np.unique(rm, axis=0), gm), name
 
 
+@pytest.mark.parametrize("ncrit", [100, 200, 700])
+def test_large_leaves_match_oracle(orc, ncrit):
+    """Leaves of more than 64 bodies take the 256-lane P2P, whose leaves of more than 128 bodies
+    run several passes of fewer targets (and more than 256 bodies several passes of 256): FP64 against
+    the oracle to 1e-10 and mixed against FP64 to 1e-6."""
+    X, A, S = CASES["leapfrog"]()
+    b_ref = orc.make_bodies(X, A, S)
+    c = orc.Calculator(P=8, theta=0.5, ncrit=ncrit)
+    c.init(b_ref)
+    c.calculate()
+    ref = c.bodies(b_ref.shape[0])
+    outs = {}
+    for precision in ("fp64", "mixed"):
+        g = FMMCalculator(P=8, theta=0.5, ncrit=ncrit, precision=precision)
+        b = make_bodies(X, A, S)
+        g.init(b)
+        g.calculate()
+        outs[precision] = g.copyBodies2Host()
+    for f, sl in (("velocity", slice(15, 18)), ("J", slice(0, 9)), ("dadt", slice(18, 21))):
+        e64 = rel(outs["fp64"][:, sl], ref[:, sl])
+        emx = rel(outs["mixed"][:, sl], outs["fp64"][:, sl])
+        print(f"ncrit {ncrit} {f}: fp64 vs oracle {e64:.2e}, mixed vs fp64 {emx:.2e}", flush=True)
+        assert e64 < 1e-10 and emx < 1e-6, (f, e64, emx)
+
+
 def test_lists_regrow_from_tiny_buffers(orc, monkeypatch):
     """The traversal runs its steps in batches without the host and, when a frontier or list
     buffer overflows, grows it and traverses again: with 64-entry first buffers (several regrowths)
