@@ -178,32 +178,36 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MIXED ? 5 :
           float        fa[15];
           const int    T2 = 2 * ((cnt + 2 * S - 1) / (2 * S));
           const float4 *ps = s32 + 2 * (lane_on ? si : 0);
-          for (int k = 0; k < 15; k++) fa[k] = 0.f;
-          for (int k = 0; k < T2; k += 2, ps += 4 * S) {
-            const float4 A0 = ps[0], B0 = ps[1], A1 = ps[2 * S], B1 = ps[2 * S + 1];
-            const float  dx0 = rxi - A0.x, dy0 = ryi - A0.y, dz0 = rzi - A0.z;
-            const float  dx1 = rxi - A1.x, dy1 = ryi - A1.y, dz1 = rzi - A1.z;
-            const float  r20 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0, r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-            const float  q0 = TSIG ? t_q : A0.w, q1 = TSIG ? t_q : A1.w;
-            const float  tq0 = fmaf(r20, q0, -1.0f), tq1 = fmaf(r21, q1, -1.0f);  // x - 1
-            float        G0, Gp0, G1, Gp1;  // G(x) and G'(x) of both pairs
-            g_near(tq0, tq1, G0, Gp0, G1, Gp1);
-            if (fmaxf(tq0, tq1) >= XPOLY - 1.0f) {  // lanes with a far pair take both far forms
-              float Gf0, Gpf0, Gf1, Gpf1;
-              g_far(r20 * q0, Gf0, Gpf0);
-              g_far(r21 * q1, Gf1, Gpf1);
-              if (tq0 >= XPOLY - 1.0f) G0 = Gf0, Gp0 = Gpf0;
-              if (tq1 >= XPOLY - 1.0f) G1 = Gf1, Gp1 = Gpf1;
+          // FP32 partial sums cover at most FLUSH entries per lane before they go into the FP64 sums
+          constexpr int FLUSH = 64;
+          for (int k0 = 0; k0 < T2; k0 += FLUSH) {
+            for (int k = 0; k < 15; k++) fa[k] = 0.f;
+            for (int k = k0; k < min(T2, k0 + FLUSH); k += 2, ps += 4 * S) {
+              const float4 A0 = ps[0], B0 = ps[1], A1 = ps[2 * S], B1 = ps[2 * S + 1];
+              const float  dx0 = rxi - A0.x, dy0 = ryi - A0.y, dz0 = rzi - A0.z;
+              const float  dx1 = rxi - A1.x, dy1 = ryi - A1.y, dz1 = rzi - A1.z;
+              const float  r20 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0, r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+              const float  q0 = TSIG ? t_q : A0.w, q1 = TSIG ? t_q : A1.w;
+              const float  tq0 = fmaf(r20, q0, -1.0f), tq1 = fmaf(r21, q1, -1.0f);  // x - 1
+              float        G0, Gp0, G1, Gp1;  // G(x) and G'(x) of both pairs
+              g_near(tq0, tq1, G0, Gp0, G1, Gp1);
+              if (fmaxf(tq0, tq1) >= XPOLY - 1.0f) {  // lanes with a far pair take both far forms
+                float Gf0, Gpf0, Gf1, Gpf1;
+                g_far(r20 * q0, Gf0, Gpf0);
+                g_far(r21 * q1, Gf1, Gpf1);
+                if (tq0 >= XPOLY - 1.0f) G0 = Gf0, Gp0 = Gpf0;
+                if (tq1 >= XPOLY - 1.0f) G1 = Gf1, Gp1 = Gpf1;
+              }
+              // coincident pairs are skipped (multipoleKernels.hpp:242): only the f1 alpha terms
+              // of J survive a zero dX, so zeroing f1 is enough
+              const float f10 = r20 == 0.0f ? 0.0f : (TSIG ? G0 * t_c1 : G0);
+              const float f11 = r21 == 0.0f ? 0.0f : (TSIG ? G1 * t_c1 : G1);
+              accumulate_pair(fa, dx0, dy0, dz0, B0.x, B0.y, B0.z, f10, Gp0 * (TSIG ? t_c2 : B0.w), WITH_J);
+              accumulate_pair(fa, dx1, dy1, dz1, B1.x, B1.y, B1.z, f11, Gp1 * (TSIG ? t_c2 : B1.w), WITH_J);
             }
-            // coincident pairs are skipped (multipoleKernels.hpp:242): only the f1 alpha terms
-            // of J survive a zero dX, so zeroing f1 is enough
-            const float f10 = r20 == 0.0f ? 0.0f : (TSIG ? G0 * t_c1 : G0);
-            const float f11 = r21 == 0.0f ? 0.0f : (TSIG ? G1 * t_c1 : G1);
-            accumulate_pair(fa, dx0, dy0, dz0, B0.x, B0.y, B0.z, f10, Gp0 * (TSIG ? t_c2 : B0.w), WITH_J);
-            accumulate_pair(fa, dx1, dy1, dz1, B1.x, B1.y, B1.z, f11, Gp1 * (TSIG ? t_c2 : B1.w), WITH_J);
+            if (WITH_J) fold_antisym(fa);
+            for (int k = 0; k < (WITH_J ? 12 : 3); k++) acc[k] += (double)fa[k];
           }
-          if (WITH_J) fold_antisym(fa);
-          for (int k = 0; k < (WITH_J ? 12 : 3); k++) acc[k] += (double)fa[k];
         } else {
           for (int j = jfirst; j < cnt; j += S) {
             const SrcF64 &sj = s64[j];
