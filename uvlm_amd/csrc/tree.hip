@@ -448,6 +448,24 @@ __global__ void k_traverse(const int32_t *__restrict__ ft, TreeView T, TreeView 
     traverse_round(ft, nf, base + threadIdx.x, T, S, theta, nxt, cap_f, p2p, cap_p, m2l, cap_m, counters, sbits);
 }
 
+// Start of a traversal: the root pair (0, 0) is the first frontier, or, when both roots are leaves,
+// the single P2P pair (traverse.hpp:41-44); the counters are cleared.
+__global__ void k_traverse_start(TreeView T, TreeView S, int32_t *__restrict__ front, uint64_t *__restrict__ p2p,
+                                 unsigned long long *__restrict__ c)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const bool leaves = T.nchild[0] == 0 && S.nchild[0] == 0;
+  if (leaves) p2p[0] = 0;
+  front[0] = 0;
+  front[1] = 0;
+  c[0] = 0;
+  c[1] = leaves ? 1 : 0;
+  c[2] = 0;
+  c[3] = leaves ? 0 : 1;
+  c[4] = 0;
+  c[5] = 0;
+}
+
 // End of a step: the next frontier becomes the current one.  counters[4] (sticky) flags a buffer
 // that overflowed (1 frontier, 2 P2P, 4 M2L); the traversal then stops (frontier 0) and the host
 // starts it again with larger buffers.  counters[5] keeps the largest frontier seen.
@@ -728,9 +746,6 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
 {
   TreeView  T = view(tt), S = view(st);
   const int sbits = bits_for(std::max<int64_t>(st.C, 1));  // list keys: (target << sbits) | source
-  int32_t  root_nchild[2];
-  to_host(s, &root_nchild[0], tt.nchild.p);
-  to_host(s, &root_nchild[1], st.nchild.p);
   int64_t cap_f = std::max<int64_t>(4096, 8 * (tt.C + st.C));
   int64_t cap_p = std::max<int64_t>(4096, 32 * tt.nleaf);
   int64_t cap_m = std::max<int64_t>(4096, 64 * tt.C);
@@ -748,18 +763,11 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
   unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
   sc.counters.reserve(64);
   unsigned long long *dcnt = (unsigned long long *)sc.counters.p;
-  if (root_nchild[0] == 0 && root_nchild[1] == 0) {
-    const uint64_t k0 = 0;
-    h2d(s, sc.out_p2p.p, &k0, 8);
-    cnt[1] = 1;
-  } else {
+  {
     // steps run in batches of STEPS without the host; a batch ends with one read of the counters
     constexpr int STEPS = 6;
     for (;;) {
-      const int32_t            pair[2] = {0, 0};
-      const unsigned long long init[6] = {0, 0, 0, 1, 0, 0};
-      h2d(s, sc.front0.p, pair, 8);
-      h2d(s, dcnt, init, sizeof(init));
+      k_traverse_start<<<1, 64, 0, s>>>(T, S, sc.front0.as<int32_t>(), sc.out_p2p.as<uint64_t>(), dcnt);
       const int grid = nblocks(std::min<int64_t>(cap_f, 2048 * (int64_t)BLK));
       int       par  = 0;
       for (;;) {
