@@ -140,6 +140,17 @@ struct M2LSchedule {
   M2LPlan plan;
   DevBuf  dev;         // slots, lev, out_off, out_src
   int64_t dev_slots = 0, dev_lev = 0, dev_out_off = 0, dev_out_src = 0;
+  DevBuf  ytab;        // singular harmonics of the lattice vectors [-K, K]^3 (M2L_TABLE_K)
+  int     ytab_P = 0;
+};
+// Half-width of the cube of integer offsets (in units of the smaller cell's half-width) whose
+// singular harmonics the M2L reads from a table; 99.9% of the 4M leapfrog tree's M2L pairs.
+constexpr int M2L_TABLE_K = 12;
+// M2L inputs for the lattice table (Y null: compute every pair's harmonics).
+struct M2LTable {
+  const double2 *Y;
+  const int32_t *level;
+  const double  *X0R0;
 };
 void make_m2l_schedule(int P, M2LSchedule &s);
 void zero_doubles(hipStream_t s, double *p, int64_t n);

@@ -591,6 +591,22 @@ __global__ __launch_bounds__(WG) void k_l2p_adj_tgt(int P, int NT, const int32_t
 
 // ------------------------------------------------------------------ M2L
 
+// Singular harmonics of the integer lattice vectors k in [-K, K]^3 (k = 0 left zero), in the
+// extended-row layout of singular_column_ext: entry (J, mu) of vector k at Y[idx(k) P^2 + J^2 + J + mu].
+__global__ __launch_bounds__(WG) void k_m2l_table(int P, double2 *__restrict__ Y)
+{
+  const int     w   = 2 * M2L_TABLE_K + 1;
+  const int64_t idx = blockIdx.x;
+  const int     kx = (int)(idx / (w * w)) - M2L_TABLE_K, ky = (int)((idx / w) % w) - M2L_TABLE_K,
+            kz = (int)(idx % w) - M2L_TABLE_K;
+  double2 *Yk = Y + idx * P * P;
+  if (kx == 0 && ky == 0 && kz == 0) {
+    for (int e = threadIdx.x; e < P * P; e += WG) Yk[e] = make_double2(0, 0);
+    return;
+  }
+  if (threadIdx.x < P) singular_column_ext(P, threadIdx.x, sph(kx, ky, kz), Yk);
+}
+
 // The n-synchronous M2L (plan in m2l_plan.hpp).  One workgroup (one wavefront) per target
 // cell; its 4 groups of 16 lanes take every 4th source.  All lanes of a group walk the source
 // degrees n and orders m in the same sequence, so M[n,m] is one LDS broadcast per group, and
@@ -639,10 +655,11 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
                                                const double2 *__restrict__ M, double2 *__restrict__ L,
                                                const int32_t *__restrict__ slots, const int32_t *__restrict__ lev,
                                                const int32_t *__restrict__ out_off,
-                                               const int32_t *__restrict__ out_src,
+                                               const int32_t *__restrict__ out_src, const M2LTable tab,
                                                const int32_t *__restrict__ seg = nullptr)
 {
   extern __shared__ double2 sm2[];
+  __shared__ double sinv[16];  // R0^-(J+1), J = 0..P-1 (lattice table only)
   const int64_t b = blockIdx.x;
   const int64_t u = (b & 7) * per_xcd + (b >> 3);
   if (u >= ncell) return;
@@ -663,18 +680,56 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
     for (int d = 0; d < 3; d++) acc[s][d] = make_double2(0, 0);
   const int    s0 = SPLIT ? seg[3 * u + 1] : off[t], nsrc = (SPLIT ? seg[3 * u + 2] : off[t + 1]) - s0;
   const double X = tx[t], Yc = ty[t], Z = tz[t];
-  // the next batch's source centre and the index of the one after it are loaded a batch ahead,
-  // so their latency overlaps the current batch's multiply-adds
+  const int    lt = tab.Y ? tab.level[t] : 0;
+  if (tab.Y) {
+    if (lane < P) {
+      const double v = 1.0 / tab.X0R0[3];
+      double       q = v;
+      for (int i = 0; i < lane; i++) q *= v;
+      sinv[lane] = q;
+    }
+    __syncthreads();
+  }
+  // the next batch's source centre (and level) and the index of the one after it are loaded a
+  // batch ahead, so their latency overlaps the current batch's multiply-adds
   int    s_cur = g < nsrc ? src[s0 + g] : 0, s_nxt = g + 4 < nsrc ? src[s0 + g + 4] : 0;
   double cx_cur = 0.0, cy_cur = 0.0, cz_cur = 0.0;
-  if (g < nsrc) cx_cur = sx[s_cur], cy_cur = sy[s_cur], cz_cur = sz[s_cur];
+  int    ls_cur = 0;
+  if (g < nsrc) {
+    cx_cur = sx[s_cur], cy_cur = sy[s_cur], cz_cur = sz[s_cur];
+    if (tab.Y) ls_cur = tab.level[s_cur];
+  }
   for (int base = 0; base < nsrc; base += 4) {
     const int  si     = base + g;
     const bool active = si < nsrc;
     if (active) {
       const double2 *Ms = M + (int64_t)s_cur * NT;
       for (int q = sl; q < NT; q += 16) Mg[q] = Ms[q];
-      if (sl < P) {
+      // same tree: X_t - X_s = r k with r = R0 / 2^l the smaller cell's half-width and k an
+      // integer vector, and Y_n(r k) = r^-(n+1) Y_n(k); offsets inside the table's cube are read
+      // from it and scaled per row, the rest (and other trees) are computed here
+      bool from_table = false;
+      if (tab.Y) {
+        const int    l  = max(lt, ls_cur);
+        const double sc = ldexp(1.0 / tab.X0R0[3], l);  // 1 / r
+        const double qx = (X - cx_cur) * sc, qy = (Yc - cy_cur) * sc, qz = (Z - cz_cur) * sc;
+        const double kx = rint(qx), ky = rint(qy), kz = rint(qz);
+        const double dev = fmax(fabs(qx - kx), fmax(fabs(qy - ky), fabs(qz - kz)));
+        const double km  = fmax(fabs(kx), fmax(fabs(ky), fabs(kz)));
+        if (dev < 1e-6 && km <= M2L_TABLE_K && km > 0.0) {
+          const int      w   = 2 * M2L_TABLE_K + 1;
+          const int64_t  idx = (((int64_t)kx + M2L_TABLE_K) * w + ((int64_t)ky + M2L_TABLE_K)) * w + ((int64_t)kz + M2L_TABLE_K);
+          const double2 *Yk  = tab.Y + idx * P * P;
+          for (int e = sl; e < P * P; e += 16) {
+            const int     J = (int)sqrtf((float)e + 0.5f);  // row of entry e: J*J <= e <= J*J + 2J
+            const double  f = ldexp(sinv[J], l * (J + 1));
+            const double2 y = Yk[e];
+            Yx[e]           = make_double2(y.x * f, y.y * f);
+          }
+          from_table = true;
+        }
+      }
+      if (!from_table && sl < P) {
         const Sph sp = sph(X - cx_cur, Yc - cy_cur, Z - cz_cur);
         singular_column_ext(P, sl, sp, Yx);
       }
@@ -682,6 +737,7 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
     if (si + 4 < nsrc) {
       s_cur  = s_nxt;
       cx_cur = sx[s_cur], cy_cur = sy[s_cur], cz_cur = sz[s_cur];
+      if (tab.Y) ls_cur = tab.level[s_cur];
       if (si + 8 < nsrc) s_nxt = src[s0 + si + 8];
     }
     __syncthreads();
@@ -910,6 +966,14 @@ void make_m2l_schedule(int P, M2LSchedule &s)
   all.insert(all.end(), pl.out_src.begin(), pl.out_src.end());
   s.dev.reserve(4 * all.size());
   UVLM_HIP(hipMemcpy(s.dev.p, all.data(), 4 * all.size(), hipMemcpyHostToDevice));
+  if (s.ytab_P != P) {
+    const int w = 2 * M2L_TABLE_K + 1;
+    s.ytab.reserve(sizeof(double2) * (size_t)w * w * w * P * P);
+    k_m2l_table<<<w * w * w, WG>>>(P, s.ytab.as<double2>());
+    UVLM_LAUNCH_CHECK();
+    UVLM_HIP(hipDeviceSynchronize());
+    s.ytab_P = P;
+  }
 }
 
 void zero_doubles(hipStream_t s, double *p, int64_t n)
@@ -1002,19 +1066,23 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
     }
     const int64_t per_xcd = (units + 7) / 8;
     const int     grid    = (int)(8 * per_xcd);
+    // the lattice table needs one tree (one root, one unit): particle evaluations and their adjoints
+    M2LTable tab{};
+    if (&tt == &st && sch.ytab_P == P)
+      tab = M2LTable{sch.ytab.as<double2>(), tt.level.as<int32_t>(), tt.X0R0.as<double>()};
 #define UVLM_M2L_NS(SS)                                                                                            \
   if (S == SS) {                                                                                                   \
     if (split)                                                                                                     \
       k_m2l_ns<SS, true><<<grid, WG, lds_ns, s>>>(                                                                 \
           P, NT, units, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(), tt.cx.as<double>(),            \
           tt.cy.as<double>(), tt.cz.as<double>(), st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M,   \
-          out, d + sch.dev_slots, d + sch.dev_lev, d + sch.dev_out_off, d + sch.dev_out_src, dseg);               \
+          out, d + sch.dev_slots, d + sch.dev_lev, d + sch.dev_out_off, d + sch.dev_out_src, tab, dseg);          \
     else                                                                                                           \
       k_m2l_ns<SS><<<grid, WG, lds_ns, s>>>(P, NT, tt.C, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),  \
                                             tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),             \
                                             st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, L,       \
                                             d + sch.dev_slots, d + sch.dev_lev, d + sch.dev_out_off,               \
-                                            d + sch.dev_out_src);                                                  \
+                                            d + sch.dev_out_src, tab);                                             \
     UVLM_LAUNCH_CHECK();                                                                                           \
     if (split) {                                                                                                   \
       k_m2l_reduce<<<(int)tt.C, WG, 0, s>>>(NT, tt.C, dsegoff, out, L);                                            \
