@@ -61,6 +61,25 @@ def test_singular_harmonics_closed_form(orc, pt):
             assert abs(Y[n * n + n + m] - v) < 1e-13 * max(1, abs(v)), (n, m)
 
 
+def test_singular_harmonics_scale_with_cell_size(orc):
+    """The M2L lattice table rests on homogeneity: Y_n^m(r k) = r^-(n+1) Y_n^m(k) for an offset r k
+    (k an integer vector, r a cell half-width R0 / 2^l), so a table of Y(k) scaled per degree gives
+    every same-tree pair's harmonics."""
+    P = 10
+    for k in [(1, 2, 3), (-4, 0, 5), (0, 0, -7), (12, -12, 1)]:
+        k = np.asarray(k, float)
+        rk = np.linalg.norm(k)
+        a, b = math.acos(k[2] / rk), math.atan2(k[1], k[0])
+        base = orc.eval_local(P, rk, a, b)
+        for l, R0 in [(3, 1.00001), (8, 2.5)]:
+            r = R0 / 2 ** l
+            Y = orc.eval_local(P, r * rk, a, b)
+            for n in range(P):
+                for m in range(-n, n + 1):
+                    want = base[n * n + n + m] * r ** -(n + 1)
+                    assert abs(Y[n * n + n + m] - want) < 1e-12 * max(1e-300, abs(want)), (tuple(k), l, n, m)
+
+
 # ------------------------------------------------------------------- P2P kernel
 def _psi(x, xs, a, s):
     """Stream function of one Gaussian vortex blob: a erf(r / (sqrt2 s)) / (4 pi r)."""
