@@ -338,14 +338,19 @@ __global__ void k_iota_u32(int64_t N, uint32_t *__restrict__ v)
   if (i < N) v[i] = (uint32_t)i;
 }
 
-// One field per grid row (blockIdx.y), so the whole state moves with full memory parallelism.
+// GATHER_FIELDS fields per grid row (blockIdx.y): the state moves with full memory parallelism
+// and each permutation entry is read once per group of fields.
+constexpr int GATHER_FIELDS = 4;
 __global__ void k_gather_fields(const double *__restrict__ src, double *__restrict__ dst, int64_t scap, int64_t dcap,
-                                const int32_t *__restrict__ perm, int64_t N)
+                                const int32_t *__restrict__ perm, int64_t N, int nfields)
 {
   const int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x;
   if (i >= N) return;
-  const int64_t f = blockIdx.y;
-  dst[f * dcap + i] = src[f * scap + perm[i]];
+  const int64_t j  = perm[i];
+  const int     f0 = blockIdx.y * GATHER_FIELDS, f1 = min(nfields, f0 + GATHER_FIELDS);
+  double        v[GATHER_FIELDS];
+  for (int f = f0; f < f1; f++) v[f - f0] = src[f * scap + j];
+  for (int f = f0; f < f1; f++) dst[f * dcap + i] = v[f - f0];
 }
 
 __global__ void k_gather_i32(const int32_t *__restrict__ src, int32_t *__restrict__ dst,
@@ -725,7 +730,8 @@ void gather_fields(hipStream_t s, const double *src, double *dst, int64_t src_ca
                    const int32_t *perm, int64_t N)
 {
   if (N <= 0 || nfields <= 0) return;
-  k_gather_fields<<<dim3(nblocks(N), nfields), BLK, 0, s>>>(src, dst, src_cap, dst_cap, perm, N);
+  k_gather_fields<<<dim3(nblocks(N), (nfields + GATHER_FIELDS - 1) / GATHER_FIELDS), BLK, 0, s>>>(
+      src, dst, src_cap, dst_cap, perm, N, nfields);
   UVLM_LAUNCH_CHECK();
 }
 
