@@ -184,7 +184,7 @@ void do_eval(uvlm_handle h, bool sync)
   double2 *M = h->M.as<double2>(), *L = h->L.as<double2>();
   UVLM_HIP(hipEventRecord(h->ev[EV_E0], s));
   p2m(s, P, t, h->st.as<double>(), h->cap, M);
-  for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M);
+  for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M, h->sched.octY.as<double2>());
   zero_doubles(s, (double *)M, 2 * NT);
   UVLM_HIP(hipEventRecord(h->ev[EV_UP], s));
   m2l(s, P, t, t, h->lists, h->sched, M, L);
@@ -192,7 +192,7 @@ void do_eval(uvlm_handle h, bool sync)
   p2p(s, t, t, h->lists, h->st.as<double>(), h->cap, h->st.as<double>(), h->cap, h->st.as<double>(), h->cap,
       h->prm.precision, 0, h->prm.ncrit);
   UVLM_HIP(hipEventRecord(h->ev[EV_P2P], s));
-  for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L);
+  for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L, h->sched.octY.as<double2>());
   l2p_stretch(s, P, t, L, h->st.as<double>(), h->cap, h->prm.stretching, h->prm.formulation, 0);
   UVLM_HIP(hipEventRecord(h->ev[EV_DOWN], s));
   h->evaluated = true;
@@ -298,7 +298,7 @@ void sensor_pass(uvlm_handle h, int64_t m)
   m2l(s, P, h->stree, h->tree, h->slists, h->sched, h->M.as<double2>(), Ls, &h->sc);
   p2p(s, h->stree, h->tree, h->slists, h->sst.as<double>(), sc, h->st.as<double>(), h->cap, h->sst.as<double>(), sc,
       h->prm.precision, 1, h->prm.ncrit, &h->sc);
-  for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls);
+  for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls, h->sched.octY.as<double2>());
   l2p_stretch(s, P, h->stree, Ls, h->sst.as<double>(), sc, h->prm.stretching, h->prm.formulation, 1);
 }
 
@@ -374,7 +374,7 @@ void probe_adjoint_core(uvlm_handle h, int64_t m)
   h->Ms.reserve(sizeof(double2) * NT * h->stree.C);
   double2 *Ms = h->Ms.as<double2>();
   p2m(s, P, h->stree, h->sst.as<double>(), sc, Ms);
-  for (int l = h->stree.nlevels - 2; l >= 0; l--) m2m_level(s, P, h->stree, l, Ms);
+  for (int l = h->stree.nlevels - 2; l >= 0; l--) m2m_level(s, P, h->stree, l, Ms, h->sched.octY.as<double2>());
   // adjoint state of the particles: positions, strengths and sigma copied in
   const int64_t cap = h->cap;
   h->ast.reserve(sizeof(double) * N_FIELDS * cap);
@@ -386,7 +386,7 @@ void probe_adjoint_core(uvlm_handle h, int64_t m)
   double2 *La = h->L.as<double2>();
   m2l(s, P, h->tree, h->stree, h->alists, h->sched, Ms, La);
   zero_doubles(s, (double *)La, 2 * NT);  // the particle root multipole is never formed forward
-  for (int l = 1; l < h->tree.nlevels; l++) l2l_level(s, P, h->tree, l, La);
+  for (int l = 1; l < h->tree.nlevels; l++) l2l_level(s, P, h->tree, l, La, h->sched.octY.as<double2>());
   p2p(s, h->tree, h->stree, h->alists, a, cap, h->sst.as<double>(), sc, a, cap, h->prm.precision, 2, h->prm.ncrit);
   l2p_stretch(s, P, h->tree, La, a, cap, UVLM_TRANSPOSED, UVLM_CVPM, 0);
   h->evaluated = false;  // L now holds adjoint locals
@@ -452,10 +452,10 @@ void vjp_core(uvlm_handle h)
   }
   double2 *M = h->M.as<double2>(), *L = h->L.as<double2>();
   p2m_adj(s, P, t, st, aux, cap, M);
-  for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M);
+  for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M, h->sched.octY.as<double2>());
   zero_doubles(s, (double *)M, 2 * NT);
   m2l(s, P, t, t, h->tlists, h->sched, M, L);
-  for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L);
+  for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L, h->sched.octY.as<double2>());
   h->ast.reserve(sizeof(double) * N_FIELDS * cap);
   double *a = h->ast.as<double>();
   zero_doubles(s, a + F_J * cap, 9 * cap);

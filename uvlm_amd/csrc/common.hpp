@@ -141,6 +141,7 @@ struct M2LSchedule {
   DevBuf  dev;         // slots, lev, out_off, out_src
   int64_t dev_slots = 0, dev_lev = 0, dev_out_off = 0, dev_out_src = 0;
   DevBuf  ytab;        // singular harmonics of the lattice vectors [-K, K]^3 (M2L_TABLE_K)
+  DevBuf  octY;        // regular harmonics of the 8 unit octant vectors (M2M, L2L)
   int     ytab_P = 0;
 };
 // Half-width of the cube of integer offsets (in units of the smaller cell's half-width) whose
@@ -155,10 +156,10 @@ struct M2LTable {
 void make_m2l_schedule(int P, M2LSchedule &s);
 void zero_doubles(hipStream_t s, double *p, int64_t n);
 void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, double2 *M);
-void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M);
+void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M, const double2 *octY);
 void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, const M2LSchedule &sch, const double2 *M,
          double2 *L, Scratch *sc = nullptr);
-void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L);
+void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L, const double2 *octY);
 // adjoint of the particle rates: vortex + vortex-dipole P2M (aux fields 0..11) and the far-field
 // target-side term (adds into aux fields 12..14)
 void p2m_adj(hipStream_t s, int P, const Tree &t, const double *st, const double *aux, int64_t cap, double2 *M);

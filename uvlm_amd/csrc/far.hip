@@ -175,6 +175,23 @@ __device__ void singular_column_ext(int P, int m, const Sph &s, double2 *Yx)
   }
 }
 
+// Regular harmonics of the offset between a cell and its parent or child, from the table of the
+// 8 unit octant vectors (sx, sy, sz) = (+-1, +-1, +-1): the offset is r (sx, sy, sz) with r the
+// child's half-width, and Y_n^m(r k) = r^n Y_n^m(k).  Octant o has bit 0/1/2 set for a positive
+// x/y/z component.  Lanes stride over the tri(P) entries.
+__device__ __forceinline__ void octant_harmonics(int P, const double2 *__restrict__ octY, int o, double r, int lane,
+                                                 int nlanes, double2 *Y)
+{
+  for (int q = lane; q < tri(P); q += nlanes) {
+    int n = 0;
+    while (tri(n + 1) <= q) n++;
+    double f = 1.0;
+    for (int i = 0; i < n; i++) f *= r;
+    const double2 y = octY[o * tri(P) + q];
+    Y[q]            = make_double2(y.x * f, y.y * f);
+  }
+}
+
 // Y[n, mu] for any mu, from the m >= 0 table.
 __device__ __forceinline__ double2 ylook(const double2 *Y, int n, int mu)
 {
@@ -257,7 +274,8 @@ __global__ __launch_bounds__(M2M_MAXWV * WG) void k_m2m(int P, int NT, const int
                                                         int64_t ncell, const int32_t *__restrict__ cnchild,
                                                         const int32_t *__restrict__ cchild,
                                                         const double *__restrict__ cx, const double *__restrict__ cy,
-                                                        const double *__restrict__ cz, double2 *__restrict__ M)
+                                                        const double *__restrict__ cz, const double *__restrict__ cR,
+                                                        const double2 *__restrict__ octY, double2 *__restrict__ M)
 {
   extern __shared__ double2 sm2[];
   const int64_t w = blockIdx.x;
@@ -275,8 +293,8 @@ __global__ __launch_bounds__(M2M_MAXWV * WG) void k_m2m(int P, int NT, const int
     const bool on = ci < nc;  // uniform over the wavefront
     if (on) {
       const int ch = cchild[p] + ci;
-      const Sph s  = sph(cx[p] - cx[ch], cy[p] - cy[ch], cz[p] - cz[ch]);
-      if (lane < P) harmonic_column<false>(P, lane, s, Y);
+      const int o  = (cx[p] > cx[ch]) | ((cy[p] > cy[ch]) << 1) | ((cz[p] > cz[ch]) << 2);
+      octant_harmonics(P, octY, o, cR[ch], lane, WG, Y);
       for (int q = lane; q < NT; q += WG) Mc[q] = M[(int64_t)ch * NT + q];
     }
     __syncthreads();
@@ -331,6 +349,7 @@ __global__ __launch_bounds__(M2M_MAXWV * WG) void k_m2m(int P, int NT, const int
 __global__ __launch_bounds__(WG) void k_l2l(int P, int NT, const int32_t *__restrict__ cells, int64_t ncell,
                                             const int32_t *__restrict__ cparent, const double *__restrict__ cx,
                                             const double *__restrict__ cy, const double *__restrict__ cz,
+                                            const double *__restrict__ cR, const double2 *__restrict__ octY,
                                             double2 *__restrict__ L)
 {
   extern __shared__ double2 sm2[];
@@ -341,8 +360,8 @@ __global__ __launch_bounds__(WG) void k_l2l(int P, int NT, const int32_t *__rest
   const int c = cells[w], p = cparent[c];
   if (p < 0) return;
   const int lane = threadIdx.x, nout = tri(P);
-  const Sph s    = sph(cx[c] - cx[p], cy[c] - cy[p], cz[c] - cz[p]);
-  if (lane < P) harmonic_column<false>(P, lane, s, Y);
+  const int o    = (cx[c] > cx[p]) | ((cy[c] > cy[p]) << 1) | ((cz[c] > cz[p]) << 2);
+  octant_harmonics(P, octY, o, cR[c], lane, WG, Y);
   for (int q = lane; q < NT; q += WG) Lp[q] = L[(int64_t)p * NT + q];
   __syncthreads();
   for (int a = 0; a < QPL; a++) {
@@ -590,6 +609,15 @@ __global__ __launch_bounds__(WG) void k_l2p_adj_tgt(int P, int NT, const int32_t
 }
 
 // ------------------------------------------------------------------ M2L
+
+// Regular harmonics of the 8 unit octant vectors (octant_harmonics).
+__global__ void k_octant_table(int P, double2 *__restrict__ Y)
+{
+  const int o = blockIdx.x;
+  if (threadIdx.x < P)
+    harmonic_column<false>(P, threadIdx.x, sph((o & 1) ? 1.0 : -1.0, (o & 2) ? 1.0 : -1.0, (o & 4) ? 1.0 : -1.0),
+                           Y + o * tri(P));
+}
 
 // Singular harmonics of the integer lattice vectors k in [-K, K]^3 (k = 0 left zero), in the
 // extended-row layout of singular_column_ext: entry (J, mu) of vector k at Y[idx(k) P^2 + J^2 + J + mu].
@@ -971,6 +999,9 @@ void make_m2l_schedule(int P, M2LSchedule &s)
     s.ytab.reserve(sizeof(double2) * (size_t)w * w * w * P * P);
     k_m2l_table<<<w * w * w, WG>>>(P, s.ytab.as<double2>());
     UVLM_LAUNCH_CHECK();
+    s.octY.reserve(sizeof(double2) * 8 * P * (P + 1) / 2);
+    k_octant_table<<<8, WG>>>(P, s.octY.as<double2>());
+    UVLM_LAUNCH_CHECK();
     UVLM_HIP(hipDeviceSynchronize());
     s.ytab_P = P;
   }
@@ -994,7 +1025,7 @@ void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, dou
   UVLM_LAUNCH_CHECK();
 }
 
-void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M)
+void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M, const double2 *octY)
 {
   const int     NT = 3 * P * (P + 1) / 2;
   const int64_t o = t.lvl_off[level], n = t.lvl_off[level + 1] - o;
@@ -1004,18 +1035,18 @@ void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M)
   const int    nwv   = slice * M2M_MAXWV <= 65536 ? M2M_MAXWV : M2M_MAXWV / 2;
   const size_t lds   = std::max(slice * nwv, sizeof(double2) * 3 * WG * nwv);
   k_m2m<<<(int)n, nwv * WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.nchild.as<int32_t>(), t.child.as<int32_t>(),
-                                t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(), M);
+                                t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(), t.R.as<double>(), octY, M);
   UVLM_LAUNCH_CHECK();
 }
 
-void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L)
+void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L, const double2 *octY)
 {
   const int     NT = 3 * P * (P + 1) / 2;
   const int64_t o = t.lvl_off[level], n = t.lvl_off[level + 1] - o;
   if (n <= 0) return;
   const size_t lds = sizeof(double2) * (P * (P + 1) / 2 + NT);
   k_l2l<<<(int)n, WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.parent.as<int32_t>(), t.cx.as<double>(),
-                                t.cy.as<double>(), t.cz.as<double>(), L);
+                                t.cy.as<double>(), t.cz.as<double>(), t.R.as<double>(), octY, L);
   UVLM_LAUNCH_CHECK();
 }
 
