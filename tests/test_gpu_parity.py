@@ -146,7 +146,7 @@ def test_fmm_fp64_parity(orc, name):
     assert rel(L1, L0) < 1e-10
 
 
-@pytest.mark.parametrize("P,S", [(4, 0), (13, 0), (16, 0), (10, 5), (10, 6), (10, 12)])
+@pytest.mark.parametrize("P,S", [(4, 0), (12, 0), (13, 0), (16, 0), (10, 5), (10, 6), (10, 12)])
 def test_fmm_fp64_parity_orders_and_plans(orc, P, S, monkeypatch):
     """Expansion orders away from the default (P = 13 and 16 need a third coefficient per lane
     in P2M/M2M/L2L; P = 4 runs with a single home slot) and the M2L plan sizes 5, 6 and 12."""

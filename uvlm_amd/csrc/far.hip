@@ -846,13 +846,17 @@ __global__ __launch_bounds__(WG) void k_m2l_reduce(int NT, int64_t ncell, const 
 }
 
 // ------------------------------------------------------------------ L2P + stretching
-__global__ __launch_bounds__(WG) void k_l2p(int P, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
+// PC > 0 fixes the expansion order at compile time (fully unrolled sums with constant LDS offsets and
+// signs); PC = 0 takes it from Prt.
+template <int PC>
+__global__ __launch_bounds__(WG) void k_l2p(int Prt, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
                                             const int32_t *__restrict__ cbody, const int32_t *__restrict__ cnbody,
                                             const double *__restrict__ cx, const double *__restrict__ cy,
                                             const double *__restrict__ cz, const double2 *__restrict__ L,
                                             double *__restrict__ st, int64_t cap, int stretching, int formulation,
                                             int sensor_mode)
 {
+  const int P = PC > 0 ? PC : Prt;
   extern __shared__ double2 sm2[];
   double2      *Lc = sm2;
   const int64_t w  = blockIdx.x;
@@ -870,7 +874,11 @@ __global__ __launch_bounds__(WG) void k_l2p(int P, int NT, const int32_t *__rest
     for (int a = 0; a < 6; a++)
       for (int d = 0; d < 3; d++) o[a][d] = make_double2(0, 0);
     const int OJ[6] = {0, 1, 1, 2, 2, 2}, OK[6] = {0, 0, 1, 0, 1, 2};
+#pragma unroll(PC > 0 ? PC : 1)
     for (int mu = 0; mu < P; mu++) {
+      // keeps each column's coefficient reads inside it (hoisting them all out of the body loop
+      // needs more registers than a wavefront has)
+      asm volatile("" ::: "memory");
       double2 e = make_double2(1.0, 0.0);
       for (int t = 0; t < mu; t++) e = cmul(e, make_double2(s.cb, s.sb));
       double p = 1.0;
@@ -878,6 +886,7 @@ __global__ __launch_bounds__(WG) void k_l2p(int P, int NT, const int32_t *__rest
       double f = c_invfact[2 * mu];
       for (int t = 1; t <= mu; t++) f *= -s.rho;
       double pp = 0.0;
+#pragma unroll(PC > 0 ? PC : 1)
       for (int nu = mu; nu < P; nu++) {
         if (nu > mu) {
           const double pn = (nu == mu + 1) ? s.x * (2 * mu + 1) * p
@@ -1137,10 +1146,19 @@ void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *
 {
   const int NT = 3 * P * (P + 1) / 2;
   if (t.nleaf == 0) return;
-  k_l2p<<<(int)t.nleaf, WG, sizeof(double2) * NT, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(),
-                                                        t.nbody.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(),
-                                                        t.cz.as<double>(), L, st, cap, stretching, formulation,
-                                                        sensor_mode);
+  auto launch = [&](auto kern) {
+    kern<<<(int)t.nleaf, WG, sizeof(double2) * NT, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(),
+                                                       t.nbody.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(),
+                                                       t.cz.as<double>(), L, st, cap, stretching, formulation,
+                                                       sensor_mode);
+  };
+  switch (P) {
+  case 6: launch(k_l2p<6>); break;
+  case 8: launch(k_l2p<8>); break;
+  case 10: launch(k_l2p<10>); break;
+  case 12: launch(k_l2p<12>); break;
+  default: launch(k_l2p<0>);
+  }
   UVLM_LAUNCH_CHECK();
 }
 
