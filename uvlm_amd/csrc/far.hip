@@ -199,7 +199,9 @@ __device__ __forceinline__ double2 ylook(const double2 *Y, int n, int mu)
 }
 
 // ------------------------------------------------------------------ P2M
-__global__ __launch_bounds__(WG) void k_p2m(int P, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
+// PC > 0 fixes the expansion order at compile time; PC = 0 takes it from Prt.
+template <int PC>
+__global__ __launch_bounds__(WG) void k_p2m(int Prt, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
                                             const int32_t *__restrict__ cbody, const int32_t *__restrict__ cnbody,
                                             const double *__restrict__ cx, const double *__restrict__ cy,
                                             const double *__restrict__ cz, const double *__restrict__ st, int64_t cap,
@@ -209,6 +211,7 @@ __global__ __launch_bounds__(WG) void k_p2m(int P, int NT, const int32_t *__rest
   // (body - centre) to LDS; lane c of a body takes columns c and P-1-c, so every lane walks
   // about P+1 degrees.  Phase 2: lane <-> coefficient (n,m) adds alpha conj(Y_n^m) over the
   // chunk, in body order (multipoleKernels.hpp:305-327).
+  const int P = PC > 0 ? PC : Prt;
   extern __shared__ double2 smq[];
   const int64_t w = blockIdx.x;
   if (w >= nleaf) return;
@@ -1029,8 +1032,18 @@ void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, dou
   if (t.nleaf == 0) return;
   const int    B   = WG / ((P + 1) / 2), ny = P * (P + 1) / 2;  // bodies per chunk (k_p2m)
   const size_t lds = sizeof(double2) * B * ny + sizeof(double) * 3 * B;
-  k_p2m<<<(int)t.nleaf, WG, lds, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(), t.nbody.as<int32_t>(),
-                                      t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(), st, cap, M);
+  auto launch = [&](auto kern) {
+    kern<<<(int)t.nleaf, WG, lds, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(),
+                                       t.nbody.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(),
+                                       st, cap, M);
+  };
+  switch (P) {
+  case 6: launch(k_p2m<6>); break;
+  case 8: launch(k_p2m<8>); break;
+  case 10: launch(k_p2m<10>); break;
+  case 12: launch(k_p2m<12>); break;
+  default: launch(k_p2m<0>);
+  }
   UVLM_LAUNCH_CHECK();
 }
 
