@@ -273,13 +273,16 @@ __global__ __launch_bounds__(WG) void k_p2m(int Prt, int NT, const int32_t *__re
 // own LDS slice, lane <-> output coefficient (n,m)), so the children are translated side by side
 // instead of one after another; the wavefronts' sums are then added in wavefront order through LDS.
 constexpr int M2M_MAXWV = 8;
-__global__ __launch_bounds__(M2M_MAXWV * WG) void k_m2m(int P, int NT, const int32_t *__restrict__ cells,
+// PC > 0 fixes the expansion order at compile time; PC = 0 takes it from Prt.
+template <int PC>
+__global__ __launch_bounds__(M2M_MAXWV * WG) void k_m2m(int Prt, int NT, const int32_t *__restrict__ cells,
                                                         int64_t ncell, const int32_t *__restrict__ cnchild,
                                                         const int32_t *__restrict__ cchild,
                                                         const double *__restrict__ cx, const double *__restrict__ cy,
                                                         const double *__restrict__ cz, const double *__restrict__ cR,
                                                         const double2 *__restrict__ octY, double2 *__restrict__ M)
 {
+  const int P = PC > 0 ? PC : Prt;
   extern __shared__ double2 sm2[];
   const int64_t w = blockIdx.x;
   if (w >= ncell) return;
@@ -349,12 +352,14 @@ __global__ __launch_bounds__(M2M_MAXWV * WG) void k_m2m(int P, int NT, const int
 }
 
 // ------------------------------------------------------------------ L2L
-__global__ __launch_bounds__(WG) void k_l2l(int P, int NT, const int32_t *__restrict__ cells, int64_t ncell,
+template <int PC>
+__global__ __launch_bounds__(WG) void k_l2l(int Prt, int NT, const int32_t *__restrict__ cells, int64_t ncell,
                                             const int32_t *__restrict__ cparent, const double *__restrict__ cx,
                                             const double *__restrict__ cy, const double *__restrict__ cz,
                                             const double *__restrict__ cR, const double2 *__restrict__ octY,
                                             double2 *__restrict__ L)
 {
+  const int P = PC > 0 ? PC : Prt;
   extern __shared__ double2 sm2[];
   double2      *Y  = sm2;
   double2      *Lp = sm2 + tri(P);
@@ -1056,8 +1061,18 @@ void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M, const
   const size_t slice = sizeof(double2) * (P * (P + 1) / 2 + NT);
   const int    nwv   = slice * M2M_MAXWV <= 65536 ? M2M_MAXWV : M2M_MAXWV / 2;
   const size_t lds   = std::max(slice * nwv, sizeof(double2) * 3 * WG * nwv);
-  k_m2m<<<(int)n, nwv * WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.nchild.as<int32_t>(), t.child.as<int32_t>(),
-                                t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(), t.R.as<double>(), octY, M);
+  auto launch = [&](auto kern) {
+    kern<<<(int)n, nwv * WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.nchild.as<int32_t>(),
+                                       t.child.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(),
+                                       t.R.as<double>(), octY, M);
+  };
+  switch (P) {
+  case 6: launch(k_m2m<6>); break;
+  case 8: launch(k_m2m<8>); break;
+  case 10: launch(k_m2m<10>); break;
+  case 12: launch(k_m2m<12>); break;
+  default: launch(k_m2m<0>);
+  }
   UVLM_LAUNCH_CHECK();
 }
 
@@ -1067,8 +1082,17 @@ void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L, const
   const int64_t o = t.lvl_off[level], n = t.lvl_off[level + 1] - o;
   if (n <= 0) return;
   const size_t lds = sizeof(double2) * (P * (P + 1) / 2 + NT);
-  k_l2l<<<(int)n, WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.parent.as<int32_t>(), t.cx.as<double>(),
-                                t.cy.as<double>(), t.cz.as<double>(), t.R.as<double>(), octY, L);
+  auto launch = [&](auto kern) {
+    kern<<<(int)n, WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.parent.as<int32_t>(), t.cx.as<double>(),
+                                 t.cy.as<double>(), t.cz.as<double>(), t.R.as<double>(), octY, L);
+  };
+  switch (P) {
+  case 6: launch(k_l2l<6>); break;
+  case 8: launch(k_l2l<8>); break;
+  case 10: launch(k_l2l<10>); break;
+  case 12: launch(k_l2l<12>); break;
+  default: launch(k_l2l<0>);
+  }
   UVLM_LAUNCH_CHECK();
 }
 
