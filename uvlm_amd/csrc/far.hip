@@ -493,13 +493,16 @@ __global__ __launch_bounds__(WG) void k_p2m_adj(int P, int NT, const int32_t *__
 // Third derivatives of Re(w_k o[3,k] Y_3^k(h)) with o[3,k] = A_k + i B_k (w_0 = 1, w_k = 2):
 //   xxx = -3/4 A1 + 1/4 A3, xxy = 1/4 B1 - 1/4 B3, xxz = 1/2 A0 - 1/2 A2, xyy = -1/4 A1 - 1/4 A3,
 //   xyz = 1/2 B2, xzz = A1, yyy = 3/4 B1 + 1/4 B3, yyz = 1/2 A0 + 1/2 A2, yzz = -B1, zzz = -A0.
-__global__ __launch_bounds__(WG) void k_l2p_adj_tgt(int P, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
+// PC > 0 fixes the expansion order at compile time (fully unrolled sums); PC = 0 takes it from Prt.
+template <int PC>
+__global__ __launch_bounds__(WG) void k_l2p_adj_tgt(int Prt, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
                                                     const int32_t *__restrict__ cbody,
                                                     const int32_t *__restrict__ cnbody, const double *__restrict__ cx,
                                                     const double *__restrict__ cy, const double *__restrict__ cz,
                                                     const double2 *__restrict__ L, const double *__restrict__ st,
                                                     double *__restrict__ aux, int64_t cap)
 {
+  const int P = PC > 0 ? PC : Prt;
   extern __shared__ double2 sm2[];
   double2      *Lc = sm2;
   const int64_t w  = blockIdx.x;
@@ -516,7 +519,11 @@ __global__ __launch_bounds__(WG) void k_l2p_adj_tgt(int P, int NT, const int32_t
     for (int a = 0; a < NO; a++)
       for (int d = 0; d < 3; d++) o[a][d] = make_double2(0, 0);
     const int OJ[NO] = {0, 1, 1, 2, 2, 2, 3, 3, 3, 3}, OK[NO] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+#pragma unroll(PC > 0 ? PC : 1)
     for (int mu = 0; mu < P; mu++) {
+      // keeps each column's coefficient reads inside it (hoisted out of the body loop they would
+      // need more registers than a wavefront has)
+      asm volatile("" ::: "memory");
       double2 e = make_double2(1.0, 0.0);
       for (int t = 0; t < mu; t++) e = cmul(e, make_double2(s.cb, s.sb));
       double p = 1.0;
@@ -524,6 +531,7 @@ __global__ __launch_bounds__(WG) void k_l2p_adj_tgt(int P, int NT, const int32_t
       double f = c_invfact[2 * mu];
       for (int t = 1; t <= mu; t++) f *= -s.rho;
       double pp = 0.0;
+#pragma unroll(PC > 0 ? PC : 1)
       for (int nu = mu; nu < P; nu++) {
         if (nu > mu) {
           const double pn = (nu == mu + 1) ? s.x * (2 * mu + 1) * p
@@ -1215,10 +1223,18 @@ void l2p_adj_tgt(hipStream_t s, int P, const Tree &t, const double2 *L, const do
 {
   const int NT = 3 * P * (P + 1) / 2;
   if (t.nleaf == 0) return;
-  k_l2p_adj_tgt<<<(int)t.nleaf, WG, sizeof(double2) * NT, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf,
-                                                               t.body.as<int32_t>(), t.nbody.as<int32_t>(),
-                                                               t.cx.as<double>(), t.cy.as<double>(), t.cz.as<double>(),
-                                                               L, st, aux, cap);
+  auto launch = [&](auto kern) {
+    kern<<<(int)t.nleaf, WG, sizeof(double2) * NT, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(),
+                                                       t.nbody.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(),
+                                                       t.cz.as<double>(), L, st, aux, cap);
+  };
+  switch (P) {
+  case 6: launch(k_l2p_adj_tgt<6>); break;
+  case 8: launch(k_l2p_adj_tgt<8>); break;
+  case 10: launch(k_l2p_adj_tgt<10>); break;
+  case 12: launch(k_l2p_adj_tgt<12>); break;
+  default: launch(k_l2p_adj_tgt<0>);
+  }
   UVLM_LAUNCH_CHECK();
 }
 
