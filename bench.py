@@ -283,7 +283,7 @@ def main():
                     help="particles in the adjoint leg (three Gaussian blobs, 4096 probes, P=8; 0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
+        print("bench.py runs on one GPU only: the library has no multi-GPU path, so --gpus must be 1.",
               file=sys.stderr)
         sys.exit(2)
 
