@@ -403,6 +403,93 @@ __global__ __launch_bounds__(WG) void k_l2l(int Prt, int NT, const int32_t *__re
   }
 }
 
+// L2L at a fixed order with the work balanced over the lanes.  Output (j, k) of the child takes
+// terms (n, m) with n = j + d, m = k - d .. k + d (2d + 1 of them) for d = 0 .. P-1-j, so lane
+// (j, k) of k_l2l runs (P - j)^2 steps while the average is far lower.  Here the terms are taken
+// in groups of equal d: group d holds the tri(P - d) outputs with j < P - d, and when they are
+// few each output's 2d + 1 terms are split over s lanes (a power of two, aligned), summed by an
+// xor butterfly and handed to the output's own lane.  Groups run in ascending d (ascending n per
+// output), so every sum is taken in a fixed order.
+template <int PC>
+__global__ __launch_bounds__(WG) void k_l2l_bal(int, int NT, const int32_t *__restrict__ cells, int64_t ncell,
+                                                const int32_t *__restrict__ cparent, const double *__restrict__ cx,
+                                                const double *__restrict__ cy, const double *__restrict__ cz,
+                                                const double *__restrict__ cR, const double2 *__restrict__ octY,
+                                                double2 *__restrict__ L)
+{
+  constexpr int P = PC;
+  static_assert(P > 0 && P * (P + 1) / 2 <= QPL * WG, "fixed order");
+  extern __shared__ double2 sm2[];
+  double2      *Y  = sm2;
+  double2      *Lp = sm2 + tri(P);
+  const int64_t w  = blockIdx.x;
+  if (w >= ncell) return;
+  const int c = cells[w], p = cparent[c];
+  if (p < 0) return;
+  const int lane = threadIdx.x;
+  const int o    = (cx[c] > cx[p]) | ((cy[c] > cy[p]) << 1) | ((cz[c] > cz[p]) << 2);
+  octant_harmonics(P, octY, o, cR[c], lane, WG, Y);
+  for (int q = lane; q < NT; q += WG) Lp[q] = L[(int64_t)p * NT + q];
+  __syncthreads();
+  // terms i = first .. nterm-1 step 'step' of output q in group d, added to acc
+  auto terms = [&](int q, int d, int first, int step, double2 *acc) {
+    int j = 0;
+    while (tri(j + 1) <= q) j++;
+    const int k = q - tri(j), n = j + d;
+    for (int i = first; i < 2 * d + 1; i += step) {
+      const int     m = k - d + i;
+      const double2 y = ylook(Y, d, m - k);
+      if (m < 0) {
+        const double  sg = sgn(k);
+        const double2 ys = make_double2(y.x * sg, y.y * sg);
+        for (int e = 0; e < 3; e++) cfma(acc[e], conj2(Lp[3 * (tri(n) - m) + e]), ys);
+      } else {
+        const double  sg = (m < k) ? sgn(m - k) : 1.0;
+        const double2 ys = make_double2(y.x * sg, y.y * sg);
+        for (int e = 0; e < 3; e++) cfma(acc[e], Lp[3 * (tri(n) + m) + e], ys);
+      }
+    }
+  };
+  double2 acc[QPL][3];
+  for (int a = 0; a < QPL; a++)
+    for (int e = 0; e < 3; e++) acc[a][e] = make_double2(0, 0);
+  for (int d = 0; d < P; d++) {
+    const int ntask = tri(P - d), nterm = 2 * d + 1;
+    int       s     = 1;
+    while (s < 16 && 2 * s * ntask <= WG && s < nterm) s *= 2;
+    if (s == 1) {
+      for (int a = 0; a < QPL; a++) {
+        const int q = lane + a * WG;
+        if (q < ntask) terms(q, d, 0, 1, acc[a]);
+      }
+    } else {  // ntask <= WG / 2: outputs q < 32, all in slot a = 0
+      const int t = lane / s, sub = lane % s;
+      double2   part[3];
+      for (int e = 0; e < 3; e++) part[e] = make_double2(0, 0);
+      if (t < ntask) terms(t, d, sub, s, part);
+      for (int off = s / 2; off > 0; off /= 2)
+        for (int e = 0; e < 3; e++) {
+          part[e].x += __shfl_xor(part[e].x, off);
+          part[e].y += __shfl_xor(part[e].y, off);
+        }
+      const int src = lane < ntask ? lane * s : lane;
+      for (int e = 0; e < 3; e++) {
+        const double vx = __shfl(part[e].x, src), vy = __shfl(part[e].y, src);
+        if (lane < ntask) acc[0][e].x += vx, acc[0][e].y += vy;
+      }
+    }
+  }
+  for (int a = 0; a < QPL; a++) {
+    const int q = lane + a * WG;
+    if (q >= tri(P)) continue;
+    for (int e = 0; e < 3; e++) {
+      double2 &out = L[(int64_t)c * NT + 3 * q + e];
+      out.x += acc[a][e].x;
+      out.y += acc[a][e].y;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ adjoint of the particle rates
 // Adjoint sources of the far field (see uvlm_eval_vjp): at each body a vortex ubar (3) and
 // vortex dipoles Jbar (3x3, row d = component, column c = direction), so that
@@ -1095,10 +1182,10 @@ void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L, const
                                  t.cy.as<double>(), t.cz.as<double>(), t.R.as<double>(), octY, L);
   };
   switch (P) {
-  case 6: launch(k_l2l<6>); break;
-  case 8: launch(k_l2l<8>); break;
-  case 10: launch(k_l2l<10>); break;
-  case 12: launch(k_l2l<12>); break;
+  case 6: launch(k_l2l_bal<6>); break;
+  case 8: launch(k_l2l_bal<8>); break;
+  case 10: launch(k_l2l_bal<10>); break;
+  case 12: launch(k_l2l_bal<12>); break;
   default: launch(k_l2l<0>);
   }
   UVLM_LAUNCH_CHECK();
