@@ -352,73 +352,23 @@ __global__ __launch_bounds__(M2M_MAXWV * WG) void k_m2m(int Prt, int NT, const i
 }
 
 // ------------------------------------------------------------------ L2L
-template <int PC>
-__global__ __launch_bounds__(WG) void k_l2l(int Prt, int NT, const int32_t *__restrict__ cells, int64_t ncell,
-                                            const int32_t *__restrict__ cparent, const double *__restrict__ cx,
-                                            const double *__restrict__ cy, const double *__restrict__ cz,
-                                            const double *__restrict__ cR, const double2 *__restrict__ octY,
-                                            double2 *__restrict__ L)
-{
-  const int P = PC > 0 ? PC : Prt;
-  extern __shared__ double2 sm2[];
-  double2      *Y  = sm2;
-  double2      *Lp = sm2 + tri(P);
-  const int64_t w  = blockIdx.x;
-  if (w >= ncell) return;
-  const int c = cells[w], p = cparent[c];
-  if (p < 0) return;
-  const int lane = threadIdx.x, nout = tri(P);
-  const int o    = (cx[c] > cx[p]) | ((cy[c] > cy[p]) << 1) | ((cz[c] > cz[p]) << 2);
-  octant_harmonics(P, octY, o, cR[c], lane, WG, Y);
-  for (int q = lane; q < NT; q += WG) Lp[q] = L[(int64_t)p * NT + q];
-  __syncthreads();
-  for (int a = 0; a < QPL; a++) {
-    const int q = lane + a * WG;
-    if (q >= nout) continue;
-    int j = 0;
-    while (tri(j + 1) <= q) j++;
-    const int k = q - tri(j);
-    double2   acc[3];
-    for (int d = 0; d < 3; d++) acc[d] = make_double2(0, 0);
-    for (int n = j; n < P; n++) {
-      for (int m = j + k - n; m < 0; m++) {
-        const double2 y  = ylook(Y, n - j, m - k);
-        const double  sg = sgn(k);
-        const double2 ys = make_double2(y.x * sg, y.y * sg);
-        for (int d = 0; d < 3; d++) cfma(acc[d], conj2(Lp[3 * (tri(n) - m) + d]), ys);
-      }
-      for (int m = 0; m <= n; m++) {
-        if (n - j < abs(m - k)) continue;
-        const double2 y  = ylook(Y, n - j, m - k);
-        const double  sg = (m < k) ? sgn(m - k) : 1.0;
-        const double2 ys = make_double2(y.x * sg, y.y * sg);
-        for (int d = 0; d < 3; d++) cfma(acc[d], Lp[3 * (tri(n) + m) + d], ys);
-      }
-    }
-    for (int d = 0; d < 3; d++) {
-      double2 &o = L[(int64_t)c * NT + 3 * q + d];
-      o.x += acc[d].x;
-      o.y += acc[d].y;
-    }
-  }
-}
 
-// L2L at a fixed order with the work balanced over the lanes.  Output (j, k) of the child takes
-// terms (n, m) with n = j + d, m = k - d .. k + d (2d + 1 of them) for d = 0 .. P-1-j, so lane
-// (j, k) of k_l2l runs (P - j)^2 steps while the average is far lower.  Here the terms are taken
+// L2L with the work balanced over the lanes.  Output (j, k) of the child takes
+// terms (n, m) with n = j + d, m = k - d .. k + d (2d + 1 of them) for d = 0 .. P-1-j, so one lane
+// per output would run (P - j)^2 steps while the average is far lower.  Here the terms are taken
 // in groups of equal d: group d holds the tri(P - d) outputs with j < P - d, and when they are
 // few each output's 2d + 1 terms are split over s lanes (a power of two, aligned), summed by an
 // xor butterfly and handed to the output's own lane.  Groups run in ascending d (ascending n per
 // output), so every sum is taken in a fixed order.
+// PC > 0 fixes the expansion order at compile time; PC = 0 takes it from Prt.
 template <int PC>
-__global__ __launch_bounds__(WG) void k_l2l_bal(int, int NT, const int32_t *__restrict__ cells, int64_t ncell,
+__global__ __launch_bounds__(WG) void k_l2l_bal(int Prt, int NT, const int32_t *__restrict__ cells, int64_t ncell,
                                                 const int32_t *__restrict__ cparent, const double *__restrict__ cx,
                                                 const double *__restrict__ cy, const double *__restrict__ cz,
                                                 const double *__restrict__ cR, const double2 *__restrict__ octY,
                                                 double2 *__restrict__ L)
 {
-  constexpr int P = PC;
-  static_assert(P > 0 && P * (P + 1) / 2 <= QPL * WG, "fixed order");
+  const int P = PC > 0 ? PC : Prt;
   extern __shared__ double2 sm2[];
   double2      *Y  = sm2;
   double2      *Lp = sm2 + tri(P);
@@ -1186,7 +1136,7 @@ void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L, const
   case 8: launch(k_l2l_bal<8>); break;
   case 10: launch(k_l2l_bal<10>); break;
   case 12: launch(k_l2l_bal<12>); break;
-  default: launch(k_l2l<0>);
+  default: launch(k_l2l_bal<0>);
   }
   UVLM_LAUNCH_CHECK();
 }
