@@ -76,6 +76,7 @@ def lib():
             "orc_p2m": (None, [I, P, L64, P, P, P]),
             "orc_m2m": (None, [I, P, P, P, P]),
             "orc_m2l": (None, [I, P, P, P, P]),
+            "orc_m2l_order": (None, [I, I, P, P, P, P]),
             "orc_l2l": (None, [I, P, P, P, P]),
             "orc_l2p": (None, [I, P, P, L64, P, P, P]),
             "orc_direct": (None, [L64, P, L64, P, P, P, P, P]),
@@ -265,9 +266,15 @@ def m2m(P, Xp, Xc, Mc):
     return Mp
 
 
-def m2l(P, Xt, Xs, Ms):
+def m2l(P, Xt, Xs, Ms, harmonic_order=None):
+    """M2L of one pair.  harmonic_order (tests only) fills the singular harmonics up to degree
+    harmonic_order-1 instead of P-1: a deliberately different truncation."""
     Lt = np.zeros(nterm(P), dtype=np.complex128)
-    lib().orc_m2l(P, _p(_c(Xt)), _p(_c(Xs)), _p(_c(Ms, np.complex128).view(np.float64)), _p(Lt.view(np.float64)))
+    if harmonic_order is None:
+        lib().orc_m2l(P, _p(_c(Xt)), _p(_c(Xs)), _p(_c(Ms, np.complex128).view(np.float64)), _p(Lt.view(np.float64)))
+    else:
+        lib().orc_m2l_order(P, int(harmonic_order), _p(_c(Xt)), _p(_c(Xs)),
+                            _p(_c(Ms, np.complex128).view(np.float64)), _p(Lt.view(np.float64)))
     return Lt
 
 

@@ -450,14 +450,17 @@ struct Ops {
   }
 
   // M2L: multipoleKernels.hpp:518-552.  Ynm2 has 4P^2 entries, zero beyond degree P-1.
-  void m2l(const real_t *Xt, const real_t *Xs, const complex_t *Ms, complex_t *Lt) const
+  // harm_order > 0 fills the singular harmonics up to degree harm_order-1 instead (at most
+  // 2P-1): a deliberately different truncation that only the tests use, to show that the
+  // truncation check tells the two apart.
+  void m2l(const real_t *Xt, const real_t *Xs, const complex_t *Ms, complex_t *Lt, int harm_order = 0) const
   {
     std::vector<complex_t> Y(4 * P * P, complex_t(0, 0));
     real_t                 dX[3];
     for (int d = 0; d < 3; d++) dX[d] = Xt[d] - Xs[d];
     real_t rho, alpha, beta;
     cart2sph(dX, rho, alpha, beta);
-    evalLocal(P, rho, alpha, beta, Y.data());
+    evalLocal(harm_order > 0 ? std::min(harm_order, 2 * P - 1) : P, rho, alpha, beta, Y.data());
     for (int j = 0; j < P; j++) {
       const real_t Cnm = oddOrEven(j);
       for (int k = 0; k <= j; k++) {
@@ -1305,6 +1308,11 @@ void orc_m2m(int P, const double *Xp, const double *Xc, const double *Mc, double
 void orc_m2l(int P, const double *Xt, const double *Xs, const double *Ms, double *Lt)
 {
   Ops(P).m2l(Xt, Xs, (const complex_t *)Ms, (complex_t *)Lt);
+}
+// M2L with the singular harmonics filled up to degree harm_order-1 (tests only, see Ops::m2l).
+void orc_m2l_order(int P, int harm_order, const double *Xt, const double *Xs, const double *Ms, double *Lt)
+{
+  Ops(P).m2l(Xt, Xs, (const complex_t *)Ms, (complex_t *)Lt, harm_order);
 }
 void orc_l2l(int P, const double *Xp, const double *Xc, const double *Lp, double *Lc)
 {
