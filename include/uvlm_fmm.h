@@ -137,6 +137,18 @@ int uvlm_sensors_aos(uvlm_handle h, void *probes, int64_t m, int64_t stride);
  * the device P2P formula (multipoleKernels.hpp:223-301); coincident points are skipped. */
 int uvlm_direct(uvlm_handle h, int64_t m, const double *xt, const double *at, double *u, double *J, double *dadt);
 
+/* uvlm_direct without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernel and results as uvlm_direct;
+ * the targets and outputs are staged in a static device array of the library instead of a
+ * per-call buffer, so m <= UVLM_DIRECT_NOALLOC_MAX (larger m fails with UVLM_ERR_ARG).  The
+ * array is 2.75 MB, one copy per GPU the library runs on, held until the process ends; its
+ * address is taken when the first handle on that GPU is created.  All calls on one GPU share
+ * it: calls that overlap in time on one GPU (from several threads or handles) race. */
+#define UVLM_DIRECT_NOALLOC_MAX 16384
+int uvlm_direct_noalloc(uvlm_handle h, int64_t m, const double *xt, const double *at, double *u, double *J,
+                        double *dadt);
+
 /* Adjoint of the probe objective J = sum_p w_p . u(x_p), with u the velocity that
  * uvlm_sensors_aos computes at probe p from the current particles (the same interaction
  * lists, transposed).  Outputs, in the current tree order of the particles:

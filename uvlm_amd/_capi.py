@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libuvlm_fmm.so")
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEPTH", 3: "ERR_STATE", 4: "ERR_HIP", 5: "ERR_NOMEM", 6: "ERR_INTERNAL"}
 
 # Every entry point declared in include/uvlm_fmm.h.
+DIRECT_NOALLOC_MAX = 16384  # UVLM_DIRECT_NOALLOC_MAX
 EXPORTS = [
     "uvlm_last_error", "uvlm_abi_version", "uvlm_create", "uvlm_destroy", "uvlm_set_bodies_aos",
     "uvlm_get_bodies_aos", "uvlm_set_bodies_soa", "uvlm_get_bodies_soa", "uvlm_get_permutation",
@@ -21,7 +22,7 @@ EXPORTS = [
     "uvlm_set_viscosity", "uvlm_viscosity", "uvlm_vorticity_aos", "uvlm_rbf_aos", "uvlm_eval_vjp",
     "uvlm_step_vjp", "uvlm_trajectory_forward", "uvlm_trajectory_vjp", "uvlm_append_bodies_soa",
     "uvlm_wing_create", "uvlm_wing_destroy", "uvlm_wing_get", "uvlm_wing_translate", "uvlm_wing_solve",
-    "uvlm_wing_shed", "uvlm_wing_step", "uvlm_wing_lift_adjoint", "uvlm_num_bodies",
+    "uvlm_wing_shed", "uvlm_wing_step", "uvlm_wing_lift_adjoint", "uvlm_num_bodies", "uvlm_direct_noalloc",
 ]
 
 
@@ -80,6 +81,7 @@ def lib():
             "uvlm_get_expansions": (I, [VP, PD, PD]),
             "uvlm_sensors_aos": (I, [VP, VP, I64, I64]),
             "uvlm_direct": (I, [VP, I64, PD, PD, PD, PD, PD]),
+            "uvlm_direct_noalloc": (I, [VP, I64, PD, PD, PD, PD, PD]),
             "uvlm_probe_adjoint": (I, [VP, I64, PD, PD, PD, PD]),
             "uvlm_update_bodies_soa": (I, [VP, I64, PD, PD]),
             "uvlm_relax": (I, [VP, D]),

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -604,6 +605,63 @@ void check_handle(uvlm_handle h)
   if (!h) throw Error(UVLM_ERR_ARG, "null handle");
 }
 
+// Device staging of one direct sum: targets, their strengths, u, J and dadt ([m][3], [m][3],
+// [m][3], [m][9], [m][3]) in consecutive blocks, the absent ones skipped.  The kernel writes
+// every element of the outputs it is given, so nothing in `d` has to start at zero.
+void direct_staged(uvlm_handle h, int64_t m, const double *xt, const double *at, double *u, double *J, double *dadt,
+                   double *d)
+{
+  const int64_t nin = 3 * m + (at ? 3 * m : 0);
+  double       *dxt = d, *dat = at ? d + 3 * m : nullptr;
+  double *du = d + nin, *dJ = J ? du + 3 * m : nullptr, *dd = dadt ? du + 3 * m + (J ? 9 * m : 0) : nullptr;
+  UVLM_HIP(hipMemcpyAsync(dxt, xt, 24 * m, hipMemcpyHostToDevice, h->s));
+  if (at) UVLM_HIP(hipMemcpyAsync(dat, at, 24 * m, hipMemcpyHostToDevice, h->s));
+  direct(h->s, m, dxt, dat, h->st.as<double>(), h->cap, h->N, du, dJ, dd);
+  UVLM_HIP(hipMemcpyAsync(u, du, 24 * m, hipMemcpyDeviceToHost, h->s));
+  if (J) UVLM_HIP(hipMemcpyAsync(J, dJ, 72 * m, hipMemcpyDeviceToHost, h->s));
+  if (dadt) UVLM_HIP(hipMemcpyAsync(dadt, dd, 24 * m, hipMemcpyDeviceToHost, h->s));
+  UVLM_HIP(hipStreamSynchronize(h->s));
+}
+
+void check_direct_args(uvlm_handle h, int64_t m, const double *xt, const double *at, const double *u,
+                       const double *dadt)
+{
+  check_handle(h);
+  if (h->N <= 0) throw Error(UVLM_ERR_STATE, "no bodies set");
+  if (m <= 0 || !xt || !u) throw Error(UVLM_ERR_ARG, "null or empty targets");
+  if (dadt && !at) throw Error(UVLM_ERR_ARG, "dadt needs target strengths");
+}
+
+}  // namespace
+
+// Staging of uvlm_direct_noalloc, one copy per GPU for the life of the process.  In host code
+// `scratch` only names the symbol; its device address on each GPU comes from
+// hipGetSymbolAddress (scratch_setup, called by uvlm_create with the handle's device current).
+static __device__ __align__(256) double scratch[(size_t)UVLM_DIRECT_NOALLOC_MAX * 21];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex    g_scratch_mu;
+double       *g_scratch_dev[kMaxDevices] = {};
+
+void scratch_setup(int device)
+{
+  if (device < 0 || device >= kMaxDevices) throw Error(UVLM_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return;
+  void *p = nullptr;
+  UVLM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw Error(UVLM_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+  g_scratch_dev[device] = static_cast<double *>(p);
+}
+
+double *scratch_on(int device)
+{
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  double *p = (device >= 0 && device < kMaxDevices) ? g_scratch_dev[device] : nullptr;
+  if (!p) throw Error(UVLM_ERR_STATE, "scratch array not set up on this device (create a handle on it first)");
+  return p;
+}
 }  // namespace
 
 extern "C" {
@@ -628,6 +686,7 @@ int uvlm_create(const uvlm_params *p, uvlm_handle *out)
     h->prm = *p;
     h->NT  = 3 * p->P * (p->P + 1) / 2;
     set_device(h);
+    scratch_setup(p->device);
     UVLM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     for (auto &e : h->ev) UVLM_HIP(hipEventCreate(&e));
     for (auto &e : h->aev) UVLM_HIP(hipEventCreate(&e));
@@ -1591,24 +1650,25 @@ int uvlm_update_bodies_soa(uvlm_handle h, int64_t n, const double *X, const doub
 int uvlm_direct(uvlm_handle h, int64_t m, const double *xt, const double *at, double *u, double *J, double *dadt)
 {
   return guarded([&] {
-    check_handle(h);
-    if (h->N <= 0) throw Error(UVLM_ERR_STATE, "no bodies set");
-    if (m <= 0 || !xt || !u) throw Error(UVLM_ERR_ARG, "null or empty targets");
-    if (dadt && !at) throw Error(UVLM_ERR_ARG, "dadt needs target strengths");
+    check_direct_args(h, m, xt, at, u, dadt);
     set_device(h);
-    DevBuf buf;
+    DevBuf        buf;
     const int64_t nin = 3 * m + (at ? 3 * m : 0), nout = 3 * m + (J ? 9 * m : 0) + (dadt ? 3 * m : 0);
     buf.reserve(sizeof(double) * (nin + nout));
-    double *d   = buf.as<double>();
-    double *dxt = d, *dat = at ? d + 3 * m : nullptr;
-    double *du = d + nin, *dJ = J ? du + 3 * m : nullptr, *dd = dadt ? du + 3 * m + (J ? 9 * m : 0) : nullptr;
-    UVLM_HIP(hipMemcpyAsync(dxt, xt, 24 * m, hipMemcpyHostToDevice, h->s));
-    if (at) UVLM_HIP(hipMemcpyAsync(dat, at, 24 * m, hipMemcpyHostToDevice, h->s));
-    direct(h->s, m, dxt, dat, h->st.as<double>(), h->cap, h->N, du, dJ, dd);
-    UVLM_HIP(hipMemcpyAsync(u, du, 24 * m, hipMemcpyDeviceToHost, h->s));
-    if (J) UVLM_HIP(hipMemcpyAsync(J, dJ, 72 * m, hipMemcpyDeviceToHost, h->s));
-    if (dadt) UVLM_HIP(hipMemcpyAsync(dadt, dd, 24 * m, hipMemcpyDeviceToHost, h->s));
-    UVLM_HIP(hipStreamSynchronize(h->s));
+    direct_staged(h, m, xt, at, u, J, dadt, buf.as<double>());
+  });
+}
+
+int uvlm_direct_noalloc(uvlm_handle h, int64_t m, const double *xt, const double *at, double *u, double *J,
+                        double *dadt)
+{
+  return guarded([&] {
+    check_direct_args(h, m, xt, at, u, dadt);
+    if (m > UVLM_DIRECT_NOALLOC_MAX)
+      throw Error(UVLM_ERR_ARG, "uvlm_direct_noalloc takes at most " + std::to_string(UVLM_DIRECT_NOALLOC_MAX) +
+                                    " targets, got " + std::to_string(m));
+    set_device(h);
+    direct_staged(h, m, xt, at, u, J, dadt, scratch_on(h->prm.device));
   });
 }
 

@@ -155,16 +155,18 @@ class FMMCalculator:
         assert s.flags.c_contiguous and s.dtype == np.float64 and s.shape[1] == BODY_DOUBLES
         _capi.check(self.lib.uvlm_sensors_aos(self._h, _dp(s), s.shape[0], BODY_BYTES))
 
-    def direct(self, xt, at=None, want_J=True):
-        """FP64 direct sum from the current bodies to points xt: (u, J or None, dadt or None)."""
+    def direct(self, xt, at=None, want_J=True, noalloc=False):
+        """FP64 direct sum from the current bodies to points xt: (u, J or None, dadt or None).
+        noalloc=True calls uvlm_direct_noalloc (static device staging, at most 16,384 targets)."""
         xt = np.ascontiguousarray(xt, dtype=np.float64)
         m = xt.shape[0]
         u = np.zeros((m, 3))
         J = np.zeros((m, 9)) if want_J else None
         D = np.zeros((m, 3)) if at is not None else None
         a = np.ascontiguousarray(at, dtype=np.float64) if at is not None else None
-        _capi.check(self.lib.uvlm_direct(self._h, m, _dp(xt), _dp(a) if a is not None else None, _dp(u),
-                                         _dp(J) if J is not None else None, _dp(D) if D is not None else None))
+        fn = self.lib.uvlm_direct_noalloc if noalloc else self.lib.uvlm_direct
+        _capi.check(fn(self._h, m, _dp(xt), _dp(a) if a is not None else None, _dp(u),
+                       _dp(J) if J is not None else None, _dp(D) if D is not None else None))
         return u, J, D
 
     def update_arrays(self, X=None, alpha=None):
