@@ -17,8 +17,10 @@
 #ifndef UVLM_FMM_CALCULATOR_HPP
 #define UVLM_FMM_CALCULATOR_HPP
 
+#include <algorithm>
 #include <array>
 #include <cstddef>
+#include <cstdint>
 #include <fstream>
 #include <stdexcept>
 #include <string>
@@ -30,6 +32,13 @@
 
 namespace exafmm {
 typedef double real_t;
+
+// The reference's compile-time defaults (exafmmTypes.hpp:12-15).  Here they are only defaults:
+// FMMCalculator, buildTree and getList take P, ncrit and theta as arguments.
+static constexpr int    P         = 4;
+static constexpr int    NTERM     = 3 * (P * (P + 1)) / 2;
+static constexpr size_t ncrit     = 128;
+static constexpr real_t MAC_theta = 0.3;
 
 struct Body {
   real_t J[3][3]{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
@@ -128,6 +137,49 @@ inline Cells buildTree(Bodies &bodies, size_t ncrit = 128, int device = 0)
   return detail::fetch_cells(h.h, bodies);
 }
 
+// Dual-tree traversal of traverse.hpp:32-85 from (targetRoot, sourceRoot): the P2P and M2L pair
+// lists as (target cell index, source cell index), sorted by target then source (the reference
+// returns the same pairs in its stack order).  The roots are the first cells of Cells vectors such
+// as buildTree returns (children after their parent, CHILD pointers into the same vector).  The
+// traversal runs on the GPU.
+typedef std::vector<std::pair<size_t, size_t>> PairList;
+inline std::pair<PairList, PairList> getList(const Cell *targetRoot, const Cell *sourceRoot, real_t theta = MAC_theta,
+                                             int device = 0)
+{
+  auto pack = [](const Cell *root, std::vector<int64_t> &ints, std::vector<double> &reals) {
+    // cell count: the root plus every child group reachable from it
+    size_t C = 1;
+    for (size_t c = 0; c < C; c++)
+      if (root[c].NCHILD) C = std::max(C, (size_t)(root[c].CHILD - root) + root[c].NCHILD);
+    ints.resize(5 * C);
+    reals.resize(4 * C);
+    for (size_t c = 0; c < C; c++) {
+      const Cell &q   = root[c];
+      ints[5 * c + 0] = (int64_t)q.NCHILD;
+      ints[5 * c + 1] = (int64_t)q.NBODY;
+      ints[5 * c + 2] = q.NCHILD ? (int64_t)(q.CHILD - root) : 0;
+      ints[5 * c + 3] = (int64_t)q.BodyOffset;
+      ints[5 * c + 4] = (int64_t)q.level;
+      for (int d = 0; d < 3; d++) reals[4 * c + d] = q.X[d];
+      reals[4 * c + 3] = q.R;
+    }
+    return (int64_t)C;
+  };
+  std::vector<int64_t> ti, si;
+  std::vector<double>  tr, sr;
+  const int64_t        ct = pack(targetRoot, ti, tr), cs = pack(sourceRoot, si, sr);
+  int64_t np = 0, nm = 0;  // sizes first, then the pairs (the traversal runs twice)
+  detail::check(uvlm_get_list(device, theta, ct, ti.data(), tr.data(), cs, si.data(), sr.data(), 0, 0, &np, &nm,
+                              nullptr, nullptr));
+  std::vector<int64_t> p2p(2 * np + 2), m2l(2 * nm + 2);
+  detail::check(uvlm_get_list(device, theta, ct, ti.data(), tr.data(), cs, si.data(), sr.data(), np, nm, &np, &nm,
+                              p2p.data(), m2l.data()));
+  PairList P2P((size_t)np), M2L((size_t)nm);
+  for (int64_t i = 0; i < np; i++) P2P[i] = {(size_t)p2p[2 * i], (size_t)p2p[2 * i + 1]};
+  for (int64_t i = 0; i < nm; i++) M2L[i] = {(size_t)m2l[2 * i], (size_t)m2l[2 * i + 1]};
+  return {std::move(P2P), std::move(M2L)};
+}
+
 // FP64 direct sum of jbodies onto bodies: velocity += u, dadt += J.alpha (traverse.hpp:88-98).
 inline void direct(Bodies &bodies, Bodies &jbodies, int device = 0)
 {
@@ -219,21 +271,21 @@ struct FMMCalculator {
   void init(exafmm::Bodies &bodies)
   {
     copyBodies2Device(bodies);
-    exafmm::detail::check(uvlm_build(h_.h));
+    call(uvlm_build(h_.h));
     copyBodies2Host(bodies);
   }
   // fmmCalculator.hpp:420-467
-  void calculate() { exafmm::detail::check(uvlm_eval(h_.h)); }
+  void calculate() { call(uvlm_eval(h_.h)); }
   // fmmCalculator.hpp:713-724 (inviscid: nothing to do)
   // (PSE: dadt += exchange over the near pairs; CSM: drdt += core spreading over the dt of the
   //  last advance(), or setTimeStep())
-  void calculateViscosity() { exafmm::detail::check(uvlm_viscosity(h_.h, dt_)); }
+  void calculateViscosity() { call(uvlm_viscosity(h_.h, dt_)); }
   void setTimeStep(exafmm::real_t dt) { dt_ = dt; }
   // calculateVorticityDualTree (fmmCalculator.hpp:746-753) at `sensors` (after init()):
   // velocity_old = -(Gaussian vorticity); the sensors come back in their tree order.
   void calculateVorticity(exafmm::Bodies &sensors)
   {
-    exafmm::detail::check(uvlm_vorticity_aos(h_.h, sensors.data(), (int64_t)sensors.size(), sizeof(exafmm::Body)));
+    call(uvlm_vorticity_aos(h_.h, sensors.data(), (int64_t)sensors.size(), sizeof(exafmm::Body)));
   }
   // fmmCalculator.hpp:855-917: init(particles), then the strengths of newPoints fitted to the
   // particles' vorticity; newPoints keeps the points above the vorticity threshold (tree order).
@@ -241,7 +293,7 @@ struct FMMCalculator {
   {
     init(particles);
     int64_t kept = 0;
-    exafmm::detail::check(uvlm_rbf_aos(h_.h, newPoints.data(), (int64_t)newPoints.size(), sizeof(exafmm::Body), &kept,
+    call(uvlm_rbf_aos(h_.h, newPoints.data(), (int64_t)newPoints.size(), sizeof(exafmm::Body), &kept,
                                        nullptr, nullptr));
     newPoints.resize((size_t)kept);
   }
@@ -250,17 +302,17 @@ struct FMMCalculator {
   {
     dt_ = dt;
     copyBodies2Device(bodies);
-    exafmm::detail::check(uvlm_advance(h_.h, dt));
+    call(uvlm_advance(h_.h, dt));
     copyBodies2Host(bodies);
   }
   // fmmCalculator.hpp:726-731: uses the multipoles of the last calculate().
   void getSensorData(exafmm::Bodies &sensors)
   {
-    exafmm::detail::check(uvlm_sensors_aos(h_.h, sensors.data(), (int64_t)sensors.size(), sizeof(exafmm::Body)));
+    call(uvlm_sensors_aos(h_.h, sensors.data(), (int64_t)sensors.size(), sizeof(exafmm::Body)));
   }
   void copyBodies2Device(exafmm::Bodies &bodies)
   {
-    exafmm::detail::check(
+    call(
         uvlm_set_bodies_aos(h_.h, bodies.data(), (int64_t)bodies.size(), sizeof(exafmm::Body)));
     n_ = bodies.size();
   }
@@ -272,6 +324,34 @@ struct FMMCalculator {
     bodies.resize(n_);
     exafmm::detail::check(uvlm_get_bodies_aos(h_.h, bodies.data(), (int64_t)n_, sizeof(exafmm::Body)));
   }
+  // Array form of copyBodies2Device: positions and strengths [n][3], core sizes [n], in the
+  // caller's order (the "original" order of the calls below that use it).
+  void setArrays(const std::vector<double> &X, const std::vector<double> &alpha, const std::vector<double> &sigma)
+  {
+    if (X.size() != alpha.size() || X.size() != 3 * sigma.size()) throw std::runtime_error("X, alpha must be [n][3]");
+    call(uvlm_set_bodies_soa(h_.h, (int64_t)sigma.size(), X.data(), alpha.data(), sigma.data()));
+    n_ = sigma.size();
+  }
+  // The current state in tree order ([n][3] arrays, sigma [n]); null outputs are skipped.
+  void getArrays(std::vector<double> *X, std::vector<double> *alpha, std::vector<double> *sigma,
+                 std::vector<double> *velocity = nullptr, std::vector<double> *dadt = nullptr) const
+  {
+    for (std::vector<double> *v : {X, alpha, velocity, dadt})
+      if (v) v->assign(3 * n_, 0.0);
+    if (sigma) sigma->assign(n_, 0.0);
+    auto p = [](std::vector<double> *v) { return v ? v->data() : nullptr; };
+    exafmm::detail::check(uvlm_get_bodies_soa(h_.h, (int64_t)n_, p(X), p(alpha), p(sigma), p(velocity), nullptr,
+                                              p(dadt), nullptr));
+  }
+  // perm[i] = caller index of the body at tree position i.
+  std::vector<int64_t> permutation() const
+  {
+    std::vector<int64_t> perm(n_);
+    exafmm::detail::check(uvlm_get_permutation(h_.h, (int64_t)n_, perm.data()));
+    return perm;
+  }
+  // Tree + lists of the bodies on the device (init() without the host copies).
+  void build() { call(uvlm_build(h_.h)); }
   // Interaction lists as (target cell, source cell) pairs (the reference keeps them in
   // _P2PPairsView / _M2LPairsView, fmmCalculator.hpp:50-51).
   std::pair<std::vector<std::pair<size_t, size_t>>, std::vector<std::pair<size_t, size_t>>> lists() const
@@ -296,7 +376,7 @@ struct FMMCalculator {
   // Calls with no counterpart in the reference class (include/uvlm_fmm.h gives the reasons).
   // Pedrizzetti relaxation toward curl u of the last calculate(), and the invariants
   // {impulse[3], angular impulse[3], energy, sum alpha[3]} of the current particles.
-  void relax(exafmm::real_t f) { exafmm::detail::check(uvlm_relax(h_.h, f)); }
+  void relax(exafmm::real_t f) { call(uvlm_relax(h_.h, f)); }
   std::array<double, 10> diagnostics() const
   {
     std::array<double, 10> out{};
@@ -310,7 +390,7 @@ struct FMMCalculator {
     if (probes.size() != w.size() || probes.size() % 3) throw std::runtime_error("probes and w must be [m][3]");
     dJ_dalpha.assign(3 * n_, 0.0);
     dJ_dx.assign(3 * n_, 0.0);
-    exafmm::detail::check(uvlm_probe_adjoint(h_.h, (int64_t)(probes.size() / 3), probes.data(), w.data(),
+    call(uvlm_probe_adjoint(h_.h, (int64_t)(probes.size() / 3), probes.data(), w.data(),
                                              dJ_dalpha.data(), dJ_dx.data()));
   }
   // xbar, alphabar ([n][3], tree order) of S = sum ubar.u + abar.dadt on the current tree
@@ -321,26 +401,26 @@ struct FMMCalculator {
     if (ubar.size() != 3 * n_ || abar.size() != 3 * n_) throw std::runtime_error("ubar and abar must be [n][3]");
     xbar.assign(3 * n_, 0.0);
     alphabar.assign(3 * n_, 0.0);
-    exafmm::detail::check(uvlm_eval_vjp(h_.h, (int64_t)n_, ubar.data(), abar.data(), xbar.data(), alphabar.data()));
+    call(uvlm_eval_vjp(h_.h, (int64_t)n_, ubar.data(), abar.data(), xbar.data(), alphabar.data()));
   }
   // One advance(dt) backwards from the device state: cotangents ([n][3], ORIGINAL body order)
   // of the state after the step in, those of the state before it out.
   void stepVJP(exafmm::real_t dt, std::vector<double> &xbar, std::vector<double> &alphabar)
   {
     if (xbar.size() != 3 * n_ || alphabar.size() != 3 * n_) throw std::runtime_error("cotangents must be [n][3]");
-    exafmm::detail::check(uvlm_step_vjp(h_.h, dt, (int64_t)n_, xbar.data(), alphabar.data()));
+    call(uvlm_step_vjp(h_.h, dt, (int64_t)n_, xbar.data(), alphabar.data()));
   }
   // Checkpointed trajectory: `steps` advances of dt from the device state with checkpoints every
   // `every` steps, then (adjointTrajectory) the cotangent of the final state mapped back to the
   // initial one, in place (original body order).
   void forwardTrajectory(size_t steps, exafmm::real_t dt, size_t every)
   {
-    exafmm::detail::check(uvlm_trajectory_forward(h_.h, (int64_t)steps, dt, (int64_t)every));
+    call(uvlm_trajectory_forward(h_.h, (int64_t)steps, dt, (int64_t)every));
   }
   void adjointTrajectory(std::vector<double> &xbar, std::vector<double> &alphabar)
   {
     if (xbar.size() != 3 * n_ || alphabar.size() != 3 * n_) throw std::runtime_error("cotangents must be [n][3]");
-    exafmm::detail::check(uvlm_trajectory_vjp(h_.h, (int64_t)n_, xbar.data(), alphabar.data()));
+    call(uvlm_trajectory_vjp(h_.h, (int64_t)n_, xbar.data(), alphabar.data()));
   }
   // Leaf size (and M2L plan size) for the bodies last passed to init()/copyBodies2Device();
   // returns the chosen ncrit.  init() again before calculate().
@@ -348,11 +428,20 @@ struct FMMCalculator {
   {
     int64_t best = 0;
     double  ms   = 0.0;
-    exafmm::detail::check(uvlm_autotune(h_.h, candidates.data(), (int)candidates.size(), reps, &best, &ms));
+    call(uvlm_autotune(h_.h, candidates.data(), (int)candidates.size(), reps, &best, &ms));
     return (size_t)best;
   }
 
+  // Incremented by every call that changes the device state (the autograd-style functions use it
+  // to tell whether the calculator still holds their forward pass).
+  uint64_t version() const { return version_; }
+
 private:
+  void call(int rc)
+  {
+    ++version_;
+    exafmm::detail::check(rc);
+  }
   static uvlm_params detail_params(int P, double theta, size_t ncrit, Precision precision, int device)
   {
     return exafmm::detail::params(P, theta, ncrit, (int)precision,
@@ -360,8 +449,9 @@ private:
                                   std::is_same<Formulation, rVPM>::value ? UVLM_RVPM : UVLM_CVPM, scheme(), device);
   }
   exafmm::detail::Handle h_;
-  size_t                 n_  = 0;
-  exafmm::real_t         dt_ = 0.0;
+  size_t                 n_       = 0;
+  exafmm::real_t         dt_      = 0.0;
+  uint64_t               version_ = 0;
 };
 }  // namespace Vortex
 

@@ -120,6 +120,18 @@ int uvlm_get_tree(uvlm_handle h, int64_t n_cells, int64_t *ints, double *reals);
 /* Pair lists as (target cell, source cell), sorted by target then source.  Pass NULL
  * arrays to query the sizes. */
 int uvlm_get_lists(uvlm_handle h, int64_t *n_p2p, int64_t *n_m2l, int64_t *p2p, int64_t *m2l);
+/* exafmm::getList(targetRoot, sourceRoot) (traverse.hpp:32-85) on two trees given by the
+ * caller (e.g. exafmm::Cells from a build): the dual-tree traversal with the near test
+ * |X_t - X_s|^2 theta^2 < (R_t + R_s)^2 and the split rule of traverse.hpp:74, run by the device
+ * traversal on `device`.  Trees are [C][5] ints (NCHILD, NBODY, ChildOffset, BodyOffset, level)
+ * and [C][4] reals (X, R), as uvlm_get_tree returns them; the children of a cell must follow it
+ * (ChildOffset > its index), as in the reference's numbering.  *n_p2p and *n_m2l receive the list
+ * sizes; p2p / m2l ([n][2] (target, source), sorted by target then source) are written only when
+ * non-NULL and at least that long (cap_p2p / cap_m2l pairs), so a caller can ask for the sizes
+ * first.  The reference returns the pairs in its stack order; as sets they are the same. */
+int uvlm_get_list(int device, double theta, int64_t ct, const int64_t *t_ints, const double *t_reals, int64_t cs,
+                  const int64_t *s_ints, const double *s_reals, int64_t cap_p2p, int64_t cap_m2l, int64_t *n_p2p,
+                  int64_t *n_m2l, int64_t *p2p, int64_t *m2l);
 /* Multipole and local expansions after uvlm_eval: [C][3*P(P+1)/2] complex doubles
  * (re, im interleaved), index 3*(n(n+1)/2 + m) + d as in multipoleKernels.hpp:468-471. */
 int uvlm_get_expansions(uvlm_handle h, double *M, double *L);

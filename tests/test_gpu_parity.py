@@ -370,6 +370,18 @@ def test_relaxation_and_diagnostics():
     assert np.allclose(np.linalg.norm(a1, axis=1), np.linalg.norm(a, axis=1), rtol=1e-12)
     cosang = np.einsum("ij,ij->i", a1, w) / (np.linalg.norm(a1, axis=1) * np.linalg.norm(w, axis=1))
     assert np.allclose(cosang, 1.0, atol=1e-12)
+    # a partial blend (the leapfrog workload relaxes with f = 0.3): relaxation.hpp:8-18's
+    # alpha <- (1 - f) alpha + f |alpha| w / |w| with w the curl of the same evaluation
+    g.update_arrays(alpha=a)
+    g.calculate()
+    Jm = g.arrays()["J"].reshape(-1, 3, 3)
+    w = np.stack([Jm[:, 2, 1] - Jm[:, 1, 2], Jm[:, 0, 2] - Jm[:, 2, 0], Jm[:, 1, 0] - Jm[:, 0, 1]], 1)
+    g.relax(0.3)
+    a3 = g.arrays()["alpha"]
+    want = 0.7 * a + 0.3 * np.linalg.norm(a, axis=1)[:, None] * w / np.linalg.norm(w, axis=1)[:, None]
+    err = np.abs(a3 - want).max() / np.abs(a).max()
+    print(f"relax f=0.3: max abs err / max |alpha| = {err:.2e}", flush=True)
+    assert err < 1e-13
 
 
 def test_autotune_keeps_a_candidate_and_the_results_stay_exact(orc):

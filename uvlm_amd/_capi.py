@@ -22,7 +22,7 @@ EXPORTS = [
     "uvlm_set_viscosity", "uvlm_viscosity", "uvlm_vorticity_aos", "uvlm_rbf_aos", "uvlm_eval_vjp",
     "uvlm_step_vjp", "uvlm_trajectory_forward", "uvlm_trajectory_vjp", "uvlm_append_bodies_soa",
     "uvlm_wing_create", "uvlm_wing_destroy", "uvlm_wing_get", "uvlm_wing_translate", "uvlm_wing_solve",
-    "uvlm_wing_shed", "uvlm_wing_step", "uvlm_wing_lift_adjoint", "uvlm_num_bodies", "uvlm_direct_noalloc",
+    "uvlm_wing_shed", "uvlm_wing_step", "uvlm_wing_lift_adjoint", "uvlm_num_bodies", "uvlm_direct_noalloc", "uvlm_get_list",
 ]
 
 
@@ -82,6 +82,7 @@ def lib():
             "uvlm_sensors_aos": (I, [VP, VP, I64, I64]),
             "uvlm_direct": (I, [VP, I64, PD, PD, PD, PD, PD]),
             "uvlm_direct_noalloc": (I, [VP, I64, PD, PD, PD, PD, PD]),
+            "uvlm_get_list": (I, [I, D, I64, PI64, PD, I64, PI64, PD, I64, I64, PI64, PI64, PI64, PI64]),
             "uvlm_probe_adjoint": (I, [VP, I64, PD, PD, PD, PD]),
             "uvlm_update_bodies_soa": (I, [VP, I64, PD, PD]),
             "uvlm_relax": (I, [VP, D]),

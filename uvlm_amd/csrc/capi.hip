@@ -929,6 +929,83 @@ int uvlm_get_lists(uvlm_handle h, int64_t *n_p2p, int64_t *n_m2l, int64_t *p2p_o
   });
 }
 
+int uvlm_get_list(int device, double theta, int64_t ct, const int64_t *t_ints, const double *t_reals, int64_t cs,
+                  const int64_t *s_ints, const double *s_reals, int64_t cap_p2p, int64_t cap_m2l, int64_t *n_p2p,
+                  int64_t *n_m2l, int64_t *p2p_out, int64_t *m2l_out)
+{
+  hipStream_t s = nullptr;
+  const int   rc = guarded([&] {
+    if (!(theta > 0.0)) throw Error(UVLM_ERR_ARG, "theta must be > 0");
+    if (!t_ints || !t_reals || !s_ints || !s_reals || !n_p2p || !n_m2l) throw Error(UVLM_ERR_ARG, "null argument");
+    if (ct < 1 || cs < 1 || ct >= (int64_t(1) << 31) || cs >= (int64_t(1) << 31))
+      throw Error(UVLM_ERR_ARG, "cell counts must be in [1, 2^31)");
+    // Children must lie after their parent and inside the array: then every descent of the
+    // traversal visits strictly larger cell indices and ends.
+    auto check_tree = [](int64_t C, const int64_t *ints, const char *which) {
+      for (int64_t c = 0; c < C; c++) {
+        const int64_t nch = ints[5 * c], ch = ints[5 * c + 2];
+        if (nch < 0 || nch > 8 || (nch > 0 && (ch <= c || ch + nch > C)))
+          throw Error(UVLM_ERR_ARG, std::string("malformed ") + which + " tree at cell " + std::to_string(c));
+      }
+    };
+    check_tree(ct, t_ints, "target");
+    check_tree(cs, s_ints, "source");
+    UVLM_HIP(hipSetDevice(device));
+    UVLM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    Tree    tt, st;
+    Scratch sc;
+    Lists   lists;
+    auto    upload = [&](Tree &t, int64_t C, const int64_t *ints, const double *reals) {
+      std::vector<int32_t> nch(C), ch(C), nb(C);
+      std::vector<double>  x(C), y(C), z(C), r(C);
+      t.nleaf = 0;
+      for (int64_t c = 0; c < C; c++) {
+        nch[c] = (int32_t)ints[5 * c];
+        nb[c]  = (int32_t)ints[5 * c + 1];
+        ch[c]  = (int32_t)ints[5 * c + 2];
+        x[c]   = reals[4 * c];
+        y[c]   = reals[4 * c + 1];
+        z[c]   = reals[4 * c + 2];
+        r[c]   = reals[4 * c + 3];
+        t.nleaf += nch[c] == 0;
+      }
+      t.C = C;
+      for (DevBuf *b : {&t.nchild, &t.child, &t.nbody}) b->reserve(4 * C);
+      for (DevBuf *b : {&t.cx, &t.cy, &t.cz, &t.R}) b->reserve(8 * C);
+      UVLM_HIP(hipMemcpyAsync(t.nchild.p, nch.data(), 4 * C, hipMemcpyHostToDevice, s));
+      UVLM_HIP(hipMemcpyAsync(t.child.p, ch.data(), 4 * C, hipMemcpyHostToDevice, s));
+      UVLM_HIP(hipMemcpyAsync(t.nbody.p, nb.data(), 4 * C, hipMemcpyHostToDevice, s));
+      UVLM_HIP(hipMemcpyAsync(t.cx.p, x.data(), 8 * C, hipMemcpyHostToDevice, s));
+      UVLM_HIP(hipMemcpyAsync(t.cy.p, y.data(), 8 * C, hipMemcpyHostToDevice, s));
+      UVLM_HIP(hipMemcpyAsync(t.cz.p, z.data(), 8 * C, hipMemcpyHostToDevice, s));
+      UVLM_HIP(hipMemcpyAsync(t.R.p, r.data(), 8 * C, hipMemcpyHostToDevice, s));
+      UVLM_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+    };
+    upload(tt, ct, t_ints, t_reals);
+    upload(st, cs, s_ints, s_reals);
+    traverse(s, sc, tt, st, theta, lists);
+    *n_p2p = lists.np2p;
+    *n_m2l = lists.nm2l;
+    auto expand = [&](DevBuf &off, DevBuf &src, int64_t n, int64_t cap, int64_t *out) {
+      if (!out || n > cap) return;
+      std::vector<int32_t> o(ct + 1), sv(std::max<int64_t>(n, 1));
+      UVLM_HIP(hipMemcpyAsync(o.data(), off.p, 4 * (ct + 1), hipMemcpyDeviceToHost, s));
+      if (n) UVLM_HIP(hipMemcpyAsync(sv.data(), src.p, 4 * n, hipMemcpyDeviceToHost, s));
+      UVLM_HIP(hipStreamSynchronize(s));
+      for (int64_t c = 0; c < ct; c++)
+        for (int q = o[c]; q < o[c + 1]; q++) {
+          out[2 * q]     = c;
+          out[2 * q + 1] = sv[q];
+        }
+    };
+    expand(lists.p2p_off, lists.p2p_src, lists.np2p, cap_p2p, p2p_out);
+    expand(lists.m2l_off, lists.m2l_src, lists.nm2l, cap_m2l, m2l_out);
+    UVLM_HIP(hipStreamSynchronize(s));
+  });
+  if (s) (void)hipStreamDestroy(s);
+  return rc;
+}
+
 int uvlm_get_expansions(uvlm_handle h, double *M, double *L)
 {
   return guarded([&] {
