@@ -251,6 +251,23 @@ int uvlm_wing_lift_adjoint(uvlm_wing w, uvlm_handle particles, const double U[3]
                            int reference_offset, const double e[3], double *CL, int64_t n, double *xbar,
                            double *alphabar);
 
+/* Design gradient of the lift coefficient through a window of the loop (no counterpart in the
+ * reference).  As uvlm_wing_lift_adjoint, from the current particles (n of them) and wing
+ * position: `steps` passes of uvlm_wing_step, CL at the final position, and its gradient with
+ * respect to the particles at the start (xbar, alphabar [n][3], original order) and to the wing's
+ * vertices (vertex_bar [(nchord+1)(nspan+1)][3]: the same lattice, moved rigidly by -U dt per
+ * step, so one set of vertices fixes the geometry of every step: the normals, control points,
+ * areas, system matrix, right-hand sides, force segments and shed particles all follow it).  The
+ * particle state at the window start is held fixed (the window's sensitivity).  The particle
+ * state and the wing position are checkpointed every `every` steps and each segment is
+ * recomputed from its checkpoint in the backward pass.  A design variable (angle of attack,
+ * twist, ...) that moves the vertices by dV gives dCL = sum vertex_bar . dV.  Any of CL, xbar,
+ * alphabar, vertex_bar may be NULL.  The particles and the wing end at the final state.  cVPM,
+ * inviscid. */
+int uvlm_wing_window_adjoint(uvlm_wing w, uvlm_handle particles, const double U[3], double dt, int64_t steps,
+                             int64_t every, int reference_offset, const double e[3], double *CL, int64_t n,
+                             double *xbar, double *alphabar, double *vertex_bar);
+
 /* Overwrite positions and strengths [n][3] in the CURRENT tree order without rebuilding
  * the tree or lists (frozen-tree perturbations, e.g. finite differences).  No counterpart
  * in the reference. */

@@ -1,10 +1,12 @@
 // The reference driver's time loop (src/main.cpp:168-295) written against the C++ facades:
 // a Vortex::FMMCalculator for the wake and a uvlm::Wing built from vertex couples the way
 // main.cpp builds its Wing (rotated LE/TE couples).  argv: npanels steps out.bin; writes
-// [steps][3] forces followed by the final particle records.
+// [steps][3] forces followed by the final particle records.  argv: design out.bin runs the
+// design gradient instead (design() below).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "../../include/uvlm/fmm_calculator.hpp"
@@ -13,8 +15,44 @@
 struct HostTag {};
 struct DeviceTag {};
 
+// The design gradient from C++ (uvlm::WingDesign + Wing::windowAdjoint): a 6 x 2 lattice of
+// span 3 at alpha = 5 degrees, 4 loop steps, then the window of 6 steps with checkpoints every 2.
+// Writes CL, dCL/dalpha and dCL/dtwist [6].
+static int design(const char *path)
+{
+  using Calc = Vortex::FMMCalculator<HostTag, DeviceTag, Vortex::RK2, Vortex::Inviscid, Vortex::cVPM, Vortex::Transposed>;
+  uvlm::WingDesign d;
+  d.nspan  = 6;
+  d.nchord = 2;
+  d.span   = 3.0;
+  const double           alpha = 5.0 * M_PI / 180.0, dt = 0.05;
+  const uvlm::Wing::Vec3 U{0.0, 1.0, 0.0}, e{0.0, 0.0, 1.0};
+  Calc                   fmm(6, 0.5, 1024);
+  uvlm::Wing             wing(d.nspan, d.nchord, d.vertices(alpha, {}));
+  for (int t = 0; t < 4; t++) wing.step(fmm.handle(), U, dt);
+  std::vector<double> vbar, dtwist;
+  const double        CL     = wing.windowAdjoint(fmm.handle(), U, dt, 6, 2, e, vbar);
+  const double        dalpha = d.gradient(vbar, alpha, {}, dtwist);
+  std::vector<double> out{CL, dalpha};
+  out.insert(out.end(), dtwist.begin(), dtwist.end());
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return 5;
+  std::fwrite(out.data(), sizeof(double), out.size(), f);
+  std::fclose(f);
+  std::printf("design: CL %.9f dCL/dalpha %.9f\n", CL, dalpha);
+  return 0;
+}
+
 int main(int argc, char **argv)
 {
+  if (argc == 3 && std::string(argv[1]) == "design") {
+    try {
+      return design(argv[2]);
+    } catch (const std::exception &e) {
+      std::fprintf(stderr, "error: %s\n", e.what());
+      return 1;
+    }
+  }
   if (argc < 4) {
     std::fprintf(stderr, "usage: %s npanels steps out.bin\n", argv[0]);
     return 2;

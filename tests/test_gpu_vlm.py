@@ -222,3 +222,124 @@ def test_wing_rejects_bad_inputs():
     w.step(p, [0.0, 1.0, 0.0], 0.05)  # the forward loop runs with rVPM ...
     with pytest.raises(Exception):
         w.lift_adjoint(p, [0.0, 1.0, 0.0], 0.05, 2)  # ... its adjoint covers cVPM only
+
+
+@pytest.mark.parametrize("ncrit,every,twisted", [(1024, 2, False), (1024, 1, True), (1024, 4, True)])
+def test_wing_design_gradient_finite_differences(ncrit, every, twisted):
+    """dC_L/d(angle of attack) and dC_L/d(strip twist) through a window of the loop
+    (uvlm_wing_window_adjoint, checkpoints every `every` steps, the wake at the window start held
+    fixed) against central differences of the loop run with a new wing built from the perturbed
+    design (design_wing).  ncrit 1024: every particle in one leaf (the FMM is the direct sum);
+    ncrit 8: multi-level trees at P = 12."""
+    from uvlm_amd.vlm import design_gradient, design_wing
+    alpha, nspan, nchord, span = np.deg2rad(5.0), 6, 2, 3.0
+    twist = 0.04 * np.sin(np.arange(nspan)) if twisted else None
+    U, dt, warm, win = np.array([0.0, 1.0, 0.0]), 0.05, 4, 6
+    P = 6 if ncrit > 100 else 12
+    w = Wing(design_wing(alpha, twist, span=span, nspan=nspan, nchord=nchord))
+    p = FMMCalculator(P=P, theta=0.5, ncrit=ncrit, scheme="rk2")
+    for _ in range(warm):
+        w.step(p, U, dt)
+    arr, perm = p.arrays(), p.permutation()
+    n = p.n
+    X0, A0, S0 = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    X0[perm], A0[perm], S0[perm] = arr["X"], arr["alpha"], arr["sigma"]
+    shift = warm * dt * U
+
+    def run(a, tw, adjoint=False):
+        w2 = Wing(design_wing(a, tw, span=span, nspan=nspan, nchord=nchord) - shift)
+        p2 = FMMCalculator(P=P, theta=0.5, ncrit=ncrit, scheme="rk2")
+        p2.set_arrays(X0, A0, S0)
+        if adjoint:
+            return w2.window_adjoint(p2, U, dt, win, every=every)
+        for _ in range(win):
+            w2.step(p2, U, dt)
+        _, F = w2.solve(p2, U)
+        return F[2] / (0.5 * U @ U * w2.geometry()[2].sum())
+
+    CL, xb, ab, vb = run(alpha, twist, adjoint=True)
+    CL0 = run(alpha, twist)
+    assert abs(CL - CL0) < 1e-12 * abs(CL)
+    ga, gt = design_gradient(vb, alpha, twist)
+    h = 1e-6
+    tw0 = np.zeros(nspan) if twist is None else twist
+    fa = (run(alpha + h, twist) - run(alpha - h, twist)) / (2 * h)
+    an, fd = [ga], [fa]
+    for c in (0, 2, nspan - 1):
+        e = np.zeros(nspan)
+        e[c] = h
+        an.append(gt[c])
+        fd.append((run(alpha, tw0 + e) - run(alpha, tw0 - e)) / (2 * h))
+    an, fd = np.array(an), np.array(fd)
+    print(f"[design ncrit {ncrit} every {every} twisted {twisted}] CL {CL:.6f} dCL/dalpha {ga:.6e} (FD {fa:.6e}); "
+          f"twist gradient {gt}; vs FD {rel(an, fd):.3e}", flush=True)
+    assert rel(an, fd) < 1e-6
+
+
+def test_cpp_wing_design_gradient(tmp_path):
+    """tests/cpp/wing_demo.cpp design: uvlm::WingDesign and Wing::windowAdjoint from C++ give the
+    design gradient that the same window gives through the Python binding (design_gradient)."""
+    import os
+    import subprocess
+    from uvlm_amd.vlm import design_gradient, design_wing
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "uvlm_amd", "lib",
+                       "uvlm_wing_demo")
+    out = tmp_path / "design.bin"
+    r = subprocess.run([exe, "design", str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    got = np.fromfile(out)
+    alpha, U, dt = np.deg2rad(5.0), np.array([0.0, 1.0, 0.0]), 0.05
+    w = Wing(design_wing(alpha, None, span=3.0, nspan=6, nchord=2))
+    p = FMMCalculator(P=6, theta=0.5, ncrit=1024, scheme="rk2")
+    for _ in range(4):
+        w.step(p, U, dt)
+    CL, _, _, vb = w.window_adjoint(p, U, dt, 6, every=2)
+    ga, gt = design_gradient(vb, alpha, None)
+    want = np.concatenate([[CL, ga], gt])
+    print(f"[design C++] {got[:2]} python {want[:2]}", flush=True)
+    assert got.shape == want.shape and rel(got, want) < 1e-13
+
+
+def test_wing_window_adjoint_checkpoints_multilevel():
+    """Multi-level trees (ncrit 4, P 12): the window adjoint's forward CL equals the loop's, and
+    its gradients (particles and vertices) do not depend on the checkpoint interval (every step,
+    every 3 steps, one segment), which changes only what is recomputed.  Central differences
+    through the loop are not used here: a perturbation that moves a pair across the acceptance
+    test switches it between the regularised and the singular kernel, a jump the differences
+    cannot see past."""
+    from uvlm_amd.vlm import design_wing
+    alpha, nspan, nchord, span = np.deg2rad(5.0), 6, 2, 3.0
+    U, dt, warm, win = np.array([0.0, 1.0, 0.0]), 0.05, 10, 6
+    V = design_wing(alpha, None, span=span, nspan=nspan, nchord=nchord)
+    w = Wing(V)
+    p = FMMCalculator(P=12, theta=0.5, ncrit=4, scheme="rk2")
+    for _ in range(warm):
+        w.step(p, U, dt)
+    arr, perm = p.arrays(), p.permutation()
+    n = p.n
+    X0, A0, S0 = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    X0[perm], A0[perm], S0[perm] = arr["X"], arr["alpha"], arr["sigma"]
+    pc = FMMCalculator(P=12, theta=0.5, ncrit=4, scheme="rk2")
+    pc.set_arrays(X0, A0, S0)
+    pc.build()
+    assert pc.timings()["n_levels"] > 2 and pc.timings()["n_m2l_pairs"] > 0
+    outs = {}
+    for every in (1, 3, 6):
+        w2 = Wing(V - warm * dt * U)
+        p2 = FMMCalculator(P=12, theta=0.5, ncrit=4, scheme="rk2")
+        p2.set_arrays(X0, A0, S0)
+        outs[every] = w2.window_adjoint(p2, U, dt, win, every=every)
+    w3 = Wing(V - warm * dt * U)
+    p3 = FMMCalculator(P=12, theta=0.5, ncrit=4, scheme="rk2")
+    p3.set_arrays(X0, A0, S0)
+    for _ in range(win):
+        w3.step(p3, U, dt)
+    _, F = w3.solve(p3, U)
+    CL = F[2] / (0.5 * U @ U * w3.geometry()[2].sum())
+    for every, (cl, xb, ab, vb) in outs.items():
+        ex, ea, ev = rel(xb, outs[6][1]), rel(ab, outs[6][2]), rel(vb, outs[6][3])
+        print(f"[window every {every}] CL {cl:.9f} (loop {CL:.9f}) vs one segment: xbar {ex:.1e} alphabar {ea:.1e} "
+              f"vertices {ev:.1e}", flush=True)
+        assert abs(cl - CL) < 1e-12 * abs(CL)
+        assert ex < 1e-9 and ea < 1e-9 and ev < 1e-9

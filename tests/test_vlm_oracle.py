@@ -102,3 +102,21 @@ def test_shedding_closed_form_for_uniform_circulation():
     assert np.allclose(X - (te + te1) / 2, [0.0, dt, dt])
     X2, _, _ = vlm.shed(w, np.full(w.n, g0), U, dt, reference_offset=False)
     assert np.allclose(X2 - (te + te1) / 2, [0.0, dt, 0.0])
+
+
+def test_design_gradient_chain_rule():
+    """design_gradient maps a vertex cotangent to (dCL/dalpha, dCL/dtwist): against central
+    differences of sum(vbar . V(alpha, twist)) for a random vbar, at a twisted design."""
+    from uvlm_amd.vlm import design_gradient, design_wing
+    rng = np.random.default_rng(4)
+    nspan, nchord = 7, 3
+    vb = rng.normal(size=(nchord + 1, nspan + 1, 3))
+    alpha, tw = 0.09, 0.05 * rng.normal(size=nspan)
+    f = lambda a, t: float(np.sum(vb * design_wing(a, t, span=4.0, chord=1.3, nspan=nspan, nchord=nchord)))
+    ga, gt = design_gradient(vb, alpha, tw, chord=1.3)
+    h = 1e-6
+    assert abs(ga - (f(alpha + h, tw) - f(alpha - h, tw)) / (2 * h)) < 1e-8 * max(1, abs(ga))
+    for c in range(nspan):
+        e = np.zeros(nspan)
+        e[c] = h
+        assert abs(gt[c] - (f(alpha, tw + e) - f(alpha, tw - e)) / (2 * h)) < 1e-8 * max(1, abs(gt[c]))

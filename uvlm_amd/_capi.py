@@ -23,6 +23,7 @@ EXPORTS = [
     "uvlm_step_vjp", "uvlm_trajectory_forward", "uvlm_trajectory_vjp", "uvlm_append_bodies_soa",
     "uvlm_wing_create", "uvlm_wing_destroy", "uvlm_wing_get", "uvlm_wing_translate", "uvlm_wing_solve",
     "uvlm_wing_shed", "uvlm_wing_step", "uvlm_wing_lift_adjoint", "uvlm_num_bodies", "uvlm_direct_noalloc", "uvlm_get_list",
+    "uvlm_wing_window_adjoint",
 ]
 
 
@@ -108,6 +109,7 @@ def lib():
             "uvlm_wing_shed": (I, [VP, VP, PD, D, I]),
             "uvlm_wing_step": (I, [VP, VP, PD, D, I, PD, PD]),
             "uvlm_wing_lift_adjoint": (I, [VP, VP, PD, D, I64, I, PD, PD, I64, PD, PD]),
+            "uvlm_wing_window_adjoint": (I, [VP, VP, PD, D, I64, I64, I, PD, PD, I64, PD, PD, PD]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -464,6 +464,118 @@ void vjp_finish(hipStream_t s, int64_t N, const double *ast, double *aux, int64_
   UVLM_LAUNCH_CHECK();
 }
 
+// Cotangent of the sources' core sizes, sbar_j = dS/dsigma_j (device order of the sources), of
+// S = sum over the near (P2P) pairs of ubar_i . u_i + Jbar_i : J_i: only the near field depends
+// on sigma (the far field applies the singular kernel).  Per pair, r = x_i - x_j and the kernel of
+// sigma_j, S_ij = -alpha_j . (f1 (ubar_i x r + e_i) + f2 (Jbar_i r) x r), e the axial vector of
+// Jbar, so dS_ij/dsigma_j = -alpha_j . (f1s (ubar_i x r + e_i) + f2s (Jbar_i r) x r) with
+// f1s = df1/dsigma = -2 s^2 t2 / (sigma 4 pi R^3), f2s = df2/dsigma = 2 s^2 t2 / (sigma^3 4 pi R^3),
+// s^2 = R^2 / (2 sigma^2), t2 = 2 s e^{-s^2} / sqrt(pi).  FP64.  One wavefront per source leaf,
+// lanes over its bodies; `off`/`ent` is CSR by source cell with target cells as entries (the
+// P2P list transposed); the target bodies pass through LDS, 64 at a time, in list order.
+template <bool WITH_JB>
+__global__ __launch_bounds__(WG) void k_sigma_bar(const int32_t *__restrict__ leaves, int64_t nleaf,
+                                                  const int32_t *__restrict__ sbody, const int32_t *__restrict__ snb,
+                                                  const int32_t *__restrict__ off, const int32_t *__restrict__ ent,
+                                                  const int32_t *__restrict__ tbody, const int32_t *__restrict__ tnb,
+                                                  const double *__restrict__ tx, int64_t tcap,
+                                                  const double *__restrict__ tcot, int64_t ccap,
+                                                  const double *__restrict__ sst, int64_t scap,
+                                                  double *__restrict__ sbar)
+{
+  __shared__ double sh[15][WG];
+  const int64_t w = blockIdx.x;
+  if (w >= nleaf) return;
+  const int lane = threadIdx.x;
+  const int sl = leaves[w], sb0 = sbody[sl], ns = snb[sl];
+  for (int p0 = 0; p0 < ns; p0 += WG) {
+    const int  j  = sb0 + p0 + lane;
+    const bool ok = p0 + lane < ns;
+    double     xj[3] = {0, 0, 0}, aj[3] = {0, 0, 0}, sg = 1.0;
+    if (ok) {
+      for (int d = 0; d < 3; d++) {
+        xj[d] = sst[(F_X + d) * scap + j];
+        aj[d] = sst[(F_A + d) * scap + j];
+      }
+      sg = sst[F_SIG * scap + j];
+    }
+    const double S2 = sg * sg;
+    double       acc = 0.0;
+    for (int q = off[sl]; q < off[sl + 1]; q++) {
+      const int t = ent[q], tb0 = tbody[t], nt = tnb[t];
+      for (int c0 = 0; c0 < nt; c0 += WG) {
+        const int cnt = min(WG, nt - c0);
+        __syncthreads();
+        if (lane < cnt) {
+          const int64_t i = tb0 + c0 + lane;
+          for (int d = 0; d < 3; d++) {
+            sh[d][lane]     = tx[(F_X + d) * tcap + i];
+            sh[3 + d][lane] = tcot[d * ccap + i];
+          }
+          if (WITH_JB)
+            for (int k = 0; k < 9; k++) sh[6 + k][lane] = tcot[(3 + k) * ccap + i];
+        }
+        __syncthreads();
+        if (!ok) continue;
+        for (int e = 0; e < cnt; e++) {
+          const double r[3] = {sh[0][e] - xj[0], sh[1][e] - xj[1], sh[2][e] - xj[2]};
+          const double R2   = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+          if (R2 == 0.0) continue;
+          const double s2 = R2 / S2 * 0.5, t2 = 2.0 * INV_SQRTPI * sqrt(s2) * exp(-s2);
+          const double q4 = 0.25 * INV_PI / (R2 * sqrt(R2));
+          const double f1s = -2.0 * s2 * t2 * q4 / sg, f2s = 2.0 * s2 * t2 * q4 / (S2 * sg);
+          const double ub[3] = {sh[3][e], sh[4][e], sh[5][e]};
+          double       W[3]  = {ub[1] * r[2] - ub[2] * r[1], ub[2] * r[0] - ub[0] * r[2], ub[0] * r[1] - ub[1] * r[0]};
+          for (int d = 0; d < 3; d++) W[d] *= f1s;
+          if (WITH_JB) {
+            const double *M = &sh[6][0];
+            auto          Jb = [&](int a, int b) { return M[(3 * a + b) * WG + e]; };
+            const double  ea[3] = {Jb(1, 2) - Jb(2, 1), Jb(2, 0) - Jb(0, 2), Jb(0, 1) - Jb(1, 0)};
+            double        Jr[3];
+            for (int a = 0; a < 3; a++) Jr[a] = Jb(a, 0) * r[0] + Jb(a, 1) * r[1] + Jb(a, 2) * r[2];
+            const double Jrxr[3] = {Jr[1] * r[2] - Jr[2] * r[1], Jr[2] * r[0] - Jr[0] * r[2],
+                                    Jr[0] * r[1] - Jr[1] * r[0]};
+            for (int d = 0; d < 3; d++) W[d] += f1s * ea[d] + f2s * Jrxr[d];
+          }
+          acc -= aj[0] * W[0] + aj[1] * W[1] + aj[2] * W[2];
+        }
+      }
+    }
+    if (ok) sbar[j] = acc;
+  }
+}
+
+__global__ void k_add_to_orig(int64_t N, const int32_t *__restrict__ orig, const double *__restrict__ src,
+                              double *__restrict__ dst)
+{
+  const int64_t p = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (p >= N) return;
+  dst[orig[p]] += src[p];
+}
+
+void add_to_orig(hipStream_t s, int64_t N, const int32_t *orig, const double *src, double *dst)
+{
+  if (N <= 0) return;
+  k_add_to_orig<<<(int)((N + 255) / 256), 256, 0, s>>>(N, orig, src, dst);
+  UVLM_LAUNCH_CHECK();
+}
+
+void sigma_bar(hipStream_t s, const Tree &stree, const Tree &ttree, const Lists &tl, const double *tx, int64_t tcap,
+               const double *tcot, int64_t ccap, bool with_jbar, const double *sst, int64_t scap, double *sbar)
+{
+  if (stree.nleaf == 0) return;
+  const int32_t *lv = stree.leaves.as<int32_t>(), *sb = stree.body.as<int32_t>(), *sn = stree.nbody.as<int32_t>();
+  const int32_t *off = tl.p2p_off.as<int32_t>(), *ent = tl.p2p_src.as<int32_t>();
+  const int32_t *tb = ttree.body.as<int32_t>(), *tn = ttree.nbody.as<int32_t>();
+  const int      nb = (int)stree.nleaf;
+  if (with_jbar)
+    k_sigma_bar<true><<<nb, WG, 0, s>>>(lv, stree.nleaf, sb, sn, off, ent, tb, tn, tx, tcap, tcot, ccap, sst, scap, sbar);
+  else
+    k_sigma_bar<false><<<nb, WG, 0, s>>>(lv, stree.nleaf, sb, sn, off, ent, tb, tn, tx, tcap, tcot, ccap, sst, scap,
+                                         sbar);
+  UVLM_LAUNCH_CHECK();
+}
+
 void p2p_adj(hipStream_t s, const Tree &t, const Lists &l, int side, int mixed, const double *st, double *aux,
              int64_t cap)
 {

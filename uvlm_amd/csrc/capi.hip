@@ -65,7 +65,7 @@ struct uvlm_handle_s {
   // (X, alpha, sigma + original index, per stage), state cotangents in the original order
   Lists   tlists;
   bool    tlists_ok = false;  // tlists is the transpose of the current build's lists
-  DevBuf  vaux, vstage, vorig, vcot, vlam;
+  DevBuf  vaux, vstage, vorig, vcot, vlam, vsig;
   // trajectory checkpoints (X, alpha, sigma in the original order, every `ck_every` steps)
   DevBuf  ck;
   int64_t ck_n = 0, ck_steps = 0, ck_every = 0, ck_N = 0;
@@ -281,7 +281,9 @@ void reserve_probes(uvlm_handle h, int64_t m)
 // getSensorData / calculateSensorPoints (fmmCalculator.hpp:469-491, 726-731) on the m probe
 // records already in h->sst: the probe tree (probes permuted into its order, h->sperm), lists
 // against the particle tree, M2L from the particle multipoles, P2P velocity, L2L, L2P.
-void sensor_pass(uvlm_handle h, int64_t m)
+// with_j: also the near-field velocity gradient (the reference's sensor P2P gives velocity only,
+// multipoleKernels.hpp:303-341; the wing's design gradient needs the whole J at its points).
+void sensor_pass(uvlm_handle h, int64_t m, bool with_j = false)
 {
   Region        region("Calculation of Sensors");  // fmmCalculator.hpp:471
   hipStream_t   s  = h->s;
@@ -298,7 +300,7 @@ void sensor_pass(uvlm_handle h, int64_t m)
   reset_sensor_rates(s, h->sst.as<double>(), sc, m);
   m2l(s, P, h->stree, h->tree, h->slists, h->sched, h->M.as<double2>(), Ls, &h->sc);
   p2p(s, h->stree, h->tree, h->slists, h->sst.as<double>(), sc, h->st.as<double>(), h->cap, h->sst.as<double>(), sc,
-      h->prm.precision, 1, h->prm.ncrit, &h->sc);
+      h->prm.precision, with_j ? 0 : 1, h->prm.ncrit, &h->sc);
   for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls, h->sched.octY.as<double2>());
   l2p_stretch(s, P, h->stree, Ls, h->sst.as<double>(), sc, h->prm.stretching, h->prm.formulation, 1);
 }
@@ -317,6 +319,23 @@ void velocity_at_points(uvlm_handle h, int64_t m, const double *pts, double *out
   points_to_soa(h->s, m, pts, h->sst.as<double>(), h->scap);
   sensor_pass(h, m);
   probe_velocity_out(h->s, m, h->sperm.as<int32_t>(), h->sst.as<double>(), h->scap, out);
+}
+
+// Velocity and its gradient J (dv/dx, near field included) of the particles at m device points
+// (AoS [m][3]): out [m][3], Jout [m][9], in the points' order.  No particles: zero.
+void velocity_grad_at_points(uvlm_handle h, int64_t m, const double *pts, double *out, double *Jout)
+{
+  if (h->N <= 0) {
+    zero_doubles(h->s, out, 3 * m);
+    zero_doubles(h->s, Jout, 9 * m);
+    return;
+  }
+  do_build(h);
+  do_eval(h, false);
+  reserve_probes(h, m);
+  points_to_soa(h->s, m, pts, h->sst.as<double>(), h->scap);
+  sensor_pass(h, m, true);
+  probe_velocity_grad_out(h->s, m, h->sperm.as<int32_t>(), h->sst.as<double>(), h->scap, out, Jout);
 }
 
 // Append k bodies (device AoS X [k][3], alpha [k][3], sigma [k]) after the current N, growing
@@ -435,7 +454,8 @@ Tableau tableau_of(int scheme)
 // Vector-Jacobian product of the last do_eval (forward J in the state, forward L in h->L):
 // on entry the adjoint fields hold ubar (0-2) and abar (15-17) in the device order, on exit
 // xbar (12-14) and alphabar (15-17).  M and L are overwritten by the adjoint expansions.
-void vjp_core(uvlm_handle h)
+// sbar (optional, device order): also the cotangent of the core sizes (near pairs only).
+void vjp_core(uvlm_handle h, double *sbar = nullptr)
 {
   hipStream_t s = h->s;
   const int   P = h->prm.P, NT = h->NT;
@@ -466,6 +486,7 @@ void vjp_core(uvlm_handle h)
   l2p_stretch(s, P, t, L, a, cap, UVLM_TRANSPOSED, UVLM_CVPM, 0);
   p2p_adj(s, t, h->tlists, 1, h->prm.precision == UVLM_MIXED, st, aux, cap);
   vjp_finish(s, N, a, aux, cap);
+  if (sbar) sigma_bar(s, t, t, h->tlists, st, cap, aux, cap, true, st, cap, sbar);
   h->evaluated = false;  // M and L hold adjoint expansions
 }
 
@@ -499,7 +520,9 @@ void load_state(uvlm_handle h, const DevBuf &store, const DevBuf &ostore, int k)
 // holds the cotangent of the state after the step, on exit that of the state before it.  The
 // bodies are left at the step's start.  Stage states are recomputed, and in reverse each stage
 // rebuilds its own tree (the build is deterministic, so it is the forward's tree).
-void step_vjp(uvlm_handle h, double dt, double *cot)
+// sbar (optional, original order, cap doubles): the core sizes' cotangent, accumulated in place
+// (sigma does not change over a cVPM inviscid step, so each stage adds its rates' part).
+void step_vjp(uvlm_handle h, double dt, double *cot, double *sbar = nullptr)
 {
   hipStream_t   s   = h->s;
   const int64_t cap = h->cap, N = h->N;
@@ -534,8 +557,10 @@ void step_vjp(uvlm_handle h, double dt, double *cot)
     do_build(h);
     do_eval(h, false);
     cot_gather(s, N, h->orig.as<int32_t>(), fb, cap, h->vaux.as<double>(), cap);
-    vjp_core(h);
+    if (sbar) h->vsig.reserve(sizeof(double) * cap);
+    vjp_core(h, sbar ? h->vsig.as<double>() : nullptr);
     cot_scatter(s, N, h->orig.as<int32_t>(), h->vaux.as<double>(), cap, Yb + (int64_t)k * 6 * cap, cap);
+    if (sbar) add_to_orig(s, N, h->orig.as<int32_t>(), h->vsig.as<double>(), sbar);
   }
   const double *p[6];
   double        c[6];
@@ -1587,6 +1612,265 @@ int uvlm_wing_step(uvlm_wing w, uvlm_handle particles, const double U[3], double
   });
 }
 
+namespace {
+// The wing's position at one step of a window: vertices and control points (the loop moves both
+// by -U dt per step; the geometry that does not depend on the position is kept once).
+struct WingPose {
+  DevBuf V, cp;
+};
+void save_pose(uvlm_wing w, WingPose &p)
+{
+  const int64_t nv = (int64_t)(w->nspan + 1) * (w->nchord + 1);
+  p.V.reserve(24 * nv);
+  p.cp.reserve(24 * w->n);
+  UVLM_HIP(hipMemcpyAsync(p.V.p, w->V.p, 24 * nv, hipMemcpyDeviceToDevice, w->s));
+  UVLM_HIP(hipMemcpyAsync(p.cp.p, w->cp.p, 24 * w->n, hipMemcpyDeviceToDevice, w->s));
+}
+void load_pose(uvlm_wing w, const WingPose &p)
+{
+  const int64_t nv = (int64_t)(w->nspan + 1) * (w->nchord + 1);
+  UVLM_HIP(hipMemcpyAsync(w->V.p, p.V.p, 24 * nv, hipMemcpyDeviceToDevice, w->s));
+  UVLM_HIP(hipMemcpyAsync(w->cp.p, p.cp.p, 24 * w->n, hipMemcpyDeviceToDevice, w->s));
+}
+
+// Adjoint of CL = e . F / (|U|^2 S / 2) at the end of `steps` passes of the loop (solve, shed,
+// RK stages, move the wing) from the current particles and wing position.  Outputs (each may be
+// null): CL; the cotangents of the particles' X and alpha at the start (original order); the
+// cotangent of the wing's vertices (vbar [nv][3]: the lattice moves rigidly, so one set of
+// vertices fixes every step's geometry).  The particle state and the wing position are kept
+// every `every` steps; each segment between two checkpoints is recomputed from its checkpoint
+// (bit for bit the forward's operations) with the per-step states it needs, then run backwards.
+// Backwards, each solve contributes through A^{-T} to the circulations' cotangent, to the
+// accumulated cotangents of A (-rbar gamma^T) and of the normals (rhs), and through the wake
+// velocity v at the control points to the particles (probe adjoint) and to the control points
+// (J^T vbar, J the velocity gradient there); each shedding contributes through the shed
+// particles' cotangents.  At the end the accumulated cotangents go through the ring influences
+// (segment VJPs), the panel geometry (control point, normal, area) and the force's segments
+// to the vertices.  The trees are held fixed per evaluation, as in the other adjoints.
+void wing_window_adjoint(uvlm_wing w, uvlm_handle h, const double U[3], double dt, int64_t steps, int64_t every,
+                         int ref_offset, const double e[3], double *CL, double *xbar, double *alphabar,
+                         double *vertex_bar)
+{
+  hipStream_t   s = h->s, ws = w->s;
+  const int     np = w->n, ns = w->nspan, nc = w->nchord;
+  const int64_t nv = (int64_t)(ns + 1) * (nc + 1);
+  const int64_t N0 = h->N;
+  const bool    geom = vertex_bar != nullptr;
+  const int64_t E = std::max<int64_t>(1, every), nck = (steps + E - 1) / E;
+  // particle capacity for the whole window first, so the cotangent stride stays fixed
+  grow_particles(h, N0 + steps * ns);
+  const int64_t cap = h->cap;
+  auto          keep = [&](DevBuf &b, int64_t &cnt) {
+    cnt = h->N;
+    b.reserve(sizeof(double) * 7 * std::max<int64_t>(cnt, 1));
+    if (cnt) save_state_orig(h, b.as<double>());
+  };
+  auto restore = [&](const DevBuf &b, int64_t cnt) {
+    if (cnt) load_state_orig_n(h, b.as<double>(), cnt, cnt);
+    else h->N = 0;
+  };
+  std::vector<int64_t> nstart(steps + 1, 0);
+  // one pass of the loop from the current state; gam / vel receive the solve's circulations and
+  // wake velocity at the control points, yshed the state right after the shedding
+  auto loop_step = [&](int64_t k, double *gam, double *vel, DevBuf *yshed) {
+    nstart[k] = h->N;
+    wing_solve(w, h, U);
+    if (gam) UVLM_HIP(hipMemcpyAsync(gam, w->gamma.p, 8 * np, hipMemcpyDeviceToDevice, ws));
+    if (vel) {
+      if (h->N > 0) UVLM_HIP(hipMemcpyAsync(vel, w->v.p, 24 * np, hipMemcpyDeviceToDevice, ws));
+      else zero_doubles(ws, vel, 3 * np);
+    }
+    UVLM_HIP(hipStreamSynchronize(ws));
+    wing_shed_into(w, h, U, dt, ref_offset);
+    if (yshed) {
+      int64_t cnt = 0;
+      keep(*yshed, cnt);
+    }
+    for (int st = 0; st < stages_of(h->prm.scheme); st++) {
+      do_build(h);
+      do_eval(h, false);
+      rk_stage(s, h->st.as<double>(), cap, h->N, h->prm.scheme, st, dt, 0);
+    }
+    const double d[3] = {-U[0] * dt, -U[1] * dt, -U[2] * dt};
+    translate3(ws, nv, w->V.as<double>(), d);
+    translate3(ws, np, w->cp.as<double>(), d);
+    UVLM_HIP(hipStreamSynchronize(ws));
+    UVLM_HIP(hipStreamSynchronize(s));
+  };
+  // forward: checkpoints (restored in the original order, as the recomputation starts from them);
+  // the last segment keeps its per-step records right away, so it is not recomputed
+  std::vector<DevBuf>   ck(nck);
+  std::vector<int64_t>  ckn(nck);
+  std::vector<WingPose> ckpose(nck);
+  std::vector<DevBuf>   yshed(E);
+  std::vector<WingPose> pose(E);
+  DevBuf                segrec;
+  segrec.reserve(sizeof(double) * 4 * np * E);  // gamma [np] and v [3 np] per step of a segment
+  const int64_t last0 = (nck - 1) * E;
+  for (int64_t k = 0; k < steps; k++) {
+    if (k % E == 0) {
+      keep(ck[k / E], ckn[k / E]);
+      restore(ck[k / E], ckn[k / E]);
+      save_pose(w, ckpose[k / E]);
+    }
+    if (k >= last0) {
+      double *rec = segrec.as<double>() + 4 * np * (k - last0);
+      save_pose(w, pose[k - last0]);
+      loop_step(k, rec, rec + np, &yshed[k - last0]);
+    } else {
+      loop_step(k, nullptr, nullptr, nullptr);
+    }
+  }
+  nstart[steps] = h->N;
+  DevBuf  yfinal;
+  int64_t nfinal = 0;
+  keep(yfinal, nfinal);
+  WingPose final_pose;
+  save_pose(w, final_pose);
+  // the lift coefficient at the final position
+  wing_solve(w, h, U);
+  std::vector<double> area(np);
+  double              F[3];
+  UVLM_HIP(hipMemcpyAsync(area.data(), w->area.p, 8 * np, hipMemcpyDeviceToHost, ws));
+  UVLM_HIP(hipMemcpyAsync(F, w->F.p, 24, hipMemcpyDeviceToHost, ws));
+  UVLM_HIP(hipStreamSynchronize(ws));
+  double S = 0.0;
+  for (double a : area) S += a;
+  const double q     = 0.5 * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]) * S;
+  const double es[3] = {e[0] / q, e[1] / q, e[2] / q};
+  const double cl    = (e[0] * F[0] + e[1] * F[1] + e[2] * F[2]) / q;
+  if (CL) *CL = cl;
+  // backward
+  DevBuf abuf;
+  abuf.reserve(sizeof(double) * (size_t)(np + 3 * np + np + np + 3 * np + 9 * np));
+  double *c = abuf.as<double>(), *vbar = c + np, *gbar = vbar + 3 * np, *rbar = gbar + np, *vtmp = rbar + np,
+         *Jcp = vtmp + 3 * np;
+  h->vlam.reserve(sizeof(double) * 7 * cap);
+  double *lam = h->vlam.as<double>();
+  zero_doubles(s, lam, 7 * cap);
+  // the particles' core-size cotangent (original order): the shed core sizes follow the geometry
+  double *sbar = geom ? lam + 6 * cap : nullptr;
+  // geometry accumulators: A's cotangent [np][np], normals, control points, shed corners,
+  // and (final solve) the force's velocity cotangent, circulations and segment cotangents
+  DevBuf gacc;
+  double *Abar = nullptr, *nacc = nullptr, *cpacc = nullptr, *pbshed = nullptr, *ubfin = nullptr, *gfin = nullptr,
+         *dlb = nullptr, *pbring = nullptr, *pbgeom = nullptr, *cpb2 = nullptr, *nb2 = nullptr, *vbdev = nullptr;
+  if (geom) {
+    gacc.reserve(sizeof(double) * ((size_t)np * np + 3 * np + 3 * np + 12 * np + 3 * np + np + 3 * np + 12 * np +
+                                   12 * np + 3 * np + 3 * np + 3 * nv));
+    Abar   = gacc.as<double>();
+    nacc   = Abar + (size_t)np * np;
+    cpacc  = nacc + 3 * np;
+    pbshed = cpacc + 3 * np;
+    ubfin  = pbshed + 12 * np;
+    gfin   = ubfin + 3 * np;
+    dlb    = gfin + np;
+    pbring = dlb + 3 * np;
+    pbgeom = pbring + 12 * np;
+    cpb2   = pbgeom + 12 * np;
+    nb2    = cpb2 + 3 * np;
+    vbdev  = nb2 + 3 * np;
+    zero_doubles(ws, Abar, (int64_t)np * np + 3 * np + 3 * np + 12 * np);
+  }
+  wing_force_vjp(ws, np, ns, w->V.as<double>(), w->B.as<double>(), w->u.as<double>(), w->gamma.as<double>(), es, c,
+                 vbar, gbar);
+  if (geom) {
+    UVLM_HIP(hipMemcpyAsync(ubfin, vbar, 24 * np, hipMemcpyDeviceToDevice, ws));
+    UVLM_HIP(hipMemcpyAsync(gfin, w->gamma.p, 8 * np, hipMemcpyDeviceToDevice, ws));
+    wing_force_dl_vjp(ws, np, ns, w->u.as<double>(), w->gamma.as<double>(), es, dlb);
+  }
+  // the solve of one step (circulations gam, wake velocity vel at the control points, npart
+  // particles on the device in that step's state, wing at that step's pose): gbar -> rbar =
+  // A^{-T} gbar -> the geometry accumulators, vbar -= rbar n, then the wake
+  auto through_solve = [&](const double *gam, const double *vel, int64_t npart) {
+    matvec_inv_t(ws, np, w->Ainv.as<double>(), gbar, rbar);
+    if (geom) wing_solve_geom_acc(ws, np, rbar, gam, U, npart ? vel : nullptr, Abar, nacc);
+    wing_rhs_vjp(ws, np, w->nrm.as<double>(), rbar, vbar);
+    UVLM_HIP(hipStreamSynchronize(ws));
+    if (npart == 0) return;
+    if (geom) {
+      velocity_grad_at_points(h, np, w->cp.as<double>(), vtmp, Jcp);
+      UVLM_HIP(hipStreamSynchronize(s));
+      wing_cp_acc(ws, np, vbar, Jcp, cpacc);
+      UVLM_HIP(hipStreamSynchronize(ws));
+    } else {
+      do_build(h);
+    }
+    reserve_probes(h, np);
+    probe_sources(s, np, w->cp.as<double>(), vbar, h->sst.as<double>(), h->scap);
+    probe_adjoint_core(h, np);
+    cot_add_probe(s, h->N, h->orig.as<int32_t>(), h->ast.as<double>(), cap, lam, cap);
+    if (sbar) {
+      // the near field of the wake velocity at the control points: probes (weights vbar) as targets
+      h->vsig.reserve(sizeof(double) * cap);
+      const int64_t sc = h->scap;
+      sigma_bar(s, h->tree, h->stree, h->alists, h->sst.as<double>(), sc, h->sst.as<double>() + F_A * sc, sc, false,
+                h->st.as<double>(), cap, h->vsig.as<double>());
+      add_to_orig(s, h->N, h->orig.as<int32_t>(), h->vsig.as<double>(), sbar);
+    }
+    UVLM_HIP(hipStreamSynchronize(s));
+  };
+  {
+    DevBuf fK;  // gamma_K [np], v_K [3 np] of the final solve
+    fK.reserve(32 * np);
+    UVLM_HIP(hipMemcpyAsync(fK.p, w->gamma.p, 8 * np, hipMemcpyDeviceToDevice, ws));
+    if (nfinal > 0) UVLM_HIP(hipMemcpyAsync(fK.as<double>() + np, w->v.p, 24 * np, hipMemcpyDeviceToDevice, ws));
+    UVLM_HIP(hipStreamSynchronize(ws));
+    through_solve(fK.as<double>(), fK.as<double>() + np, nstart[steps]);
+  }
+  // segments backwards: recompute each (the last was recorded by the forward pass), then its
+  // steps in reverse
+  for (int64_t g = nck - 1; g >= 0; g--) {
+    const int64_t k0 = g * E, k1 = std::min(steps, k0 + E);
+    if (g != nck - 1) {
+      restore(ck[g], ckn[g]);
+      load_pose(w, ckpose[g]);
+      UVLM_HIP(hipStreamSynchronize(ws));
+      for (int64_t k = k0; k < k1; k++) {
+        double *rec = segrec.as<double>() + 4 * np * (k - k0);
+        save_pose(w, pose[k - k0]);
+        loop_step(k, rec, rec + np, &yshed[k - k0]);
+      }
+    }
+    for (int64_t k = k1 - 1; k >= k0; k--) {
+      const double *gam = segrec.as<double>() + 4 * np * (k - k0), *vel = gam + np;
+      const int64_t nshed = nstart[k] + ns;
+      load_pose(w, pose[k - k0]);
+      UVLM_HIP(hipStreamSynchronize(ws));
+      restore(yshed[k - k0], nshed);
+      step_vjp(h, dt, lam, sbar);
+      UVLM_HIP(hipStreamSynchronize(s));
+      // the shed particles' cotangents are entries nstart[k] .. nstart[k] + ns - 1
+      wing_shed_vjp(ws, ns, nc, w->V.as<double>(), lam + 3 * cap + nstart[k], cap, gbar);
+      if (geom) wing_shed_geom_vjp(ws, ns, nc, w->V.as<double>(), gam, lam, cap, nstart[k], sbar, pbshed);
+      zero_doubles(ws, vbar, 3 * np);
+      UVLM_HIP(hipStreamSynchronize(ws));
+      if (nstart[k]) load_state_orig_n(h, yshed[k - k0].as<double>(), nshed, nstart[k]);
+      else h->N = 0;
+      through_solve(gam, vel, nstart[k]);
+    }
+  }
+  h->N = N0;
+  download_cot(h, lam, xbar, alphabar);
+  if (geom) {
+    // the accumulated cotangents through the ring influences, the panel geometry and the
+    // force's segments to the vertices (any pose: the influences are translation invariant)
+    const double areab = -cl / S;
+    wing_influence_vjp(ws, np, ns, w->V.as<double>(), w->cp.as<double>(), w->nrm.as<double>(), w->B.as<double>(), Abar,
+                       ubfin, gfin, pbring, cpb2, nb2);
+    wing_geom_vjp(ws, np, ns, w->V.as<double>(), cpacc, cpb2, nacc, nb2, areab, dlb, pbgeom);
+    wing_vertex_gather(ws, ns, nc, pbgeom, pbring, pbshed, vbdev);
+    UVLM_HIP(hipMemcpyAsync(vertex_bar, vbdev, 24 * nv, hipMemcpyDeviceToHost, ws));
+    UVLM_HIP(hipStreamSynchronize(ws));
+  }
+  // particles and wing back at the final state
+  load_pose(w, final_pose);
+  UVLM_HIP(hipStreamSynchronize(ws));
+  restore(yfinal, nfinal);
+  UVLM_HIP(hipStreamSynchronize(s));
+}
+}  // namespace
+
 int uvlm_wing_lift_adjoint(uvlm_wing w, uvlm_handle particles, const double U[3], double dt, int64_t steps,
                            int reference_offset, const double e[3], double *CL, int64_t n, double *xbar,
                            double *alphabar)
@@ -1595,111 +1879,27 @@ int uvlm_wing_lift_adjoint(uvlm_wing w, uvlm_handle particles, const double U[3]
     check_pair(w, particles);
     if (!particles || !U || !e) throw Error(UVLM_ERR_ARG, "null particles, freestream or direction");
     if (steps < 0) throw Error(UVLM_ERR_ARG, "negative step count");
-    uvlm_handle h = particles;
-    if (n != h->N) throw Error(UVLM_ERR_ARG, "size mismatch");
-    check_vjp_config(h);
+    if (n != particles->N) throw Error(UVLM_ERR_ARG, "size mismatch");
+    check_vjp_config(particles);
     UVLM_HIP(hipSetDevice(w->device));
-    hipStream_t   s = h->s, ws = w->s;
-    const int     np = w->n, ns = w->nspan;
-    const int64_t N0 = h->N;
-    // particle capacity for the whole window first, so the cotangent stride stays fixed
-    grow_particles(h, N0 + steps * ns);
-    const int64_t cap = h->cap;
-    // forward: the step-start states y_k and the post-shedding states y_k' in the original order
-    std::vector<DevBuf>  ystart(steps + 1), yshed(steps);
-    std::vector<int64_t> nstart(steps + 1), nshed(steps);
-    auto                 keep = [&](DevBuf &b, int64_t &cnt) {
-      cnt = h->N;
-      b.reserve(sizeof(double) * 7 * std::max<int64_t>(cnt, 1));
-      if (cnt) save_state_orig(h, b.as<double>());
-    };
-    auto restore = [&](const DevBuf &b, int64_t cnt) {
-      if (cnt) load_state_orig_n(h, b.as<double>(), cnt, cnt);
-      else h->N = 0;
-    };
-    keep(ystart[0], nstart[0]);
-    restore(ystart[0], nstart[0]);
-    for (int64_t k = 0; k < steps; k++) {
-      wing_solve(w, h, U);
-      UVLM_HIP(hipStreamSynchronize(ws));
-      wing_shed_into(w, h, U, dt, reference_offset);
-      keep(yshed[k], nshed[k]);
-      restore(yshed[k], nshed[k]);
-      for (int st = 0; st < stages_of(h->prm.scheme); st++) {
-        do_build(h);
-        do_eval(h, false);
-        rk_stage(s, h->st.as<double>(), cap, h->N, h->prm.scheme, st, dt, 0);
-      }
-      const double d[3] = {-U[0] * dt, -U[1] * dt, -U[2] * dt};
-      translate3(ws, (int64_t)(w->nspan + 1) * (w->nchord + 1), w->V.as<double>(), d);
-      translate3(ws, np, w->cp.as<double>(), d);
-      UVLM_HIP(hipStreamSynchronize(ws));
-      keep(ystart[k + 1], nstart[k + 1]);
-      restore(ystart[k + 1], nstart[k + 1]);
-    }
-    // the lift coefficient at the final position
-    wing_solve(w, h, U);
-    std::vector<double> area(np);
-    double              F[3];
-    UVLM_HIP(hipMemcpyAsync(area.data(), w->area.p, 8 * np, hipMemcpyDeviceToHost, ws));
-    UVLM_HIP(hipMemcpyAsync(F, w->F.p, 24, hipMemcpyDeviceToHost, ws));
-    UVLM_HIP(hipStreamSynchronize(ws));
-    double S = 0.0;
-    for (double a : area) S += a;
-    const double q     = 0.5 * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]) * S;
-    const double es[3] = {e[0] / q, e[1] / q, e[2] / q};
-    if (CL) *CL = (e[0] * F[0] + e[1] * F[1] + e[2] * F[2]) / q;
-    // backward
-    DevBuf abuf;
-    abuf.reserve(sizeof(double) * (size_t)(np + 3 * np + np + np));
-    double *c = abuf.as<double>(), *vbar = c + np, *gbar = vbar + 3 * np, *rbar = gbar + np;
-    h->vlam.reserve(sizeof(double) * 6 * cap);
-    double *lam = h->vlam.as<double>();
-    zero_doubles(s, lam, 6 * cap);
-    wing_force_vjp(ws, np, ns, w->V.as<double>(), w->B.as<double>(), w->u.as<double>(), w->gamma.as<double>(), es, c,
-                   vbar, gbar);
-    auto through_solve_and_probes = [&](int64_t npart) {
-      // gbar -> rbar = A^{-T} gbar -> vbar -= rbar n -> particle cotangents (probe adjoint)
-      matvec_inv_t(ws, np, w->Ainv.as<double>(), gbar, rbar);
-      wing_rhs_vjp(ws, np, w->nrm.as<double>(), rbar, vbar);
-      UVLM_HIP(hipStreamSynchronize(ws));
-      if (npart == 0) return;
-      do_build(h);
-      reserve_probes(h, np);
-      probe_sources(s, np, w->cp.as<double>(), vbar, h->sst.as<double>(), h->scap);
-      probe_adjoint_core(h, np);
-      cot_add_probe(s, h->N, h->orig.as<int32_t>(), h->ast.as<double>(), cap, lam, cap);
-      UVLM_HIP(hipStreamSynchronize(s));
-    };
-    restore(ystart[steps], nstart[steps]);
-    through_solve_and_probes(nstart[steps]);
-    for (int64_t k = steps - 1; k >= 0; k--) {
-      const double d[3] = {U[0] * dt, U[1] * dt, U[2] * dt};
-      translate3(ws, (int64_t)(w->nspan + 1) * (w->nchord + 1), w->V.as<double>(), d);
-      translate3(ws, np, w->cp.as<double>(), d);
-      UVLM_HIP(hipStreamSynchronize(ws));
-      restore(yshed[k], nshed[k]);
-      step_vjp(h, dt, lam);
-      UVLM_HIP(hipStreamSynchronize(s));
-      // the shed strengths' cotangents are entries nstart[k] .. nstart[k] + ns - 1
-      wing_shed_vjp(ws, ns, w->nchord, w->V.as<double>(), lam + 3 * cap + nstart[k], cap, gbar);
-      zero_doubles(ws, vbar, 3 * np);
-      UVLM_HIP(hipStreamSynchronize(ws));
-      if (nstart[k]) load_state_orig_n(h, yshed[k].as<double>(), nshed[k], nstart[k]);
-      else h->N = 0;
-      through_solve_and_probes(nstart[k]);
-    }
-    h->N = N0;
-    download_cot(h, lam, xbar, alphabar);
-    // particles and wing back at the final state
-    for (int64_t k = 0; k < steps; k++) {
-      const double d[3] = {-U[0] * dt, -U[1] * dt, -U[2] * dt};
-      translate3(ws, (int64_t)(w->nspan + 1) * (w->nchord + 1), w->V.as<double>(), d);
-      translate3(ws, np, w->cp.as<double>(), d);
-    }
-    UVLM_HIP(hipStreamSynchronize(ws));
-    restore(ystart[steps], nstart[steps]);
-    UVLM_HIP(hipStreamSynchronize(s));
+    // one segment over the whole window: every state kept, nothing recomputed
+    wing_window_adjoint(w, particles, U, dt, steps, std::max<int64_t>(steps, 1), reference_offset, e, CL, xbar,
+                        alphabar, nullptr);
+  });
+}
+
+int uvlm_wing_window_adjoint(uvlm_wing w, uvlm_handle particles, const double U[3], double dt, int64_t steps,
+                             int64_t every, int reference_offset, const double e[3], double *CL, int64_t n,
+                             double *xbar, double *alphabar, double *vertex_bar)
+{
+  return guarded([&] {
+    check_pair(w, particles);
+    if (!particles || !U || !e) throw Error(UVLM_ERR_ARG, "null particles, freestream or direction");
+    if (steps < 0 || every < 1) throw Error(UVLM_ERR_ARG, "steps must be >= 0 and every >= 1");
+    if ((xbar || alphabar) && n != particles->N) throw Error(UVLM_ERR_ARG, "size mismatch");
+    check_vjp_config(particles);
+    UVLM_HIP(hipSetDevice(w->device));
+    wing_window_adjoint(w, particles, U, dt, steps, every, reference_offset, e, CL, xbar, alphabar, vertex_bar);
   });
 }
 

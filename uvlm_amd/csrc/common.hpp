@@ -187,6 +187,13 @@ void vjp_finish(hipStream_t s, int64_t N, const double *ast, double *aux, int64_
 // mixed: FP32 pair arithmetic (FP64 accumulation), else FP64
 void p2p_adj(hipStream_t s, const Tree &t, const Lists &l, int side, int mixed, const double *st, double *aux,
              int64_t cap);
+// sbar_j (device order of the sources) = d/dsigma_j of the near-pair sum of ubar_i . u_i +
+// Jbar_i : J_i (Jbar when with_jbar), over tl = the P2P list CSR by source cell (entries = target
+// cells); tcot holds ubar (3 fields) then Jbar (9 fields) with stride ccap, tx the targets' X.
+void sigma_bar(hipStream_t s, const Tree &stree, const Tree &ttree, const Lists &tl, const double *tx, int64_t tcap,
+               const double *tcot, int64_t ccap, bool with_jbar, const double *sst, int64_t scap, double *sbar);
+// dst[orig[p]] += src[p] (device order -> original order)
+void add_to_orig(hipStream_t s, int64_t N, const int32_t *orig, const double *src, double *dst);
 // state cotangents in the original body order (6 fields of ccap doubles) <-> adjoint fields
 void cot_gather(hipStream_t s, int64_t N, const int32_t *orig, const double *cot, int64_t ccap, double *aux,
                 int64_t cap);
@@ -229,6 +236,24 @@ void wing_shed_vjp(hipStream_t s, int nspan, int nchord, const double *V, const 
 void probe_sources(hipStream_t s, int64_t m, const double *pts, const double *wts, double *st, int64_t cap);
 void cot_add_probe(hipStream_t s, int64_t N, const int32_t *orig, const double *ast, int64_t cap, double *cot,
                    int64_t ccap);
+// design gradient: cotangents of the wing's vertices.  Corner cotangents per panel are
+// [n][12] (le, le1, te, te1); wing_vertex_gather sums up to three such buffers into [nv][3].
+void wing_influence_vjp(hipStream_t s, int n, int nspan, const double *V, const double *cp, const double *nrm,
+                        const double *B, const double *Abar, const double *ub, const double *gamma, double *pb_ring,
+                        double *cpb, double *nb);
+void wing_geom_vjp(hipStream_t s, int n, int nspan, const double *V, const double *cpb1, const double *cpb2,
+                   const double *nb1, const double *nb2, double areab, const double *dlb, double *pb);
+void wing_shed_geom_vjp(hipStream_t s, int nspan, int nchord, const double *V, const double *gamma, const double *lam,
+                        int64_t stride, int64_t base, const double *sb, double *pb);
+void wing_vertex_gather(hipStream_t s, int nspan, int nchord, const double *pb0, const double *pb1, const double *pb2,
+                        double *vb);
+void wing_solve_geom_acc(hipStream_t s, int n, const double *rbar, const double *gamma, const double U[3],
+                         const double *v, double *Abar, double *nb);
+void wing_cp_acc(hipStream_t s, int n, const double *vb, const double *J, double *cpb);
+void wing_force_dl_vjp(hipStream_t s, int n, int nspan, const double *u, const double *gamma, const double e[3],
+                       double *dlb);
+void probe_velocity_grad_out(hipStream_t s, int64_t m, const int32_t *perm, const double *st, int64_t cap, double *out,
+                             double *Jout);
 
 // body.hip
 void aos_to_soa(hipStream_t s, const void *aos, int64_t n, int64_t stride_doubles, double *st, int64_t cap);

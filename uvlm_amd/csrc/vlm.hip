@@ -442,6 +442,257 @@ __global__ void k_cot_add_probe(int64_t N, const int32_t *__restrict__ orig, con
   }
 }
 
+// ---- design gradient: cotangents of the wing's vertices ----
+// Per-panel corner cotangents are written as [n][12] (le, le1, te, te1, 3 doubles each) and
+// summed into the vertices by k_wing_vertex_gather, so every sum runs in a fixed order.
+
+// Vector-Jacobian product of the unit segment velocity w = segment(a, b, x) (main.cpp:11-25):
+// adds (dw/da)^T wb, (dw/db)^T wb, (dw/dx)^T wb.  w = k c with c = r1 x r2, r1 = x - a,
+// r2 = x - b, k = r0 . (r1/|r1| - r2/|r2|) / (4 pi |c|^2), r0 = b - a.  Zero in the reference's
+// singular case, where the forward returns zero.
+__device__ inline void segment_vjp(V3 a, V3 b, V3 x, V3 wb, V3 &ab, V3 &bb, V3 &xb)
+{
+  const V3     r1 = x - a, r2 = x - b, r0 = b - a;
+  const V3     c  = cross(r1, r2);
+  const double n1 = sqrt(dot(r1, r1)), n2 = sqrt(dot(r2, r2)), c2 = dot(c, c);
+  if (n1 < 1e-6 || n2 < 1e-6 || sqrt(c2) < 1e-6) return;
+  const V3     u1 = (1.0 / n1) * r1, u2 = (1.0 / n2) * r2;
+  const V3     d  = u1 - u2;
+  const double k  = dot(r0, d) / c2 * INV_4PI;
+  const double kb = dot(wb, c);
+  const double Nb = kb * INV_4PI / c2;    // cotangent of r0 . d
+  const double cb2 = -kb * k / c2;        // cotangent of |c|^2
+  const V3     cb  = k * wb + (2.0 * cb2) * c;
+  const V3     r0b = Nb * d, db = Nb * r0;
+  V3           r1b = (1.0 / n1) * (db - dot(db, u1) * u1);
+  V3           r2b = (-1.0 / n2) * (db - dot(db, u2) * u2);
+  r1b              = r1b + cross(r2, cb);
+  r2b              = r2b + cross(cb, r1);
+  xb               = xb + r1b + r2b;
+  ab               = ab - r1b - r0b;
+  bb               = bb - r2b + r0b;
+}
+
+// Cotangents of the corners of ring j at control point x for a cotangent wb of the ring's
+// velocity (its four segments, in the order of ring()); xb collects the point's cotangent.
+__device__ inline void ring_vjp(const Corners &q, V3 x, V3 wb, V3 cb[4], V3 &xb)
+{
+  segment_vjp(q.te1, q.te, x, wb, cb[3], cb[2], xb);
+  segment_vjp(q.te, q.le, x, wb, cb[2], cb[0], xb);
+  segment_vjp(q.le, q.le1, x, wb, cb[0], cb[1], xb);
+  segment_vjp(q.le1, q.te1, x, wb, cb[1], cb[3], xb);
+}
+
+// Cotangent of ring j's velocity at control point i from A[i][j] = B[i][j] . n_i (Abar) and from
+// the force's u_i = sum_j B[i][j] gamma_j (ub_i gamma_j); either part may be absent.
+__device__ inline V3 influence_cot(int n, int i, int j, const double *Abar, const double *nrm, const double *ub,
+                                   const double *gamma)
+{
+  V3 wb{0, 0, 0};
+  if (Abar) wb = Abar[(int64_t)i * n + j] * ld3(nrm + 3 * i);
+  if (ub) wb = wb + gamma[j] * ld3(ub + 3 * i);
+  return wb;
+}
+
+// One workgroup per ring j, threads over the control points: corner cotangents of ring j.
+__global__ __launch_bounds__(256) void k_wing_infl_vjp_ring(int n, int nspan, const double *__restrict__ V,
+                                                            const double *__restrict__ cp,
+                                                            const double *__restrict__ nrm,
+                                                            const double *__restrict__ Abar,
+                                                            const double *__restrict__ ub,
+                                                            const double *__restrict__ gamma, double *__restrict__ pb)
+{
+  __shared__ double red[12][256];
+  const int         j = blockIdx.x, tid = threadIdx.x;
+  const Corners     q = corners(V, nspan, j);
+  V3                cb[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = tid; i < n; i += 256) {
+    const V3 wb = influence_cot(n, i, j, Abar, nrm, ub, gamma);
+    V3       xb{0, 0, 0};
+    ring_vjp(q, ld3(cp + 3 * i), wb, cb, xb);
+  }
+  for (int k = 0; k < 4; k++) {
+    red[3 * k][tid]     = cb[k].x;
+    red[3 * k + 1][tid] = cb[k].y;
+    red[3 * k + 2][tid] = cb[k].z;
+  }
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o)
+      for (int k = 0; k < 12; k++) red[k][tid] += red[k][tid + o];
+    __syncthreads();
+  }
+  if (tid < 12) pb[12 * j + tid] = red[tid][0];
+}
+
+// One wavefront per control point i, lanes over the rings: the point's cotangent and that of its
+// normal (A[i][j] = B[i][j] . n_i).
+__global__ __launch_bounds__(64) void k_wing_infl_vjp_point(int n, int nspan, const double *__restrict__ V,
+                                                            const double *__restrict__ cp,
+                                                            const double *__restrict__ nrm,
+                                                            const double *__restrict__ B,
+                                                            const double *__restrict__ Abar,
+                                                            const double *__restrict__ ub,
+                                                            const double *__restrict__ gamma,
+                                                            double *__restrict__ cpb, double *__restrict__ nb)
+{
+  const int i = blockIdx.x, lane = threadIdx.x;
+  if (i >= n) return;
+  const V3 x = ld3(cp + 3 * i);
+  V3       xb{0, 0, 0}, nnb{0, 0, 0};
+  for (int j = lane; j < n; j += 64) {
+    const V3 wb = influence_cot(n, i, j, Abar, nrm, ub, gamma);
+    V3       cb[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    ring_vjp(corners(V, nspan, j), x, wb, cb, xb);
+    if (Abar) nnb = nnb + Abar[(int64_t)i * n + j] * ld3(B + 3 * ((int64_t)i * n + j));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    xb.x += __shfl_xor(xb.x, o, 64);
+    xb.y += __shfl_xor(xb.y, o, 64);
+    xb.z += __shfl_xor(xb.z, o, 64);
+    nnb.x += __shfl_xor(nnb.x, o, 64);
+    nnb.y += __shfl_xor(nnb.y, o, 64);
+    nnb.z += __shfl_xor(nnb.z, o, 64);
+  }
+  if (lane == 0) {
+    cpb[3 * i] = xb.x, cpb[3 * i + 1] = xb.y, cpb[3 * i + 2] = xb.z;
+    nb[3 * i] = nnb.x, nb[3 * i + 1] = nnb.y, nb[3 * i + 2] = nnb.z;
+  }
+}
+
+// Corner cotangents of panel p from the cotangents of its control point (cpb1 + cpb2), unit
+// normal (nb1 + nb2), area (areab, the same for every panel) and leading-edge segment
+// LE_c1 - LE_c (dlb, may be null): the VJP of k_wing_geom and of the force's dl.
+__global__ void k_wing_geom_vjp(int n, int nspan, const double *__restrict__ V, const double *__restrict__ cpb1,
+                                const double *__restrict__ cpb2, const double *__restrict__ nb1,
+                                const double *__restrict__ nb2, double areab, const double *__restrict__ dlb,
+                                double *__restrict__ pb)
+{
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const Corners q  = corners(V, nspan, p);
+  const V3      v1 = q.le1 - q.te, v2 = q.le - q.te1;
+  const V3      v  = cross(v1, v2);
+  const double  l  = sqrt(dot(v, v));
+  const V3      nn = (1.0 / l) * v;
+  const V3      mb = 0.25 * (ld3(cpb1 + 3 * p) + ld3(cpb2 + 3 * p));
+  const V3      ng = ld3(nb1 + 3 * p) + ld3(nb2 + 3 * p);
+  const V3      vb = (1.0 / l) * (ng - dot(ng, nn) * nn) + (0.5 * areab) * nn;
+  const V3      v1b = cross(v2, vb), v2b = cross(vb, v1);
+  V3            le = mb + v2b, le1 = mb + v1b, te = mb - v1b, te1 = mb - v2b;
+  if (dlb) {
+    const V3 g = ld3(dlb + 3 * p);
+    le1        = le1 + g;
+    le         = le - g;
+  }
+  double *o = pb + 12 * p;
+  o[0] = le.x, o[1] = le.y, o[2] = le.z, o[3] = le1.x, o[4] = le1.y, o[5] = le1.z;
+  o[6] = te.x, o[7] = te.y, o[8] = te.z, o[9] = te1.x, o[10] = te1.y, o[11] = te1.z;
+}
+
+// Corner cotangents (added into pb) of the trailing-edge panels from the cotangents of the
+// particles they shed (k_wing_shed): position xb, strength ab (AoS-by-field with `stride`,
+// first shed particle at index `base`) and core size sb (may be null).
+__global__ void k_wing_shed_geom_vjp(int nspan, int nchord, const double *__restrict__ V,
+                                     const double *__restrict__ gamma, const double *__restrict__ lam, int64_t stride,
+                                     int64_t base, const double *__restrict__ sb, double *__restrict__ pb)
+{
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= nspan) return;
+  const int     p  = (nchord - 1) * nspan + c;
+  const Corners q  = corners(V, nspan, p);
+  const double *g  = gamma + (nchord - 1) * nspan;
+  const int64_t k  = base + c;
+  const V3      xb{lam[k], lam[stride + k], lam[2 * stride + k]};
+  const V3      abv{lam[3 * stride + k], lam[4 * stride + k], lam[5 * stride + k]};
+  const double  wl = c > 0 ? 0.5 * (g[c] - g[c - 1]) : g[c];
+  const double  wr = c < nspan - 1 ? 0.5 * (g[c] - g[c + 1]) : g[c];
+  V3            te = 0.5 * xb + g[c] * abv - wl * abv;
+  V3            te1 = 0.5 * xb - g[c] * abv + wr * abv;
+  const V3      le = wl * abv, le1 = -wr * abv;
+  if (sb) {
+    const V3 d  = q.te - q.te1;
+    const V3 dd = (2.5 * sb[k] / sqrt(dot(d, d))) * d;
+    te          = te + dd;
+    te1         = te1 - dd;
+  }
+  double *o = pb + 12 * p;
+  o[0] += le.x, o[1] += le.y, o[2] += le.z, o[3] += le1.x, o[4] += le1.y, o[5] += le1.z;
+  o[6] += te.x, o[7] += te.y, o[8] += te.z, o[9] += te1.x, o[10] += te1.y, o[11] += te1.z;
+}
+
+// Vertex cotangents: vertex (r, k) is corner le of panel (r, k), le1 of (r, k-1), te of (r-1, k)
+// and te1 of (r-1, k-1), where they exist; nb buffers of corner cotangents added in order.
+__global__ void k_wing_vertex_gather(int nspan, int nchord, const double *__restrict__ pb0,
+                                     const double *__restrict__ pb1, const double *__restrict__ pb2,
+                                     double *__restrict__ vb)
+{
+  const int w = nspan + 1, v = blockIdx.x * 256 + threadIdx.x;
+  if (v >= w * (nchord + 1)) return;
+  const int r = v / w, k = v % w;
+  double    acc[3] = {0, 0, 0};
+  auto      add    = [&](int pr, int pc, int corner) {
+    if (pr < 0 || pr >= nchord || pc < 0 || pc >= nspan) return;
+    const int64_t o = 12 * (int64_t)(pr * nspan + pc) + 3 * corner;
+    for (const double *pb : {pb0, pb1, pb2})
+      if (pb)
+        for (int d = 0; d < 3; d++) acc[d] += pb[o + d];
+  };
+  add(r, k, 0);
+  add(r, k - 1, 1);
+  add(r - 1, k, 2);
+  add(r - 1, k - 1, 3);
+  for (int d = 0; d < 3; d++) vb[3 * v + d] = acc[d];
+}
+
+// Abar[i][j] -= rbar_i gamma_j (A gamma = rhs), nb_i -= rbar_i (U + v_i) (rhs_i = -(U + v_i) . n_i).
+__global__ void k_wing_solve_geom_acc(int n, const double *__restrict__ rbar, const double *__restrict__ gamma,
+                                      double Ux, double Uy, double Uz, const double *__restrict__ v,
+                                      double *__restrict__ Abar, double *__restrict__ nb)
+{
+  const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  Abar[e] -= rbar[i] * gamma[j];
+  if (j == 0) {
+    V3 uv{Ux, Uy, Uz};
+    if (v) uv = uv + ld3(v + 3 * i);
+    nb[3 * i] -= rbar[i] * uv.x, nb[3 * i + 1] -= rbar[i] * uv.y, nb[3 * i + 2] -= rbar[i] * uv.z;
+  }
+}
+
+// cpb_b += sum_a vb_a J_i[a][b]: the control point's cotangent through v(cp), J = dv/dx there.
+__global__ void k_wing_cp_acc(int n, const double *__restrict__ vb, const double *__restrict__ J,
+                              double *__restrict__ cpb)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  for (int b = 0; b < 3; b++)
+    cpb[3 * i + b] += vb[3 * i] * J[9 * i + b] + vb[3 * i + 1] * J[9 * i + 3 + b] + vb[3 * i + 2] * J[9 * i + 6 + b];
+}
+
+// dlb_i = Gamma_i (e x u_i): the cotangent of panel i's leading-edge segment in J = e . F.
+__global__ void k_wing_force_dl_vjp(int n, int nspan, const double *__restrict__ uc, const double *__restrict__ gamma,
+                                    double ex, double ey, double ez, double *__restrict__ dlb)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double Gn = gamma[i] - (i >= nspan ? gamma[i - nspan] : 0.0);
+  const V3     g  = Gn * cross(V3{ex, ey, ez}, ld3(uc + 3 * i));
+  dlb[3 * i] = g.x, dlb[3 * i + 1] = g.y, dlb[3 * i + 2] = g.z;
+}
+
+// Velocity and its gradient at probe i, tree order -> the given order (out AoS [m][3], Jout [m][9]).
+__global__ void k_probe_velocity_grad_out(int64_t m, const int32_t *__restrict__ perm, const double *__restrict__ st,
+                                          int64_t cap, double *__restrict__ out, double *__restrict__ Jout)
+{
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i >= m) return;
+  const int64_t o = perm[i];
+  for (int d = 0; d < 3; d++) out[3 * o + d] = st[(F_U + d) * cap + i];
+  for (int k = 0; k < 9; k++) Jout[9 * o + k] = st[(F_J + k) * cap + i];
+}
+
 inline int nb256(int64_t n) { return (int)std::max<int64_t>(1, (n + 255) / 256); }
 
 }  // namespace
@@ -562,6 +813,64 @@ void cot_add_probe(hipStream_t s, int64_t N, const int32_t *orig, const double *
 {
   if (N <= 0) return;
   k_cot_add_probe<<<nb256(N), 256, 0, s>>>(N, orig, ast, cap, cot, ccap);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_influence_vjp(hipStream_t s, int n, int nspan, const double *V, const double *cp, const double *nrm,
+                        const double *B, const double *Abar, const double *ub, const double *gamma, double *pb_ring,
+                        double *cpb, double *nb)
+{
+  k_wing_infl_vjp_ring<<<n, 256, 0, s>>>(n, nspan, V, cp, nrm, Abar, ub, gamma, pb_ring);
+  UVLM_LAUNCH_CHECK();
+  k_wing_infl_vjp_point<<<n, 64, 0, s>>>(n, nspan, V, cp, nrm, B, Abar, ub, gamma, cpb, nb);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_geom_vjp(hipStream_t s, int n, int nspan, const double *V, const double *cpb1, const double *cpb2,
+                   const double *nb1, const double *nb2, double areab, const double *dlb, double *pb)
+{
+  k_wing_geom_vjp<<<nb256(n), 256, 0, s>>>(n, nspan, V, cpb1, cpb2, nb1, nb2, areab, dlb, pb);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_shed_geom_vjp(hipStream_t s, int nspan, int nchord, const double *V, const double *gamma, const double *lam,
+                        int64_t stride, int64_t base, const double *sb, double *pb)
+{
+  k_wing_shed_geom_vjp<<<nb256(nspan), 256, 0, s>>>(nspan, nchord, V, gamma, lam, stride, base, sb, pb);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_vertex_gather(hipStream_t s, int nspan, int nchord, const double *pb0, const double *pb1, const double *pb2,
+                        double *vb)
+{
+  k_wing_vertex_gather<<<nb256((int64_t)(nspan + 1) * (nchord + 1)), 256, 0, s>>>(nspan, nchord, pb0, pb1, pb2, vb);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_solve_geom_acc(hipStream_t s, int n, const double *rbar, const double *gamma, const double U[3],
+                         const double *v, double *Abar, double *nb)
+{
+  k_wing_solve_geom_acc<<<nb256((int64_t)n * n), 256, 0, s>>>(n, rbar, gamma, U[0], U[1], U[2], v, Abar, nb);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_cp_acc(hipStream_t s, int n, const double *vb, const double *J, double *cpb)
+{
+  k_wing_cp_acc<<<nb256(n), 256, 0, s>>>(n, vb, J, cpb);
+  UVLM_LAUNCH_CHECK();
+}
+
+void wing_force_dl_vjp(hipStream_t s, int n, int nspan, const double *u, const double *gamma, const double e[3],
+                       double *dlb)
+{
+  k_wing_force_dl_vjp<<<nb256(n), 256, 0, s>>>(n, nspan, u, gamma, e[0], e[1], e[2], dlb);
+  UVLM_LAUNCH_CHECK();
+}
+
+void probe_velocity_grad_out(hipStream_t s, int64_t m, const int32_t *perm, const double *st, int64_t cap, double *out,
+                             double *Jout)
+{
+  k_probe_velocity_grad_out<<<nb256(m), 256, 0, s>>>(m, perm, st, cap, out, Jout);
   UVLM_LAUNCH_CHECK();
 }
 
