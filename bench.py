@@ -9,8 +9,8 @@ A step = uvlm_build (tree + interaction lists) + uvlm_eval on device-resident pa
 Prints ONE JSON line (rank 0, N=1) with the contract fields plus:
   roofline     -- the dominant kernel's algorithmic FLOP rate vs the MI355X peak, from HIP
                   events recorded by the library on its stream around that kernel
-  cpu_baseline -- the oracle (CPU restatement of the reference) timed on the host cores on
-                  a bounded sample of the same workload, scaled to ms per 4M-particle eval
+  cpu_baseline -- the oracle (CPU restatement of the reference) timed on the host cores: one
+                  build + eval of the whole 4M-particle workload after a small warm-up
   extra        -- FP64-path time, mixed-vs-FP64 difference, error vs an FP64 direct sum on
                   sampled targets, per-phase times, list sizes, and the adjoint/forward time
                   ratio of the probe objective on 1M particles in three Gaussian blobs
@@ -189,17 +189,20 @@ def leapfrog_scenario(args, X, A, S, tuned):
     print(json.dumps(line), flush=True)
 
 
-def vlm_run(steps, window, precision, log=True):
-    """The wing-and-wake loop (the reference's main.cpp:168-295 time loop): a 40 x 10 vortex
-    lattice (span 10, chord 1, 5 degrees, freestream (0, 1, 0)) solved every step against the
-    FMM velocity of its particle wake at the control points, 40 particles shed per step, the
-    wake advanced by RK2 with dt = 0.02 (P = 8, theta = 0.5, ncrit 64).  After steps - window
-    steps, the lift-coefficient adjoint runs over the last window (its forward steps recomputed
-    inside).  Returns the measurements as a dict."""
+def vlm_run(steps, window, precision, every=10, log=True):
+    """The wing-and-wake loop (the reference's main.cpp:168-295 time loop) on the benchmark wing:
+    a 40 x 10 vortex lattice of aspect ratio 6 (span 6, chord 1) at 5 degrees angle of attack,
+    freestream (0, 1, 0), solved every step against the FMM velocity of its particle wake at the
+    control points, 40 particles shed per step, the wake advanced by RK2 with dt = 0.02 (P = 8,
+    theta = 0.5, ncrit 64).  After steps - window steps: the lift adjoint over the last window
+    (every state kept), then the design gradient over the same window (dCL/dalpha and dCL/d(40
+    strip twists), checkpoints every `every` steps) from the same starting state.  Returns the
+    measurements as a dict."""
     from uvlm_amd import FMMCalculator
-    from uvlm_amd.vlm import Wing, reference_wing
+    from uvlm_amd.vlm import Wing, design_gradient, design_wing
 
-    V = reference_wing(npanels=40, span=10.0, chord=1.0, angle_deg=5.0, nchord=10)
+    alpha = np.deg2rad(5.0)
+    V = design_wing(alpha, None, span=6.0, chord=1.0, nspan=40, nchord=10)
     U, dt = np.array([0.0, 1.0, 0.0]), 0.02
     win = min(window, steps)
     w = Wing(V)
@@ -217,22 +220,57 @@ def vlm_run(steps, window, precision, log=True):
             print(f"[vlm] step {k + 1}: {step_ms[-1]:.2f} ms, {p.n} particles, CL {cl[-1]:.5f}", file=sys.stderr,
                   flush=True)
     n0 = p.n
+    # the window's starting state, so the design adjoint starts where the lift adjoint did
+    arr, perm = p.arrays(), p.permutation()
+    X0, A0, S0 = np.empty((n0, 3)), np.empty((n0, 3)), np.empty(n0)
+    X0[perm], A0[perm], S0[perm] = arr["X"], arr["alpha"], arr["sigma"]
+    Vw = design_wing(alpha, None, span=6.0, chord=1.0, nspan=40, nchord=10) - (steps - win) * dt * U
+    # the forward window alone (for the ratios)
+    wf = Wing(Vw)
+    pf = FMMCalculator(P=8, theta=0.5, ncrit=64, precision=precision, scheme="rk2")
+    pf.set_arrays(X0, A0, S0)
+    pf.sync()
+    t0 = time.perf_counter()
+    for _ in range(win):
+        wf.step(pf, U, dt)
+    wf.solve(pf, U)
+    pf.sync()
+    fwd_s = time.perf_counter() - t0
+    wf.close()
+    pf.close()
     t0 = time.perf_counter()
     CL, xb, ab = w.lift_adjoint(p, U, dt, win)
     p.sync()
     adj_s = time.perf_counter() - t0
-    fwd_ms = float(np.mean(step_ms[-win:])) if step_ms else float("nan")
-    out = {"steps": steps, "panels": w.n, "particles_final": p.n,
+    wd = Wing(Vw)
+    pd = FMMCalculator(P=8, theta=0.5, ncrit=64, precision=precision, scheme="rk2")
+    pd.set_arrays(X0, A0, S0)
+    pd.sync()
+    t0 = time.perf_counter()
+    CLd, _, _, vb = wd.window_adjoint(pd, U, dt, win, every=every)
+    pd.sync()
+    des_s = time.perf_counter() - t0
+    ga, gt = design_gradient(vb, alpha, None, chord=1.0)
+    wd.close()
+    pd.close()
+    out = {"steps": steps, "panels": w.n, "particles_final": p.n, "aspect_ratio": 6.0, "alpha_deg": 5.0,
            "step_ms": {"mean": round(float(np.mean(step_ms)), 3) if step_ms else None,
                        "last": round(step_ms[-1], 3) if step_ms else None},
            "CL_final": CL, "CL_history_every_100": [round(c, 6) for c in cl[::100]],
+           "forward_window_s": round(fwd_s, 3),
            "lift_adjoint": {"window_steps": win, "particles_at_window_start": n0, "wall_s": round(adj_s, 3),
-                            "ratio_to_forward_window": round(adj_s * 1e3 / (win * fwd_ms), 3)
-                            if step_ms and win else None,
+                            "ratio_to_forward_window": round(adj_s / fwd_s, 3) if win else None,
                             "grad_norm_x": float(np.linalg.norm(xb)), "grad_norm_alpha": float(np.linalg.norm(ab)),
-                            "note": "wall time of uvlm_wing_lift_adjoint: forward window recomputed with states kept "
-                                    "in HBM, then force VJP, A^-T solves, probe adjoints and RK2 step VJPs back to "
-                                    "the window start"},
+                            "note": "wall time of uvlm_wing_lift_adjoint (every state of the window kept): its "
+                                    "forward window, then force VJP, A^-T solves, probe adjoints and RK2 step VJPs "
+                                    "back to the window start; ratio over the forward window alone"},
+           "design_gradient": {"window_steps": win, "checkpoint_every": every, "wall_s": round(des_s, 3),
+                               "ratio_to_forward_window": round(des_s / fwd_s, 3) if win else None,
+                               "CL": CLd, "dCL_dalpha_per_rad": ga, "dCL_dalpha_per_deg": ga * np.pi / 180.0,
+                               "dCL_dtwist_per_rad": [round(float(x), 8) for x in gt],
+                               "dCL_dtwist_norm": float(np.linalg.norm(gt)),
+                               "note": "uvlm_wing_window_adjoint: the wake state at the window start held fixed; "
+                                       "checkpoints every 10 steps, segments recomputed in the backward pass"},
            "wall_s": round(time.perf_counter() - t_all, 2)}
     w.close()
     p.close()
@@ -247,9 +285,9 @@ def vlm_scenario(args):
             "ms_per_step": r["step_ms"]["mean"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32 P2P + f64 expansions, f64 VLM" if args.precision == "mixed" else "f64",
             "data": "synthetic (40 x 10 panel wing, wake grown from nothing)",
-            "config": {"workload": "vortex-lattice wing 40x10 panels + particle wake, RK2 dt=0.02, P=8, theta=0.5, "
-                                   f"ncrit=64, {args.vlm_steps} steps, lift adjoint over the last "
-                                   f"{r['lift_adjoint']['window_steps']}",
+            "config": {"workload": "vortex-lattice wing 40x10 panels (aspect ratio 6, 5 deg) + particle wake, RK2 "
+                                   f"dt=0.02, P=8, theta=0.5, ncrit=64, {args.vlm_steps} steps, lift adjoint and "
+                                   f"design gradient over the last {r['lift_adjoint']['window_steps']}",
                        "panels": r["panels"], "particles_final": r["particles_final"]},
             "extra": r}
     print(json.dumps(line), flush=True)
@@ -267,7 +305,8 @@ def main():
                     help="leaf size; 0 (default): chosen by uvlm_autotune among --tune-ncrit on this workload")
     ap.add_argument("--tune-ncrit", default="64,96,128,160,192,224,256", help="autotuner candidates")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--cpu-n", type=int, default=131072, help="particles in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-n", type=int, default=-1,
+                    help="particles in the CPU baseline sample (-1: the whole workload, 0: skip)")
     ap.add_argument("--err-targets", type=int, default=2048, help="sampled targets for the direct-sum error")
     ap.add_argument("--no-fp64", action="store_true", help="skip timing the FP64 path")
     ap.add_argument("--scenario-steps", type=int, default=0,
@@ -275,7 +314,7 @@ def main():
     ap.add_argument("--vlm-steps", type=int, default=0,
                     help="instead of the eval benchmark: this many steps of the wing + wake loop (40 x 10 panels)")
     ap.add_argument("--vlm-window", type=int, default=50, help="steps of the lift adjoint at the end of --vlm-steps")
-    ap.add_argument("--vlm-leg-steps", type=int, default=250,
+    ap.add_argument("--vlm-leg-steps", type=int, default=2000,
                     help="wing + wake loop steps reported beside the eval benchmark (0: skip)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint leg (three Gaussian blobs) and print its measurements")
@@ -363,13 +402,35 @@ def main():
         peak = PEAK["fp32"] if args.precision == "mixed" else PEAK["fp64"]
     achieved = flop / (kms * 1e-3) / 1e12
     wl = {"particles": N, "P": args.P, "theta": args.theta, "ncrit": ncrit_used, "precision": args.precision}
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": profiled_traffic(wl, key), "kernel": kern,
-                "roof": "vector ALU (no MFMA in this path; FP64 vector and matrix peaks coincide on MI355X)",
+                "roof": "vector ALU: this path has no dense contraction and no MFMA (FP64 vector and matrix peaks "
+                        "coincide on MI355X); neither the HBM nor an MFMA roof bounds the dominant kernel",
                 "traffic_note": "HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in "
                                 "profiles/ for this workload (null if none)"}
+    # whole-eval roofline: the sum over the phases of the time each would take at its roof
+    # (FLOPs at the vector peak or algorithmic bytes at 8 TB/s, whichever is larger)
+    NT = 3 * args.P * (args.P + 1) // 2
+    bw = PEAK["hbm"] * 1e9
+    C, Q = tm["n_cells"], tm["n_m2l_pairs"]
+    t_roof = {
+        "tree": N * 128 / bw,  # read X, alpha, sigma; write them permuted; keys
+        "lists": (tm["n_p2p_pairs"] + Q) * 8 / bw,
+        "upward": (N * 56 + C * NT * 16 * 3) / bw,  # P2M reads the bodies; M2M reads children, writes parents
+        "m2l": max(m2l_flop / (PEAK["fp64"] * 1e12), Q * (16 * NT + 8) / bw + C * NT * 16 / bw),
+        "p2p": p2p_flop / ((PEAK["fp32"] if args.precision == "mixed" else PEAK["fp64"]) * 1e12),
+        "downward": (N * (56 + 120) + C * NT * 16 * 3) / bw,  # L2L; L2P reads bodies, writes u, J, dadt
+    }
+    t_roof_ms = {k: round(v * 1e3, 4) for k, v in t_roof.items()}
+    eval_roofline = {"roof_ms": t_roof_ms, "sum_roof_ms": round(sum(t_roof.values()) * 1e3, 3),
+                     "measured_ms": None, "frac": None,
+                     "note": "sum over the phases of algorithmic FLOPs / vector peak (M2L FP64, P2P in the path's "
+                             "precision) or algorithmic HBM bytes / 8 TB/s, whichever bounds the phase; "
+                             "frac = that sum / the measured ms per build + eval"}
 
-    extra = {"phases_ms": {k: round(v, 3) for k, v in ph.items()},
+    eval_roofline["measured_ms"] = round(ms, 3)
+    eval_roofline["frac"] = round(sum(t_roof.values()) * 1e3 / ms, 4)
+    extra = {"eval_roofline": eval_roofline, "phases_ms": {k: round(v, 3) for k, v in ph.items()},
              "eval_only_ms": round(ph["eval_ms"], 3), "tree_lists_ms": round(ph["tree_ms"] + ph["lists_ms"], 3),
              "cells": tm["n_cells"], "leaves": tm["n_leaves"], "levels": tm["n_levels"],
              "p2p_pairs": tm["n_p2p_pairs"], "m2l_pairs": tm["n_m2l_pairs"], "interactions": tm["n_interactions"],
@@ -414,21 +475,35 @@ def main():
         print(f"[bench] wing + wake: {extra['wing_wake']}", file=sys.stderr, flush=True)
 
     cpu = None
-    if args.cpu_n > 0:
+    if args.cpu_n != 0:
         from oracle import oracle as O  # the CPU baseline leg: the oracle, timed on the host
 
-        Xc, Ac, Sc = W.scaled("leapfrog", args.cpu_n)
         threads = O.max_threads()
+        # warm-up: a small eval of the same workload (threads, pages)
+        Xw, Aw, Sw = W.scaled("leapfrog", 65536)
+        cw = O.Calculator(P=args.P, theta=args.theta, ncrit=ncrit_used)
+        cw.init(O.make_bodies(Xw, Aw, Sw))
+        cw.calculate()
+        del cw
+        full = args.cpu_n < 0 or args.cpu_n >= N
+        Xc, Ac, Sc = (X, A, S) if full else W.scaled("leapfrog", args.cpu_n)
         b = O.make_bodies(Xc, Ac, Sc)
         c = O.Calculator(P=args.P, theta=args.theta, ncrit=ncrit_used)
         t0 = time.perf_counter()
         c.init(b)
         c.calculate()
         cpu_ms = (time.perf_counter() - t0) * 1e3
-        cpu = {"value": round(cpu_ms * N / Xc.shape[0], 1), "unit": "ms/eval (scaled to 4,194,304 particles)",
-               "cores": threads, "kind": "port",
-               "sample": f"one FP64 eval (tree + lists + FMM) of the same two-ring workload at {Xc.shape[0]} "
-                         f"particles took {cpu_ms:.0f} ms on {threads} threads; scaled linearly in N"}
+        del c, b
+        if full:
+            cpu = {"value": round(cpu_ms, 1), "unit": "ms/eval", "cores": threads, "kind": "port",
+                   "sample": f"one FP64 build + eval (tree + lists + FMM) of the whole {N}-particle workload at "
+                             f"ncrit {ncrit_used} on {threads} host threads, after a 65,536-particle warm-up"}
+        else:
+            cpu = {"value": round(cpu_ms * N / Xc.shape[0], 1), "unit": "ms/eval (scaled to 4,194,304 particles)",
+                   "cores": threads, "kind": "port",
+                   "sample": f"one FP64 eval (tree + lists + FMM) of the same two-ring workload at {Xc.shape[0]} "
+                             f"particles took {cpu_ms:.0f} ms on {threads} threads; scaled linearly in N"}
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     line = {"metric": "ms/FMM eval (u+stretch) @4M particles, rel err<=1e-6; adjoint/fwd time ratio",
             "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

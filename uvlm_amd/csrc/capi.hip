@@ -1532,12 +1532,14 @@ void check_pair(uvlm_wing w, uvlm_handle h)
 
 // main.cpp:196-244 at the wing's current position: particle velocity at the control points,
 // right-hand side, circulations, force.
-void wing_solve(uvlm_wing w, uvlm_handle h, const double U[3])
+// Jcp (optional, [n][9]): also the gradient of the particle velocity at the control points.
+void wing_solve(uvlm_wing w, uvlm_handle h, const double U[3], double *Jcp = nullptr)
 {
   const int n    = w->n;
   double   *vptr = nullptr;
   if (h && h->N > 0) {
-    velocity_at_points(h, n, w->cp.as<double>(), w->v.as<double>());
+    if (Jcp) velocity_grad_at_points(h, n, w->cp.as<double>(), w->v.as<double>(), Jcp);
+    else velocity_at_points(h, n, w->cp.as<double>(), w->v.as<double>());
     UVLM_HIP(hipStreamSynchronize(h->s));
     vptr = w->v.as<double>();
   }
@@ -1670,11 +1672,16 @@ void wing_window_adjoint(uvlm_wing w, uvlm_handle h, const double U[3], double d
     else h->N = 0;
   };
   std::vector<int64_t> nstart(steps + 1, 0);
-  // one pass of the loop from the current state; gam / vel receive the solve's circulations and
-  // wake velocity at the control points, yshed the state right after the shedding
-  auto loop_step = [&](int64_t k, double *gam, double *vel, DevBuf *yshed) {
+  // per-step record of a solve: circulations [np], wake velocity [3 np] and (design gradient)
+  // its gradient [9 np] at the control points
+  const int64_t RS = geom ? 13 * np : 4 * np;
+  // one pass of the loop from the current state; rec receives the solve's record, yshed the
+  // state right after the shedding
+  auto loop_step = [&](int64_t k, double *rec, DevBuf *yshed) {
     nstart[k] = h->N;
-    wing_solve(w, h, U);
+    double *gam = rec, *vel = rec ? rec + np : nullptr, *Jr = (rec && geom) ? rec + 4 * np : nullptr;
+    wing_solve(w, h, U, Jr);
+    if (Jr && h->N == 0) zero_doubles(ws, Jr, 9 * np);
     if (gam) UVLM_HIP(hipMemcpyAsync(gam, w->gamma.p, 8 * np, hipMemcpyDeviceToDevice, ws));
     if (vel) {
       if (h->N > 0) UVLM_HIP(hipMemcpyAsync(vel, w->v.p, 24 * np, hipMemcpyDeviceToDevice, ws));
@@ -1705,7 +1712,7 @@ void wing_window_adjoint(uvlm_wing w, uvlm_handle h, const double U[3], double d
   std::vector<DevBuf>   yshed(E);
   std::vector<WingPose> pose(E);
   DevBuf                segrec;
-  segrec.reserve(sizeof(double) * 4 * np * E);  // gamma [np] and v [3 np] per step of a segment
+  segrec.reserve(sizeof(double) * RS * E);
   const int64_t last0 = (nck - 1) * E;
   for (int64_t k = 0; k < steps; k++) {
     if (k % E == 0) {
@@ -1714,11 +1721,10 @@ void wing_window_adjoint(uvlm_wing w, uvlm_handle h, const double U[3], double d
       save_pose(w, ckpose[k / E]);
     }
     if (k >= last0) {
-      double *rec = segrec.as<double>() + 4 * np * (k - last0);
       save_pose(w, pose[k - last0]);
-      loop_step(k, rec, rec + np, &yshed[k - last0]);
+      loop_step(k, segrec.as<double>() + RS * (k - last0), &yshed[k - last0]);
     } else {
-      loop_step(k, nullptr, nullptr, nullptr);
+      loop_step(k, nullptr, nullptr);
     }
   }
   nstart[steps] = h->N;
@@ -1728,7 +1734,9 @@ void wing_window_adjoint(uvlm_wing w, uvlm_handle h, const double U[3], double d
   WingPose final_pose;
   save_pose(w, final_pose);
   // the lift coefficient at the final position
-  wing_solve(w, h, U);
+  DevBuf fK;  // the final solve's record
+  fK.reserve(sizeof(double) * RS);
+  wing_solve(w, h, U, (geom && h->N > 0) ? fK.as<double>() + 4 * np : nullptr);
   std::vector<double> area(np);
   double              F[3];
   UVLM_HIP(hipMemcpyAsync(area.data(), w->area.p, 8 * np, hipMemcpyDeviceToHost, ws));
@@ -1742,9 +1750,8 @@ void wing_window_adjoint(uvlm_wing w, uvlm_handle h, const double U[3], double d
   if (CL) *CL = cl;
   // backward
   DevBuf abuf;
-  abuf.reserve(sizeof(double) * (size_t)(np + 3 * np + np + np + 3 * np + 9 * np));
-  double *c = abuf.as<double>(), *vbar = c + np, *gbar = vbar + 3 * np, *rbar = gbar + np, *vtmp = rbar + np,
-         *Jcp = vtmp + 3 * np;
+  abuf.reserve(sizeof(double) * (size_t)(np + 3 * np + np + np));
+  double *c = abuf.as<double>(), *vbar = c + np, *gbar = vbar + 3 * np, *rbar = gbar + np;
   h->vlam.reserve(sizeof(double) * 7 * cap);
   double *lam = h->vlam.as<double>();
   zero_doubles(s, lam, 7 * cap);
@@ -1782,20 +1789,15 @@ void wing_window_adjoint(uvlm_wing w, uvlm_handle h, const double U[3], double d
   // the solve of one step (circulations gam, wake velocity vel at the control points, npart
   // particles on the device in that step's state, wing at that step's pose): gbar -> rbar =
   // A^{-T} gbar -> the geometry accumulators, vbar -= rbar n, then the wake
-  auto through_solve = [&](const double *gam, const double *vel, int64_t npart) {
+  auto through_solve = [&](const double *rec, int64_t npart) {
+    const double *gam = rec, *vel = rec + np, *Jr = rec + 4 * np;
     matvec_inv_t(ws, np, w->Ainv.as<double>(), gbar, rbar);
     if (geom) wing_solve_geom_acc(ws, np, rbar, gam, U, npart ? vel : nullptr, Abar, nacc);
     wing_rhs_vjp(ws, np, w->nrm.as<double>(), rbar, vbar);
+    if (geom && npart) wing_cp_acc(ws, np, vbar, Jr, cpacc);
     UVLM_HIP(hipStreamSynchronize(ws));
     if (npart == 0) return;
-    if (geom) {
-      velocity_grad_at_points(h, np, w->cp.as<double>(), vtmp, Jcp);
-      UVLM_HIP(hipStreamSynchronize(s));
-      wing_cp_acc(ws, np, vbar, Jcp, cpacc);
-      UVLM_HIP(hipStreamSynchronize(ws));
-    } else {
-      do_build(h);
-    }
+    do_build(h);
     reserve_probes(h, np);
     probe_sources(s, np, w->cp.as<double>(), vbar, h->sst.as<double>(), h->scap);
     probe_adjoint_core(h, np);
@@ -1810,14 +1812,10 @@ void wing_window_adjoint(uvlm_wing w, uvlm_handle h, const double U[3], double d
     }
     UVLM_HIP(hipStreamSynchronize(s));
   };
-  {
-    DevBuf fK;  // gamma_K [np], v_K [3 np] of the final solve
-    fK.reserve(32 * np);
-    UVLM_HIP(hipMemcpyAsync(fK.p, w->gamma.p, 8 * np, hipMemcpyDeviceToDevice, ws));
-    if (nfinal > 0) UVLM_HIP(hipMemcpyAsync(fK.as<double>() + np, w->v.p, 24 * np, hipMemcpyDeviceToDevice, ws));
-    UVLM_HIP(hipStreamSynchronize(ws));
-    through_solve(fK.as<double>(), fK.as<double>() + np, nstart[steps]);
-  }
+  UVLM_HIP(hipMemcpyAsync(fK.p, w->gamma.p, 8 * np, hipMemcpyDeviceToDevice, ws));
+  if (nfinal > 0) UVLM_HIP(hipMemcpyAsync(fK.as<double>() + np, w->v.p, 24 * np, hipMemcpyDeviceToDevice, ws));
+  UVLM_HIP(hipStreamSynchronize(ws));
+  through_solve(fK.as<double>(), nstart[steps]);
   // segments backwards: recompute each (the last was recorded by the forward pass), then its
   // steps in reverse
   for (int64_t g = nck - 1; g >= 0; g--) {
@@ -1827,13 +1825,12 @@ void wing_window_adjoint(uvlm_wing w, uvlm_handle h, const double U[3], double d
       load_pose(w, ckpose[g]);
       UVLM_HIP(hipStreamSynchronize(ws));
       for (int64_t k = k0; k < k1; k++) {
-        double *rec = segrec.as<double>() + 4 * np * (k - k0);
         save_pose(w, pose[k - k0]);
-        loop_step(k, rec, rec + np, &yshed[k - k0]);
+        loop_step(k, segrec.as<double>() + RS * (k - k0), &yshed[k - k0]);
       }
     }
     for (int64_t k = k1 - 1; k >= k0; k--) {
-      const double *gam = segrec.as<double>() + 4 * np * (k - k0), *vel = gam + np;
+      const double *rec = segrec.as<double>() + RS * (k - k0), *gam = rec;
       const int64_t nshed = nstart[k] + ns;
       load_pose(w, pose[k - k0]);
       UVLM_HIP(hipStreamSynchronize(ws));
@@ -1847,7 +1844,7 @@ void wing_window_adjoint(uvlm_wing w, uvlm_handle h, const double U[3], double d
       UVLM_HIP(hipStreamSynchronize(ws));
       if (nstart[k]) load_state_orig_n(h, yshed[k - k0].as<double>(), nshed, nstart[k]);
       else h->N = 0;
-      through_solve(gam, vel, nstart[k]);
+      through_solve(rec, nstart[k]);
     }
   }
   h->N = N0;
