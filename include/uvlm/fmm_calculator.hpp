@@ -422,6 +422,9 @@ struct FMMCalculator {
     if (xbar.size() != 3 * n_ || alphabar.size() != 3 * n_) throw std::runtime_error("cotangents must be [n][3]");
     call(uvlm_trajectory_vjp(h_.h, (int64_t)n_, xbar.data(), alphabar.data()));
   }
+  // Near-field option (uvlm_set_near_field): pairs whose bodies can come closer than c * max sigma
+  // use the regularised kernel (P2P); 0, the default, is the reference's acceptance test alone.
+  void setNearField(exafmm::real_t c) { call(uvlm_set_near_field(h_.h, c)); }
   // Leaf size (and M2L plan size) for the bodies last passed to init()/copyBodies2Device();
   // returns the chosen ncrit.  init() again before calculate().
   size_t autotune(const std::vector<int64_t> &candidates, int reps = 2)

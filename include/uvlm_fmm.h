@@ -283,6 +283,16 @@ int uvlm_relax(uvlm_handle h, double f);
  * counterpart (the reference prints only strength extrema, src/main.cpp:219-221). */
 int uvlm_diagnostics(uvlm_handle h, double *out);
 
+/* Near-field option (no counterpart in the reference, off by default): with c > 0, a cell pair
+ * whose bodies can come closer than c * (the largest core size sigma of the particles) is treated
+ * as near (P2P with the regularised kernel) even where the acceptance test of traverse.hpp:56-57
+ * would take it as far: |X_t - X_s| < c sigma_max + sqrt(3) (R_t + R_s).  The reference's far field
+ * applies the singular kernel, which differs from the regularised one by a factor
+ * 1 - erf(s) + 2 s e^{-s^2} / sqrt(pi), s = R / (sqrt(2) sigma) (about 1e-6 at R = 6 sigma), so c
+ * bounds the far field's regularisation error; the expansion order P and theta bound its
+ * truncation.  Applies from the next build; c = 0 restores the reference's lists exactly. */
+int uvlm_set_near_field(uvlm_handle h, double c);
+
 /* Viscous schemes (the ViscousScheme tag and _nu of fmmCalculator.hpp:9-13, 56; _nu is never
  * initialised there).  Once set, uvlm_advance applies the scheme after every stage's eval, as
  * advance() calls calculateViscosity() (fmmCalculator.hpp:919-1005):

@@ -77,6 +77,7 @@ def lib():
             "orc_m2m": (None, [I, P, P, P, P]),
             "orc_m2l": (None, [I, P, P, P, P]),
             "orc_m2l_order": (None, [I, I, P, P, P, P]),
+            "orc_set_near_gap": (None, [VP, D]),
             "orc_l2l": (None, [I, P, P, P, P]),
             "orc_l2p": (None, [I, P, P, L64, P, P, P]),
             "orc_direct": (None, [L64, P, L64, P, P, P, P, P]),
@@ -134,6 +135,11 @@ class Calculator:
         if getattr(self, "_h", None):
             lib().orc_destroy(self._h)
             self._h = None
+
+    def set_near_field(self, c):
+        """Near-field option: pairs whose bodies can come closer than c * max sigma are near (P2P);
+        0 is the reference's acceptance test alone.  Applies from the next init."""
+        lib().orc_set_near_gap(self._h, float(c))
 
     def init(self, bodies):
         """Builds tree + lists; permutes `bodies` (n,37) in place like FMMCalculator::init."""

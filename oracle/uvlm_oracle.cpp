@@ -190,10 +190,15 @@ static int build_tree(Body *bodies, int64_t n, int64_t ncrit, std::vector<Cell> 
 // near leaf-leaf -> P2P, near otherwise -> push, far -> M2L.  Split the target if the
 // source is a leaf or (Rt >= Rs and the target is not a leaf), else split the source (74).
 // Two leaf roots give the single P2P pair (0,0) (41-44).
+// gap > 0 (an option of this build, off in the reference): a pair is also near when its bodies
+// can come closer than gap, |dX| < gap + sqrt(3) (Ri + Rj) (cells are cubes of half-width R),
+// tested on squares so the decision rounds the same on every machine.
 struct Pair { int64_t t, s; };
 
-static void get_list(const std::vector<Cell> &tc, const std::vector<Cell> &sc, real_t theta, std::vector<Pair> &P2P,
-                     std::vector<Pair> &M2L)
+static constexpr real_t SQRT3 = 1.7320508075688772;
+
+static void get_list(const std::vector<Cell> &tc, const std::vector<Cell> &sc, real_t theta, real_t gap,
+                     std::vector<Pair> &P2P, std::vector<Pair> &M2L)
 {
   P2P.clear();
   M2L.clear();
@@ -208,8 +213,9 @@ static void get_list(const std::vector<Cell> &tc, const std::vector<Cell> &sc, r
     const Cell &Cj = sc[j];
     real_t      dX[3];
     for (int d = 0; d < 3; d++) dX[d] = Ci.X[d] - Cj.X[d];
-    const real_t R2 = norm3(dX) * theta * theta;
-    if (R2 < (Ci.R + Cj.R) * (Ci.R + Cj.R)) {
+    const real_t D2 = norm3(dX), R2 = D2 * theta * theta, Rs = Ci.R + Cj.R;
+    const real_t Gs = gap + SQRT3 * Rs;
+    if (R2 < Rs * Rs || (gap > 0 && D2 < Gs * Gs)) {
       if (Ci.NCHILD == 0 && Cj.NCHILD == 0) P2P.push_back({i, j});
       else stack.push_back({i, j});
     } else {
@@ -701,6 +707,7 @@ struct Calculator {
   std::vector<complex_t>               M, L;
   std::vector<std::vector<int64_t>>    levels;  // fmmCalculator.hpp:75-79
   bool              ready = false;
+  real_t            near_c = 0, gap = 0;  // near-field option: gap = near_c * max sigma of the bodies
   // sensor state (fmmCalculator.hpp:59-64)
   std::vector<Body>      sensors;
   std::vector<Cell>      scells;
@@ -721,7 +728,13 @@ struct Calculator {
       if ((int64_t)levels.size() <= l) levels.resize(l + 1);
       levels[l].push_back(i);
     }
-    get_list(cells, cells, theta, p2p, m2l);
+    gap = 0;
+    if (near_c > 0) {
+      real_t smax = 0;
+      for (int64_t i = 0; i < n; i++) smax = std::max(smax, caller[i].radius);
+      gap = near_c * smax;
+    }
+    get_list(cells, cells, theta, gap, p2p, m2l);
     bodies.assign(caller, caller + n);
     M.assign(cells.size() * NTERM, complex_t(0, 0));
     L.assign(cells.size() * NTERM, complex_t(0, 0));
@@ -1034,7 +1047,7 @@ struct Calculator {
     if (!ready) return ORC_ERR_STATE;
     int st = build_tree(caller, m, ncrit, scells);
     if (st != ORC_OK) return st;
-    get_list(scells, cells, theta, sp2p, sm2l);
+    get_list(scells, cells, theta, gap, sp2p, sm2l);
     sensors.assign(caller, caller + m);
     for (auto &b : sensors)
       for (int d = 0; d < 3; d++) b.velocity_old[d] = 0;
@@ -1065,7 +1078,7 @@ struct Calculator {
     st = build_tree(np.data(), k, ncrit, scells);  // createSensorTree(sensorOnly) 164-201
     if (st != ORC_OK) return st;
     std::vector<Pair> self_p2p, self_m2l;
-    get_list(scells, scells, theta, self_p2p, self_m2l);
+    get_list(scells, scells, theta, gap, self_p2p, self_m2l);
     auto single = [&]() {  // calculateVorticitySingleTree (788-795): resetSensors, then P2P
       for (auto &b : np)
         for (int d = 0; d < 3; d++) b.velocity[d] = b.dadt[d] = 0;
@@ -1126,7 +1139,7 @@ struct Calculator {
       if ((int64_t)slevels.size() <= l) slevels.resize(l + 1);
       slevels[l].push_back(i);
     }
-    get_list(scells, cells, theta, sp2p, sm2l);
+    get_list(scells, cells, theta, gap, sp2p, sm2l);
     sensors.assign(caller, caller + m);
     const int64_t C = scells.size();
     Ls.assign(C * NTERM, complex_t(0, 0));
@@ -1191,6 +1204,9 @@ void *orc_create(int P, double theta, int64_t ncrit, int stretching, int formula
   return new (std::nothrow) Calculator(P, theta, ncrit, stretching, formulation);
 }
 void orc_destroy(void *h) { delete (Calculator *)h; }
+// Near-field option (see get_list): pairs whose bodies can come closer than c * max sigma are
+// near; 0 (the default) is the reference's acceptance test alone.  Applies from the next init.
+void orc_set_near_gap(void *h, double c) { ((Calculator *)h)->near_c = c > 0 ? c : 0; }
 
 int orc_init(void *h, Body *bodies, int64_t n) { return ((Calculator *)h)->init(bodies, n); }
 int orc_calculate(void *h) { return ((Calculator *)h)->calculate(); }

@@ -405,3 +405,49 @@ def test_autotune_keeps_a_candidate_and_the_results_stay_exact(orc):
     ref = c.bodies(len(X))
     assert rel(out[:, 15:18], ref[:, 15:18]) < 1e-10 and rel(out[:, 0:9], ref[:, 0:9]) < 1e-10
 
+
+
+@pytest.mark.parametrize("name", ["cube", "blobs"])
+def test_near_field_option_matches_oracle(orc, name):
+    """The near-field option (uvlm_set_near_field, c = 5): lists bit-exact against the oracle's
+    traversal with the same option, FP64 results within 1e-10, mixed within 1e-6 of FP64, and the
+    probes' lists and velocities too; c = 0 afterwards gives the reference's lists again."""
+    X, A, S = CASES[name]()
+    P, th, nc = PARAMS[name]
+    b_ref = orc.make_bodies(X, A, S)
+    c = orc.Calculator(P=P, theta=th, ncrit=nc)
+    c.set_near_field(5.0)
+    c.init(b_ref)
+    c.calculate()
+    ref = c.bodies(len(X))
+    out = {}
+    for prec in ("fp64", "mixed"):
+        g = FMMCalculator(P=P, theta=th, ncrit=nc, precision=prec)
+        g.set_near_field(5.0)
+        b = make_bodies(X, A, S)
+        g.init(b)
+        if prec == "fp64":
+            assert np.array_equal(b, b_ref)
+            rp, rm = c.pairs()
+            gp, gm = g.lists()
+            assert np.array_equal(np.unique(rp, axis=0), gp) and np.array_equal(np.unique(rm, axis=0), gm)
+            print(f"[near field {name}] {len(gp)} P2P, {len(gm)} M2L pairs", flush=True)
+        g.calculate()
+        out[prec] = g.copyBodies2Host()
+        if prec == "fp64":
+            probes = X[::97] + 0.003
+            s_ref = orc.make_bodies(probes, np.zeros_like(probes), np.zeros(len(probes)))
+            s_gpu = s_ref.copy()
+            c.sensor_data(s_ref)
+            g.getSensorData(s_gpu)
+            assert np.array_equal(s_gpu[:, 9:12], s_ref[:, 9:12])
+            assert rel(s_gpu[:, 15:18], s_ref[:, 15:18]) < 1e-10
+            g.set_near_field(0.0)
+            g.init(make_bodies(X, A, S))
+            c0 = orc.Calculator(P=P, theta=th, ncrit=nc)
+            c0.init(orc.make_bodies(X, A, S))
+            assert np.array_equal(np.unique(c0.pairs()[1], axis=0), g.lists()[1])
+    for f, sl in (("velocity", slice(15, 18)), ("J", slice(0, 9)), ("dadt", slice(18, 21))):
+        e64, emx = rel(out["fp64"][:, sl], ref[:, sl]), rel(out["mixed"][:, sl], out["fp64"][:, sl])
+        print(f"  {f}: fp64 vs oracle {e64:.2e}, mixed vs fp64 {emx:.2e}", flush=True)
+        assert e64 < 1e-10 and emx < 1e-6

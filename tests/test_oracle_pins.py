@@ -278,3 +278,50 @@ def test_rk_stages_follow_the_butcher_form(orc, scheme):
         tol = 1e-14 * (np.abs(y0) + dt * sum(np.abs(k) for k in ks))
         assert np.all(np.abs(y - want) <= tol + 1e-300), (scheme, s, np.max(np.abs(y - want) / (tol + 1e-300)))
     assert np.abs(ks[-1][:, 6]).max() > 0  # the radius equation was exercised
+
+
+# ------------------------------------------------------------------- near-field option
+def test_near_field_option_keeps_the_lists_complete_and_removes_the_kernel_mismatch(orc):
+    """With the near-field option (pairs whose bodies can come closer than c * sigma_max are near),
+    every M2L pair's bodies stay at least c * sigma_max apart, the lists still cover every source
+    body exactly once per target leaf, c = 0 gives the reference's lists, and the FMM error against
+    the direct sum falls from the regularisation mismatch (~1e-2) to the expansion truncation."""
+    X, A, S = W.scaled("cube", 8000)
+    errs = {}
+    for c in (0.0, 6.0):
+        b = orc.make_bodies(X, A, S)
+        calc = orc.Calculator(P=12, theta=0.4, ncrit=32)
+        calc.set_near_field(c)
+        calc.init(b)
+        calc.calculate()
+        out = calc.bodies(len(X))
+        ints, reals = calc.cells()
+        p2p, m2l = calc.pairs()
+        nchild, nbody, child, body, level = ints.T
+        Xc, R = reals[:, :3], reals[:, 3]
+        gap = c * S.max()
+        d = np.linalg.norm(Xc[m2l[:, 0]] - Xc[m2l[:, 1]], axis=1)
+        if c > 0:
+            assert np.all(d >= gap + np.sqrt(3) * (R[m2l[:, 0]] + R[m2l[:, 1]]) - 1e-12)
+            # bodies of every M2L pair are at least gap apart (checked on a sample of pairs)
+            for t, s in m2l[:: max(1, len(m2l) // 200)]:
+                xt, xs = out[body[t]:body[t] + nbody[t], 9:12], out[body[s]:body[s] + nbody[s], 9:12]
+                dmin = np.min(np.linalg.norm(xt[:, None, :] - xs[None, :, :], axis=2))
+                assert dmin >= gap, (t, s, dmin, gap)
+        parent = np.full(len(ints), -1)
+        for q in range(len(ints)):
+            parent[child[q]:child[q] + nchild[q]] = q
+        by_t = {}
+        for t, s in np.concatenate([p2p, m2l]):
+            by_t.setdefault(t, []).append(s)
+        for t in np.nonzero(nchild == 0)[0][::5]:
+            cnt = np.zeros(len(X), dtype=int)
+            a = t
+            while a >= 0:
+                for s in by_t.get(a, []):
+                    cnt[body[s]:body[s] + nbody[s]] += 1
+                a = parent[a]
+            assert (cnt == 1).all(), (c, t)
+        ud, Jd = orc.direct(out[:500, 9:12], out[:, 9:12], out[:, 12:15], out[:, 33])
+        errs[c] = np.linalg.norm(out[:500, 15:18] - ud) / np.linalg.norm(ud)
+    assert errs[0.0] > 1e-3 and errs[6.0] < 2e-5, errs

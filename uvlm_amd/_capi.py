@@ -23,7 +23,7 @@ EXPORTS = [
     "uvlm_step_vjp", "uvlm_trajectory_forward", "uvlm_trajectory_vjp", "uvlm_append_bodies_soa",
     "uvlm_wing_create", "uvlm_wing_destroy", "uvlm_wing_get", "uvlm_wing_translate", "uvlm_wing_solve",
     "uvlm_wing_shed", "uvlm_wing_step", "uvlm_wing_lift_adjoint", "uvlm_num_bodies", "uvlm_direct_noalloc", "uvlm_get_list",
-    "uvlm_wing_window_adjoint",
+    "uvlm_wing_window_adjoint", "uvlm_set_near_field",
 ]
 
 
@@ -92,6 +92,7 @@ def lib():
             "uvlm_sync": (I, [VP]),
             "uvlm_autotune": (I, [VP, PI64, I, I, PI64, PD]),
             "uvlm_set_viscosity": (I, [VP, I, D]),
+            "uvlm_set_near_field": (I, [VP, D]),
             "uvlm_viscosity": (I, [VP, D]),
             "uvlm_vorticity_aos": (I, [VP, VP, I64, I64]),
             "uvlm_rbf_aos": (I, [VP, VP, I64, I64, PI64, ctypes.POINTER(ctypes.c_int), PD]),

@@ -58,6 +58,8 @@ struct uvlm_handle_s {
   DevBuf  sst, sst2, Ls, Ms, sperm, ast;
   int64_t scap = 0;
   hipEvent_t aev[2]{};
+  // near-field option (uvlm_set_near_field): near_c, and the gap of the last build = near_c * max sigma
+  double near_c = 0.0, gap = 0.0;
   // viscous scheme applied after each eval of uvlm_advance (UVLM_INVISCID / UVLM_PSE / UVLM_CSM)
   int    viscous = UVLM_INVISCID;
   double nu      = 0.0;
@@ -155,7 +157,8 @@ void do_build(uvlm_handle h)
   swap_buf(h->st, h->st2);
   swap_buf(h->orig, h->orig2);
   UVLM_HIP(hipEventRecord(h->ev[EV_TREE], s));
-  traverse(s, h->sc, h->tree, h->tree, h->prm.theta, h->lists);
+  h->gap = h->near_c > 0.0 ? h->near_c * max_value(s, h->sc, h->F(F_SIG), h->N) : 0.0;
+  traverse(s, h->sc, h->tree, h->tree, h->prm.theta, h->lists, h->gap);
   UVLM_HIP(hipEventRecord(h->ev[EV_LISTS], s));
   count_interactions(s, h->sc, h->tree, h->tree, h->lists, &h->lists.n_interactions);
   UVLM_HIP(hipStreamSynchronize(s));
@@ -261,7 +264,7 @@ void copy_out_records(const std::vector<double> &rec, int64_t m, void *dst, int6
 void probe_vorticity(uvlm_handle h, int64_t m)
 {
   Region region("Calculation of Vorticity");  // fmmCalculator.hpp:735
-  traverse(h->s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
+  traverse(h->s, h->sc, h->stree, h->tree, h->prm.theta, h->slists, h->gap);
   zero_fields(h->s, h->sst.as<double>(), h->scap, m, F_UOLD, 3);
   gauss_near(h->s, h->stree, h->tree, h->slists, h->sst.as<double>(), h->scap, h->st.as<double>(), h->cap,
              h->sst.as<double>(), h->scap, 1, F_UOLD, 0.0);
@@ -293,7 +296,7 @@ void sensor_pass(uvlm_handle h, int64_t m, bool with_j = false)
              h->sperm.as<int32_t>());
   gather_fields(s, h->sst.as<double>(), h->sst2.as<double>(), sc, sc, N_FIELDS, h->sperm.as<int32_t>(), m);
   swap_buf(h->sst, h->sst2);
-  traverse(s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
+  traverse(s, h->sc, h->stree, h->tree, h->prm.theta, h->slists, h->gap);
   const int P = h->prm.P;
   h->Ls.reserve(sizeof(double2) * h->NT * h->stree.C);
   double2 *Ls = h->Ls.as<double2>();
@@ -388,7 +391,7 @@ void probe_adjoint_core(uvlm_handle h, int64_t m)
   gather_fields(s, h->sst.as<double>(), h->sst2.as<double>(), sc, sc, N_FIELDS, h->sperm.as<int32_t>(), m);
   swap_buf(h->sst, h->sst2);
   // the forward probe lists, transposed: targets = particle cells, sources = probe cells
-  traverse(s, h->sc, h->stree, h->tree, h->prm.theta, h->slists);
+  traverse(s, h->sc, h->stree, h->tree, h->prm.theta, h->slists, h->gap);
   transpose_lists(s, h->sc, h->stree, h->tree, h->slists, h->alists);
   // probe multipoles (root included: a probe cell is a source here)
   h->Ms.reserve(sizeof(double2) * NT * h->stree.C);
@@ -1173,6 +1176,16 @@ int uvlm_autotune(uvlm_handle h, const int64_t *ncrit, int n, int reps, int64_t 
   });
 }
 
+int uvlm_set_near_field(uvlm_handle h, double c)
+{
+  return guarded([&] {
+    check_handle(h);
+    if (!(c >= 0.0)) throw Error(UVLM_ERR_ARG, "the near-field factor must be >= 0");
+    h->near_c = c;
+    h->built = h->evaluated = false;
+  });
+}
+
 int uvlm_set_viscosity(uvlm_handle h, int scheme, double nu)
 {
   return guarded([&] {
@@ -1237,7 +1250,7 @@ int uvlm_rbf_aos(uvlm_handle h, void *points, int64_t m, int64_t stride, int64_t
     if (k == 0) return;
     // tree over the kept points and its self lists (createSensorTree(sensorOnly), 164-201)
     upload_probes(h, keep.data(), k, 296);
-    traverse(s, h->sc, h->stree, h->stree, h->prm.theta, h->slists);
+    traverse(s, h->sc, h->stree, h->stree, h->prm.theta, h->slists, h->gap);
     double       *st = h->sst.as<double>();
     const int64_t sc = h->scap;
     h->sc.partial.reserve(sizeof(double) * 3 * 512);

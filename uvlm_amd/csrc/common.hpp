@@ -125,7 +125,10 @@ struct Scratch {
 // tree.hip
 void build_tree(hipStream_t s, Scratch &sc, const double *x, const double *y, const double *z, int64_t N, int64_t ncrit,
                 Tree &tree, int32_t *perm_out);
-void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double theta, Lists &lists);
+// gap > 0: pairs whose bodies can come closer than gap are near too (the near-field option)
+void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double theta, Lists &lists,
+              double gap = 0.0);
+double max_value(hipStream_t s, Scratch &sc, const double *x, int64_t n);
 void gather_fields(hipStream_t s, const double *src, double *dst, int64_t src_cap, int64_t dst_cap, int nfields,
                    const int32_t *perm, int64_t N);
 void gather_i32(hipStream_t s, const int32_t *src, int32_t *dst, const int32_t *perm, int64_t N);

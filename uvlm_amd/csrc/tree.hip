@@ -379,8 +379,11 @@ __device__ __forceinline__ int wave_excl_scan(int v, int &total)
 }
 
 // One BFS step.  counters: [0] next frontier, [1] P2P, [2] M2L (64-bit).
+// gap > 0 (near-field option, off in the reference): a pair is also near when its bodies can come
+// closer than gap, |dX| < gap + sqrt(3) (Rt + Rs), tested on squares (rounds as the oracle does).
 __device__ __forceinline__ void traverse_round(const int32_t *__restrict__ ft, int64_t nf, int64_t i, TreeView T,
-                                               TreeView S, double theta, int32_t *__restrict__ nxt, int64_t cap_f,
+                                               TreeView S, double theta, double gap, int32_t *__restrict__ nxt,
+                                               int64_t cap_f,
                                                uint64_t *__restrict__ p2p, int64_t cap_p, uint64_t *__restrict__ m2l,
                                                int64_t cap_m, unsigned long long *__restrict__ counters, int sbits)
 {
@@ -396,10 +399,11 @@ __device__ __forceinline__ void traverse_round(const int32_t *__restrict__ ft, i
       const int    a = split_t ? first + c : t;
       const int    b = split_t ? s : first + c;
       const double dx = T.cx[a] - S.cx[b], dy = T.cy[a] - S.cy[b], dz = T.cz[a] - S.cz[b];
-      const double R2 = (dx * dx + dy * dy + dz * dz) * theta * theta;
+      const double D2 = dx * dx + dy * dy + dz * dz, R2 = D2 * theta * theta;
       const double Rs = T.R[a] + S.R[b];
+      const double Gs = gap + 1.7320508075688772 * Rs;
       int          k;
-      if (R2 < Rs * Rs) k = (T.nchild[a] == 0 && S.nchild[b] == 0) ? 1 : 0;
+      if (R2 < Rs * Rs || (gap > 0.0 && D2 < Gs * Gs)) k = (T.nchild[a] == 0 && S.nchild[b] == 0) ? 1 : 0;
       else k = 2;
       cls[nout] = k;
       ct[nout]  = a;
@@ -443,14 +447,15 @@ __device__ __forceinline__ void traverse_round(const int32_t *__restrict__ ft, i
 // One breadth-first step.  counters: [0] next frontier size, [1] P2P and [2] M2L list sizes so far,
 // [3] this step's frontier size (read here, so a batch of steps runs without the host).  The
 // blocks stride over the frontier; every lane of a block takes part in each round.
-__global__ void k_traverse(const int32_t *__restrict__ ft, TreeView T, TreeView S, double theta,
+__global__ void k_traverse(const int32_t *__restrict__ ft, TreeView T, TreeView S, double theta, double gap,
                            int32_t *__restrict__ nxt, int64_t cap_f, uint64_t *__restrict__ p2p, int64_t cap_p,
                            uint64_t *__restrict__ m2l, int64_t cap_m, unsigned long long *__restrict__ counters,
                            int sbits)
 {
   const int64_t nf = (int64_t)counters[3];
   for (int64_t base = blockIdx.x * (int64_t)BLK; base < nf; base += (int64_t)gridDim.x * BLK)
-    traverse_round(ft, nf, base + threadIdx.x, T, S, theta, nxt, cap_f, p2p, cap_p, m2l, cap_m, counters, sbits);
+    traverse_round(ft, nf, base + threadIdx.x, T, S, theta, gap, nxt, cap_f, p2p, cap_p, m2l, cap_m, counters,
+                   sbits);
 }
 
 // Start of a traversal: the root pair (0, 0) is the first frontier, or, when both roots are leaves,
@@ -741,6 +746,35 @@ void gather_i32(hipStream_t s, const int32_t *src, int32_t *dst, const int32_t *
   UVLM_LAUNCH_CHECK();
 }
 
+// Largest value of x[0..n) (non-negative values): per-block maxima, then one block over them.
+__global__ void k_max_partial(const double *__restrict__ x, int64_t n, double *__restrict__ part)
+{
+  __shared__ double red[BLK];
+  double            m = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) m = fmax(m, x[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = BLK / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+double max_value(hipStream_t s, Scratch &sc, const double *x, int64_t n)
+{
+  if (n <= 0) return 0.0;
+  const int nb = (int)std::min<int64_t>(512, (n + BLK - 1) / BLK);
+  sc.partial.reserve(sizeof(double) * (nb + 1));
+  double *part = sc.partial.as<double>();
+  k_max_partial<<<nb, BLK, 0, s>>>(x, n, part);
+  k_max_partial<<<1, BLK, 0, s>>>(part, nb, part + nb);
+  UVLM_LAUNCH_CHECK();
+  double m = 0.0;
+  to_host(s, &m, part + nb);
+  return m;
+}
+
 static TreeView view(const Tree &t)
 {
   return TreeView{(const int32_t *)t.nchild.p, (const int32_t *)t.child.p, (const int32_t *)t.nbody.p,
@@ -748,7 +782,7 @@ static TreeView view(const Tree &t)
                   (const double *)t.R.p};
 }
 
-void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double theta, Lists &lists)
+void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double theta, Lists &lists, double gap)
 {
   TreeView  T = view(tt), S = view(st);
   const int sbits = bits_for(std::max<int64_t>(st.C, 1));  // list keys: (target << sbits) | source
@@ -779,7 +813,7 @@ void traverse(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &st, double
       for (;;) {
         for (int k = 0; k < STEPS; k++, par ^= 1) {
           DevBuf &fin = par ? sc.front1 : sc.front0, &fout = par ? sc.front0 : sc.front1;
-          k_traverse<<<grid, BLK, 0, s>>>(fin.as<int32_t>(), T, S, theta, fout.as<int32_t>(), cap_f,
+          k_traverse<<<grid, BLK, 0, s>>>(fin.as<int32_t>(), T, S, theta, gap, fout.as<int32_t>(), cap_f,
                                           sc.out_p2p.as<uint64_t>(), cap_p, sc.out_m2l.as<uint64_t>(), cap_m, dcnt,
                                           sbits);
           k_step_end<<<1, 64, 0, s>>>(dcnt, cap_f, cap_p, cap_m);

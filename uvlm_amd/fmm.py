@@ -261,6 +261,11 @@ class FMMCalculator:
         _capi.check(self.lib.uvlm_timings(self._h, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
 
+    def set_near_field(self, c):
+        """Near-field option: cell pairs whose bodies can come closer than c * max sigma are near
+        (P2P, regularised kernel); 0 restores the reference's lists.  Applies from the next build."""
+        _capi.check(self.lib.uvlm_set_near_field(self._h, float(c)))
+
     def set_viscosity(self, scheme, nu):
         """"inviscid", "pse" or "csm" with viscosity nu; uvlm_advance applies it after each eval."""
         _capi.check(self.lib.uvlm_set_viscosity(self._h, VISCOUS[scheme], float(nu)))
