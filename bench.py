@@ -293,6 +293,64 @@ def vlm_scenario(args):
     print(json.dumps(line), flush=True)
 
 
+def accuracy_sweep(args):
+    """Error against the FP64 direct sum on 16,384 sampled targets, and the build + eval time, over a
+    grid of (P, theta, near-field factor c) on one workload (--accuracy-sweep cube: the uniform cube
+    of 1,048,576 particles, sigma = 2h; leapfrog: the two rings of 4,194,304 particles, sigma =
+    0.08).  One JSON line per setting, then the cheapest setting whose velocity, gradient and
+    stretching errors are all <= 1e-6."""
+    from uvlm_amd import FMMCalculator, workloads as W
+
+    if args.accuracy_sweep == "cube":
+        X, A, S, idx = W.cube()
+        idx = np.asarray(idx)
+    else:
+        X, A, S = W.leapfrog(n_stations=args.stations)
+        idx = np.random.default_rng(1003).choice(X.shape[0], 16384, replace=False)
+    grid = [tuple(float(v) for v in g.split(":")) for g in args.sweep_grid.split(",")]
+    ncrit = args.ncrit or 64
+    g = FMMCalculator(P=4, theta=0.5, ncrit=ncrit)
+    g.set_arrays(X, A, S)
+    t0 = time.perf_counter()
+    ud, Jd, _ = g.direct(X[idx])
+    Dd = np.einsum("iba,ib->ia", Jd.reshape(-1, 3, 3), A[idx])  # transposed stretching J^T alpha
+    print(f"[sweep] direct sum on {len(idx)} targets: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    g.close()
+    best = None
+    for P, theta, c in grid:
+        res = {}
+        for prec in ("mixed", "fp64"):
+            h = FMMCalculator(P=int(P), theta=theta, ncrit=ncrit, precision=prec)
+            h.set_near_field(c)
+            h.set_arrays(X, A, S)
+            h.build()
+            h.calculate()
+            h.sync()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                h.build()
+                h.calculate()
+            h.sync()
+            ms = (time.perf_counter() - t0) * 1e3 / 2
+            tm = h.timings()
+            arr, perm = h.arrays(), h.permutation()
+            inv = np.empty(len(X), dtype=np.int64)
+            inv[perm] = np.arange(len(X))
+            o = inv[idx]
+            res[prec] = {"ms": round(ms, 3), "u": rel(arr["velocity"][o], ud), "J": rel(arr["J"][o], Jd),
+                         "dadt": rel(arr["dadt"][o], Dd), "p2p_ms": round(tm["p2p_ms"], 3),
+                         "m2l_ms": round(tm["m2l_ms"], 3), "interactions": tm["n_interactions"],
+                         "m2l_pairs": tm["n_m2l_pairs"]}
+            h.close()
+        line = {"workload": args.accuracy_sweep, "particles": int(X.shape[0]), "P": int(P), "theta": theta,
+                "near_field_c": c, "ncrit": ncrit, **{k: v for k, v in res.items()}}
+        print(json.dumps(line), flush=True)
+        ok = max(res["mixed"]["u"], res["mixed"]["J"], res["mixed"]["dadt"]) <= 1e-6
+        if ok and (best is None or res["mixed"]["ms"] < best["mixed"]["ms"]):
+            best = line
+    print(json.dumps({"cheapest_at_1e-6": best}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,6 +378,10 @@ def main():
                     help="run only the adjoint leg (three Gaussian blobs) and print its measurements")
     ap.add_argument("--adjoint-n", type=int, default=1 << 20,
                     help="particles in the adjoint leg (three Gaussian blobs, 4096 probes, P=8; 0: skip)")
+    ap.add_argument("--accuracy-sweep", choices=["cube", "leapfrog"], default=None,
+                    help="instead of the eval benchmark: error vs the FP64 direct sum and time over --sweep-grid")
+    ap.add_argument("--sweep-grid", default="10:0.4:0,10:0.4:6,12:0.4:6,14:0.4:6,16:0.4:6,12:0.3:6,14:0.3:6",
+                    help="comma-separated P:theta:c settings of --accuracy-sweep (c: near-field factor)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path, so --gpus must be 1.",
@@ -328,6 +390,9 @@ def main():
 
     if args.vlm_steps > 0:
         vlm_scenario(args)
+        return
+    if args.accuracy_sweep:
+        accuracy_sweep(args)
         return
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(args.adjoint_n, args.theta, args.ncrit or 64, max(1, args.steps))), flush=True)
