@@ -46,6 +46,18 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// Hand-over of LDS data between the lanes of a one-wavefront workgroup: a wavefront's LDS
+// operations execute in order, so waiting for its own (lgkmcnt 0) is enough, and global loads
+// issued before this point stay in flight (__syncthreads would also wait for those).  The
+// signal fences keep the compiler from moving memory accesses across it.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt 63, expcnt 7, lgkmcnt 0
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 __device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b)
 {
   acc.x = fma(a.x, b.x, acc.x);
@@ -769,7 +781,7 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
       for (int i = 0; i < lane; i++) q *= v;
       sinv[lane] = q;
     }
-    __syncthreads();
+    wave_lds_sync();
   }
   // the next batch's source centre (and level) and the index of the one after it are loaded a
   // batch ahead, so their latency overlaps the current batch's multiply-adds
@@ -785,7 +797,15 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
     const bool active = si < nsrc;
     if (active) {
       const double2 *Ms = M + (int64_t)s_cur * NT;
-      for (int q = sl; q < NT; q += 16) Mg[q] = Ms[q];
+      // four loads in flight per lane before the first LDS store; indices past the end are
+      // clamped to the last entry, whose owner writes the same value (no branches)
+      for (int q0 = sl; q0 < NT; q0 += 64) {
+        double2 v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] = Ms[min(q0 + 16 * i, NT - 1)];
+#pragma unroll
+        for (int i = 0; i < 4; i++) Mg[min(q0 + 16 * i, NT - 1)] = v[i];
+      }
       // same tree: X_t - X_s = r k with r = R0 / 2^l the smaller cell's half-width and k an
       // integer vector, and Y_n(r k) = r^-(n+1) Y_n(k); offsets inside the table's cube are read
       // from it and scaled per row, the rest (and other trees) are computed here
@@ -801,11 +821,17 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
           const int      w   = 2 * M2L_TABLE_K + 1;
           const int64_t  idx = (((int64_t)kx + M2L_TABLE_K) * w + ((int64_t)ky + M2L_TABLE_K)) * w + ((int64_t)kz + M2L_TABLE_K);
           const double2 *Yk  = tab.Y + idx * P * P;
-          for (int e = sl; e < P * P; e += 16) {
-            const int     J = (int)sqrtf((float)e + 0.5f);  // row of entry e: J*J <= e <= J*J + 2J
-            const double  f = ldexp(sinv[J], l * (J + 1));
-            const double2 y = Yk[e];
-            Yx[e]           = make_double2(y.x * f, y.y * f);
+          for (int e0 = sl; e0 < P * P; e0 += 64) {
+            double2 y[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) y[i] = Yk[min(e0 + 16 * i, P * P - 1)];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              const int    e = min(e0 + 16 * i, P * P - 1);
+              const int    J = (int)sqrtf((float)e + 0.5f);  // row of entry e: J*J <= e <= J*J + 2J
+              const double f = ldexp(sinv[J], l * (J + 1));
+              Yx[e]          = make_double2(y[i].x * f, y[i].y * f);
+            }
           }
           from_table = true;
         }
@@ -821,7 +847,7 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
       if (tab.Y) ls_cur = tab.level[s_cur];
       if (si + 8 < nsrc) s_nxt = src[s0 + si + 8];
     }
-    __syncthreads();
+    wave_lds_sync();
     if (active) {
       for (int n = 0; n < P; n++) {
         const int      lv = lev[n], cn = lv & 255, hr = (lv >> 8) & 255, used = lv >> 16;
@@ -850,7 +876,7 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
         }
       }
     }
-    __syncthreads();
+    wave_lds_sync();
   }
   // combine the 4 source groups (same slot in lanes sl, sl+16, sl+32, sl+48)
   for (int s = 0; s < S; s++)
@@ -1037,10 +1063,8 @@ __global__ void k_zero(double *p, int64_t n)
 }  // namespace
 
 // ---------------------------------------------------------------------------------
-// M2L schedule: items (j, k0, w, nlo, nhi) with cost = (number of paired m steps, n+1 per
-// source degree n) * W, packed
-// longest-first into 16 lane slots with at most K items each; the item cap is searched
-// to minimise the largest slot load.
+// M2L schedule: the work plan of k_m2l_ns (m2l_plan.hpp) for the handle's slot budget, copied
+// to the device, and the lattice and octant harmonic tables for P.
 void make_m2l_schedule(int P, M2LSchedule &s)
 {
   if (!make_m2l_plan(P, s.budget, s.plan)) throw Error(UVLM_ERR_INTERNAL, "no M2L plan");
