@@ -351,6 +351,131 @@ def accuracy_sweep(args):
     print(json.dumps({"cheapest_at_1e-6": best}), flush=True)
 
 
+def configs_run(args):
+    """Configs 1-4 side by side (one JSON line each, then a summary line): build + eval time on the
+    GPU in FP64 and mixed precision (median of 3 after one warm-up), per-phase times, the error of
+    both against the FP64 direct sum (uvlm_direct) on the config's targets, mixed against FP64,
+    the whole-eval roofline fraction, and one FP64 build + eval of the oracle on the host threads.
+      1 ring      8,192 particles, P 8, theta 0.5, ncrit 64; all 8,192 targets; also 10 RK3 steps
+      2 cube      1,048,576 particles, P 10, theta 0.4, ncrit 64; the 16,384 sampled targets
+      3 leapfrog  4,194,304 particles, P 10, theta 0.5, ncrit 64 (as stated) and 256; 16,384
+                  targets drawn with seed 1003
+      4 blobs     the adjoint leg (1,048,576 particles, 4,096 probes, P 8)"""
+    from oracle import oracle as O  # the CPU baseline: the oracle, timed on the host
+    from uvlm_amd import FMMCalculator, workloads as W
+
+    threads = O.max_threads()
+    rows = []
+
+    def one(name, X, A, S, P, theta, ncrit, idx, rk3_steps=0):
+        t0 = time.perf_counter()
+        g = FMMCalculator(P=4, theta=0.5, ncrit=ncrit)
+        g.set_arrays(X, A, S)
+        ud, Jd, _ = g.direct(X[idx])
+        Dd = np.einsum("iba,ib->ia", Jd.reshape(-1, 3, 3), A[idx])  # transposed stretching J^T alpha
+        g.close()
+        print(f"[configs] {name}: direct sum on {len(idx)} targets {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        res = {}
+        for prec in ("fp64", "mixed"):
+            h = FMMCalculator(P=P, theta=theta, ncrit=ncrit, precision=prec, scheme="rk3")
+            h.set_arrays(X, A, S)
+            h.build()
+            h.calculate()
+            h.sync()
+            walls, phs = [], []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                h.build()
+                h.calculate()
+                h.sync()
+                walls.append((time.perf_counter() - t0) * 1e3)
+                phs.append(h.timings())
+            ms = float(np.median(walls))
+            tm = phs[int(np.argsort(walls)[1])]
+            arr, perm = h.arrays(), h.permutation()
+            inv = np.empty(len(X), dtype=np.int64)
+            inv[perm] = np.arange(len(X))
+            o = inv[idx]
+            r = {"ms": round(ms, 3), "phases_ms": {k: round(tm[k], 3) for k in ("tree_ms", "lists_ms", "upward_ms",
+                                                                                 "m2l_ms", "p2p_ms", "downward_ms")},
+                 "interactions": tm["n_interactions"], "m2l_pairs": tm["n_m2l_pairs"],
+                 "interactions_per_s": tm["n_interactions"] / (tm["p2p_ms"] * 1e-3) if tm["p2p_ms"] > 0 else None,
+                 "vs_direct": {"u": rel(arr["velocity"][o], ud), "J": rel(arr["J"][o], Jd),
+                               "dadt": rel(arr["dadt"][o], Dd)},
+                 "eval_roofline_frac": round(eval_roofline_ms(tm, len(X), P, prec) / ms, 4)}
+            res[prec] = r
+            res[prec + "_fields"] = {f: arr[f][o].copy() for f in ("velocity", "J", "dadt")}
+            if rk3_steps:
+                h.set_arrays(X, A, S)
+                h.sync()
+                t0 = time.perf_counter()
+                for _ in range(rk3_steps):
+                    h.step(0.01)
+                h.sync()
+                r["rk3_step_ms"] = round((time.perf_counter() - t0) * 1e3 / rk3_steps, 3)
+            h.close()
+            print(f"[configs] {name} {prec}: {r}", file=sys.stderr, flush=True)
+        f64, mx = res.pop("fp64_fields"), res.pop("mixed_fields")
+        res["mixed_vs_fp64"] = {k: rel(mx[k], f64[k]) for k in f64}
+        b = O.make_bodies(X, A, S)
+        c = O.Calculator(P=P, theta=theta, ncrit=ncrit)
+        t0 = time.perf_counter()
+        c.init(b)
+        c.calculate()
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        del c, b
+        line = {"config": name, "particles": int(len(X)), "P": P, "theta": theta, "ncrit": ncrit,
+                "targets": int(len(idx)), **res,
+                "cpu_oracle": {"ms": round(cpu_ms, 1), "threads": threads, "kind": "port",
+                               "what": "one FP64 build + eval (tree + lists + FMM)"}}
+        print(json.dumps(line), flush=True)
+        rows.append(line)
+
+    # warm the host threads and pages once
+    Xw, Aw, Sw = W.scaled("leapfrog", 65536)
+    cw = O.Calculator(P=8, theta=0.5, ncrit=64)
+    cw.init(O.make_bodies(Xw, Aw, Sw))
+    cw.calculate()
+    del cw
+    X, A, S = W.ring()
+    one("1 ring", X, A, S, 8, 0.5, 64, np.arange(len(X)), rk3_steps=10)
+    X, A, S, idx = W.cube()
+    one("2 cube", X, A, S, 10, 0.4, 64, np.asarray(idx))
+    X, A, S = W.leapfrog(n_stations=args.stations)
+    idx = np.random.default_rng(1003).choice(X.shape[0], 16384, replace=False)
+    for nc in (64, 256):
+        one("3 leapfrog", X, A, S, 10, 0.5, nc, idx)
+    del X, A, S
+    adj = adjoint_leg(args.adjoint_n or (1 << 20), args.theta, 64, 3)
+    print(json.dumps({"config": "4 blobs (adjoint)", **adj}), flush=True)
+
+
+def eval_roof_phases(tm, N, P, precision):
+    """The time (s) each phase of one build + eval would take at its roof: algorithmic FLOPs at the
+    vector peak (M2L in FP64, P2P in the path's precision) or algorithmic HBM bytes at 8 TB/s,
+    whichever is larger."""
+    NT = 3 * P * (P + 1) // 2
+    bw = PEAK["hbm"] * 1e9
+    C, Q = tm["n_cells"], tm["n_m2l_pairs"]
+    m2l_flop = Q * m2l_flop_per_pair(P)
+    p2p_flop = tm["n_interactions"] * P2P_FLOP
+    t = {
+        "tree": N * 128 / bw,  # read X, alpha, sigma; write them permuted; keys
+        "lists": (tm["n_p2p_pairs"] + Q) * 8 / bw,
+        "upward": (N * 56 + C * NT * 16 * 3) / bw,  # P2M reads the bodies; M2M reads children, writes parents
+        "m2l": max(m2l_flop / (PEAK["fp64"] * 1e12), Q * (16 * NT + 8) / bw + C * NT * 16 / bw),
+        "p2p": p2p_flop / ((PEAK["fp32"] if precision == "mixed" else PEAK["fp64"]) * 1e12),
+        "downward": (N * (56 + 120) + C * NT * 16 * 3) / bw,  # L2L; L2P reads bodies, writes u, J, dadt
+    }
+    return t
+
+
+def eval_roofline_ms(tm, N, P, precision):
+    """Sum over the phases of their roof times (ms)."""
+    return sum(eval_roof_phases(tm, N, P, precision).values()) * 1e3
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -380,6 +505,8 @@ def main():
                     help="particles in the adjoint leg (three Gaussian blobs, 4096 probes, P=8; 0: skip)")
     ap.add_argument("--accuracy-sweep", choices=["cube", "leapfrog"], default=None,
                     help="instead of the eval benchmark: error vs the FP64 direct sum and time over --sweep-grid")
+    ap.add_argument("--configs", action="store_true",
+                    help="instead of the eval benchmark: configs 1-4 (ring, cube, leapfrog, blobs) on GPU and host")
     ap.add_argument("--sweep-grid", default="10:0.4:0,10:0.4:6,12:0.4:6,14:0.4:6,16:0.4:6,12:0.3:6,14:0.3:6",
                     help="comma-separated P:theta:c settings of --accuracy-sweep (c: near-field factor)")
     args = ap.parse_args()
@@ -393,6 +520,9 @@ def main():
         return
     if args.accuracy_sweep:
         accuracy_sweep(args)
+        return
+    if args.configs:
+        configs_run(args)
         return
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(args.adjoint_n, args.theta, args.ncrit or 64, max(1, args.steps))), flush=True)
@@ -474,18 +604,7 @@ def main():
                 "traffic_note": "HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in "
                                 "profiles/ for this workload (null if none)"}
     # whole-eval roofline: the sum over the phases of the time each would take at its roof
-    # (FLOPs at the vector peak or algorithmic bytes at 8 TB/s, whichever is larger)
-    NT = 3 * args.P * (args.P + 1) // 2
-    bw = PEAK["hbm"] * 1e9
-    C, Q = tm["n_cells"], tm["n_m2l_pairs"]
-    t_roof = {
-        "tree": N * 128 / bw,  # read X, alpha, sigma; write them permuted; keys
-        "lists": (tm["n_p2p_pairs"] + Q) * 8 / bw,
-        "upward": (N * 56 + C * NT * 16 * 3) / bw,  # P2M reads the bodies; M2M reads children, writes parents
-        "m2l": max(m2l_flop / (PEAK["fp64"] * 1e12), Q * (16 * NT + 8) / bw + C * NT * 16 / bw),
-        "p2p": p2p_flop / ((PEAK["fp32"] if args.precision == "mixed" else PEAK["fp64"]) * 1e12),
-        "downward": (N * (56 + 120) + C * NT * 16 * 3) / bw,  # L2L; L2P reads bodies, writes u, J, dadt
-    }
+    t_roof = eval_roof_phases(tm, N, args.P, args.precision)
     t_roof_ms = {k: round(v * 1e3, 4) for k, v in t_roof.items()}
     eval_roofline = {"roof_ms": t_roof_ms, "sum_roof_ms": round(sum(t_roof.values()) * 1e3, 3),
                      "measured_ms": None, "frac": None,
