@@ -129,17 +129,20 @@ def adjoint_leg(n, theta, ncrit, steps):
 
 def leapfrog_scenario(args, X, A, S, tuned):
     """The two-ring time integration: `--scenario-steps` RK3 steps of dt = 0.005 on the device,
-    a Pedrizzetti relaxation (f = 0.3) every 5 steps from the J of an eval at the current state,
-    linear and angular impulse every step and the energy at every relaxation eval.  Prints one
-    JSON line (ms per RK3 step)."""
+    a Pedrizzetti relaxation (f = --relax-f, 0.3 by default; 0 keeps the eval and skips the
+    blend) every 5 steps from the J of an eval at the current state, linear and angular impulse
+    every step and the energy at every relaxation eval; --near-field c > 0 turns on the
+    near-field option for every build.  Prints one JSON line (ms per RK3 step)."""
     from uvlm_amd import FMMCalculator
 
     if tuned is not None:
         os.environ["UVLM_M2L_SLOTS"] = str(tuned["m2l_slots"])
     nc = tuned["ncrit"] if tuned is not None else args.ncrit
     g = FMMCalculator(P=args.P, theta=args.theta, ncrit=nc, precision=args.precision, scheme="rk3")
+    if args.near_field > 0:
+        g.set_near_field(args.near_field)
     g.set_arrays(X, A, S)
-    dt, every, f = 0.005, 5, 0.3
+    dt, every, f = 0.005, 5, args.relax_f
     d0 = g.diagnostics()
     I0, A0 = d0["impulse"], d0["angular_impulse"]
     step_ms, relax_ms, imp, ang, energy = [], [], [], [], []
@@ -158,7 +161,8 @@ def leapfrog_scenario(args, X, A, S, tuned):
             g.build()
             g.calculate()
             energy.append(g.diagnostics()["energy"])
-            g.relax(f)
+            if f > 0:
+                g.relax(f)
             g.sync()
             relax_ms.append((time.perf_counter() - t0) * 1e3)
         if k % 20 == 0:
@@ -173,8 +177,10 @@ def leapfrog_scenario(args, X, A, S, tuned):
             "dtype": "f32 P2P + f64 expansions" if args.precision == "mixed" else "f64",
             "data": "synthetic (seeded leapfrogging vortex rings)",
             "config": {"workload": f"leapfrogging vortex rings, {X.shape[0]} particles, P={args.P}, theta={args.theta}, "
-                                   f"ncrit={nc}, RK3 dt={dt}, relaxation f={f} every {every} steps",
-                       "particles": X.shape[0], "P": args.P, "theta": args.theta, "ncrit": nc},
+                                   f"ncrit={nc}, RK3 dt={dt}, relaxation f={f} every {every} steps, "
+                                   f"near-field c={args.near_field}",
+                       "particles": X.shape[0], "P": args.P, "theta": args.theta, "ncrit": nc,
+                       "relax_f": f, "near_field_c": args.near_field},
             "extra": {"step_ms": {"mean": round(float(np.mean(step_ms)), 3), "min": round(float(np.min(step_ms)), 3),
                                   "max": round(float(np.max(step_ms)), 3)},
                       "relax_eval_ms_mean": round(float(np.mean(relax_ms)), 3) if relax_ms else None,
@@ -494,6 +500,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip timing the FP64 path")
     ap.add_argument("--scenario-steps", type=int, default=0,
                     help="instead of the eval benchmark: this many RK3 steps of the two-ring time integration")
+    ap.add_argument("--relax-f", type=float, default=0.3, help="relaxation factor of --scenario-steps (0: none)")
+    ap.add_argument("--near-field", type=float, default=0.0,
+                    help="near-field factor c of --scenario-steps (0: the reference's lists)")
     ap.add_argument("--vlm-steps", type=int, default=0,
                     help="instead of the eval benchmark: this many steps of the wing + wake loop (40 x 10 panels)")
     ap.add_argument("--vlm-window", type=int, default=50, help="steps of the lift adjoint at the end of --vlm-steps")
