@@ -50,6 +50,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 // operations execute in order, so waiting for its own (lgkmcnt 0) is enough, and global loads
 // issued before this point stay in flight (__syncthreads would also wait for those).  The
 // signal fences keep the compiler from moving memory accesses across it.
+static_assert(WG == 64, "wave_lds_sync is for one-wavefront (64-lane) workgroups");
 __device__ __forceinline__ void wave_lds_sync()
 {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
