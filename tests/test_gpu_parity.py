@@ -146,12 +146,15 @@ def test_fmm_fp64_parity(orc, name):
     assert rel(L1, L0) < 1e-10
 
 
-@pytest.mark.parametrize("P,S", [(4, 0), (12, 0), (13, 0), (16, 0), (10, 5), (10, 6), (10, 12)])
-def test_fmm_fp64_parity_orders_and_plans(orc, P, S, monkeypatch):
+@pytest.mark.parametrize("P,S,Wv", [(4, 0, 2), (12, 0, 2), (13, 0, 2), (16, 0, 2), (10, 4, 2), (10, 5, 2), (10, 6, 2),
+                                    (10, 12, 2), (10, 6, 1), (12, 0, 1), (16, 0, 1)])
+def test_fmm_fp64_parity_orders_and_plans(orc, P, S, Wv, monkeypatch):
     """Expansion orders away from the default (P = 13 and 16 need a third coefficient per lane
-    in P2M/M2M/L2L; P = 4 runs with a single home slot) and the M2L plan sizes 5, 6 and 12."""
+    in P2M/M2M/L2L; P = 4 runs with a single home slot), the M2L plan sizes 4, 5, 6 and 12, and
+    one or two wavefronts per M2L workgroup (the second takes the odd source degrees)."""
     if S:
         monkeypatch.setenv("UVLM_M2L_SLOTS", str(S))
+    monkeypatch.setenv("UVLM_M2L_WAVES", str(Wv))
     X, A, S_ = W.scaled("cube", 6000)
     b_ref = orc.make_bodies(X, A, S_)
     c = orc.Calculator(P=P, theta=0.5, ncrit=32)
@@ -164,7 +167,7 @@ def test_fmm_fp64_parity_orders_and_plans(orc, P, S, monkeypatch):
     g.calculate()
     out = g.copyBodies2Host()
     for sl in (slice(15, 18), slice(0, 9), slice(18, 21)):
-        assert rel(out[:, sl], ref[:, sl]) < 1e-10, (P, S, sl)
+        assert rel(out[:, sl], ref[:, sl]) < 1e-10, (P, S, Wv, sl)
     M0, L0 = c.expansions()
     M1, L1 = g.expansions()
     assert rel(M1, M0) < 1e-10 and rel(L1, L0) < 1e-10

@@ -718,8 +718,10 @@ int uvlm_create(const uvlm_params *p, uvlm_handle *out)
     UVLM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     for (auto &e : h->ev) UVLM_HIP(hipEventCreate(&e));
     for (auto &e : h->aev) UVLM_HIP(hipEventCreate(&e));
-    // UVLM_M2L_SLOTS overrides the M2L plan's accumulator-slot budget (uvlm_autotune also sets it)
+    // UVLM_M2L_SLOTS overrides the M2L plan's accumulator-slot budget (uvlm_autotune also sets
+    // it); UVLM_M2L_WAVES = 1 runs the M2L with one wavefront per workgroup instead of two
     if (const char *sb = std::getenv("UVLM_M2L_SLOTS")) h->sched.budget = std::atoi(sb);
+    if (const char *sw = std::getenv("UVLM_M2L_WAVES")) h->sched.waves = std::atoi(sw) == 1 ? 1 : 2;
     make_m2l_schedule(p->P, h->sched);
   });
   if (rc != UVLM_OK) {
@@ -1131,14 +1133,16 @@ int uvlm_autotune(uvlm_handle h, const int64_t *ncrit, int n, int reps, int64_t 
     int           bb = keep_budget;
     hipEvent_t    e1 = h->ev[EV_DOWN];
     try {
-      for (int b : {5, 6, 8}) {
+      M2LPlan prev;
+      bool    have_prev = false;
+      for (int b : {4, 5, 6, 8}) {
         M2LPlan probe;
         if (!make_m2l_plan(h->prm.P, b, probe)) continue;
-        if (b != 5) {  // skip budgets whose plan equals the previous one
-          M2LPlan prev;
-          make_m2l_plan(h->prm.P, b == 6 ? 5 : 6, prev);
-          if (prev.S == probe.S && prev.slots == probe.slots && prev.lev == probe.lev) continue;
-        }
+        // skip budgets whose plan equals the previous one
+        const bool same = have_prev && prev.S == probe.S && prev.slots == probe.slots && prev.lev == probe.lev;
+        prev      = probe;
+        have_prev = true;
+        if (same) continue;
         h->sched.budget = b;
         make_m2l_schedule(h->prm.P, h->sched);
         for (int i = 0; i < n; i++) {

@@ -140,6 +140,7 @@ void count_interactions(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &
 // far.hip
 struct M2LSchedule {
   int     budget = 0;  // accumulator slots per lane (0: default)
+  int     waves  = 2;  // wavefronts per M2L workgroup sharing the staged sources (1 or 2)
   M2LPlan plan;
   DevBuf  dev;         // slots, lev, out_off, out_src
   int64_t dev_slots = 0, dev_lev = 0, dev_out_off = 0, dev_out_src = 0;

@@ -51,6 +51,15 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 // issued before this point stay in flight (__syncthreads would also wait for those).  The
 // signal fences keep the compiler from moving memory accesses across it.
 static_assert(WG == 64, "wave_lds_sync is for one-wavefront (64-lane) workgroups");
+// The same for a workgroup of several wavefronts: each waits for its own LDS operations, then
+// all meet at the barrier (global loads stay in flight).
+__device__ __forceinline__ void block_lds_sync()
+{
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 __device__ __forceinline__ void wave_lds_sync()
 {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -701,8 +710,8 @@ __global__ __launch_bounds__(WG) void k_m2l_table(int P, double2 *__restrict__ Y
   if (threadIdx.x < P) singular_column_ext(P, threadIdx.x, sph(kx, ky, kz), Yk);
 }
 
-// The n-synchronous M2L (plan in m2l_plan.hpp).  One workgroup (one wavefront) per target
-// cell; its 4 groups of 16 lanes take every 4th source.  All lanes of a group walk the source
+// The n-synchronous M2L (plan in m2l_plan.hpp).  One workgroup (W wavefronts) per target
+// cell; the 4 groups of 16 lanes of a wavefront take every 4th source.  All lanes of a group walk the source
 // degrees n and orders m in the same sequence, so M[n,m] is one LDS broadcast per group, and
 // each lane adds the terms of the outputs bound to its S accumulator slots.  Blocks are
 // mapped so that each of the 8 XCDs takes one contiguous range of targets: spatially close
@@ -740,8 +749,8 @@ __device__ __forceinline__ void m2l_terms(double2 (&a)[3], const double2 *__rest
 // SPLIT: the unit of work is a list segment (seg[3u] = target, [seg[3u+1], seg[3u+2]) = list
 // entries), ncell counts segments, and the segment's local expansion goes to L + u * NT;
 // k_m2l_reduce adds the segments of each target in order.
-template <int S, bool SPLIT = false>
-__global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int64_t per_xcd,
+template <int S, bool SPLIT = false, int W = 1>
+__global__ __launch_bounds__(W * WG) void k_m2l_ns(int P, int NT, int64_t ncell, int64_t per_xcd,
                                                const int32_t *__restrict__ off, const int32_t *__restrict__ src,
                                                const double *__restrict__ tx, const double *__restrict__ ty,
                                                const double *__restrict__ tz, const double *__restrict__ sx,
@@ -758,8 +767,18 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
   const int64_t u = (b & 7) * per_xcd + (b >> 3);
   if (u >= ncell) return;
   const int64_t t = SPLIT ? seg[3 * u] : u;  // target cell; the output row is u
-  const int lane = threadIdx.x, g = lane >> 4, sl = lane & 15;
+  // W wavefronts share the staged sources: wavefront wv takes the source degrees n = wv mod W
+  // of the same 16-lane plan, and their partial sums are added at the end in wavefront order
+  // (W = 2 halves the LDS per wavefront, so 4 slots run at 4 waves per SIMD); pl numbers the
+  // W * 16 lanes of a group for the staging
+  const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, sl = lane & 15;
+  const int wv = W == 1 ? 0 : __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform in the wavefront
+  const int pl = sl + 16 * wv;
   const int ny = tri(P), per = P * P + NT;
+  auto      sync = [] {
+    if (W == 1) wave_lds_sync();
+    else block_lds_sync();
+  };
   double2  *Yx = sm2 + g * per;  // extended singular rows, P*P entries
   double2  *Mg = Yx + P * P;
   // slot s of this lane: packed output (j, k), first order m0 and the degree it serves
@@ -776,13 +795,13 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
   const double X = tx[t], Yc = ty[t], Z = tz[t];
   const int    lt = tab.Y ? tab.level[t] : 0;
   if (tab.Y) {
-    if (lane < P) {
+    if (tid < P) {
       const double v = 1.0 / tab.X0R0[3];
       double       q = v;
-      for (int i = 0; i < lane; i++) q *= v;
-      sinv[lane] = q;
+      for (int i = 0; i < tid; i++) q *= v;
+      sinv[tid] = q;
     }
-    wave_lds_sync();
+    sync();
   }
   // the next batch's source centre (and level) and the index of the one after it are loaded a
   // batch ahead, so their latency overlaps the current batch's multiply-adds
@@ -800,12 +819,12 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
       const double2 *Ms = M + (int64_t)s_cur * NT;
       // four loads in flight per lane before the first LDS store; indices past the end are
       // clamped to the last entry, whose owner writes the same value (no branches)
-      for (int q0 = sl; q0 < NT; q0 += 64) {
+      for (int q0 = pl; q0 < NT; q0 += 64 * W) {
         double2 v[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) v[i] = Ms[min(q0 + 16 * i, NT - 1)];
+        for (int i = 0; i < 4; i++) v[i] = Ms[min(q0 + 16 * W * i, NT - 1)];
 #pragma unroll
-        for (int i = 0; i < 4; i++) Mg[min(q0 + 16 * i, NT - 1)] = v[i];
+        for (int i = 0; i < 4; i++) Mg[min(q0 + 16 * W * i, NT - 1)] = v[i];
       }
       // same tree: X_t - X_s = r k with r = R0 / 2^l the smaller cell's half-width and k an
       // integer vector, and Y_n(r k) = r^-(n+1) Y_n(k); offsets inside the table's cube are read
@@ -822,13 +841,13 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
           const int      w   = 2 * M2L_TABLE_K + 1;
           const int64_t  idx = (((int64_t)kx + M2L_TABLE_K) * w + ((int64_t)ky + M2L_TABLE_K)) * w + ((int64_t)kz + M2L_TABLE_K);
           const double2 *Yk  = tab.Y + idx * P * P;
-          for (int e0 = sl; e0 < P * P; e0 += 64) {
+          for (int e0 = pl; e0 < P * P; e0 += 64 * W) {
             double2 y[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) y[i] = Yk[min(e0 + 16 * i, P * P - 1)];
+            for (int i = 0; i < 4; i++) y[i] = Yk[min(e0 + 16 * W * i, P * P - 1)];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-              const int    e = min(e0 + 16 * i, P * P - 1);
+              const int    e = min(e0 + 16 * W * i, P * P - 1);
               const int    J = (int)sqrtf((float)e + 0.5f);  // row of entry e: J*J <= e <= J*J + 2J
               const double f = ldexp(sinv[J], l * (J + 1));
               Yx[e]          = make_double2(y[i].x * f, y[i].y * f);
@@ -837,9 +856,9 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
           from_table = true;
         }
       }
-      if (!from_table && sl < P) {
+      if (!from_table && pl < P) {
         const Sph sp = sph(X - cx_cur, Yc - cy_cur, Z - cz_cur);
-        singular_column_ext(P, sl, sp, Yx);
+        singular_column_ext(P, pl, sp, Yx);
       }
     }
     if (si + 4 < nsrc) {
@@ -848,9 +867,9 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
       if (tab.Y) ls_cur = tab.level[s_cur];
       if (si + 8 < nsrc) s_nxt = src[s0 + si + 8];
     }
-    wave_lds_sync();
+    sync();
     if (active) {
-      for (int n = 0; n < P; n++) {
+      for (int n = wv; n < P; n += W) {
         const int      lv = lev[n], cn = lv & 255, hr = (lv >> 8) & 255, used = lv >> 16;
         const double2 *Mn = Mg + 3 * tri(n);
 #pragma unroll
@@ -877,7 +896,7 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
         }
       }
     }
-    wave_lds_sync();
+    sync();
   }
   // combine the 4 source groups (same slot in lanes sl, sl+16, sl+32, sl+48)
   for (int s = 0; s < S; s++)
@@ -889,11 +908,28 @@ __global__ __launch_bounds__(WG) void k_m2l_ns(int P, int NT, int64_t ncell, int
       a.y += __shfl_xor(a.y, 32, 64);
     }
   double2 *part = sm2;  // [16][S][3]
-  if (g == 0)
+  if (W > 1) {  // wavefront 0 adds the others' partial sums, in wavefront order
+    sync();
+    for (int w = 1; w < W; w++) {
+      if (wv == w && g == 0)
+        for (int s = 0; s < S; s++)
+          for (int d = 0; d < 3; d++) part[(sl * S + s) * 3 + d] = acc[s][d];
+      sync();
+      if (wv == 0)
+        for (int s = 0; s < S; s++)
+          for (int d = 0; d < 3; d++) {
+            const double2 o = part[(sl * S + s) * 3 + d];
+            acc[s][d].x += o.x;
+            acc[s][d].y += o.y;
+          }
+      sync();
+    }
+  }
+  if (wv == 0 && g == 0)
     for (int s = 0; s < S; s++)
       for (int d = 0; d < 3; d++) part[(sl * S + s) * 3 + d] = acc[s][d];
   __syncthreads();
-  for (int q = lane; q < ny; q += WG) {
+  for (int q = tid; q < ny; q += W * WG) {
     int j = 0;
     while (tri(j + 1) <= q) j++;
     double2 r[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
@@ -1068,7 +1104,9 @@ __global__ void k_zero(double *p, int64_t n)
 // to the device, and the lattice and octant harmonic tables for P.
 void make_m2l_schedule(int P, M2LSchedule &s)
 {
-  if (!make_m2l_plan(P, s.budget, s.plan)) throw Error(UVLM_ERR_INTERNAL, "no M2L plan");
+  // default budget: with two wavefronts per workgroup 4 slots keep 4 waves per SIMD at P = 10
+  const int budget = s.budget > 0 ? s.budget : (s.waves == 2 ? 4 : 0);
+  if (!make_m2l_plan(P, budget, s.plan)) throw Error(UVLM_ERR_INTERNAL, "no M2L plan");
   const M2LPlan &pl = s.plan;
   s.dev_slots       = 0;
   s.dev_lev         = pl.slots.size();
@@ -1177,7 +1215,7 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
   if (tt.C == 0) return;
   const int32_t *d = sch.dev.as<int32_t>();
   {
-    const int    S      = sch.plan.S;
+    const int    S = sch.plan.S, W = sch.waves;
     const size_t lds_ns = std::max(sizeof(double2) * 4 * (P * P + NT), sizeof(double2) * 16 * S * 3);
     // few targets with long lists: one workgroup per list segment, partial expansions reduced
     const bool     split = sc && tt.C < M2L_SPLIT_TARGETS && l.nm2l >= 4 * tt.C;
@@ -1217,25 +1255,30 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
     M2LTable tab{};
     if (&tt == &st && sch.ytab_P == P)
       tab = M2LTable{sch.ytab.as<double2>(), tt.level.as<int32_t>(), tt.X0R0.as<double>()};
-#define UVLM_M2L_NS(SS)                                                                                            \
-  if (S == SS) {                                                                                                   \
-    if (split)                                                                                                     \
-      k_m2l_ns<SS, true><<<grid, WG, lds_ns, s>>>(                                                                 \
-          P, NT, units, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(), tt.cx.as<double>(),            \
-          tt.cy.as<double>(), tt.cz.as<double>(), st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M,   \
-          out, d + sch.dev_slots, d + sch.dev_lev, d + sch.dev_out_off, d + sch.dev_out_src, tab, dseg);          \
-    else                                                                                                           \
-      k_m2l_ns<SS><<<grid, WG, lds_ns, s>>>(P, NT, tt.C, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(),  \
-                                            tt.cx.as<double>(), tt.cy.as<double>(), tt.cz.as<double>(),             \
-                                            st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, L,       \
-                                            d + sch.dev_slots, d + sch.dev_lev, d + sch.dev_out_off,               \
-                                            d + sch.dev_out_src, tab);                                             \
-    UVLM_LAUNCH_CHECK();                                                                                           \
-    if (split) {                                                                                                   \
-      k_m2l_reduce<<<(int)tt.C, WG, 0, s>>>(NT, tt.C, dsegoff, out, L);                                            \
-      UVLM_LAUNCH_CHECK();                                                                                         \
-    }                                                                                                              \
-    return;                                                                                                        \
+#define UVLM_M2L_LAUNCH(SS, WW)                                                                              \
+  if (split)                                                                                               \
+    k_m2l_ns<SS, true, WW><<<grid, WW * WG, lds_ns, s>>>(                                                  \
+        P, NT, units, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(), tt.cx.as<double>(),       \
+        tt.cy.as<double>(), tt.cz.as<double>(), st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, \
+        out, d + sch.dev_slots, d + sch.dev_lev, d + sch.dev_out_off, d + sch.dev_out_src, tab, dseg);    \
+  else                                                                                                     \
+    k_m2l_ns<SS, false, WW><<<grid, WW * WG, lds_ns, s>>>(                                                 \
+        P, NT, tt.C, per_xcd, l.m2l_off.as<int32_t>(), l.m2l_src.as<int32_t>(), tt.cx.as<double>(),        \
+        tt.cy.as<double>(), tt.cz.as<double>(), st.cx.as<double>(), st.cy.as<double>(), st.cz.as<double>(), M, \
+        L, d + sch.dev_slots, d + sch.dev_lev, d + sch.dev_out_off, d + sch.dev_out_src, tab);
+#define UVLM_M2L_NS(SS)                                                                                    \
+  if (S == SS) {                                                                                           \
+    if (W == 2) {                                                                                          \
+      UVLM_M2L_LAUNCH(SS, 2)                                                                               \
+    } else {                                                                                               \
+      UVLM_M2L_LAUNCH(SS, 1)                                                                               \
+    }                                                                                                      \
+    UVLM_LAUNCH_CHECK();                                                                                   \
+    if (split) {                                                                                           \
+      k_m2l_reduce<<<(int)tt.C, WG, 0, s>>>(NT, tt.C, dsegoff, out, L);                                    \
+      UVLM_LAUNCH_CHECK();                                                                                 \
+    }                                                                                                      \
+    return;                                                                                                \
   }
     UVLM_M2L_NS(4)
     UVLM_M2L_NS(5)
@@ -1244,6 +1287,7 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
     UVLM_M2L_NS(12)
     UVLM_M2L_NS(16)
 #undef UVLM_M2L_NS
+#undef UVLM_M2L_LAUNCH
     throw Error(UVLM_ERR_INTERNAL, "unsupported M2L plan size");
   }
 }
