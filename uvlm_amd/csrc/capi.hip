@@ -719,9 +719,9 @@ int uvlm_create(const uvlm_params *p, uvlm_handle *out)
     for (auto &e : h->ev) UVLM_HIP(hipEventCreate(&e));
     for (auto &e : h->aev) UVLM_HIP(hipEventCreate(&e));
     // UVLM_M2L_SLOTS overrides the M2L plan's accumulator-slot budget (uvlm_autotune also sets
-    // it); UVLM_M2L_WAVES = 1 runs the M2L with one wavefront per workgroup instead of two
+    // it); UVLM_M2L_WAVES (1 or 2) the wavefronts per M2L workgroup, chosen by P otherwise
     if (const char *sb = std::getenv("UVLM_M2L_SLOTS")) h->sched.budget = std::atoi(sb);
-    if (const char *sw = std::getenv("UVLM_M2L_WAVES")) h->sched.waves = std::atoi(sw) == 1 ? 1 : 2;
+    if (const char *sw = std::getenv("UVLM_M2L_WAVES")) h->sched.waves = std::atoi(sw);
     make_m2l_schedule(p->P, h->sched);
   });
   if (rc != UVLM_OK) {

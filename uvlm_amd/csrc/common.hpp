@@ -140,7 +140,8 @@ void count_interactions(hipStream_t s, Scratch &sc, const Tree &tt, const Tree &
 // far.hip
 struct M2LSchedule {
   int     budget = 0;  // accumulator slots per lane (0: default)
-  int     waves  = 2;  // wavefronts per M2L workgroup sharing the staged sources (1 or 2)
+  int     waves  = 0;  // wavefronts per M2L workgroup (1 or 2; 0: 1 for P <= 8, else 2)
+  int     W      = 1;  // the wavefronts per workgroup of the current schedule
   M2LPlan plan;
   DevBuf  dev;         // slots, lev, out_off, out_src
   int64_t dev_slots = 0, dev_lev = 0, dev_out_off = 0, dev_out_src = 0;

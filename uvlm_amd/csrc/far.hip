@@ -1104,8 +1104,12 @@ __global__ void k_zero(double *p, int64_t n)
 // to the device, and the lattice and octant harmonic tables for P.
 void make_m2l_schedule(int P, M2LSchedule &s)
 {
-  // default budget: with two wavefronts per workgroup 4 slots keep 4 waves per SIMD at P = 10
-  const int budget = s.budget > 0 ? s.budget : (s.waves == 2 ? 4 : 0);
+  // Two wavefronts per workgroup when one wavefront's LDS (16 (P^2 + NT) bytes per staged
+  // source, 4 sources) would cap the occupancy below what its registers allow: from P = 9 on
+  // (measured: P = 8 is 12% faster with one, P = 10 10% faster with two, P = 12 and 14 a third).
+  // Default budget: with two, 4 slots keep 4 waves per SIMD at P = 10.
+  s.W              = s.waves == 1 || s.waves == 2 ? s.waves : (P <= 8 ? 1 : 2);
+  const int budget = s.budget > 0 ? s.budget : (s.W == 2 ? 4 : 0);
   if (!make_m2l_plan(P, budget, s.plan)) throw Error(UVLM_ERR_INTERNAL, "no M2L plan");
   const M2LPlan &pl = s.plan;
   s.dev_slots       = 0;
@@ -1215,7 +1219,7 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
   if (tt.C == 0) return;
   const int32_t *d = sch.dev.as<int32_t>();
   {
-    const int    S = sch.plan.S, W = sch.waves;
+    const int    S = sch.plan.S, W = sch.W;
     const size_t lds_ns = std::max(sizeof(double2) * 4 * (P * P + NT), sizeof(double2) * 16 * S * 3);
     // few targets with long lists: one workgroup per list segment, partial expansions reduced
     const bool     split = sc && tt.C < M2L_SPLIT_TARGETS && l.nm2l >= 4 * tt.C;
