@@ -749,8 +749,10 @@ __device__ __forceinline__ void m2l_terms(double2 (&a)[3], const double2 *__rest
 // SPLIT: the unit of work is a list segment (seg[3u] = target, [seg[3u+1], seg[3u+2]) = list
 // entries), ncell counts segments, and the segment's local expansion goes to L + u * NT;
 // k_m2l_reduce adds the segments of each target in order.
+// With two wavefronts, up to 5 slots are held to 128 VGPRs (4 waves per SIMD, a few bytes of
+// spill at 5); more slots keep their registers.
 template <int S, bool SPLIT = false, int W = 1>
-__global__ __launch_bounds__(W * WG) void k_m2l_ns(int P, int NT, int64_t ncell, int64_t per_xcd,
+__global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 && S <= 5 ? 4 : 1))) void k_m2l_ns(int P, int NT, int64_t ncell, int64_t per_xcd,
                                                const int32_t *__restrict__ off, const int32_t *__restrict__ src,
                                                const double *__restrict__ tx, const double *__restrict__ ty,
                                                const double *__restrict__ tz, const double *__restrict__ sx,
@@ -1107,9 +1109,9 @@ void make_m2l_schedule(int P, M2LSchedule &s)
   // Two wavefronts per workgroup when one wavefront's LDS (16 (P^2 + NT) bytes per staged
   // source, 4 sources) would cap the occupancy below what its registers allow: from P = 9 on
   // (measured: P = 8 is 12% faster with one, P = 10 10% faster with two, P = 12 and 14 a third).
-  // Default budget: with two, 4 slots keep 4 waves per SIMD at P = 10.
+  // Default budget: with two, 5 slots run at 4 waves per SIMD at P = 10.
   s.W              = s.waves == 1 || s.waves == 2 ? s.waves : (P <= 8 ? 1 : 2);
-  const int budget = s.budget > 0 ? s.budget : (s.W == 2 ? 4 : 0);
+  const int budget = s.budget > 0 ? s.budget : (s.W == 2 ? 5 : 0);
   if (!make_m2l_plan(P, budget, s.plan)) throw Error(UVLM_ERR_INTERNAL, "no M2L plan");
   const M2LPlan &pl = s.plan;
   s.dev_slots       = 0;
