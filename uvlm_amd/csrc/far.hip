@@ -752,7 +752,8 @@ __device__ __forceinline__ void m2l_terms(double2 (&a)[3], const double2 *__rest
 // With two wavefronts, up to 5 slots are held to 128 VGPRs (4 waves per SIMD, a few bytes of
 // spill at 5); more slots keep their registers.
 template <int S, bool SPLIT = false, int W = 1>
-__global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 && S <= 5 ? 4 : 1))) void k_m2l_ns(int P, int NT, int64_t ncell, int64_t per_xcd,
+__global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 && S <= 5 ? 4 : 1))) void k_m2l_ns(
+                                               int P, int NT, int64_t ncell, int64_t per_xcd,
                                                const int32_t *__restrict__ off, const int32_t *__restrict__ src,
                                                const double *__restrict__ tx, const double *__restrict__ ty,
                                                const double *__restrict__ tz, const double *__restrict__ sx,
@@ -771,8 +772,8 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
   const int64_t t = SPLIT ? seg[3 * u] : u;  // target cell; the output row is u
   // W wavefronts share the staged sources: wavefront wv takes the source degrees n = wv mod W
   // of the same 16-lane plan, and their partial sums are added at the end in wavefront order
-  // (W = 2 halves the LDS per wavefront, so 4 slots run at 4 waves per SIMD); pl numbers the
-  // W * 16 lanes of a group for the staging
+  // (W = 2 halves the LDS per wavefront, so up to 5 slots run at 4 waves per SIMD); pl numbers
+  // the W * 16 lanes of a group for the staging
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, sl = lane & 15;
   const int wv = W == 1 ? 0 : __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform in the wavefront
   const int pl = sl + 16 * wv;
