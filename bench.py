@@ -529,6 +529,8 @@ def main():
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--ncrit", type=int, default=0,
                     help="leaf size; 0 (default): chosen by uvlm_autotune among --tune-ncrit on this workload")
+    ap.add_argument("--m2l-slots", type=int, default=0,
+                    help="with --ncrit: the M2L plan's accumulator slots per lane (0: the library's default)")
     ap.add_argument("--tune-ncrit", default="64,96,128,160,192,224,256", help="autotuner candidates")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--cpu-n", type=int, default=-1,
@@ -605,6 +607,8 @@ def main():
         """cfg: autotuner result (ncrit + M2L plan size) to run with; else ncrit or --ncrit."""
         if cfg is not None:
             os.environ["UVLM_M2L_SLOTS"] = str(cfg["m2l_slots"])
+        elif args.m2l_slots > 0:
+            os.environ["UVLM_M2L_SLOTS"] = str(args.m2l_slots)
         else:
             os.environ.pop("UVLM_M2L_SLOTS", None)
         nc = cfg["ncrit"] if cfg is not None else (ncrit or args.ncrit)
