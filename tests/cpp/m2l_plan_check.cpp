@@ -2,7 +2,9 @@
 // lanes per group, P = 1..16 and every slot budget: every (output j,k; degree n; order m) term
 // of the truncated M2L sum is owned by exactly one (lane, slot), every harmonic and multipole
 // index the kernel forms stays inside its row, and the per-output piece lists name each active
-// (lane, slot) once.  Exit status 0 when all plans pass.
+// (lane, slot) once.  Two rules the kernel's uniform trip counts rest on are checked too: a
+// home slot s of lane l is active at degree n exactly when G s + l < tri(P - n), and every
+// helper chunk holds exactly cn orders.  Exit status 0 when all plans pass.
 #include "../../uvlm_amd/csrc/m2l_plan.hpp"
 
 #include <cstdio>
@@ -30,8 +32,15 @@ int main()
             if (v < 0) continue;
             const int j = v & 31, k = (v >> 5) & 31, m0 = (v >> 10) & 31, lvl = (v >> 15) & 31;
             bool act; int a, b;
-            if (lvl == M2LPlan::HOME) { act = s < hr && j + n <= P - 1; a = 0; b = n + 1; }
-            else { act = lvl == n; a = m0; b = std::min(m0 + cn, n + 1); }
+            if (lvl == M2LPlan::HOME) {
+              act = s < hr && j + n <= P - 1;
+              a = 0; b = n + 1;
+              if (act != (s < hr && G * s + l < plan_tri(P - n))) { printf("P=%d home activity rule\n", P); bad++; }
+            } else {
+              act = lvl == n;
+              a = m0; b = std::min(m0 + cn, n + 1);
+              if (act && m0 + cn > n + 1) { printf("P=%d budget=%d: unequal chunk at n=%d\n", P, budget, n); bad++; }
+            }
             if (!act) continue;
             const int J = j + n;
             if (J > P - 1 || k > j) { printf("P=%d bad J/k\n", P); bad++; }

@@ -716,34 +716,52 @@ __global__ __launch_bounds__(WG) void k_m2l_table(int P, double2 *__restrict__ Y
 // each lane adds the terms of the outputs bound to its S accumulator slots.  Blocks are
 // mapped so that each of the 8 XCDs takes one contiguous range of targets: spatially close
 // targets share most of their sources, and those multipoles then hit the XCD's own L2.
-__device__ __forceinline__ void m2l_terms(double2 (&a)[3], const double2 *__restrict__ Yrow,
-                                          const double2 *__restrict__ Mn, int k, double sk, int m0, int m1)
+
+// Order m >= 1 of output (j,k) at degree n: terms m and -m taken together (multipoleKernels.hpp:534-546),
+//   conj(M[n,m]) Y[J,-m-k] + s M[n,m] Y[J,m-k] = (-1)^k (A S' + i B D'),  M = A + iB,
+//   S' = Yx[J,m-k] + (-1)^k conj(Y[J,m+k]),  D' = Yx[J,m-k] - (-1)^k conj(Y[J,m+k]),
+// with ya = Yx[J,m-k], yb = Yx[J,m+k], Mm = M[n,m] (3 components) and sk = (-1)^k.
+__device__ __forceinline__ void m2l_step(double2 (&a)[3], double2 ya, double2 yb, const double2 *__restrict__ Mm,
+                                         double sk)
 {
-  // Sums (-1)^k times the terms of output (j,k) at degree n (multipoleKernels.hpp:534-546),
-  // with Yrow = Yx[J, 0], J = j + n.  Terms m and -m are taken together:
-  //   conj(M[n,m]) Y[J,-m-k] + s M[n,m] Y[J,m-k] = (-1)^k (A S' + i B D'),  M = A + iB,
-  //   S' = Yx[J,m-k] + (-1)^k conj(Y[J,m+k]),  D' = Yx[J,m-k] - (-1)^k conj(Y[J,m+k]).
-  const double2 *Ya = Yrow - k, *Yb = Yrow + k;
-  int            m  = m0;
-  if (m == 0) {  // M[n,0] Y[J,-k] = (-1)^k M[n,0] Yx[J,-k] (Y_J^0 is real)
-    const double2 y = Ya[0];
-    for (int d = 0; d < 3; d++) cfma(a[d], Mn[d], y);
-    m = 1;
+  const double2 Sv = make_double2(fma(sk, yb.x, ya.x), fma(-sk, yb.y, ya.y));
+  const double2 Dv = make_double2(fma(-sk, yb.x, ya.x), fma(sk, yb.y, ya.y));
+  for (int d = 0; d < 3; d++) {
+    const double2 mv = Mm[d];
+    a[d].x           = fma(mv.x, Sv.x, a[d].x);
+    a[d].x           = fma(-mv.y, Dv.y, a[d].x);
+    a[d].y           = fma(mv.x, Sv.y, a[d].y);
+    a[d].y           = fma(mv.y, Dv.x, a[d].y);
   }
-  // unrolled by two, so the LDS reads of step m+1 are in flight during the multiply-adds of m
+}
+
+// Sums (-1)^k times the terms of output (j,k) at degree n over the orders m0 .. m0 + T, with
+// Ya = Yx[J,m0-k], Yb = Yx[J,m0+k], J = j + n, and Mm = M[n,m0]; T is the same in every lane of
+// the wavefront, so the loop needs no per-lane control.  The first order is m0 = 0 in a home
+// slot: M[n,0] Y[J,-k] = (-1)^k M[n,0] Yx[J,-k] (Y_J^0 is real).  In a helper chunk it is a
+// step with sign sk0: sk for m0 > 0, and 0 for m0 = 0, which makes S' = D' = Yx[J,-k] and the
+// step's four multiply-adds those of cfma(M[n,0], Yx[J,-k]), in the same order.
+__device__ __forceinline__ void m2l_terms_run(double2 (&a)[3], const double2 *__restrict__ Ya,
+                                              const double2 *__restrict__ Yb, const double2 *__restrict__ Mm,
+                                              double sk, bool home, double sk0, int T)
+{
+  // the first step's reads are issued ahead of the branch on the slot kind
+  const double2 ya = Ya[0], m0v[3] = {Mm[0], Mm[1], Mm[2]};
+  double2       Sv = ya, Dv = ya;  // home: cfma(M[n,0], ya) is the step with S' = D' = ya
+  if (!home) {
+    const double2 yb = Yb[0];
+    Sv               = make_double2(fma(sk0, yb.x, ya.x), fma(-sk0, yb.y, ya.y));
+    Dv               = make_double2(fma(-sk0, yb.x, ya.x), fma(sk0, yb.y, ya.y));
+  }
+  for (int d = 0; d < 3; d++) {
+    a[d].x = fma(m0v[d].x, Sv.x, a[d].x);
+    a[d].x = fma(-m0v[d].y, Dv.y, a[d].x);
+    a[d].y = fma(m0v[d].x, Sv.y, a[d].y);
+    a[d].y = fma(m0v[d].y, Dv.x, a[d].y);
+  }
+  // unrolled by two, so the LDS reads of step i+1 are in flight during the multiply-adds of i
 #pragma unroll 2
-  for (; m < m1; m++) {
-    const double2 ya = Ya[m], yb = Yb[m];
-    const double2 Sv = make_double2(fma(sk, yb.x, ya.x), fma(-sk, yb.y, ya.y));
-    const double2 Dv = make_double2(fma(-sk, yb.x, ya.x), fma(sk, yb.y, ya.y));
-    for (int d = 0; d < 3; d++) {
-      const double2 mv = Mn[3 * m + d];
-      a[d].x           = fma(mv.x, Sv.x, a[d].x);
-      a[d].x           = fma(-mv.y, Dv.y, a[d].x);
-      a[d].y           = fma(mv.x, Sv.y, a[d].y);
-      a[d].y           = fma(mv.y, Dv.x, a[d].y);
-    }
-  }
+  for (int i = 1; i <= T; i++) m2l_step(a, Ya[i], Yb[i], Mm + 3 * i, sk);
 }
 
 // SPLIT: the unit of work is a list segment (seg[3u] = target, [seg[3u+1], seg[3u+2]) = list
@@ -784,12 +802,33 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
   };
   double2  *Yx = sm2 + g * per;  // extended singular rows, P*P entries
   double2  *Mg = Yx + P * P;
-  // slot s of this lane: packed output (j, k), first order m0 and the degree it serves
-  // (31: home slot); an unused slot gets j = 31, which no degree activates
-  int32_t sv[S];
-  for (int s = 0; s < S; s++) {
-    const int32_t v = slots[sl * S + s];
-    sv[s]           = v >= 0 ? v : (31 | (31 << 15));
+  // Slot s of lane sl, decoded once per workgroup into a table of two int4 after the 4 groups'
+  // areas (byte offsets; hi(x) = high word of the double x).  At degree n the slot adds the
+  // orders m0 .. m0 + c - 1 of output (j, k), J = j + n, starting at
+  //   Ya = Yx[J, m0 - k] = Yx + x + y n + 16 n^2,  Yb = Yx[J, m0 + k] = Ya + z,  M[n, m0] = Mn + 48 m0,
+  // in lanes with a < A (A: outputs active at the degree for a home slot, 1 for a helper):
+  //  * home slot (s < H), output w = 16 s + sl = tri(j) + k, m0 = 0, c = n + 1:
+  //    {16 (j(j+1) - k), 16 (2j + 1), 32 k, hi((-1)^k)}, {w, 0, 0, 0};
+  //  * helper slot of degree d (a chunk of c = cn orders):
+  //    {16 (J(J+1) - k + m0 - d^2), 0, 32 k, hi((-1)^k)}, {0, 48 m0, hi(first sign), 0}, where the
+  //    first step's sign is 0 for m0 = 0 (the step is then the order-0 term, see m2l_terms_run);
+  //  * unused: a = INT_MAX.
+  int4 *htab = reinterpret_cast<int4 *>(sm2 + 4 * per);
+  for (int q = tid; q < 16 * S; q += W * WG) {
+    const int s = q >> 4, l = q & 15, v = slots[l * S + s];
+    const int j = v & 31, k = (v >> 5) & 31, m0 = (v >> 10) & 31, d = (v >> 15) & 31;
+    const int hk = (k & 1) ? (int)0xBFF00000 : 0x3FF00000;
+    int4      e0 = make_int4(0, 0, 0, 0x3FF00000), e1 = make_int4(0x7fffffff, 0, 0, 0);
+    if (v >= 0 && d == 31) {
+      e0 = make_int4(16 * (j * (j + 1) - k), 16 * (2 * j + 1), 32 * k, hk);
+      e1 = make_int4(q, 0, 0, 0);
+    } else if (v >= 0) {
+      const int J = j + d;
+      e0 = make_int4(16 * (J * (J + 1) - k + m0 - d * d), 0, 32 * k, hk);
+      e1 = make_int4(0, 48 * m0, m0 ? hk : 0, 0);
+    }
+    htab[2 * q]     = e0;
+    htab[2 * q + 1] = e1;
   }
   double2 acc[S][3];
   for (int s = 0; s < S; s++)
@@ -875,27 +914,34 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
       for (int n = wv; n < P; n += W) {
         const int      lv = lev[n], cn = lv & 255, hr = (lv >> 8) & 255, used = lv >> 16;
         const double2 *Mn = Mg + 3 * tri(n);
+        const int      A = tri(P - n), nn = 16 * n * n;  // outputs w < A are active at degree n
+        // the table is read per degree: the empty asm on a (zero) uniform offset keeps the
+        // compiler from hoisting the S invariant reads out of the degree loop (8 S registers);
+        // the slots' entries are then immediate offsets from one lane address
+        int tq = 0;
+        asm volatile("" : "+s"(tq));
+        const int4 *hl = htab + tq + 2 * sl;
 #pragma unroll
         for (int s = 0; s < S; s++) {
           if (!((used >> s) & 1)) continue;  // no lane has work in slot s at this degree
-          // decoded here, per degree: the empty asm keeps the compiler from hoisting S copies
-          // of the decoded fields out of the loops, which would cost registers (occupancy)
-          int v = sv[s];
-          asm volatile("" : "+v"(v));
-          const int sj = v & 31, sk = (v >> 5) & 31, lvl = (v >> 15) & 31;
-          bool      act;
-          int       m0, m1;
-          if (lvl == 31) {  // home slot: every m of an active, unsplit output
-            act = s < hr && sj + n <= P - 1;
-            m0  = 0;
-            m1  = n + 1;
-          } else {  // helper slot of degree lvl: one chunk of the m-range
-            act = lvl == n;
-            m0  = (v >> 10) & 31;
-            m1  = min(m0 + cn, n + 1);
+          // home slot (s < hr): orders 0..n of the active outputs (a prefix of the lanes);
+          // otherwise a helper slot of this degree: one chunk of cn orders
+          // (a home slot's activity and first order need no table read: w = 16 s + sl < A, m0 = 0)
+          const bool home = s < hr;
+          int        ea = 16 * s + sl, em = 0, es = 0;
+          if (!home) {
+            const int4 e1 = hl[32 * s + 1];
+            ea = e1.x, em = e1.y, es = e1.z;
           }
-          const int J = sj + n;
-          if (act) m2l_terms(acc[s], Yx + J * J + J, Mn, sk, (sk & 1) ? -1.0 : 1.0, m0, m1);
+          if (ea < (home ? A : 1)) {
+            const int4     e0 = hl[32 * s];
+            const double2 *Ya = reinterpret_cast<const double2 *>(
+                reinterpret_cast<const char *>(Yx) + e0.x + __umul24(e0.y, n) + nn);
+            const double2 *Yb = reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(Ya) + e0.z);
+            const double2 *Mm = reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(Mn) + em);
+            m2l_terms_run(acc[s], Ya, Yb, Mm, __hiloint2double(e0.w, 0), home, __hiloint2double(es, 0),
+                          home ? n : cn - 1);
+          }
         }
       }
     }
@@ -1223,7 +1269,8 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
   const int32_t *d = sch.dev.as<int32_t>();
   {
     const int    S = sch.plan.S, W = sch.W;
-    const size_t lds_ns = std::max(sizeof(double2) * 4 * (P * P + NT), sizeof(double2) * 16 * S * 3);
+    const size_t lds_ns = std::max(sizeof(double2) * 4 * (P * P + NT) + 2 * sizeof(int4) * 16 * S,
+                                   sizeof(double2) * 16 * S * 3);
     // few targets with long lists: one workgroup per list segment, partial expansions reduced
     const bool     split = sc && tt.C < M2L_SPLIT_TARGETS && l.nm2l >= 4 * tt.C;
     int64_t        units = tt.C;

@@ -68,10 +68,12 @@ inline bool make_m2l_plan(int P, int budget, M2LPlan &pl, int G = 16)
   for (int n = 0; n < P; n++) {
     const int A = plan_tri(P - n), rem = A % L;
     if (!rem) continue;
-    const int c0 = std::min(L / rem, n + 1);
-    const int cn = (n + 1 + c0 - 1) / c0;
-    const int c  = (n + 1 + cn - 1) / cn;
-    if (cn < n + 1) opts.push_back({n + 1 - cn, n, cn, c});
+    // c chunks of equal length cn = (n+1)/c per output (every lane of a split degree then runs
+    // the same trip count), as many as the idle lanes allow
+    int c = std::min(L / rem, n + 1);
+    while (c > 1 && (n + 1) % c) c--;
+    const int cn = (n + 1) / c;
+    if (c > 1) opts.push_back({n + 1 - cn, n, cn, c});
   }
   std::stable_sort(opts.begin(), opts.end(), [](const Split &a, const Split &b) { return a.saving > b.saving; });
   std::vector<Split> chosen;
