@@ -836,12 +836,22 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
   const int    s0 = SPLIT ? seg[3 * u + 1] : off[t], nsrc = (SPLIT ? seg[3 * u + 2] : off[t + 1]) - s0;
   const double X = tx[t], Yc = ty[t], Z = tz[t];
   const int    lt = tab.Y ? tab.level[t] : 0;
+  // lattice table only: R0^-(J+1) in sinv, and the row J of each harmonic entry this lane copies,
+  // e = min(pl + 16 W i, P*P - 1) (J*J <= e <= J*J + 2J), 4 bits per entry i
+  using JRows = typename std::conditional<W == 1, uint64_t, uint32_t>::type;
+  JRows jrow = 0;
   if (tab.Y) {
+    const double inv0 = 1.0 / tab.X0R0[3];
     if (tid < P) {
-      const double v = 1.0 / tab.X0R0[3];
-      double       q = v;
-      for (int i = 0; i < tid; i++) q *= v;
+      double q = inv0;
+      for (int i = 0; i < tid; i++) q *= inv0;
       sinv[tid] = q;
+    }
+    for (int i = 0; i < 2 * (int)sizeof(JRows); i++) {
+      const int e = min(pl + 16 * W * i, P * P - 1);
+      int       J = 0;
+      while ((J + 1) * (J + 1) <= e) J++;
+      jrow |= (JRows)J << (4 * i);
     }
     sync();
   }
@@ -858,15 +868,28 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
     const int  si     = base + g;
     const bool active = si < nsrc;
     if (active) {
-      const double2 *Ms = M + (int64_t)s_cur * NT;
-      // four loads in flight per lane before the first LDS store; indices past the end are
+      // four loads in flight per lane before the first LDS store: at immediate offsets from the
+      // lane's first entry while every lane's four entries exist, then with indices past the end
       // clamped to the last entry, whose owner writes the same value (no branches)
-      for (int q0 = pl; q0 < NT; q0 += 64 * W) {
+      const double2 *Ms = M + (int64_t)s_cur * NT;
+      // (the scheduling barriers keep the four loads ahead of the stores)
+      int            q0 = 0;
+#pragma unroll 1
+      for (; q0 + 64 * W <= NT; q0 += 64 * W) {
         double2 v[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) v[i] = Ms[min(q0 + 16 * W * i, NT - 1)];
+        for (int i = 0; i < 4; i++) v[i] = Ms[pl + q0 + 16 * W * i];
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int i = 0; i < 4; i++) Mg[min(q0 + 16 * W * i, NT - 1)] = v[i];
+        for (int i = 0; i < 4; i++) Mg[pl + q0 + 16 * W * i] = v[i];
+      }
+      if (q0 < NT) {
+        double2 v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] = Ms[min(pl + q0 + 16 * W * i, NT - 1)];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) Mg[min(pl + q0 + 16 * W * i, NT - 1)] = v[i];
       }
       // same tree: X_t - X_s = r k with r = R0 / 2^l the smaller cell's half-width and k an
       // integer vector, and Y_n(r k) = r^-(n+1) Y_n(k); offsets inside the table's cube are read
@@ -874,7 +897,7 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
       bool from_table = false;
       if (tab.Y) {
         const int    l  = max(lt, ls_cur);
-        const double sc = ldexp(1.0 / tab.X0R0[3], l);  // 1 / r
+        const double sc = ldexp(sinv[0], l);  // 1 / r
         const double qx = (X - cx_cur) * sc, qy = (Yc - cy_cur) * sc, qz = (Z - cz_cur) * sc;
         const double kx = rint(qx), ky = rint(qy), kz = rint(qz);
         const double dev = fmax(fabs(qx - kx), fmax(fabs(qy - ky), fabs(qz - kz)));
@@ -883,14 +906,15 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
           const int      w   = 2 * M2L_TABLE_K + 1;
           const int64_t  idx = (((int64_t)kx + M2L_TABLE_K) * w + ((int64_t)ky + M2L_TABLE_K)) * w + ((int64_t)kz + M2L_TABLE_K);
           const double2 *Yk  = tab.Y + idx * P * P;
-          for (int e0 = pl; e0 < P * P; e0 += 64 * W) {
+          // indices past the end are clamped to the last entry (its row is in jrow too)
+          for (int e0 = 0, b0 = 0; e0 < P * P; e0 += 64 * W, b0 += 16) {
             double2 y[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) y[i] = Yk[min(e0 + 16 * W * i, P * P - 1)];
+            for (int i = 0; i < 4; i++) y[i] = Yk[min(pl + e0 + 16 * W * i, P * P - 1)];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-              const int    e = min(e0 + 16 * W * i, P * P - 1);
-              const int    J = (int)sqrtf((float)e + 0.5f);  // row of entry e: J*J <= e <= J*J + 2J
+              const int    e = min(pl + e0 + 16 * W * i, P * P - 1);
+              const int    J = (int)(jrow >> (b0 + 4 * i)) & 15;
               const double f = ldexp(sinv[J], l * (J + 1));
               Yx[e]          = make_double2(y[i].x * f, y[i].y * f);
             }
