@@ -23,7 +23,7 @@ int main()
       long cost = 0;
       std::vector<int> owner_ok(G * S, 0);
       for (int n = 0; n < P; n++) {
-        const int lv = pl.lev[n], cn = lv & 255, hr = (lv >> 8) & 255, used = lv >> 16;
+        const int lv = pl.lev[n], cn = lv & 255, hr = (lv >> 8) & 127, used = lv >> 16;
         for (int s = 0; s < S; s++) {
           if (!((used >> s) & 1)) continue;
           int mx = 0;

@@ -788,8 +788,9 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
   const int64_t u = (b & 7) * per_xcd + (b >> 3);
   if (u >= ncell) return;
   const int64_t t = SPLIT ? seg[3 * u] : u;  // target cell; the output row is u
-  // W wavefronts share the staged sources: wavefront wv takes the source degrees n = wv mod W
-  // of the same 16-lane plan, and their partial sums are added at the end in wavefront order
+  // W wavefronts share the staged sources: with W = 2 each takes the source degrees the plan
+  // gives it (balanced by lock-step cost) of the same 16-lane plan, and their partial sums are
+  // added at the end in wavefront order
   // (W = 2 halves the LDS per wavefront, so up to 5 slots run at 4 waves per SIMD); pl numbers
   // the W * 16 lanes of a group for the staging
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, sl = lane & 15;
@@ -833,6 +834,9 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
   double2 acc[S][3];
   for (int s = 0; s < S; s++)
     for (int d = 0; d < 3; d++) acc[s][d] = make_double2(0, 0);
+  int wdeg = 0;  // the source degrees of this wavefront (bit n)
+  for (int n = 0; n < P; n++)
+    if (W == 1 || ((lev[n] >> 15) & 1) == wv) wdeg |= 1 << n;
   const int    s0 = SPLIT ? seg[3 * u + 1] : off[t], nsrc = (SPLIT ? seg[3 * u + 2] : off[t + 1]) - s0;
   const double X = tx[t], Yc = ty[t], Z = tz[t];
   const int    lt = tab.Y ? tab.level[t] : 0;
@@ -903,9 +907,9 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
         const double dev = fmax(fabs(qx - kx), fmax(fabs(qy - ky), fabs(qz - kz)));
         const double km  = fmax(fabs(kx), fmax(fabs(ky), fabs(kz)));
         if (dev < 1e-6 && km <= M2L_TABLE_K && km > 0.0) {
-          const int      w   = 2 * M2L_TABLE_K + 1;
-          const int64_t  idx = (((int64_t)kx + M2L_TABLE_K) * w + ((int64_t)ky + M2L_TABLE_K)) * w + ((int64_t)kz + M2L_TABLE_K);
-          const double2 *Yk  = tab.Y + idx * P * P;
+          const int      w   = 2 * M2L_TABLE_K + 1;  // |k| <= K: 32-bit index arithmetic is exact
+          const int      idx = ((int)kx + M2L_TABLE_K) * w * w + ((int)ky + M2L_TABLE_K) * w + ((int)kz + M2L_TABLE_K);
+          const double2 *Yk  = tab.Y + (int64_t)idx * (P * P);
           // indices past the end are clamped to the last entry (its row is in jrow too)
           for (int e0 = 0, b0 = 0; e0 < P * P; e0 += 64 * W, b0 += 16) {
             double2 y[4];
@@ -935,8 +939,9 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
     }
     sync();
     if (active) {
-      for (int n = wv; n < P; n += W) {
-        const int      lv = lev[n], cn = lv & 255, hr = (lv >> 8) & 255, used = lv >> 16;
+      for (int dm = wdeg; dm; dm &= dm - 1) {
+        const int      n = __builtin_ctz(dm), lv = lev[n];
+        const int      cn = lv & 255, hr = (lv >> 8) & 127, used = lv >> 16;
         const double2 *Mn = Mg + 3 * tri(n);
         const int      A = tri(P - n), nn = 16 * n * n;  // outputs w < A are active at degree n
         // the table is read per degree: the empty asm on a (zero) uniform offset keeps the

@@ -25,7 +25,7 @@ struct M2LPlan {
   static constexpr int HOME = 31;   // level field of a home slot
   int P = 0, S = 0, H = 0, G = 16;  // slots per lane (kernel template), home slots, group lanes
   std::vector<int32_t> slots;       // [G][S]: pack_slot(j, k, m0, level) or -1
-  std::vector<int32_t> lev;         // [P]: chunk length | home rounds << 8 | active slots << 16
+  std::vector<int32_t> lev;         // [P]: chunk length | home rounds << 8 | wavefront << 15 | active slots << 16
   std::vector<int32_t> out_off;     // [P(P+1)/2 + 1]
   std::vector<int32_t> out_src;     // pieces of each output: lane * S + slot
 };
@@ -114,13 +114,36 @@ inline bool make_m2l_plan(int P, int budget, M2LPlan &pl, int G = 16)
   }
   // slots that any lane uses at degree n (a uniform skip test in the kernel)
   for (int n = 0; n < P; n++) {
-    const int hr   = (pl.lev[n] >> 8) & 255;
+    const int hr   = (pl.lev[n] >> 8) & 127;
     int       mask = 0;
     for (int s = 0; s < H && s < hr; s++) mask |= 1 << s;
     for (size_t t = 0; t < chosen.size(); t++)
       if (chosen[t].n == n) mask |= 1 << (H + (int)t);
     pl.lev[n] |= mask << 16;
   }
+  // A workgroup of two wavefronts splits the degrees between them (bit 15: the second one) so
+  // that the larger of the two lock-step costs is least (per degree: n+1 steps per home slot,
+  // cn per helper, and about half a step of set-up per slot; all 2^P splits are tried, ties
+  // keep the degree-parity split's neighbourhood by preferring the smaller mask).
+  std::vector<double> dc(P, 0.0);
+  for (int n = 0; n < P; n++) {
+    const int lv = pl.lev[n], cn = lv & 255, hr = (lv >> 8) & 127, used = lv >> 16;
+    for (int s = 0; s < S; s++)
+      if ((used >> s) & 1) dc[n] += (s < hr ? n + 1 : cn) + 0.5;
+  }
+  double total = 0;
+  for (double c : dc) total += c;
+  int    best_mask = 0;
+  double best      = total;
+  for (int mask = 0; mask < (1 << P); mask++) {
+    double c1 = 0;
+    for (int n = 0; n < P; n++)
+      if ((mask >> n) & 1) c1 += dc[n];
+    const double worst = std::max(c1, total - c1);
+    if (worst < best - 1e-9) best = worst, best_mask = mask;
+  }
+  for (int n = 0; n < P; n++)
+    if ((best_mask >> n) & 1) pl.lev[n] |= 1 << 15;
   pl.out_off.assign(ny + 1, 0);
   pl.out_src.clear();
   for (int w = 0; w < ny; w++) {
