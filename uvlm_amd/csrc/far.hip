@@ -859,19 +859,15 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
     }
     sync();
   }
-  // the next batch's source centre (and level) and the index of the one after it are loaded a
-  // batch ahead, so their latency overlaps the current batch's multiply-adds
-  int    s_cur = g < nsrc ? src[s0 + g] : 0, s_nxt = g + 4 < nsrc ? src[s0 + g + 4] : 0;
-  double cx_cur = 0.0, cy_cur = 0.0, cz_cur = 0.0;
-  int    ls_cur = 0;
-  if (g < nsrc) {
-    cx_cur = sx[s_cur], cy_cur = sy[s_cur], cz_cur = sz[s_cur];
-    if (tab.Y) ls_cur = tab.level[s_cur];
-  }
+  // the next batch's source index is loaded a batch ahead, so its latency overlaps the current
+  // batch's multiply-adds; the source's centre and level are loaded with its multipole
+  int s_cur = g < nsrc ? src[s0 + g] : 0;
   for (int base = 0; base < nsrc; base += 4) {
     const int  si     = base + g;
     const bool active = si < nsrc;
     if (active) {
+      const double cx_cur = sx[s_cur], cy_cur = sy[s_cur], cz_cur = sz[s_cur];
+      const int    ls_cur = tab.Y ? tab.level[s_cur] : 0;
       // four loads in flight per lane before the first LDS store: at immediate offsets from the
       // lane's first entry while every lane's four entries exist, then with indices past the end
       // clamped to the last entry, whose owner writes the same value (no branches)
@@ -931,12 +927,7 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
         singular_column_ext(P, pl, sp, Yx);
       }
     }
-    if (si + 4 < nsrc) {
-      s_cur  = s_nxt;
-      cx_cur = sx[s_cur], cy_cur = sy[s_cur], cz_cur = sz[s_cur];
-      if (tab.Y) ls_cur = tab.level[s_cur];
-      if (si + 8 < nsrc) s_nxt = src[s0 + si + 8];
-    }
+    const int s_next = si + 4 < nsrc ? src[s0 + si + 4] : 0;
     sync();
     if (active) {
       for (int dm = wdeg; dm; dm &= dm - 1) {
@@ -975,6 +966,7 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
       }
     }
     sync();
+    s_cur = s_next;
   }
   // combine the 4 source groups (same slot in lanes sl, sl+16, sl+32, sl+48)
   for (int s = 0; s < S; s++)
