@@ -1137,7 +1137,7 @@ int uvlm_autotune(uvlm_handle h, const int64_t *ncrit, int n, int reps, int64_t 
       bool    have_prev = false;
       for (int b : {4, 5, 6, 8}) {
         M2LPlan probe;
-        if (!make_m2l_plan(h->prm.P, b, probe)) continue;
+        if (!make_m2l_plan(h->prm.P, b, probe, 16, m2l_waves(h->prm.P, h->sched.waves))) continue;
         // skip budgets whose plan equals the previous one
         const bool same = have_prev && prev.S == probe.S && prev.slots == probe.slots && prev.lev == probe.lev;
         prev      = probe;

@@ -158,6 +158,7 @@ struct M2LTable {
   const int32_t *level;
   const double  *X0R0;
 };
+int  m2l_waves(int P, int waves);  // wavefronts per M2L workgroup for order P (waves: 0 = by P)
 void make_m2l_schedule(int P, M2LSchedule &s);
 void zero_doubles(hipStream_t s, double *p, int64_t n);
 void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, double2 *M);

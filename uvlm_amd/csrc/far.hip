@@ -789,10 +789,9 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
   if (u >= ncell) return;
   const int64_t t = SPLIT ? seg[3 * u] : u;  // target cell; the output row is u
   // W wavefronts share the staged sources: with W = 2 each takes the source degrees the plan
-  // gives it (balanced by lock-step cost) of the same 16-lane plan, and their partial sums are
-  // added at the end in wavefront order
-  // (W = 2 halves the LDS per wavefront, so up to 5 slots run at 4 waves per SIMD); pl numbers
-  // the W * 16 lanes of a group for the staging
+  // gives it, with its own 16-lane plan for them, and the output loop adds both wavefronts'
+  // pieces of each output in a fixed order (W = 2 halves the LDS per wavefront, so up to 5
+  // slots run at 4 waves per SIMD); pl numbers the W * 16 lanes of a group for the staging
   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, sl = lane & 15;
   const int wv = W == 1 ? 0 : __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform in the wavefront
   const int pl = sl + 16 * wv;
@@ -803,40 +802,38 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
   };
   double2  *Yx = sm2 + g * per;  // extended singular rows, P*P entries
   double2  *Mg = Yx + P * P;
-  // Slot s of lane sl, decoded once per workgroup into a table of two int4 after the 4 groups'
-  // areas (byte offsets; hi(x) = high word of the double x).  At degree n the slot adds the
-  // orders m0 .. m0 + c - 1 of output (j, k), J = j + n, starting at
-  //   Ya = Yx[J, m0 - k] = Yx + x + y n + 16 n^2,  Yb = Yx[J, m0 + k] = Ya + z,  M[n, m0] = Mn + 48 m0,
-  // in lanes with a < A (A: outputs active at the degree for a home slot, 1 for a helper):
-  //  * home slot (s < H), output w = 16 s + sl = tri(j) + k, m0 = 0, c = n + 1:
-  //    {16 (j(j+1) - k), 16 (2j + 1), 32 k, hi((-1)^k)}, {w, 0, 0, 0};
+  // This wavefront's plan: slots [16][S] and per-degree words [P] (0: a degree of the other).
+  const int32_t *wslots = slots + wv * 16 * S, *wlev = lev + wv * P;
+  // Slot s of lane sl, decoded once per workgroup into an int4 of this wavefront's table after
+  // the 4 groups' areas (byte offsets; hi(x) = high word of the double x).  At degree n the
+  // slot adds orders m0 .. m0 + c - 1 of output (j, k), J = j + n, from Ya = Yx[J, m0 - k] and
+  // Yb = Yx[J, m0 + k] = Ya + 32 k, with M[n, m0] = Mn + 48 m0:
+  //  * home slot, output w = 16 s + sl = tri(j) + k, active where w < tri(P - n), m0 = 0,
+  //    c = n + 1: {16 (j(j+1) - k), 16 (2j + 1), 32 k, hi((-1)^k)}, Ya = Yx + x + y n + 16 n^2;
   //  * helper slot of degree d (a chunk of c = cn orders):
-  //    {16 (J(J+1) - k + m0 - d^2), 0, 32 k, hi((-1)^k)}, {0, 48 m0, hi(first sign), 0}, where the
-  //    first step's sign is 0 for m0 = 0 (the step is then the order-0 term, see m2l_terms_run);
-  //  * unused: a = INT_MAX.
-  int4 *htab = reinterpret_cast<int4 *>(sm2 + 4 * per);
-  for (int q = tid; q < 16 * S; q += W * WG) {
-    const int s = q >> 4, l = q & 15, v = slots[l * S + s];
+  //    {16 (J(J+1) - k + m0 - d^2), 48 m0, 32 k, hi((-1)^k)}, Ya = Yx + x + 16 d^2;
+  //    its first step's sign is 0 for m0 = 0 (the step is then the order-0 term, see
+  //    m2l_terms_run);
+  //  * unused helper lane: {0, 0, 0, 0} (a sign word of 0 marks it idle).
+  int4 *htab = reinterpret_cast<int4 *>(sm2 + 4 * per) + wv * 16 * S;
+  for (int q = lane; q < 16 * S; q += WG) {
+    const int s = q >> 4, l = q & 15, v = wslots[l * S + s];
     const int j = v & 31, k = (v >> 5) & 31, m0 = (v >> 10) & 31, d = (v >> 15) & 31;
     const int hk = (k & 1) ? (int)0xBFF00000 : 0x3FF00000;
-    int4      e0 = make_int4(0, 0, 0, 0x3FF00000), e1 = make_int4(0x7fffffff, 0, 0, 0);
-    if (v >= 0 && d == 31) {
-      e0 = make_int4(16 * (j * (j + 1) - k), 16 * (2 * j + 1), 32 * k, hk);
-      e1 = make_int4(q, 0, 0, 0);
-    } else if (v >= 0) {
+    int4      e  = make_int4(0, 0, 0, 0);
+    if (v >= 0 && d == 31) e = make_int4(16 * (j * (j + 1) - k), 16 * (2 * j + 1), 32 * k, hk);
+    else if (v >= 0) {
       const int J = j + d;
-      e0 = make_int4(16 * (J * (J + 1) - k + m0 - d * d), 0, 32 * k, hk);
-      e1 = make_int4(0, 48 * m0, m0 ? hk : 0, 0);
+      e = make_int4(16 * (J * (J + 1) - k + m0 - d * d), 48 * m0, 32 * k, hk);
     }
-    htab[2 * q]     = e0;
-    htab[2 * q + 1] = e1;
+    htab[q] = e;
   }
   double2 acc[S][3];
   for (int s = 0; s < S; s++)
     for (int d = 0; d < 3; d++) acc[s][d] = make_double2(0, 0);
   int wdeg = 0;  // the source degrees of this wavefront (bit n)
   for (int n = 0; n < P; n++)
-    if (W == 1 || ((lev[n] >> 15) & 1) == wv) wdeg |= 1 << n;
+    if (wlev[n]) wdeg |= 1 << n;
   const int    s0 = SPLIT ? seg[3 * u + 1] : off[t], nsrc = (SPLIT ? seg[3 * u + 2] : off[t + 1]) - s0;
   const double X = tx[t], Yc = ty[t], Z = tz[t];
   const int    lt = tab.Y ? tab.level[t] : 0;
@@ -931,8 +928,8 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
     sync();
     if (active) {
       for (int dm = wdeg; dm; dm &= dm - 1) {
-        const int      n = __builtin_ctz(dm), lv = lev[n];
-        const int      cn = lv & 255, hr = (lv >> 8) & 127, used = lv >> 16;
+        const int      n = __builtin_ctz(dm), lv = wlev[n];
+        const int      cn = lv & 255, hr = (lv >> 8) & 255, used = lv >> 16;
         const double2 *Mn = Mg + 3 * tri(n);
         const int      A = tri(P - n), nn = 16 * n * n;  // outputs w < A are active at degree n
         // the table is read per degree: the empty asm on a (zero) uniform offset keeps the
@@ -940,26 +937,22 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
         // the slots' entries are then immediate offsets from one lane address
         int tq = 0;
         asm volatile("" : "+s"(tq));
-        const int4 *hl = htab + tq + 2 * sl;
+        const int4 *hl = htab + tq + sl;
 #pragma unroll
         for (int s = 0; s < S; s++) {
           if (!((used >> s) & 1)) continue;  // no lane has work in slot s at this degree
           // home slot (s < hr): orders 0..n of the active outputs (a prefix of the lanes);
           // otherwise a helper slot of this degree: one chunk of cn orders
-          // (a home slot's activity and first order need no table read: w = 16 s + sl < A, m0 = 0)
+          // (a home slot's activity needs no table read: w = 16 s + sl < A)
           const bool home = s < hr;
-          int        ea = 16 * s + sl, em = 0, es = 0;
-          if (!home) {
-            const int4 e1 = hl[32 * s + 1];
-            ea = e1.x, em = e1.y, es = e1.z;
-          }
-          if (ea < (home ? A : 1)) {
-            const int4     e0 = hl[32 * s];
+          const int4 e    = hl[16 * s];
+          if (home ? 16 * s + sl < A : e.w != 0) {
             const double2 *Ya = reinterpret_cast<const double2 *>(
-                reinterpret_cast<const char *>(Yx) + e0.x + __umul24(e0.y, n) + nn);
-            const double2 *Yb = reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(Ya) + e0.z);
+                reinterpret_cast<const char *>(Yx) + e.x + (home ? __umul24(e.y, n) : 0) + nn);
+            const double2 *Yb = reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(Ya) + e.z);
+            const int      em = home ? 0 : e.y;
             const double2 *Mm = reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(Mn) + em);
-            m2l_terms_run(acc[s], Ya, Yb, Mm, __hiloint2double(e0.w, 0), home, __hiloint2double(es, 0),
+            m2l_terms_run(acc[s], Ya, Yb, Mm, __hiloint2double(e.w, 0), home, __hiloint2double(em ? e.w : 0, 0),
                           home ? n : cn - 1);
           }
         }
@@ -977,27 +970,11 @@ __global__ __launch_bounds__(W * WG) __attribute__((amdgpu_waves_per_eu(W == 2 &
       a.x += __shfl_xor(a.x, 32, 64);
       a.y += __shfl_xor(a.y, 32, 64);
     }
-  double2 *part = sm2;  // [16][S][3]
-  if (W > 1) {  // wavefront 0 adds the others' partial sums, in wavefront order
-    sync();
-    for (int w = 1; w < W; w++) {
-      if (wv == w && g == 0)
-        for (int s = 0; s < S; s++)
-          for (int d = 0; d < 3; d++) part[(sl * S + s) * 3 + d] = acc[s][d];
-      sync();
-      if (wv == 0)
-        for (int s = 0; s < S; s++)
-          for (int d = 0; d < 3; d++) {
-            const double2 o = part[(sl * S + s) * 3 + d];
-            acc[s][d].x += o.x;
-            acc[s][d].y += o.y;
-          }
-      sync();
-    }
-  }
-  if (wv == 0 && g == 0)
+  double2 *part = sm2;  // [W][16][S][3]: every wavefront's slots, read by the output loop
+  if (W > 1) sync();
+  if (g == 0)
     for (int s = 0; s < S; s++)
-      for (int d = 0; d < 3; d++) part[(sl * S + s) * 3 + d] = acc[s][d];
+      for (int d = 0; d < 3; d++) part[((wv * 16 + sl) * S + s) * 3 + d] = acc[s][d];
   __syncthreads();
   for (int q = tid; q < ny; q += W * WG) {
     int j = 0;
@@ -1172,15 +1149,17 @@ __global__ void k_zero(double *p, int64_t n)
 // ---------------------------------------------------------------------------------
 // M2L schedule: the work plan of k_m2l_ns (m2l_plan.hpp) for the handle's slot budget, copied
 // to the device, and the lattice and octant harmonic tables for P.
+// Two wavefronts per workgroup when one wavefront's LDS (16 (P^2 + NT) bytes per staged source,
+// 4 sources) would cap the occupancy below what its registers allow: from P = 9 on (measured:
+// P = 8 is 12% faster with one, P = 10 10% faster with two, P = 12 and 14 a third).
+int m2l_waves(int P, int waves) { return waves == 1 || waves == 2 ? waves : (P <= 8 ? 1 : 2); }
+
 void make_m2l_schedule(int P, M2LSchedule &s)
 {
-  // Two wavefronts per workgroup when one wavefront's LDS (16 (P^2 + NT) bytes per staged
-  // source, 4 sources) would cap the occupancy below what its registers allow: from P = 9 on
-  // (measured: P = 8 is 12% faster with one, P = 10 10% faster with two, P = 12 and 14 a third).
-  // Default budget: with two, 5 slots run at 4 waves per SIMD at P = 10.
-  s.W              = s.waves == 1 || s.waves == 2 ? s.waves : (P <= 8 ? 1 : 2);
+  // Default budget: with two wavefronts, 5 slots run at 4 waves per SIMD at P = 10.
+  s.W              = m2l_waves(P, s.waves);
   const int budget = s.budget > 0 ? s.budget : (s.W == 2 ? 5 : 0);
-  if (!make_m2l_plan(P, budget, s.plan)) throw Error(UVLM_ERR_INTERNAL, "no M2L plan");
+  if (!make_m2l_plan(P, budget, s.plan, 16, s.W)) throw Error(UVLM_ERR_INTERNAL, "no M2L plan");
   const M2LPlan &pl = s.plan;
   s.dev_slots       = 0;
   s.dev_lev         = pl.slots.size();
@@ -1290,8 +1269,8 @@ void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, c
   const int32_t *d = sch.dev.as<int32_t>();
   {
     const int    S = sch.plan.S, W = sch.W;
-    const size_t lds_ns = std::max(sizeof(double2) * 4 * (P * P + NT) + 2 * sizeof(int4) * 16 * S,
-                                   sizeof(double2) * 16 * S * 3);
+    const size_t lds_ns = std::max(sizeof(double2) * 4 * (P * P + NT) + sizeof(int4) * W * 16 * S,
+                                   sizeof(double2) * W * 16 * S * 3);
     // few targets with long lists: one workgroup per list segment, partial expansions reduced
     const bool     split = sc && tt.C < M2L_SPLIT_TARGETS && l.nm2l >= 4 * tt.C;
     int64_t        units = tt.C;
