@@ -61,6 +61,11 @@ def test_tree_and_lists_bitexact(orc, name):
     assert np.array_equal(np.unique(rp, axis=0), gp)
     assert np.array_equal(np.unique(rm, axis=0), gm)
     assert len(np.unique(rp, axis=0)) == len(rp) and len(np.unique(rm, axis=0)) == len(rm)
+    # the counts the bench's interactions/s rest on: pairs, and bodies(t) * bodies(s) over P2P pairs
+    t = g.timings()
+    nbody = ri[:, 1]
+    assert t["n_p2p_pairs"] == len(rp) and t["n_m2l_pairs"] == len(rm)
+    assert t["n_interactions"] == int(np.sum(nbody[rp[:, 0]] * nbody[rp[:, 1]])), t["n_interactions"]
 
 
 def test_tree_rebuilds_on_one_handle_stay_bitexact(orc):

@@ -385,7 +385,8 @@ __device__ __forceinline__ void traverse_round(const int32_t *__restrict__ ft, i
                                                TreeView S, double theta, double gap, int32_t *__restrict__ nxt,
                                                int64_t cap_f,
                                                uint64_t *__restrict__ p2p, int64_t cap_p, uint64_t *__restrict__ m2l,
-                                               int64_t cap_m, unsigned long long *__restrict__ counters, int sbits)
+                                               int64_t cap_m, unsigned long long *__restrict__ counters, int sbits,
+                                               unsigned long long (*blk)[BLK / 64][3])
 {
   int           cls[8];  // 0 push, 1 P2P, 2 M2L
   int           ct[8], cs[8];
@@ -416,16 +417,28 @@ __device__ __forceinline__ void traverse_round(const int32_t *__restrict__ ft, i
   }
   int tf, tp, tm;
   int of = wave_excl_scan(npush, tf), op = wave_excl_scan(np2p, tp), om = wave_excl_scan(nm2l, tm);
-  unsigned long long bf = 0, bp = 0, bm = 0;
-  const int lane = threadIdx.x & 63;
+  // one atomic per class and block: the wavefronts' totals meet in LDS, three lanes reserve the
+  // block's ranges and hand each wavefront its base (the counters are the traversal's only
+  // serialisation point)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) {
-    if (tf) bf = atomicAdd(&counters[0], (unsigned long long)tf);
-    if (tp) bp = atomicAdd(&counters[1], (unsigned long long)tp);
-    if (tm) bm = atomicAdd(&counters[2], (unsigned long long)tm);
+    blk[0][wv][0] = tf;
+    blk[0][wv][1] = tp;
+    blk[0][wv][2] = tm;
   }
-  bf = __shfl(bf, 0, 64);
-  bp = __shfl(bp, 0, 64);
-  bm = __shfl(bm, 0, 64);
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int          cl  = threadIdx.x;
+    unsigned long long tot = 0;
+    for (int w = 0; w < BLK / 64; w++) tot += blk[0][w][cl];
+    unsigned long long base = tot ? atomicAdd(&counters[cl], tot) : 0;
+    for (int w = 0; w < BLK / 64; w++) {
+      blk[1][w][cl] = base;
+      base += blk[0][w][cl];
+    }
+  }
+  __syncthreads();
+  const unsigned long long bf = blk[1][wv][0], bp = blk[1][wv][1], bm = blk[1][wv][2];
   for (int c = 0; c < nout; c++) {
     const uint64_t key = ((uint64_t)(uint32_t)ct[c] << sbits) | (uint64_t)(uint32_t)cs[c];
     if (cls[c] == 0) {
@@ -452,10 +465,11 @@ __global__ void k_traverse(const int32_t *__restrict__ ft, TreeView T, TreeView 
                            uint64_t *__restrict__ m2l, int64_t cap_m, unsigned long long *__restrict__ counters,
                            int sbits)
 {
+  __shared__ unsigned long long blk[2][BLK / 64][3];  // per wavefront: totals, then bases
   const int64_t nf = (int64_t)counters[3];
   for (int64_t base = blockIdx.x * (int64_t)BLK; base < nf; base += (int64_t)gridDim.x * BLK)
     traverse_round(ft, nf, base + threadIdx.x, T, S, theta, gap, nxt, cap_f, p2p, cap_p, m2l, cap_m, counters,
-                   sbits);
+                   sbits, blk);
 }
 
 // Start of a traversal: the root pair (0, 0) is the first frontier, or, when both roots are leaves,
@@ -538,7 +552,15 @@ __global__ void k_count_inter(int64_t C, const int32_t *__restrict__ off, const 
   if (t < C)
     for (int q = off[t]; q < off[t + 1]; q++) acc += (unsigned long long)tn[t] * (unsigned long long)sn[src[q]];
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
+  // one atomic per block (the wavefronts' sums meet in LDS)
+  __shared__ unsigned long long wsum[BLK / 64];
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < BLK / 64; w++) tot += wsum[w];
+    if (tot) atomicAdd(out, tot);
+  }
 }
 
 void h2d(hipStream_t s, void *d, const void *h, size_t bytes)
