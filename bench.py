@@ -8,7 +8,8 @@ A step = uvlm_build (tree + interaction lists) + uvlm_eval on device-resident pa
 
 Prints ONE JSON line (rank 0, N=1) with the contract fields plus:
   roofline     -- the dominant kernel's algorithmic FLOP rate vs the MI355X peak, from HIP
-                  events recorded by the library on its stream around that kernel
+                  events recorded by the library on its stream around that kernel; "kernels"
+                  holds the same for both M2L and P2P
   cpu_baseline -- the oracle (CPU restatement of the reference) timed on the host cores: one
                   build + eval of the whole 4M-particle workload after a small warm-up
   extra        -- FP64-path time, mixed-vs-FP64 difference, error vs an FP64 direct sum on
@@ -641,22 +642,31 @@ def main():
         g64.close()
         ncrit64 = {"ms": round(ms64n, 3), "phases_ms": {k: round(v, 3) for k, v in ph64n.items()}}
 
-    # roofline of the dominant kernel (M2L or P2P), from the library's HIP events
-    m2l_flop = tm["n_m2l_pairs"] * m2l_flop_per_pair(args.P)
-    p2p_flop = tm["n_interactions"] * P2P_FLOP
-    if ph["m2l_ms"] >= ph["p2p_ms"]:
-        kern, key, flop, kms, peak = "M2L (FP64)", "k_m2l_ns", m2l_flop, ph["m2l_ms"], PEAK["fp64"]
-    else:
-        kern, key, flop, kms = "P2P", "k_p2p", p2p_flop, ph["p2p_ms"]
-        peak = PEAK["fp32"] if args.precision == "mixed" else PEAK["fp64"]
-    achieved = flop / (kms * 1e-3) / 1e12
+    # roofline of the dominant kernel (M2L or P2P), from the library's HIP events; both are
+    # reported under "kernels" (at the autotuned leaf size they take about the same time)
     wl = {"particles": N, "P": args.P, "theta": args.theta, "ncrit": ncrit_used, "precision": args.precision}
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": profiled_traffic(wl, key), "kernel": kern,
-                "roof": "vector ALU: this path has no dense contraction and no MFMA (FP64 vector and matrix peaks "
-                        "coincide on MI355X); neither the HBM nor an MFMA roof bounds the dominant kernel",
-                "traffic_note": "HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in "
-                                "profiles/ for this workload (null if none)"}
+    p2p_peak = PEAK["fp32"] if args.precision == "mixed" else PEAK["fp64"]
+    roofs = {
+        "M2L": ("k_m2l_ns", tm["n_m2l_pairs"] * m2l_flop_per_pair(args.P), ph["m2l_ms"], PEAK["fp64"],
+                "FP64 vector ALU: no dense contraction and no MFMA (FP64 vector and matrix peaks coincide on "
+                "MI355X); not HBM-bound (profiles/)"),
+        "P2P": ("k_p2p", tm["n_interactions"] * P2P_FLOP, ph["p2p_ms"], p2p_peak,
+                ("FP32 vector ALU (pair math in FP32, sums in FP64); the 157.3 TFLOP/s peak counts packed FP32, "
+                 "which this build leaves off, so the unpacked ceiling is half of it; not HBM-bound (profiles/)")
+                if args.precision == "mixed" else "FP64 vector ALU; not HBM-bound (profiles/)"),
+    }
+    kernels = {}
+    for name, (key, flop, kms, peak, roof) in roofs.items():
+        achieved = flop / (kms * 1e-3) / 1e12
+        kernels[name] = {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                         "frac": round(achieved / peak, 4), "ms": round(kms, 3),
+                         "traffic": profiled_traffic(wl, key), "roof": roof}
+    dom = "M2L" if ph["m2l_ms"] >= ph["p2p_ms"] else "P2P"
+    roofline = {k: v for k, v in kernels[dom].items() if k != "ms"}
+    roofline["kernel"] = dom + (" (FP64)" if dom == "M2L" or args.precision != "mixed" else " (FP32 pairs)")
+    roofline["kernels"] = kernels
+    roofline["traffic_note"] = ("HBM bytes per launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in "
+                                "profiles/ for this workload (null if none)")
     # whole-eval roofline: the sum over the phases of the time each would take at its roof
     t_roof = eval_roof_phases(tm, N, args.P, args.precision)
     t_roof_ms = {k: round(v * 1e3, 4) for k, v in t_roof.items()}
