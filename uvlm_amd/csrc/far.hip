@@ -134,8 +134,10 @@ __constant__ double c_invfact[32] = {
 // Column m of the regular (singular=false) or singular harmonics, degrees m..P-1, written
 // to Y[tri(n) + m].  Associated Legendre recurrence in n at fixed m:
 // (n-m) P_n^m = x (2n-1) P_{n-1}^m - (n+m-1) P_{n-2}^m, from P_m^m = (-1)^m (2m-1)!! y^m.
+// inv: the table 1/k (c_inv, or a copy of it in LDS where m differs between the lanes: read from
+// __constant__ memory at a per-lane index it is a vector load, waited for inside the recurrence).
 template <bool SINGULAR>
-__device__ void harmonic_column(int P, int m, const Sph &s, double2 *Y)
+__device__ void harmonic_column(int P, int m, const Sph &s, double2 *Y, const double *__restrict__ inv_k)
 {
   // e^{imb}
   double2 e = make_double2(1.0, 0.0);
@@ -151,17 +153,17 @@ __device__ void harmonic_column(int P, int m, const Sph &s, double2 *Y)
     for (int i = 0; i < m; i++) f *= -inv;  // (-1)^m / rho^{m+1}
   } else {
     f = 1.0;
-    for (int i = 1; i <= m; i++) f *= -s.rho * c_inv[2 * i - 1] * c_inv[2 * i];  // (-1)^m rho^m / (2m)!
+    for (int i = 1; i <= m; i++) f *= -s.rho * inv_k[2 * i - 1] * inv_k[2 * i];  // (-1)^m rho^m / (2m)!
   }
   double p_prev = 0.0, p = pmm;
   for (int n = m; n < P; n++) {
     if (n > m) {
       const double pn = (n == m + 1) ? s.x * (2 * m + 1) * p
-                                     : (s.x * (2 * n - 1) * p - (n + m - 1) * p_prev) * c_inv[n - m];
+                                     : (s.x * (2 * n - 1) * p - (n + m - 1) * p_prev) * inv_k[n - m];
       p_prev = p;
       p      = pn;
       if (SINGULAR) f *= -(double)(n - m) * inv;  // (-1)^n (n-m)! / rho^{n+1}
-      else f *= -s.rho * c_inv[n + m];            // (-1)^n rho^n / (n+m)!
+      else f *= -s.rho * inv_k[n + m];            // (-1)^n rho^n / (n+m)!
     }
     const double a = f * p;
     Y[tri(n) + m]  = make_double2(a * e.x, a * e.y);
@@ -235,8 +237,11 @@ __global__ __launch_bounds__(WG) void k_p2m(int Prt, int NT, const int32_t *__re
   // chunk, in body order (multipoleKernels.hpp:305-327).
   const int P = PC > 0 ? PC : Prt;
   extern __shared__ double2 smq[];
+  __shared__ double tinv[64];  // 1/k: the columns' recurrences index it per lane
   const int64_t w = blockIdx.x;
   if (w >= nleaf) return;
+  tinv[threadIdx.x] = c_inv[threadIdx.x];
+  wave_lds_sync();
   const int    c = leaves[w], b0 = cbody[c], nb = cnbody[c];
   const double X = cx[c], Yc = cy[c], Z = cz[c];
   const int    lane = threadIdx.x, ny = tri(P);
@@ -257,8 +262,8 @@ __global__ __launch_bounds__(WG) void k_p2m(int Prt, int NT, const int32_t *__re
     const int i = c0 + pb;
     if (pb < B && i < nb) {
       const Sph sp = sph(xb[0] - X, xb[1] - Yc, xb[2] - Z);
-      harmonic_column<false>(P, pc, sp, Yb + pb * ny);
-      if (P - 1 - pc > pc) harmonic_column<false>(P, P - 1 - pc, sp, Yb + pb * ny);
+      harmonic_column<false>(P, pc, sp, Yb + pb * ny, tinv);
+      if (P - 1 - pc > pc) harmonic_column<false>(P, P - 1 - pc, sp, Yb + pb * ny, tinv);
       if (pc == 0)
         for (int d = 0; d < 3; d++) al[pb * 3 + d] = ab[d];
     }
@@ -478,8 +483,11 @@ __global__ __launch_bounds__(WG) void k_p2m_adj(int P, int NT, const int32_t *__
                                                 const double *__restrict__ aux, int64_t cap, double2 *__restrict__ M)
 {
   extern __shared__ double2 smq[];
+  __shared__ double tinv[64];  // 1/k: the columns' recurrences index it per lane
   const int64_t w = blockIdx.x;
   if (w >= nleaf) return;
+  tinv[threadIdx.x] = c_inv[threadIdx.x];
+  wave_lds_sync();
   const int    c = leaves[w], b0 = cbody[c], nb = cnbody[c];
   const double X = cx[c], Yc = cy[c], Z = cz[c];
   const int    lane = threadIdx.x, ny = tri(P);
@@ -495,8 +503,8 @@ __global__ __launch_bounds__(WG) void k_p2m_adj(int P, int NT, const int32_t *__
     if (pb < B && i < nb) {
       const int64_t bi = b0 + i;
       const Sph     sp = sph(st[(F_X + 0) * cap + bi] - X, st[(F_X + 1) * cap + bi] - Yc, st[(F_X + 2) * cap + bi] - Z);
-      harmonic_column<false>(P, pc, sp, Yb + pb * ny);
-      if (P - 1 - pc > pc) harmonic_column<false>(P, P - 1 - pc, sp, Yb + pb * ny);
+      harmonic_column<false>(P, pc, sp, Yb + pb * ny, tinv);
+      if (P - 1 - pc > pc) harmonic_column<false>(P, P - 1 - pc, sp, Yb + pb * ny, tinv);
       if (pc == 0)
         for (int f = 0; f < 12; f++) wt[pb * 12 + f] = aux[f * cap + bi];
     }
@@ -691,7 +699,7 @@ __global__ void k_octant_table(int P, double2 *__restrict__ Y)
   const int o = blockIdx.x;
   if (threadIdx.x < P)
     harmonic_column<false>(P, threadIdx.x, sph((o & 1) ? 1.0 : -1.0, (o & 2) ? 1.0 : -1.0, (o & 4) ? 1.0 : -1.0),
-                           Y + o * tri(P));
+                           Y + o * tri(P), c_inv);
 }
 
 // Singular harmonics of the integer lattice vectors k in [-K, K]^3 (k = 0 left zero), in the
