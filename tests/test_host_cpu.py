@@ -188,3 +188,53 @@ def test_m2l_plan_owns_every_term_once(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout[-2000:]
     assert "all terms owned exactly once, in bounds" in r.stdout
+
+
+def _unbound_names(path):
+    """(function, name, line) for every name a top-level function of the module loads that is
+    bound neither in that function (arguments, assignments, imports, nested definitions) nor at
+    module level nor as a builtin: a NameError waiting for the line to run."""
+    import ast
+    import builtins
+
+    tree = ast.parse(open(path).read())
+
+    def bound(nodes):
+        b = set()
+        for top in nodes:
+            for n in ast.walk(top):
+                if isinstance(n, (ast.FunctionDef, ast.AsyncFunctionDef, ast.ClassDef)):
+                    b.add(n.name)
+                elif isinstance(n, (ast.Import, ast.ImportFrom)):
+                    b.update((a.asname or a.name).split(".")[0] for a in n.names)
+                elif isinstance(n, ast.Name) and isinstance(n.ctx, (ast.Store, ast.Del)):
+                    b.add(n.id)
+                elif isinstance(n, ast.arg):
+                    b.add(n.arg)
+                elif isinstance(n, ast.ExceptHandler) and n.name:
+                    b.add(n.name)
+                elif isinstance(n, (ast.Global, ast.Nonlocal)):
+                    b.update(n.names)
+        return b
+
+    module = set(dir(builtins)) | {"__file__", "__name__"}
+    for n in tree.body:
+        if isinstance(n, (ast.FunctionDef, ast.AsyncFunctionDef, ast.ClassDef)):
+            module.add(n.name)
+        else:
+            module |= bound([n])
+    bad = []
+    for f in tree.body:
+        if isinstance(f, (ast.FunctionDef, ast.AsyncFunctionDef)):
+            local = bound([f])
+            bad += [(f.name, n.id, n.lineno) for n in ast.walk(f)
+                    if isinstance(n, ast.Name) and isinstance(n.ctx, ast.Load) and n.id not in local | module]
+    return bad
+
+
+@pytest.mark.parametrize("rel", ["bench.py", "__graft_entry__.py", "uvlm_amd/fmm.py", "uvlm_amd/vlm.py",
+                                 "uvlm_amd/_capi.py", "uvlm_amd/workloads.py", "uvlm_amd/vtk.py"])
+def test_no_unbound_names(rel):
+    """bench.py and the package run only on the GPU box; a name left over from an edit would
+    first fail there.  Every top-level function loads only names it or the module binds."""
+    assert _unbound_names(os.path.join(ROOT, rel)) == []
