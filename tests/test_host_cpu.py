@@ -114,24 +114,30 @@ def test_p2p_polynomial_matches_erf_form():
         assert abs(ppp - _g_series(x, 2)) < 1e-7
 
 
-def test_p2p_degree8_polynomial_matches_erf_form():
-    """The forward P2P's degree-8 polynomial for G and G' (c_gpoly8 in pair_kernel.hpp), evaluated
-    as the kernel does (Horner in FP32, chains started after their first step), against the erf
-    form in FP64 and (small x, where the erf form cancels) G's power series."""
+def test_p2p_near_polynomials_match_erf_form():
+    """The forward P2P's degree-7 polynomials for G and G' (c_gnear and c_gpnear in
+    pair_kernel.hpp), evaluated as the kernel does (Horner in FP32, each chain started after its
+    first step), against the erf form in FP64 and (small x, where the erf form cancels) G's power
+    series; and G' against the derivative of G's polynomial, so that a swapped or mistyped
+    coefficient in either table fails."""
     import math
     src = open(os.path.join(ROOT, "uvlm_amd", "csrc", "pair_kernel.hpp")).read()
-    key = "c_gpoly8[NPOLY8] = {"
-    body = src[src.index(key) + len(key): src.index("};", src.index(key))]
-    coef = np.array([float(v.strip().rstrip("f")) for v in body.split(",")], dtype=np.float32)
-    assert len(coef) == 9
+
+    def table(name):
+        m = re.search(re.escape(name) + r"\[NNEAR\]\s*=\s*\{([^}]*)\}", src)
+        body = m.group(1)
+        return np.array([float(v.strip().rstrip("f")) for v in body.split(",")], dtype=np.float32)
+
+    cg, cp = table("c_gnear"), table("c_gpnear")
+    assert len(cg) == 8 and len(cp) == 8
     f32 = np.float32
     for x in np.linspace(0.0, 2.0, 201):
         t = f32(f32(x) - f32(1.0))
-        gp = coef[8]
-        g = f32(gp * t + coef[7])
-        for k in range(6, -1, -1):
-            gp = f32(gp * t + g)
-            g = f32(g * t + coef[k])
+        g = f32(cg[7] * t + cg[6])
+        p = f32(cp[7] * t + cp[6])
+        for k in range(5, -1, -1):
+            g = f32(g * t + cg[k])
+            p = f32(p * t + cp[k])
         if x < 0.05:
             G, Gp = _g_series(x, 0), _g_series(x, 1)
         else:
@@ -140,7 +146,11 @@ def test_p2p_degree8_polynomial_matches_erf_form():
             G = gr / s ** 3
             Gp = (s * 4 * x * math.exp(-x) / math.sqrt(math.pi) - 3 * gr) / (2 * s ** 5)
         assert abs(float(g) - G) < 2e-7 * abs(G)
-        assert abs(float(gp) - Gp) < 1.2e-7
+        assert abs(float(p) - Gp) < 1.2e-7
+        # G' against the derivative of the G polynomial (exact arithmetic; the derivative of
+        # the degree-7 interpolant is within 7e-7 of G' at the ends of [0, 2])
+        dg = sum(i * float(c) * (x - 1.0) ** (i - 1) for i, c in enumerate(cg) if i)
+        assert abs(dg - Gp) < 1e-6
 
 
 def test_p2p_far_form_matches_erf_form():

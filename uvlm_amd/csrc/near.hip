@@ -166,10 +166,15 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MIXED ? 5 :
           }
         }
         if (MIXED) {
-          // zero entries (no strength) past the chunk: every lane runs the same trip count
+          // entries of no strength past the chunk: every lane runs the same trip count.  Their
+          // offset and q put x = |r - (1024, 0, 0)|^2 / 2^20 near 1 for any target within a few
+          // hundred units of the leaf centre, so they stay on the pair loop's common path (a pad
+          // at x = 0 would take its coincident-pair test); TSIG takes q from the target, where
+          // a pad at the centre gives x = |r|^2 q of the same order as the real pairs.
           const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (lane >= cnt) s32[2 * lane] = s32[2 * lane + 1] = zero;
-          s32[2 * (BLK + lane)] = s32[2 * (BLK + lane) + 1] = zero;
+          const float4 pad  = TSIG ? zero : make_float4(1024.f, 0.f, 0.f, 0x1p-20f);
+          if (lane >= cnt) s32[2 * lane] = pad, s32[2 * lane + 1] = zero;
+          s32[2 * (BLK + lane)] = pad, s32[2 * (BLK + lane) + 1] = zero;
         }
         __syncthreads();
         if (MIXED) {
@@ -191,17 +196,24 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MIXED ? 5 :
               const float  tq0 = fmaf(r20, q0, -1.0f), tq1 = fmaf(r21, q1, -1.0f);  // x - 1
               float        G0, Gp0, G1, Gp1;  // G(x) and G'(x) of both pairs
               g_near(tq0, tq1, G0, Gp0, G1, Gp1);
-              if (fmaxf(tq0, tq1) >= XPOLY - 1.0f) {  // lanes with a far pair take both far forms
-                float Gf0, Gpf0, Gf1, Gpf1;
-                g_far(r20 * q0, Gf0, Gpf0);
-                g_far(r21 * q1, Gf1, Gpf1);
-                if (tq0 >= XPOLY - 1.0f) G0 = Gf0, Gp0 = Gpf0;
-                if (tq1 >= XPOLY - 1.0f) G1 = Gf1, Gp1 = Gpf1;
+              // x >= 2 (t >= 1) takes the far form, and a coincident pair (R^2 = 0, so t = -1
+              // exactly) is skipped (multipoleKernels.hpp:242).  t >= -1 always, so one test
+              // |t| >= 1 over both pairs sends the rare steps that need either fix-up off the
+              // common path.
+              if (fmaxf(fabsf(tq0), fabsf(tq1)) >= XPOLY - 1.0f) {
+                if (fmaxf(tq0, tq1) >= XPOLY - 1.0f) {  // lanes with a far pair take both far forms
+                  float Gf0, Gpf0, Gf1, Gpf1;
+                  g_far(r20 * q0, Gf0, Gpf0);
+                  g_far(r21 * q1, Gf1, Gpf1);
+                  if (tq0 >= XPOLY - 1.0f) G0 = Gf0, Gp0 = Gpf0;
+                  if (tq1 >= XPOLY - 1.0f) G1 = Gf1, Gp1 = Gpf1;
+                }
+                // only the f1 alpha terms of J survive a zero dX, so zeroing f1 is enough
+                if (r20 == 0.0f) G0 = 0.0f;
+                if (r21 == 0.0f) G1 = 0.0f;
               }
-              // coincident pairs are skipped (multipoleKernels.hpp:242): only the f1 alpha terms
-              // of J survive a zero dX, so zeroing f1 is enough
-              const float f10 = r20 == 0.0f ? 0.0f : (TSIG ? G0 * t_c1 : G0);
-              const float f11 = r21 == 0.0f ? 0.0f : (TSIG ? G1 * t_c1 : G1);
+              const float f10 = TSIG ? G0 * t_c1 : G0;
+              const float f11 = TSIG ? G1 * t_c1 : G1;
               accumulate_pair(fa, dx0, dy0, dz0, B0.x, B0.y, B0.z, f10, Gp0 * (TSIG ? t_c2 : B0.w), WITH_J);
               accumulate_pair(fa, dx1, dy1, dz1, B1.x, B1.y, B1.z, f11, Gp1 * (TSIG ? t_c2 : B1.w), WITH_J);
             }

@@ -10,10 +10,10 @@
 // because 2 s^2 t2 - 3 g = s g'(s) - 3 g = 2 s^5 G'(x), and
 //   G(x) = (2/sqrt(pi)) sum_{i>=0} (-1)^i 2(i+1) / ((2i+3) (i+1)!) x^i.
 // For x < 2, G is approximated on [0, 2] by its degree-9 Chebyshev interpolant (kernel_g, which
-// also needs G''; the forward kernels use a degree-8 one, c_gpoly8 below) (interpolation
-// at the Chebyshev nodes of [0, 2], re-expanded in powers of t = x - 1), with no cancellation
-// between erf and t2: max error 8e-11 (G), 5e-9 (G') and 7.6e-8 (G'', |G''| <= 0.33) in exact
-// arithmetic, 1.4e-7 and 1.1e-7 (G, G') when evaluated by Horner in FP32.
+// also needs G''; the forward kernels interpolate G and G' separately, c_gnear and c_gpnear
+// below) (interpolation at the Chebyshev nodes of [0, 2], re-expanded in powers of t = x - 1),
+// with no cancellation between erf and t2: max error 8e-11 (G), 5e-9 (G') and 7.6e-8 (G'',
+// |G''| <= 0.33) in exact arithmetic, 1.4e-7 and 1.1e-7 (G, G') when evaluated by Horner in FP32.
 // For x >= 2 the hardware rsq/exp2 and a short polynomial: with w = x^{-1/2} = 1/s, e = e^{-x},
 //   g = erf(s) - (2/sqrt(pi)) s e = 1 - e (erfcx(s) + (2/sqrt(pi)) s),
 //   G = g w^3,   G' = w^2 ((2/sqrt(pi)) e - (3/2) G),
@@ -37,13 +37,18 @@ __constant__ float c_gpoly[NPOLY] = {4.27593295550599495e-01f,  -2.2628244588289
                                      8.69427232794296300e-05f,  -1.08135023163257920e-05f, 1.22398144588401693e-06f,
                                      -1.21730778572683114e-07f};
 
-// The forward near-field kernels need G and G' only: the degree-8 interpolant at the Chebyshev
-// nodes of [0, 2] (coefficients of t^i, t = x - 1) is accurate to 5e-10 (G) and 4e-8 (G') in
-// exact arithmetic and to 7e-8 and 8e-8 when evaluated by Horner in FP32, as the degree-9 one.
-constexpr int   NPOLY8             = 9;
-__constant__ float c_gpoly8[NPOLY8] = {4.27593295529120287e-01f,  -2.26282441609536422e-01f, 7.52993082456607232e-02f,
-                                       -1.86646673148319273e-02f, 3.70154622223736220e-03f,  -6.12261825120658876e-04f,
-                                       8.69482296212105497e-05f,  -1.10873394265394848e-05f, 1.22122772547046681e-06f};
+// The forward near-field kernels need G and G' only: each has its own degree-7 interpolant at
+// the Chebyshev nodes of [0, 2] (coefficients of t^i, t = x - 1), accurate to 1.1e-8 (G) and
+// 9.5e-9 (G') in exact arithmetic and to 7.3e-8 and 4.3e-8 by fused Horner in FP32.  Two chains
+// of 7 multiply-adds, independent of each other, instead of one degree-8 polynomial and its
+// derivative (15 multiply-adds, the derivative's chain waiting on the value's).
+constexpr int   NNEAR              = 8;
+__constant__ float c_gnear[NNEAR]  = {4.27593286009723905e-01f,  -2.26282444927409232e-01f, 7.52996131662352769e-02f,
+                                      -1.86646407540406734e-02f, 3.70002071831887491e-03f,  -6.12314968091220990e-04f,
+                                      8.93899977314482896e-05f,  -1.10569638636093259e-05f};
+__constant__ float c_gpnear[NNEAR] = {-2.26282437333498510e-01f, 1.50598616404860219e-01f,  -5.59941046870113882e-02f,
+                                      1.48061917699502943e-02f,  -3.06096641491382382e-03f, 5.21663223968849010e-04f,
+                                      -7.78856703251840088e-05f, 9.79185156779749377e-06f};
 
 // erfcx(1/w) ~ sum_k c_far[k] (w - FAR_WC)^k, w in [FAR_WMIN, 1/sqrt(2)].
 constexpr int   NFAR     = 8;
@@ -83,20 +88,20 @@ __device__ __forceinline__ void kernel_g(float xq, float &G, float &Gp, float &G
   }
 }
 
-// G, G' at t0 = x0 - 1 and t1 = x1 - 1 (x < XPOLY) by the degree-8 interpolant, both Horner
-// chains interleaved; the chains start after their first step (G' = c8, G = c8 t + c7).
+// G, G' at t0 = x0 - 1 and t1 = x1 - 1 (x < XPOLY) by the degree-7 interpolants: four
+// independent Horner chains, each started after its first step.
 __device__ __forceinline__ void g_near(float t0, float t1, float &G0, float &Gp0, float &G1, float &Gp1)
 {
-  float gp0 = c_gpoly8[NPOLY8 - 1], gp1 = gp0;
-  float g0 = fmaf(gp0, t0, c_gpoly8[NPOLY8 - 2]), g1 = fmaf(gp1, t1, c_gpoly8[NPOLY8 - 2]);
+  float g0 = fmaf(c_gnear[NNEAR - 1], t0, c_gnear[NNEAR - 2]), g1 = fmaf(c_gnear[NNEAR - 1], t1, c_gnear[NNEAR - 2]);
+  float p0 = fmaf(c_gpnear[NNEAR - 1], t0, c_gpnear[NNEAR - 2]), p1 = fmaf(c_gpnear[NNEAR - 1], t1, c_gpnear[NNEAR - 2]);
 #pragma unroll
-  for (int k = NPOLY8 - 3; k >= 0; k--) {
-    gp0 = fmaf(gp0, t0, g0);
-    gp1 = fmaf(gp1, t1, g1);
-    g0  = fmaf(g0, t0, c_gpoly8[k]);
-    g1  = fmaf(g1, t1, c_gpoly8[k]);
+  for (int k = NNEAR - 3; k >= 0; k--) {
+    g0 = fmaf(g0, t0, c_gnear[k]);
+    g1 = fmaf(g1, t1, c_gnear[k]);
+    p0 = fmaf(p0, t0, c_gpnear[k]);
+    p1 = fmaf(p1, t1, c_gpnear[k]);
   }
-  G0 = g0, Gp0 = gp0, G1 = g1, Gp1 = gp1;
+  G0 = g0, Gp0 = p0, G1 = g1, Gp1 = p1;
 }
 
 // G, G' at x >= XPOLY from rsq/exp2 and the erfcx polynomial (the x >= XPOLY branch of kernel_g).

@@ -339,8 +339,9 @@ __global__ void k_iota_u32(int64_t N, uint32_t *__restrict__ v)
 }
 
 // GATHER_FIELDS fields per grid row (blockIdx.y): the state moves with full memory parallelism
-// and each permutation entry is read once per group of fields.
-constexpr int GATHER_FIELDS = 4;
+// and each permutation entry is read once per group of fields.  Two fields per row moved 37
+// fields of 4M bodies in 0.45 ms (5.6 TB/s, a plain copy 5.8 TB/s), four in 0.54 ms.
+constexpr int GATHER_FIELDS = 2;
 __global__ void k_gather_fields(const double *__restrict__ src, double *__restrict__ dst, int64_t scap, int64_t dcap,
                                 const int32_t *__restrict__ perm, int64_t N, int nfields)
 {
