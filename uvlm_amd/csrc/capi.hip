@@ -188,7 +188,7 @@ void do_eval(uvlm_handle h, bool sync)
   double2 *M = h->M.as<double2>(), *L = h->L.as<double2>();
   UVLM_HIP(hipEventRecord(h->ev[EV_E0], s));
   p2m(s, P, t, h->st.as<double>(), h->cap, M);
-  for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M, h->sched.octY.as<double2>());
+  for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M, h->sched);
   zero_doubles(s, (double *)M, 2 * NT);
   UVLM_HIP(hipEventRecord(h->ev[EV_UP], s));
   m2l(s, P, t, t, h->lists, h->sched, M, L);
@@ -397,7 +397,7 @@ void probe_adjoint_core(uvlm_handle h, int64_t m)
   h->Ms.reserve(sizeof(double2) * NT * h->stree.C);
   double2 *Ms = h->Ms.as<double2>();
   p2m(s, P, h->stree, h->sst.as<double>(), sc, Ms);
-  for (int l = h->stree.nlevels - 2; l >= 0; l--) m2m_level(s, P, h->stree, l, Ms, h->sched.octY.as<double2>());
+  for (int l = h->stree.nlevels - 2; l >= 0; l--) m2m_level(s, P, h->stree, l, Ms, h->sched);
   // adjoint state of the particles: positions, strengths and sigma copied in
   const int64_t cap = h->cap;
   h->ast.reserve(sizeof(double) * N_FIELDS * cap);
@@ -476,7 +476,7 @@ void vjp_core(uvlm_handle h, double *sbar = nullptr)
   }
   double2 *M = h->M.as<double2>(), *L = h->L.as<double2>();
   p2m_adj(s, P, t, st, aux, cap, M);
-  for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M, h->sched.octY.as<double2>());
+  for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M, h->sched);
   zero_doubles(s, (double *)M, 2 * NT);
   m2l(s, P, t, t, h->tlists, h->sched, M, L);
   for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L, h->sched.octY.as<double2>());

@@ -147,6 +147,8 @@ struct M2LSchedule {
   int64_t dev_slots = 0, dev_lev = 0, dev_out_off = 0, dev_out_src = 0;
   DevBuf  ytab;        // singular harmonics of the lattice vectors [-K, K]^3 (M2L_TABLE_K)
   DevBuf  octY;        // regular harmonics of the 8 unit octant vectors (M2M, L2L)
+  DevBuf  m2mtab;      // term table of the M2M for P <= 10: 64 counts, then (m2m_rp + m2m_rc) rows
+  int     m2m_rp = 0, m2m_rc = 0;
   int     ytab_P = 0;
 };
 // Half-width of the cube of integer offsets (in units of the smaller cell's half-width) whose
@@ -162,7 +164,7 @@ int  m2l_waves(int P, int waves);  // wavefronts per M2L workgroup for order P (
 void make_m2l_schedule(int P, M2LSchedule &s);
 void zero_doubles(hipStream_t s, double *p, int64_t n);
 void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, double2 *M);
-void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M, const double2 *octY);
+void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M, const M2LSchedule &sch);
 void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, const M2LSchedule &sch, const double2 *M,
          double2 *L, Scratch *sc = nullptr);
 void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L, const double2 *octY);

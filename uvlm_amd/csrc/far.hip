@@ -378,6 +378,96 @@ __global__ __launch_bounds__(M2M_MAXWV * WG) void k_m2m(int Prt, int NT, const i
   }
 }
 
+// M2M for P <= M2M_TAB_MAXP from a term table (make_m2m_table): the translation of k_m2m with
+// the terms of output q = (j,k) listed once on the host, in k_m2m's (n, m) order, as 16-bit
+// entries q' | v << 6: Mc entry q' = (j-n, |k-m|) and v = c * nq + (n, |m|) in one of four
+// copies of the child's octant harmonics (c: Y, -Y, conj Y, -conj Y), so that each term is a
+// table read, four LDS reads and three complex multiply-adds, with no index arithmetic, sign or
+// branch.  The terms k_m2m takes with conj(Mc) (m >= k) are summed as B = sum Mc conj(Y') and
+// added as conj(B) after those with Mc (m < k), so each lane runs two loops of its own lengths
+// (cnt: plain | conjugated << 8, rows RP and RC of 64 lanes).  One wavefront per child as in
+// k_m2m; the wavefronts' sums are added in wavefront order.
+constexpr int M2M_TAB_MAXP = 10;
+__global__ __launch_bounds__(M2M_MAXWV * WG) void k_m2m_tab(int P, int NT, const int32_t *__restrict__ cells,
+                                                            int64_t ncell, const int32_t *__restrict__ cnchild,
+                                                            const int32_t *__restrict__ cchild,
+                                                            const double *__restrict__ cx, const double *__restrict__ cy,
+                                                            const double *__restrict__ cz, const double *__restrict__ cR,
+                                                            const double2 *__restrict__ octY,
+                                                            const int32_t *__restrict__ tcnt,
+                                                            const uint32_t *__restrict__ tab, int RP, int RC,
+                                                            double2 *__restrict__ M)
+{
+  extern __shared__ double2 sm2[];
+  const int64_t w = blockIdx.x;
+  if (w >= ncell) return;
+  const int p = cells[w], nc = cnchild[p];
+  if (nc == 0) return;
+  const int nwv = blockDim.x / WG, wv = threadIdx.x / WG, lane = threadIdx.x % WG, nq = tri(P);
+  double2  *Yv  = sm2 + wv * (4 * nq + NT);  // [4][nq]
+  double2  *Mc  = Yv + 4 * nq;               // [NT]
+  uint32_t *ts  = reinterpret_cast<uint32_t *>(sm2 + nwv * (4 * nq + NT));  // (RP + RC) rows of 64 halves
+  for (int i = threadIdx.x; i < (RP + RC) * WG / 2; i += blockDim.x) ts[i] = tab[i];
+  const unsigned short *tl = reinterpret_cast<const unsigned short *>(ts) + lane;
+  const int cn = tcnt[lane], cp = cn & 255, cc = cn >> 8;
+  const char *yb = reinterpret_cast<const char *>(Yv), *mb = reinterpret_cast<const char *>(Mc);
+  double2 A[3], B[3];
+  for (int d = 0; d < 3; d++) A[d] = B[d] = make_double2(0, 0);
+  for (int c0 = 0; c0 < nc; c0 += nwv) {
+    const int  ci = c0 + wv;
+    const bool on = ci < nc;  // uniform over the wavefront
+    if (on) {
+      const int    ch = cchild[p] + ci;
+      const int    o  = (cx[p] > cx[ch]) | ((cy[p] > cy[ch]) << 1) | ((cz[p] > cz[ch]) << 2);
+      const double r  = cR[ch];
+      for (int q = lane; q < nq; q += WG) {
+        int n = 0;
+        while (tri(n + 1) <= q) n++;
+        double f = 1.0;
+        for (int i = 0; i < n; i++) f *= r;
+        const double2 y = octY[o * nq + q];
+        const double  a = y.x * f, b = y.y * f;
+        Yv[q]          = make_double2(a, b);
+        Yv[nq + q]     = make_double2(-a, -b);
+        Yv[2 * nq + q] = make_double2(a, -b);
+        Yv[3 * nq + q] = make_double2(-a, b);
+      }
+      for (int q = lane; q < NT; q += WG) Mc[q] = M[(int64_t)ch * NT + q];
+    }
+    __syncthreads();
+    if (on) {
+      for (int i = 0; i < cp; i++) {
+        const int      e  = tl[i * WG];
+        const double2  y  = *reinterpret_cast<const double2 *>(yb + ((e >> 6) << 4));
+        const double2 *mc = reinterpret_cast<const double2 *>(mb + (e & 63) * 48);
+        for (int d = 0; d < 3; d++) cfma(A[d], mc[d], y);
+      }
+      for (int i = 0; i < cc; i++) {
+        const int      e  = tl[(RP + i) * WG];
+        const double2  y  = *reinterpret_cast<const double2 *>(yb + ((e >> 6) << 4));
+        const double2 *mc = reinterpret_cast<const double2 *>(mb + (e & 63) * 48);
+        for (int d = 0; d < 3; d++) cfma(B[d], mc[d], y);
+      }
+    }
+    __syncthreads();
+  }
+  // sum over the wavefronts in order: part[wv][lane][3]
+  double2  *part = sm2;
+  const int nw   = min(nwv, nc);
+  if (wv < nw)
+    for (int d = 0; d < 3; d++) part[(wv * WG + lane) * 3 + d] = make_double2(A[d].x + B[d].x, A[d].y - B[d].y);
+  __syncthreads();
+  if (wv == 0 && lane < nq)
+    for (int d = 0; d < 3; d++) {
+      double2 r = part[lane * 3 + d];
+      for (int v = 1; v < nw; v++) {
+        r.x += part[(v * WG + lane) * 3 + d].x;
+        r.y += part[(v * WG + lane) * 3 + d].y;
+      }
+      M[(int64_t)p * NT + 3 * lane + d] = r;
+    }
+}
+
 // ------------------------------------------------------------------ L2L
 
 // L2L with the work balanced over the lanes.  Output (j, k) of the child takes
@@ -1162,6 +1252,51 @@ __global__ void k_zero(double *p, int64_t n)
 // P = 8 is 12% faster with one, P = 10 10% faster with two, P = 12 and 14 a third).
 int m2l_waves(int P, int waves) { return waves == 1 || waves == 2 ? waves : (P <= 8 ? 1 : 2); }
 
+// Term table of k_m2m_tab for P <= M2M_TAB_MAXP: per lane (output q = (j,k) = lane, tri(P) <= 64)
+// the counts of its plain and conjugated terms, then RP rows of plain terms and RC rows of
+// conjugated terms, 64 lanes per row, in the order in which k_m2m takes them (n, then m).
+static void make_m2m_table(int P, std::vector<int32_t> &cnt, std::vector<uint16_t> &tab, int &RP, int &RC)
+{
+  const int nq = P * (P + 1) / 2;
+  auto tr  = [](int n) { return n * (n + 1) / 2; };
+  auto neg = [](int e) { return (e & 1) != 0; };  // (-1)^e < 0
+  std::vector<std::vector<uint16_t>> pl(WG), cj(WG);
+  // copy c of the octant harmonics: 0 Y, 1 -Y, 2 conj Y, 3 -conj Y
+  auto entry = [&](int qs, int idx, bool conjY, bool minus) {
+    return (uint16_t)(qs | (((conjY ? 2 : 0) + (minus ? 1 : 0)) * nq + idx) << 6);
+  };
+  for (int j = 0; j < P; j++)
+    for (int k = 0; k <= j; k++) {
+      const int q = tr(j) + k;
+      for (int n = 0; n <= j; n++) {
+        // Mc[j-n, k-m] Y[n,-m] (-1)^{m<0 ? m : 0} (-1)^n for m < k
+        for (int m = std::max(-n, -j + k + n); m <= std::min(k - 1, n); m++) {
+          const bool minus = (m < 0 && neg(m)) != neg(n);
+          pl[q].push_back(m <= 0 ? entry(tr(j - n) + k - m, tr(n) - m, false, minus)
+                                 : entry(tr(j - n) + k - m, tr(n) + m, true, minus));
+        }
+        // conj(Mc[j-n, m-k]) Y[n,-m] (-1)^{k+n+m} for m >= k, summed as Mc conj(Y[n,-m] (-1)^{k+n+m})
+        for (int m = k; m <= std::min(n, j + k - n); m++) {
+          const bool minus = neg(k + n + m);
+          cj[q].push_back(m == 0 ? entry(tr(j - n) - k + m, tr(n), true, minus)
+                                 : entry(tr(j - n) - k + m, tr(n) + m, false, minus));
+        }
+      }
+    }
+  RP = RC = 0;
+  cnt.assign(WG, 0);
+  for (int q = 0; q < nq; q++) {
+    RP     = std::max(RP, (int)pl[q].size());
+    RC     = std::max(RC, (int)cj[q].size());
+    cnt[q] = (int32_t)pl[q].size() | ((int32_t)cj[q].size() << 8);
+  }
+  tab.assign((size_t)(RP + RC) * WG, 0);
+  for (int q = 0; q < nq; q++) {
+    for (size_t i = 0; i < pl[q].size(); i++) tab[i * WG + q] = pl[q][i];
+    for (size_t i = 0; i < cj[q].size(); i++) tab[(RP + i) * WG + q] = cj[q][i];
+  }
+}
+
 void make_m2l_schedule(int P, M2LSchedule &s)
 {
   // Default budget: with two wavefronts, 5 slots run at 4 waves per SIMD at P = 10.
@@ -1188,6 +1323,15 @@ void make_m2l_schedule(int P, M2LSchedule &s)
     s.octY.reserve(sizeof(double2) * 8 * P * (P + 1) / 2);
     k_octant_table<<<8, WG>>>(P, s.octY.as<double2>());
     UVLM_LAUNCH_CHECK();
+    s.m2m_rp = s.m2m_rc = 0;
+    if (P <= M2M_TAB_MAXP) {
+      std::vector<int32_t>  cnt;
+      std::vector<uint16_t> tab;
+      make_m2m_table(P, cnt, tab, s.m2m_rp, s.m2m_rc);
+      s.m2mtab.reserve(4 * cnt.size() + 2 * tab.size());
+      UVLM_HIP(hipMemcpy(s.m2mtab.p, cnt.data(), 4 * cnt.size(), hipMemcpyHostToDevice));
+      UVLM_HIP(hipMemcpy((char *)s.m2mtab.p + 4 * cnt.size(), tab.data(), 2 * tab.size(), hipMemcpyHostToDevice));
+    }
     UVLM_HIP(hipDeviceSynchronize());
     s.ytab_P = P;
   }
@@ -1221,11 +1365,24 @@ void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, dou
   UVLM_LAUNCH_CHECK();
 }
 
-void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M, const double2 *octY)
+void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M, const M2LSchedule &sch)
 {
   const int     NT = 3 * P * (P + 1) / 2;
   const int64_t o = t.lvl_off[level], n = t.lvl_off[level + 1] - o;
   if (n <= 0) return;
+  const double2 *octY = sch.octY.as<double2>();
+  if (P <= M2M_TAB_MAXP && sch.m2m_rp + sch.m2m_rc > 0) {
+    const size_t slice = sizeof(double2) * (4 * P * (P + 1) / 2 + NT);
+    const size_t lds   = std::max(slice * M2M_MAXWV, sizeof(double2) * 3 * WG * M2M_MAXWV) +
+                       sizeof(uint16_t) * WG * (sch.m2m_rp + sch.m2m_rc);
+    const int32_t *cnt = reinterpret_cast<const int32_t *>(sch.m2mtab.p);
+    k_m2m_tab<<<(int)n, M2M_MAXWV * WG, lds, s>>>(
+        P, NT, t.lvl_cells.as<int32_t>() + o, n, t.nchild.as<int32_t>(), t.child.as<int32_t>(), t.cx.as<double>(),
+        t.cy.as<double>(), t.cz.as<double>(), t.R.as<double>(), octY, cnt, reinterpret_cast<const uint32_t *>(cnt + WG),
+        sch.m2m_rp, sch.m2m_rc, M);
+    UVLM_LAUNCH_CHECK();
+    return;
+  }
   // wavefronts per workgroup: 8 LDS slices of (harmonics + multipole) up to 64 KB, else 4
   const size_t slice = sizeof(double2) * (P * (P + 1) / 2 + NT);
   const int    nwv   = slice * M2M_MAXWV <= 65536 ? M2M_MAXWV : M2M_MAXWV / 2;
