@@ -1109,6 +1109,19 @@ __global__ __launch_bounds__(WG) void k_m2l_reduce(int NT, int64_t ncell, const 
 // ------------------------------------------------------------------ L2P + stretching
 // PC > 0 fixes the expansion order at compile time (fully unrolled sums with constant LDS offsets and
 // signs); PC = 0 takes it from Prt.
+// The L2P's U/V table holds, for output order kk in {1, 2}, the entries (n, mu) with n in
+// kk+1 .. P-1 and mu in 1 .. n-kk: tri(P-1-kk) of them (none below P = kk + 2).
+__host__ __device__ __forceinline__ int l2p_uv_count(int P, int kk)
+{
+  const int r = P - 1 - kk;
+  return r > 0 ? r * (r + 1) / 2 : 0;
+}
+// Offset of entry (n, mu) of order kk: the kk = 1 entries come first, each in (n, mu) order.
+__device__ __forceinline__ int l2p_uv_index(int P, int kk, int n, int mu)
+{
+  return (kk == 2 ? l2p_uv_count(P, 1) : 0) + tri(n - kk - 1) + mu - 1;
+}
+
 template <int PC>
 __global__ __launch_bounds__(WG) void k_l2p(int Prt, int NT, const int32_t *__restrict__ leaves, int64_t nleaf,
                                             const int32_t *__restrict__ cbody, const int32_t *__restrict__ cnbody,
@@ -1119,18 +1132,45 @@ __global__ __launch_bounds__(WG) void k_l2p(int Prt, int NT, const int32_t *__re
 {
   const int P = PC > 0 ? PC : Prt;
   extern __shared__ double2 sm2[];
-  double2      *Lc = sm2;
+  double2      *Lc = sm2;       // [NT]
+  double2      *UV = sm2 + NT;  // [l2p_uv_count(P, 1) + l2p_uv_count(P, 2)][3][U, V]
   const int64_t w  = blockIdx.x;
   if (w >= nleaf) return;
   const int c = leaves[w], b0 = cbody[c], nb = cnbody[c];
   for (int q = threadIdx.x; q < NT; q += WG) Lc[q] = L[(int64_t)c * NT + q];
+  __syncthreads();
+  // Output (j,k), k >= 1, takes the terms mu and -mu of Y[nu, mu] = (p, q) together:
+  //   L[n, mu+k] Y + W conj(Y) = (p U.x - q V.y, q V.x + p U.y),  n = nu + j,
+  // with W the sign-adjusted entry of the -mu term, U = L[n, mu+k] + W and V = L[n, mu+k] - W.
+  // U and V depend on the leaf only, so they are formed here once for all its bodies, which then
+  // take 4 multiply-adds per (nu, mu) instead of 8.
+  const int nuv1 = l2p_uv_count(P, 1), nuv = nuv1 + l2p_uv_count(P, 2);
+  for (int e = threadIdx.x; e < 3 * nuv; e += WG) {
+    const int ent = e / 3, d = e % 3;
+    const int kk  = ent < nuv1 ? 1 : 2;
+    int       r = kk == 1 ? ent : ent - nuv1, n = kk + 1;
+    while (r >= n - kk) r -= n - kk, n++;
+    const int     mu = r + 1, m2 = kk - mu;
+    const double2 l1 = Lc[3 * (tri(n) + mu + kk) + d];
+    double2       wv;
+    if (m2 < 0) {  // conj(L[n, mu-k]) (-1)^k
+      const double2 l = Lc[3 * (tri(n) - m2) + d];
+      wv              = make_double2(l.x * sgn(kk), -l.y * sgn(kk));
+    } else {  // L[n, k-mu] (-1)^mu
+      const double2 l = Lc[3 * (tri(n) + m2) + d];
+      wv              = make_double2(l.x * sgn(mu), l.y * sgn(mu));
+    }
+    UV[2 * e + 0] = make_double2(l1.x + wv.x, l1.y + wv.y);
+    UV[2 * e + 1] = make_double2(l1.x - wv.x, l1.y - wv.y);
+  }
   __syncthreads();
   const double X = cx[c], Yc = cy[c], Z = cz[c];
   for (int i = threadIdx.x; i < nb; i += WG) {
     const int64_t bi = b0 + i;
     const double  xb[3] = {st[(F_X + 0) * cap + bi], st[(F_X + 1) * cap + bi], st[(F_X + 2) * cap + bi]};
     const Sph     s     = sph(xb[0] - X, xb[1] - Yc, xb[2] - Z);
-    // outputs (j,k) = (0,0),(1,0),(1,1),(2,0),(2,1),(2,2) for each component d
+    // outputs (j,k) = (1,0),(1,1),(2,0),(2,1),(2,2) in slots 1..5 for each component d; the
+    // potential itself (slot 0) is not needed
     double2 o[6][3];
     for (int a = 0; a < 6; a++)
       for (int d = 0; d < 3; d++) o[a][d] = make_double2(0, 0);
@@ -1158,25 +1198,27 @@ __global__ __launch_bounds__(WG) void k_l2p(int Prt, int NT, const int32_t *__re
         }
         const double  v  = f * p;
         const double2 yp = make_double2(v * e.x, v * e.y);  // Y[nu, +mu]
-        for (int sgm = 0; sgm < (mu > 0 ? 2 : 1); sgm++) {
-          const int     mup = sgm ? -mu : mu;
-          const double2 yv  = sgm ? conj2(yp) : yp;
-          for (int a = 0; a < 6; a++) {
-            const int j = OJ[a], k = OK[a];
-            const int n = nu + j;
-            if (n >= P) continue;
-            const int m = mup + k;
-            double    sg;
-            double2   lv[3];
-            if (m < 0) {
-              sg = sgn(k);
-              for (int d = 0; d < 3; d++) lv[d] = conj2(Lc[3 * (tri(n) - m) + d]);
-            } else {
-              sg = (m < k) ? sgn(m - k) : 1.0;
-              for (int d = 0; d < 3; d++) lv[d] = Lc[3 * (tri(n) + m) + d];
+        for (int a = 1; a < 6; a++) {
+          const int j = OJ[a], k = OK[a];
+          const int n = nu + j;
+          if (n >= P) continue;
+          if (mu == 0) {  // the single term m = k (sign +1)
+            for (int d = 0; d < 3; d++) cfma(o[a][d], Lc[3 * (tri(n) + k) + d], yp);
+          } else if (k == 0) {  // the -mu term is the conjugate of the +mu term: 2 Re(L[n, mu] Y)
+            for (int d = 0; d < 3; d++) {
+              const double2 l = Lc[3 * (tri(n) + mu) + d];
+              o[a][d].x       = fma(l.x, 2.0 * yp.x, o[a][d].x);
+              o[a][d].x       = fma(-l.y, 2.0 * yp.y, o[a][d].x);
             }
-            const double2 ys = make_double2(yv.x * sg, yv.y * sg);
-            for (int d = 0; d < 3; d++) cfma(o[a][d], lv[d], ys);
+          } else {
+            const double2 *uv = UV + 6 * l2p_uv_index(P, k, n, mu);
+            for (int d = 0; d < 3; d++) {
+              const double2 U = uv[2 * d], V = uv[2 * d + 1];
+              o[a][d].x       = fma(yp.x, U.x, o[a][d].x);
+              o[a][d].x       = fma(-yp.y, V.y, o[a][d].x);
+              o[a][d].y       = fma(yp.y, V.x, o[a][d].y);
+              o[a][d].y       = fma(yp.x, U.y, o[a][d].y);
+            }
           }
         }
       }
@@ -1516,8 +1558,10 @@ void l2p_stretch(hipStream_t s, int P, const Tree &t, const double2 *L, double *
 {
   const int NT = 3 * P * (P + 1) / 2;
   if (t.nleaf == 0) return;
+  // local expansion, then the U/V table of k_l2p: 3 components x (U, V) per (order, n, mu)
+  const size_t lds = sizeof(double2) * (NT + 6 * (l2p_uv_count(P, 1) + l2p_uv_count(P, 2)));
   auto launch = [&](auto kern) {
-    kern<<<(int)t.nleaf, WG, sizeof(double2) * NT, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(),
+    kern<<<(int)t.nleaf, WG, lds, s>>>(P, NT, t.leaves.as<int32_t>(), t.nleaf, t.body.as<int32_t>(),
                                                        t.nbody.as<int32_t>(), t.cx.as<double>(), t.cy.as<double>(),
                                                        t.cz.as<double>(), L, st, cap, stretching, formulation,
                                                        sensor_mode);
