@@ -269,7 +269,10 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(MIXED ? 5 :
       __syncthreads();
       for (int k = 0; k < 12; k++) red[k * BLK + lane] = acc[k];
       __syncthreads();
+      // not unrolled: unrolled, the compiler kept twelve loop-invariant LDS addresses from the
+      // kernel start and spilled them to scratch (an extra 3 GB of HBM traffic per 4M launch)
       if (si == 0)
+#pragma unroll 1
         for (int k = 0; k < (WITH_J ? 12 : 3); k++) {
           double v = 0.0;
           for (int q = 0; q < S; q++) v += red[k * BLK + ti + q * ntp];
