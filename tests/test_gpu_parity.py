@@ -151,12 +151,14 @@ def test_fmm_fp64_parity(orc, name):
     assert rel(L1, L0) < 1e-10
 
 
-@pytest.mark.parametrize("P,S,Wv", [(4, 0, 2), (12, 0, 2), (13, 0, 2), (16, 0, 2), (10, 4, 2), (10, 5, 2), (10, 6, 2),
-                                    (10, 12, 2), (10, 6, 1), (12, 0, 1), (16, 0, 1)])
+@pytest.mark.parametrize("P,S,Wv", [(2, 0, 1), (3, 0, 2), (4, 0, 2), (6, 0, 1), (12, 0, 2), (13, 0, 2), (16, 0, 2),
+                                    (10, 4, 2), (10, 5, 2), (10, 6, 2), (10, 12, 2), (10, 6, 1), (12, 0, 1),
+                                    (16, 0, 1)])
 def test_fmm_fp64_parity_orders_and_plans(orc, P, S, Wv, monkeypatch):
     """Expansion orders away from the default (P = 13 and 16 need a third coefficient per lane
-    in P2M/M2M/L2L; P = 4 runs with a single home slot), the M2L plan sizes 4, 5, 6 and 12, and
-    one or two wavefronts per M2L workgroup (the second takes the odd source degrees)."""
+    in P2M/M2M/L2L and take the M2M without its term table; P = 2 and 3 leave the L2P's table of
+    paired +-mu terms empty or with one entry; P = 4 runs with a single home slot), the M2L plan
+    sizes 4, 5, 6 and 12, and one or two wavefronts per M2L workgroup (each with its own degrees)."""
     if S:
         monkeypatch.setenv("UVLM_M2L_SLOTS", str(S))
     monkeypatch.setenv("UVLM_M2L_WAVES", str(Wv))
