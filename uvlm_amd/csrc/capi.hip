@@ -196,7 +196,7 @@ void do_eval(uvlm_handle h, bool sync)
   p2p(s, t, t, h->lists, h->st.as<double>(), h->cap, h->st.as<double>(), h->cap, h->st.as<double>(), h->cap,
       h->prm.precision, 0, h->prm.ncrit);
   UVLM_HIP(hipEventRecord(h->ev[EV_P2P], s));
-  for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L, h->sched.octY.as<double2>());
+  for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L, h->sched);
   l2p_stretch(s, P, t, L, h->st.as<double>(), h->cap, h->prm.stretching, h->prm.formulation, 0);
   UVLM_HIP(hipEventRecord(h->ev[EV_DOWN], s));
   h->evaluated = true;
@@ -304,7 +304,7 @@ void sensor_pass(uvlm_handle h, int64_t m, bool with_j = false)
   m2l(s, P, h->stree, h->tree, h->slists, h->sched, h->M.as<double2>(), Ls, &h->sc);
   p2p(s, h->stree, h->tree, h->slists, h->sst.as<double>(), sc, h->st.as<double>(), h->cap, h->sst.as<double>(), sc,
       h->prm.precision, with_j ? 0 : 1, h->prm.ncrit, &h->sc);
-  for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls, h->sched.octY.as<double2>());
+  for (int l = 1; l < h->stree.nlevels; l++) l2l_level(s, P, h->stree, l, Ls, h->sched);
   l2p_stretch(s, P, h->stree, Ls, h->sst.as<double>(), sc, h->prm.stretching, h->prm.formulation, 1);
 }
 
@@ -409,7 +409,7 @@ void probe_adjoint_core(uvlm_handle h, int64_t m)
   double2 *La = h->L.as<double2>();
   m2l(s, P, h->tree, h->stree, h->alists, h->sched, Ms, La);
   zero_doubles(s, (double *)La, 2 * NT);  // the particle root multipole is never formed forward
-  for (int l = 1; l < h->tree.nlevels; l++) l2l_level(s, P, h->tree, l, La, h->sched.octY.as<double2>());
+  for (int l = 1; l < h->tree.nlevels; l++) l2l_level(s, P, h->tree, l, La, h->sched);
   p2p(s, h->tree, h->stree, h->alists, a, cap, h->sst.as<double>(), sc, a, cap, h->prm.precision, 2, h->prm.ncrit);
   l2p_stretch(s, P, h->tree, La, a, cap, UVLM_TRANSPOSED, UVLM_CVPM, 0);
   h->evaluated = false;  // L now holds adjoint locals
@@ -479,7 +479,7 @@ void vjp_core(uvlm_handle h, double *sbar = nullptr)
   for (int l = t.nlevels - 2; l >= 1; l--) m2m_level(s, P, t, l, M, h->sched);
   zero_doubles(s, (double *)M, 2 * NT);
   m2l(s, P, t, t, h->tlists, h->sched, M, L);
-  for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L, h->sched.octY.as<double2>());
+  for (int l = 1; l < t.nlevels; l++) l2l_level(s, P, t, l, L, h->sched);
   h->ast.reserve(sizeof(double) * N_FIELDS * cap);
   double *a = h->ast.as<double>();
   zero_doubles(s, a + F_J * cap, 9 * cap);

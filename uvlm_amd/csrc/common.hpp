@@ -149,6 +149,8 @@ struct M2LSchedule {
   DevBuf  octY;        // regular harmonics of the 8 unit octant vectors (M2M, L2L)
   DevBuf  m2mtab;      // term table of the M2M for P <= 10: 64 counts, then (m2m_rp + m2m_rc) rows
   int     m2m_rp = 0, m2m_rc = 0;
+  DevBuf  l2ltab;      // term table of the L2L for P = 6, 8, 10: 32 entries per lane
+  bool    l2l_tab = false;
   int     ytab_P = 0;
 };
 // Half-width of the cube of integer offsets (in units of the smaller cell's half-width) whose
@@ -167,7 +169,7 @@ void p2m(hipStream_t s, int P, const Tree &t, const double *st, int64_t cap, dou
 void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M, const M2LSchedule &sch);
 void m2l(hipStream_t s, int P, const Tree &tt, const Tree &st, const Lists &l, const M2LSchedule &sch, const double2 *M,
          double2 *L, Scratch *sc = nullptr);
-void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L, const double2 *octY);
+void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L, const M2LSchedule &sch);
 // adjoint of the particle rates: vortex + vortex-dipole P2M (aux fields 0..11) and the far-field
 // target-side term (adds into aux fields 12..14)
 void p2m_adj(hipStream_t s, int P, const Tree &t, const double *st, const double *aux, int64_t cap, double2 *M);

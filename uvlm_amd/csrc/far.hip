@@ -557,6 +557,110 @@ __global__ __launch_bounds__(WG) void k_l2l_bal(int Prt, int NT, const int32_t *
   }
 }
 
+// k_l2l_bal with its terms read from a host-built table (make_l2l_table) for the fixed orders
+// P = 6, 8, 10: the same groups d, lane split s and butterfly, but each lane's terms over all
+// groups (at most L2L_TAB_ROWS; 31 at P = 10) are 16-bit entries loaded into registers at kernel
+// start: q' | v << 6 | conj << 14, with Lp entry q' (conjugated when conj is set) times copy v
+// of the child's octant harmonics (Y, -Y, conj Y, -conj Y in LDS).  A term is then three LDS
+// reads of Lp, one of Y and three complex multiply-adds, without the per-term row search, sign
+// and conjugation branches of k_l2l_bal.
+constexpr int L2L_TAB_ROWS = 32;
+// Lane split of group d (k_l2l_bal): s aligned lanes per output, s a power of two, at most 16.
+__host__ __device__ __forceinline__ int l2l_split(int P, int d)
+{
+  const int ntask = (P - d) * (P - d + 1) / 2, nterm = 2 * d + 1;
+  int       s     = 1;
+  while (s < 16 && 2 * s * ntask <= WG && s < nterm) s *= 2;
+  return s;
+}
+template <int PC>
+__global__ __launch_bounds__(WG) void k_l2l_tab(int NT, const int32_t *__restrict__ cells, int64_t ncell,
+                                                const int32_t *__restrict__ cparent, const double *__restrict__ cx,
+                                                const double *__restrict__ cy, const double *__restrict__ cz,
+                                                const double *__restrict__ cR, const double2 *__restrict__ octY,
+                                                const uint4 *__restrict__ tab, double2 *__restrict__ L)
+{
+  constexpr int P = PC, nq = PC * (PC + 1) / 2;
+  static_assert(nq <= WG, "one output per lane");
+  extern __shared__ double2 sm2[];
+  double2      *Yv = sm2;           // [4][nq]
+  double2      *Lp = sm2 + 4 * nq;  // [NT]
+  const int64_t w  = blockIdx.x;
+  if (w >= ncell) return;
+  const int c = cells[w], p = cparent[c];
+  if (p < 0) return;
+  const int lane = threadIdx.x;
+  uint4     tw[L2L_TAB_ROWS / 8];  // this lane's entries, two per 32-bit word
+  for (int i = 0; i < L2L_TAB_ROWS / 8; i++) tw[i] = tab[lane * (L2L_TAB_ROWS / 8) + i];
+  const int    o = (cx[c] > cx[p]) | ((cy[c] > cy[p]) << 1) | ((cz[c] > cz[p]) << 2);
+  const double r = cR[c];
+  for (int q = lane; q < nq; q += WG) {
+    int n = 0;
+    while (tri(n + 1) <= q) n++;
+    double f = 1.0;
+    for (int i = 0; i < n; i++) f *= r;
+    const double2 y = octY[o * nq + q];
+    const double  a = y.x * f, b = y.y * f;
+    Yv[q]          = make_double2(a, b);
+    Yv[nq + q]     = make_double2(-a, -b);
+    Yv[2 * nq + q] = make_double2(a, -b);
+    Yv[3 * nq + q] = make_double2(-a, b);
+  }
+  for (int q = lane; q < NT; q += WG) Lp[q] = L[(int64_t)p * NT + q];
+  __syncthreads();
+  auto entry = [&](int row) -> int {
+    const uint4    v = tw[row >> 3];
+    const unsigned x = ((row >> 1) & 3) == 0 ? v.x : ((row >> 1) & 3) == 1 ? v.y : ((row >> 1) & 3) == 2 ? v.z : v.w;
+    return (row & 1) ? (int)(x >> 16) : (int)(x & 0xffff);
+  };
+  auto term = [&](int e, double2 *acc) {
+    const double2  y  = Yv[(e >> 6) & 255];
+    const double2 *lp = Lp + 3 * (e & 63);
+    const double   sy = (e >> 14) ? -1.0 : 1.0;
+    for (int k = 0; k < 3; k++) {
+      const double2 l = lp[k];
+      cfma(acc[k], make_double2(l.x, l.y * sy), y);
+    }
+  };
+  double2 acc[3];
+  for (int e = 0; e < 3; e++) acc[e] = make_double2(0, 0);
+  int row = 0;
+#pragma unroll
+  for (int d = 0; d < P; d++) {
+    const int ntask = tri(P - d), nterm = 2 * d + 1, s = l2l_split(P, d);
+    if (s == 1) {
+#pragma unroll
+      for (int i = 0; i < nterm; i++)
+        if (lane < ntask) term(entry(row + i), acc);
+      row += nterm;
+    } else {  // ntask <= WG / 2: outputs q < 32
+      const int t = lane / s, sub = lane % s, nr = (nterm + s - 1) / s;
+      double2   part[3];
+      for (int e = 0; e < 3; e++) part[e] = make_double2(0, 0);
+#pragma unroll
+      for (int i = 0; i < nr; i++)
+        if (t < ntask && sub + i * s < nterm) term(entry(row + i), part);
+      row += nr;
+      for (int off = s / 2; off > 0; off /= 2)
+        for (int e = 0; e < 3; e++) {
+          part[e].x += __shfl_xor(part[e].x, off);
+          part[e].y += __shfl_xor(part[e].y, off);
+        }
+      const int src = lane < ntask ? lane * s : lane;
+      for (int e = 0; e < 3; e++) {
+        const double vx = __shfl(part[e].x, src), vy = __shfl(part[e].y, src);
+        if (lane < ntask) acc[e].x += vx, acc[e].y += vy;
+      }
+    }
+  }
+  if (lane < nq)
+    for (int e = 0; e < 3; e++) {
+      double2 &out = L[(int64_t)c * NT + 3 * lane + e];
+      out.x += acc[e].x;
+      out.y += acc[e].y;
+    }
+}
+
 // ------------------------------------------------------------------ adjoint of the particle rates
 // Adjoint sources of the far field (see uvlm_eval_vjp): at each body a vortex ubar (3) and
 // vortex dipoles Jbar (3x3, row d = component, column c = direction), so that
@@ -1339,6 +1443,46 @@ static void make_m2m_table(int P, std::vector<int32_t> &cnt, std::vector<uint16_
   }
 }
 
+// Term table of k_l2l_tab: for each lane, its terms over the groups d of k_l2l_bal in the order
+// k_l2l_bal takes them (row offsets fixed by P), L2L_TAB_ROWS 16-bit entries per lane.
+static bool make_l2l_table(int P, std::vector<uint16_t> &tab)
+{
+  const int nq = P * (P + 1) / 2;
+  auto tr  = [](int n) { return n * (n + 1) / 2; };
+  auto neg = [](int e) { return (e & 1) != 0; };
+  tab.assign((size_t)WG * L2L_TAB_ROWS, 0);
+  // term i of output q in group d: n = j + d, m = k - d + i, Y[d, m - k] (k_l2l_bal's terms)
+  auto entry = [&](int q, int d, int i) {
+    int j = 0;
+    while (tr(j + 1) <= q) j++;
+    const int  k = q - tr(j), n = j + d, m = k - d + i, mu = m - k;
+    const int  yidx  = tr(d) + (mu >= 0 ? mu : -mu);
+    const bool conjY = mu < 0;  // ylook(Y, d, mu)
+    bool       minus, conjL;
+    int        lq;
+    if (m < 0) minus = neg(k), conjL = true, lq = tr(n) - m;
+    else minus = m < k ? neg(m - k) : false, conjL = false, lq = tr(n) + m;
+    const int v = ((conjY ? 2 : 0) + (minus ? 1 : 0)) * nq + yidx;
+    return (uint16_t)(lq | (v << 6) | ((conjL ? 1 : 0) << 14));
+  };
+  int row = 0;
+  for (int d = 0; d < P; d++) {
+    const int ntask = tr(P - d), nterm = 2 * d + 1, s = l2l_split(P, d);
+    const int nr = s == 1 ? nterm : (nterm + s - 1) / s;
+    if (row + nr > L2L_TAB_ROWS) return false;
+    for (int lane = 0; lane < WG; lane++) {
+      const int q = s == 1 ? lane : lane / s, first = s == 1 ? 0 : lane % s;
+      if (q >= ntask) continue;
+      for (int i = 0; i < nr; i++) {
+        const int t = first + i * s;
+        if (t < nterm) tab[(size_t)lane * L2L_TAB_ROWS + row + i] = entry(q, d, t);
+      }
+    }
+    row += nr;
+  }
+  return true;
+}
+
 void make_m2l_schedule(int P, M2LSchedule &s)
 {
   // Default budget: with two wavefronts, 5 slots run at 4 waves per SIMD at P = 10.
@@ -1365,6 +1509,15 @@ void make_m2l_schedule(int P, M2LSchedule &s)
     s.octY.reserve(sizeof(double2) * 8 * P * (P + 1) / 2);
     k_octant_table<<<8, WG>>>(P, s.octY.as<double2>());
     UVLM_LAUNCH_CHECK();
+    s.l2l_tab = false;
+    if (P == 6 || P == 8 || P == 10) {
+      std::vector<uint16_t> tab;
+      if (make_l2l_table(P, tab)) {
+        s.l2ltab.reserve(2 * tab.size());
+        UVLM_HIP(hipMemcpy(s.l2ltab.p, tab.data(), 2 * tab.size(), hipMemcpyHostToDevice));
+        s.l2l_tab = true;
+      }
+    }
     s.m2m_rp = s.m2m_rc = 0;
     if (P <= M2M_TAB_MAXP) {
       std::vector<int32_t>  cnt;
@@ -1444,11 +1597,25 @@ void m2m_level(hipStream_t s, int P, const Tree &t, int level, double2 *M, const
   UVLM_LAUNCH_CHECK();
 }
 
-void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L, const double2 *octY)
+void l2l_level(hipStream_t s, int P, const Tree &t, int level, double2 *L, const M2LSchedule &sch)
 {
   const int     NT = 3 * P * (P + 1) / 2;
   const int64_t o = t.lvl_off[level], n = t.lvl_off[level + 1] - o;
   if (n <= 0) return;
+  const double2 *octY = sch.octY.as<double2>();
+  if (sch.l2l_tab && (P == 6 || P == 8 || P == 10)) {
+    const size_t lds = sizeof(double2) * (4 * P * (P + 1) / 2 + NT);
+    auto launch = [&](auto kern) {
+      kern<<<(int)n, WG, lds, s>>>(NT, t.lvl_cells.as<int32_t>() + o, n, t.parent.as<int32_t>(), t.cx.as<double>(),
+                                   t.cy.as<double>(), t.cz.as<double>(), t.R.as<double>(), octY,
+                                   reinterpret_cast<const uint4 *>(sch.l2ltab.p), L);
+    };
+    if (P == 6) launch(k_l2l_tab<6>);
+    else if (P == 8) launch(k_l2l_tab<8>);
+    else launch(k_l2l_tab<10>);
+    UVLM_LAUNCH_CHECK();
+    return;
+  }
   const size_t lds = sizeof(double2) * (P * (P + 1) / 2 + NT);
   auto launch = [&](auto kern) {
     kern<<<(int)n, WG, lds, s>>>(P, NT, t.lvl_cells.as<int32_t>() + o, n, t.parent.as<int32_t>(), t.cx.as<double>(),
