@@ -565,6 +565,19 @@ __global__ __launch_bounds__(WG) void k_l2l_bal(int Prt, int NT, const int32_t *
 // reads of Lp, one of Y and three complex multiply-adds, without the per-term row search, sign
 // and conjugation branches of k_l2l_bal.
 constexpr int L2L_TAB_ROWS = 32;
+// v + v of the partner lane at distance OFF in a butterfly over aligned groups of up to 16 lanes,
+// by DPP moves instead of LDS permutes: OFF = 1 and 2 swap within quads; OFF = 4 and 8 take
+// the mirror lane of the 8- and 16-lane half row, which pairs the same partial sums once the
+// smaller steps are done (every lane then holds its group's sum, as after the xor butterfly).
+template <int OFF>
+__device__ __forceinline__ double dpp_pair_add(double v)
+{
+  constexpr int ctrl = OFF == 1 ? 0xB1 : OFF == 2 ? 0x4E : OFF == 4 ? 0x141 : 0x140;
+  const int     lo = __double2loint(v), hi = __double2hiint(v);
+  const int     plo = __builtin_amdgcn_mov_dpp(lo, ctrl, 0xF, 0xF, false);
+  const int     phi = __builtin_amdgcn_mov_dpp(hi, ctrl, 0xF, 0xF, false);
+  return v + __hiloint2double(phi, plo);
+}
 // Lane split of group d (k_l2l_bal): s aligned lanes per output, s a power of two, at most 16.
 __host__ __device__ __forceinline__ int l2l_split(int P, int d)
 {
@@ -641,11 +654,12 @@ __global__ __launch_bounds__(WG) void k_l2l_tab(int NT, const int32_t *__restric
       for (int i = 0; i < nr; i++)
         if (t < ntask && sub + i * s < nterm) term(entry(row + i), part);
       row += nr;
-      for (int off = s / 2; off > 0; off /= 2)
-        for (int e = 0; e < 3; e++) {
-          part[e].x += __shfl_xor(part[e].x, off);
-          part[e].y += __shfl_xor(part[e].y, off);
-        }
+      for (int e = 0; e < 3; e++) {
+        if (s >= 2) part[e].x = dpp_pair_add<1>(part[e].x), part[e].y = dpp_pair_add<1>(part[e].y);
+        if (s >= 4) part[e].x = dpp_pair_add<2>(part[e].x), part[e].y = dpp_pair_add<2>(part[e].y);
+        if (s >= 8) part[e].x = dpp_pair_add<4>(part[e].x), part[e].y = dpp_pair_add<4>(part[e].y);
+        if (s >= 16) part[e].x = dpp_pair_add<8>(part[e].x), part[e].y = dpp_pair_add<8>(part[e].y);
+      }
       const int src = lane < ntask ? lane * s : lane;
       for (int e = 0; e < 3; e++) {
         const double vx = __shfl(part[e].x, src), vy = __shfl(part[e].y, src);
