@@ -68,6 +68,48 @@ def test_tree_and_lists_bitexact(orc, name):
     assert t["n_interactions"] == int(np.sum(nbody[rp[:, 0]] * nbody[rp[:, 1]])), t["n_interactions"]
 
 
+def test_build_defers_the_output_permutation():
+    """uvlm_build permutes the fields an evaluation overwrites (J, velocity, dadt, drdt) only when
+    something reads them before the next evaluation. After a rebuild on moved bodies the last
+    evaluation's outputs must follow their bodies exactly, through the SoA and AoS readers, also
+    after two rebuilds in a row with nothing in between, and an evaluation after a deferred
+    build must match one after a full build."""
+    X, A, S = W.scaled("cube", 6000)
+    g = FMMCalculator(P=6, theta=0.5, ncrit=32)
+    g.set_arrays(X, A, S)
+    g.build()
+    g.calculate()
+    a1, o1 = g.arrays(), g.permutation()
+    pos1 = np.empty_like(o1)
+    pos1[o1] = np.arange(len(o1))
+    rng = np.random.default_rng(3)
+    X2 = a1["X"] + rng.normal(scale=0.05, size=a1["X"].shape)
+    g.update_arrays(X=X2)
+    g.build()
+    g.build()  # the second build applies the first one's pending permutation first
+    o2 = g.permutation()
+    assert not np.array_equal(o1, o2)
+    src = pos1[o2]  # tree position before the rebuilds of the body now at each position
+    a2 = g.arrays()
+    for k in ("velocity", "J", "dadt", "drdt"):
+        assert np.array_equal(a2[k], a1[k][src]), k
+    assert np.array_equal(a2["X"], X2[src])
+    g.build()
+    b = g.copyBodies2Host()
+    o3 = g.permutation()
+    pos2 = np.empty_like(o2)
+    pos2[o2] = np.arange(len(o2))
+    src3 = src[pos2[o3]]
+    assert np.array_equal(b[:, 15:18], a1["velocity"][src3]) and np.array_equal(b[:, 0:9], a1["J"][src3])
+    # an evaluation after a deferred build equals one after a full build (set_bodies resets)
+    g.calculate()
+    u_def = g.arrays()["velocity"]
+    h = FMMCalculator(P=6, theta=0.5, ncrit=32)
+    h.init(b.copy())
+    h.calculate()
+    assert np.array_equal(u_def, h.arrays()["velocity"])
+
+
 def test_tree_rebuilds_on_one_handle_stay_bitexact(orc):
     """A handle sorts only the key digits its previous tree used (plus one level); a much deeper
     tree on the next build (uniform cube -> clustered blobs) must fall back to the full sort, and

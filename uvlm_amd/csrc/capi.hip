@@ -50,6 +50,9 @@ struct uvlm_handle_s {
   DevBuf      M, L;
   M2LSchedule sched;
   bool        built = false, evaluated = false;
+  // uvlm_build permutes the fields the evaluation overwrites (J, velocity, dadt, drdt) only when
+  // something other than uvlm_eval needs them: until then they stay in st2 in the previous order
+  bool        out_deferred = false;
   hipEvent_t  ev[EV_COUNT]{};
   uvlm_timing tm{};
   // probe (sensor) evaluation and its adjoint
@@ -145,17 +148,42 @@ void iota(hipStream_t s, int32_t *p, int64_t n)
   UVLM_LAUNCH_CHECK();
 }
 
-void do_build(uvlm_handle h)
+// Fields of the state that an evaluation overwrites for every body (J, velocity, dadt, drdt)
+// and the rest, as [first, last) field ranges.
+constexpr int OUT_FIELDS[3][2]   = {{F_J, F_J + 9}, {F_U, F_DADT + 3}, {F_DSIG, F_DSIG + 1}};
+constexpr int STATE_FIELDS[3][2] = {{F_X, F_A + 3}, {F_UOLD, F_SIG + 1}, {F_SIGOLD, N_FIELDS}};
+
+// Applies a deferred build permutation to the output fields (from the previous order in st2).
+void resolve_outputs(uvlm_handle h)
+{
+  if (!h->out_deferred) return;
+  UVLM_HIP(hipSetDevice(h->prm.device));
+  for (const auto &r : OUT_FIELDS)
+    gather_fields(h->s, h->st2.as<double>() + (int64_t)r[0] * h->cap, h->st.as<double>() + (int64_t)r[0] * h->cap,
+                  h->cap, h->cap, r[1] - r[0], h->perm.as<int32_t>(), h->N);
+  h->out_deferred = false;
+}
+
+// defer_outputs: permute only the fields the next evaluation does not overwrite (uvlm_build);
+// resolve_outputs permutes the rest if anything else touches the state first.
+void do_build(uvlm_handle h, bool defer_outputs = false)
 {
   Region      region("Tree Construction on CPU");  // fmmCalculator.hpp:73
   if (h->N <= 0) throw Error(UVLM_ERR_STATE, "no bodies set");
   hipStream_t s = h->s;
   UVLM_HIP(hipEventRecord(h->ev[EV_B0], s));
+  resolve_outputs(h);
   build_tree(s, h->sc, h->F(F_X), h->F(F_X + 1), h->F(F_X + 2), h->N, h->prm.ncrit, h->tree, h->perm.as<int32_t>());
-  gather_fields(s, h->st.as<double>(), h->st2.as<double>(), h->cap, h->cap, N_FIELDS, h->perm.as<int32_t>(), h->N);
+  if (defer_outputs)
+    for (const auto &r : STATE_FIELDS)
+      gather_fields(s, h->st.as<double>() + (int64_t)r[0] * h->cap, h->st2.as<double>() + (int64_t)r[0] * h->cap,
+                    h->cap, h->cap, r[1] - r[0], h->perm.as<int32_t>(), h->N);
+  else
+    gather_fields(s, h->st.as<double>(), h->st2.as<double>(), h->cap, h->cap, N_FIELDS, h->perm.as<int32_t>(), h->N);
   gather_i32(s, h->orig.as<int32_t>(), h->orig2.as<int32_t>(), h->perm.as<int32_t>(), h->N);
   swap_buf(h->st, h->st2);
   swap_buf(h->orig, h->orig2);
+  h->out_deferred = defer_outputs;
   UVLM_HIP(hipEventRecord(h->ev[EV_TREE], s));
   h->gap = h->near_c > 0.0 ? h->near_c * max_value(s, h->sc, h->F(F_SIG), h->N) : 0.0;
   traverse(s, h->sc, h->tree, h->tree, h->prm.theta, h->lists, h->gap);
@@ -180,6 +208,7 @@ void do_eval(uvlm_handle h, bool sync)
 {
   Region      region("Calculation of FMM");  // fmmCalculator.hpp:422
   if (!h->built) throw Error(UVLM_ERR_STATE, "uvlm_eval before uvlm_build");
+  h->out_deferred = false;  // P2P stores and L2P completes every output field of every body
   hipStream_t s = h->s;
   const int   P = h->prm.P, NT = h->NT;
   const Tree &t = h->tree;
@@ -347,6 +376,7 @@ void velocity_grad_at_points(uvlm_handle h, int64_t m, const double *pts, double
 // are kept (the field stride changes).
 void grow_particles(uvlm_handle h, int64_t want)
 {
+  resolve_outputs(h);
   const int64_t N = h->N;
   if (want > h->cap) {
     const int64_t ncap = std::max<int64_t>(want, h->cap + h->cap / 2);
@@ -628,7 +658,15 @@ void download_cot(uvlm_handle h, const double *dev, double *xb, double *ab)
     }
 }
 
+// The handle check of the entry points that read or change the state: a permutation that
+// uvlm_build left pending on the output fields is applied first.
 void check_handle(uvlm_handle h)
+{
+  if (!h) throw Error(UVLM_ERR_ARG, "null handle");
+  resolve_outputs(h);
+}
+// The handle check of uvlm_build, uvlm_eval and the queries that leave the state alone.
+void check_handle_keep(uvlm_handle h)
 {
   if (!h) throw Error(UVLM_ERR_ARG, "null handle");
 }
@@ -848,16 +886,16 @@ int uvlm_build(uvlm_handle h)
 {
   Region region("Initialization of FMM");  // fmmCalculator.hpp:284
   return guarded([&] {
-    check_handle(h);
+    check_handle_keep(h);
     set_device(h);
-    do_build(h);
+    do_build(h, true);
   });
 }
 
 int uvlm_eval(uvlm_handle h)
 {
   return guarded([&] {
-    check_handle(h);
+    check_handle_keep(h);
     set_device(h);
     do_eval(h, true);
   });
@@ -902,7 +940,7 @@ int uvlm_num_bodies(uvlm_handle h, int64_t *n)
 int uvlm_num_cells(uvlm_handle h, int64_t *n_cells)
 {
   return guarded([&] {
-    check_handle(h);
+    check_handle_keep(h);
     if (!h->built) throw Error(UVLM_ERR_STATE, "no tree");
     *n_cells = h->tree.C;
   });
@@ -1545,6 +1583,7 @@ void check_pair(uvlm_wing w, uvlm_handle h)
 {
   if (!w) throw Error(UVLM_ERR_ARG, "null wing");
   if (h && h->prm.device != w->device) throw Error(UVLM_ERR_ARG, "wing and particles on different devices");
+  if (h) resolve_outputs(h);
 }
 
 // main.cpp:196-244 at the wing's current position: particle velocity at the control points,
@@ -1975,7 +2014,7 @@ int uvlm_timings(uvlm_handle h, uvlm_timing *out)
 int uvlm_sync(uvlm_handle h)
 {
   return guarded([&] {
-    check_handle(h);
+    check_handle_keep(h);
     UVLM_HIP(hipStreamSynchronize(h->s));
   });
 }
